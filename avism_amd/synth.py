@@ -1,0 +1,161 @@
+"""Seeded synthetic inputs for the benchmark and test configurations.
+
+There is no network here, so every workload is generated:
+
+* ``zeldovich``   -- lattice particles displaced by a Gaussian random field with
+  P(k) ~ k^slope exp(-k^2/kc^2) (Zel'dovich approximation), rms displacement
+  fixed, velocities f*H0*psi, periodic box, positions wrapped to [-L/2, L/2).
+* ``mock_voids``  -- uniform particles in a non-periodic box thinned inside
+  spherical voids with delta(r) = -0.85 (1 - (r/R)^3) and a linear radial outflow
+  v = -(1/3) f H0 delta(r) r plus Gaussian velocity noise (the reference's test1
+  idea, test1/mock_voids.f90, with our own generator and profile).
+* ``grid_fields`` -- a log-normal density contrast and its linear-theory velocity
+  field on a grid, for grid-input runs (reader.f90:523-552).
+
+Arrays are float32 (positions Mpc/h, velocities km/s, masses Msun), as the
+reference's binary particle format stores them (reader.f90:447-453).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+__all__ = ["Particles", "zeldovich", "mock_voids", "grid_fields", "particle_mass"]
+
+RHO_CRIT = 2.775e11  # h^2 Msun / Mpc^3
+
+
+@dataclass
+class Particles:
+    x: torch.Tensor
+    y: torch.Tensor
+    z: torch.Tensor
+    vx: torch.Tensor
+    vy: torch.Tensor
+    vz: torch.Tensor
+    m: torch.Tensor
+    L0: float
+    periodic: int
+
+    @property
+    def n(self) -> int:
+        return int(self.x.numel())
+
+    def to(self, device) -> "Particles":
+        f = lambda t: t.to(device)
+        return Particles(f(self.x), f(self.y), f(self.z), f(self.vx), f(self.vy), f(self.vz),
+                         f(self.m), self.L0, self.periodic)
+
+    def numpy(self):
+        return [t.detach().cpu().numpy() for t in (self.x, self.y, self.z, self.vx, self.vy, self.vz, self.m)]
+
+
+def particle_mass(L0: float, np_total: int, omega_m: float = 0.31) -> float:
+    return omega_m * RHO_CRIT * L0 ** 3 / np_total
+
+
+def _kgrid(n: int, L: float, device):
+    k1 = torch.fft.fftfreq(n, d=L / n, device=device) * (2 * math.pi)
+    kr = torch.fft.rfftfreq(n, d=L / n, device=device) * (2 * math.pi)
+    kx = k1.view(n, 1, 1)
+    ky = k1.view(1, n, 1)
+    kz = kr.view(1, 1, -1)
+    return kx, ky, kz
+
+
+def _gaussian_k(n: int, L: float, seed: int, slope: float, kc: float, device):
+    """Fourier amplitudes of a real Gaussian field with P(k) ~ k^slope exp(-k^2/kc^2).
+
+    Index order of the returned arrays is (x, y, z); callers flatten them in
+    Fortran order (x fastest)."""
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    white = torch.randn((n, n, n), generator=g, device=device, dtype=torch.float32)
+    wk = torch.fft.rfftn(white)
+    del white
+    kx, ky, kz = _kgrid(n, L, device)
+    k2 = kx * kx + ky * ky + kz * kz
+    k2c = k2.clone()
+    k2c[0, 0, 0] = 1.0
+    amp = torch.pow(k2c, slope / 4.0) * torch.exp(-k2c / (2.0 * kc * kc))
+    amp[0, 0, 0] = 0.0
+    return wk * amp, (kx, ky, kz, k2c)
+
+
+def _fortran_flat(a: torch.Tensor) -> torch.Tensor:
+    """(x, y, z)-indexed array -> flat, x fastest."""
+    return a.permute(2, 1, 0).contiguous().view(-1)
+
+
+def zeldovich(n_side: int, L0: float, seed: int, kc: float = 0.5, slope: float = -1.5,
+              rms_disp: float = 3.0, f: float = 0.5, H0: float = 100.0,
+              device="cpu", omega_m: float = 0.31) -> Particles:
+    n = n_side
+    dk, (kx, ky, kz, k2) = _gaussian_k(n, L0, seed, slope, kc, device)
+    psi = []
+    for kj in (kx, ky, kz):
+        comp = torch.fft.irfftn(dk * (1j * kj / k2), s=(n, n, n))
+        psi.append(comp)
+    del dk
+    rms = torch.sqrt(sum((p.double() ** 2).mean() for p in psi)).item()
+    scale = rms_disp / max(rms, 1e-30)
+    q = (torch.arange(n, device=device, dtype=torch.float64) + 0.5) * (L0 / n) - L0 / 2.0
+    qs = (q.view(n, 1, 1), q.view(1, n, 1), q.view(1, 1, n))
+    out = []
+    vel = []
+    for j in range(3):
+        d = psi[j].double() * scale
+        pos = qs[j] + d
+        pos = torch.remainder(pos + L0 / 2.0, L0) - L0 / 2.0
+        out.append(_fortran_flat(pos.float()))
+        vel.append(_fortran_flat((d * (f * H0)).float()))
+    m = torch.full_like(out[0], particle_mass(L0, n ** 3, omega_m))
+    return Particles(out[0], out[1], out[2], vel[0], vel[1], vel[2], m, float(L0), 1)
+
+
+def mock_voids(n0: int = 1_000_000, L0: float = 200.0, nvoids: int = 40, rmin: float = 8.0,
+               rmax: float = 25.0, seed: int = 1, f: float = 0.5, H0: float = 100.0,
+               sigma_v: float = 50.0, omega_m: float = 0.31):
+    """Returns (Particles, centres[nvoids,3], radii[nvoids]) -- non-periodic box."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-L0 / 2, L0 / 2, size=(nvoids, 3))
+    radii = rng.uniform(rmin, rmax, size=nvoids)
+    pos = rng.uniform(-L0 / 2, L0 / 2, size=(n0, 3))
+    delta = np.zeros(n0)
+    vel = np.zeros((n0, 3))
+    for c, R in zip(centres, radii):
+        d = pos - c
+        r = np.sqrt((d * d).sum(1))
+        inside = r < R
+        dv = -0.85 * (1.0 - (r[inside] / R) ** 3)
+        # overlapping voids: keep the deepest contrast
+        upd = dv < delta[inside]
+        idx = np.nonzero(inside)[0][upd]
+        delta[idx] = dv[upd]
+        vel[idx] = (-(1.0 / 3.0) * f * H0 * dv[upd])[:, None] * d[idx]
+    keep = rng.uniform(size=n0) < (1.0 + delta)
+    pos = pos[keep]
+    vel = vel[keep] + rng.normal(0.0, sigma_v, size=vel[keep].shape)
+    npk = pos.shape[0]
+    m = np.full(npk, particle_mass(L0, npk, omega_m), dtype=np.float32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32))
+    parts = Particles(t(pos[:, 0]), t(pos[:, 1]), t(pos[:, 2]), t(vel[:, 0]), t(vel[:, 1]),
+                      t(vel[:, 2]), t(m), float(L0), 0)
+    return parts, centres, radii
+
+
+def grid_fields(n: int, L0: float, seed: int, sigma: float = 0.8, kc: float = 0.5,
+                slope: float = -1.5, f: float = 0.5, H0: float = 100.0, device="cpu"):
+    """Log-normal delta (> -1) and linear-theory velocity (km/s), flat x-fastest."""
+    dk, (kx, ky, kz, k2) = _gaussian_k(n, L0, seed, slope, kc, device)
+    g = torch.fft.irfftn(dk, s=(n, n, n)).double()
+    g = g * (sigma / g.std())
+    delta = torch.exp(g - 0.5 * sigma * sigma) - 1.0
+    dl = torch.fft.rfftn(delta)
+    vel = []
+    for kj in (kx, ky, kz):
+        vel.append(_fortran_flat(torch.fft.irfftn(dl * (1j * f * H0 * kj / k2), s=(n, n, n)).float()))
+    return _fortran_flat(delta.float()), vel[0], vel[1], vel[2]

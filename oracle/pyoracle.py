@@ -1,0 +1,394 @@
+"""ctypes bindings to the C oracle (oracle/avism_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+
+``run_level`` restates the reference driver's single-level flow on the CPU
+(avism.f90:1177-2085 for particle input, with reader.f90:523-552 for grid input)
+and returns every intermediate the GPU path is compared against.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle_avism.so"
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> Path:
+    if force or not LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+class _VP(C.Structure):
+    _fields_ = [("dens_thre", C.c_float), ("dens_thre2", C.c_float), ("grad_thre", C.c_float),
+                ("div_thre", C.c_float), ("rmin", C.c_float), ("rmin_sub", C.c_float),
+                ("nvoid_max", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    L.orc_grid_new.restype = vp
+    L.orc_grid_new.argtypes = [C.c_int32, C.c_float, C.c_int32]
+    L.orc_grid_free.argtypes = [vp]
+    L.orc_grid_info.argtypes = [vp, _i32p, _f32p, C.c_void_p]
+    L.orc_grid_rad.restype = C.c_float
+    L.orc_grid_rad.argtypes = [vp, C.c_int32]
+    L.orc_knn_new.restype = vp
+    L.orc_knn_new.argtypes = [C.c_int64, _f32p, _f32p, _f32p, C.c_float, C.c_int32]
+    L.orc_knn_free.argtypes = [vp]
+    L.orc_knn_query.restype = C.c_int32
+    L.orc_knn_query.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_int32, _i64p, _f32p]
+    L.orc_ball_query.restype = C.c_int64
+    L.orc_ball_query.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int64, _i64p, _f32p]
+    L.orc_convert_units.argtypes = [C.c_int64, _f32p, _f32p, _f32p, _f32p]
+    L.orc_ppart_dens.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, C.c_int32, _f32p, _f32p]
+    L.orc_vvel_vw.argtypes = [vp, vp, C.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_void_p,
+                              _f32p, _f32p, _f32p, _f32p]
+    L.orc_h_distance.argtypes = [vp, vp, C.c_int32, C.c_void_p, _f32p]
+    L.orc_fill_h_zeros.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, C.c_int32, _f32p]
+    L.orc_ddens_sph.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f64p]
+    L.orc_dens_tsc.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.orc_vel_tsc.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.orc_normalise.restype = C.c_float
+    L.orc_normalise.argtypes = [vp, C.c_float, C.c_float, C.c_float, _f32p, _f32p, C.POINTER(C.c_double)]
+    L.orc_diver_uniform.argtypes = [C.c_int32, C.c_float, _f32p, _f32p, _f32p, C.c_int32, _f32p]
+    L.orc_extend_f32.argtypes = [vp, _f32p, _f32p]
+    L.orc_extend_i32.argtypes = [vp, _i32p, _i32p]
+    L.orc_indexx.restype = C.c_int32
+    L.orc_indexx.argtypes = [C.c_int64, _f32p, _i32p]
+    L.orc_voidfind.restype = C.c_int32
+    L.orc_voidfind.argtypes = [vp, C.POINTER(_VP), _f32p, _f32p, _i32p, _i8p, C.c_void_p, _i32p, _f32p,
+                               _i32p, _i64p, C.c_int64, C.c_void_p, C.c_void_p]
+    L.orc_merge_post.restype = C.c_int32
+    L.orc_merge_post.argtypes = [vp, C.POINTER(_VP), C.c_int32, _i32p, _f32p, _i32p, _f32p, C.c_float,
+                                 C.c_float, _i32p, _i32p, _i32p, _f32p, _i32p, _i32p, _f64p, C.c_int32]
+    L.orc_cosmo.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                            C.POINTER(C.c_float)]
+    _lib = L
+    return L
+
+
+class Grid:
+    """MALLA geometry (grid.f90:3-76) with the PBC extension (avism.f90:780-865)."""
+
+    def __init__(self, n: int, L0: float, periodic: int):
+        self._p = lib().orc_grid_new(int(n), float(L0), int(periodic))
+        ints = np.zeros(6, np.int32)
+        fl = np.zeros(4, np.float32)
+        lib().orc_grid_info(self._p, ints, fl, None)
+        rad = np.zeros(int(ints[5]), np.float32)
+        lib().orc_grid_info(self._p, ints, fl, rad.ctypes.data_as(C.c_void_p))
+        self.n, self.periodic, self.nplus, self.low1, self.low2, self.next = (int(v) for v in ints)
+        self.L0, self.lado0plus, self.dx, self.rx1 = (float(v) for v in fl)
+        self.rad_ext = rad
+
+    def rad(self, i: int) -> float:
+        return float(lib().orc_grid_rad(self._p, int(i)))
+
+    def __del__(self):
+        try:
+            lib().orc_grid_free(self._p)
+        except Exception:
+            pass
+
+
+def cosmo(ache, omega0, zeta):
+    r = [C.c_float(), C.c_float(), C.c_float()]
+    lib().orc_cosmo(ache, omega0, zeta, C.byref(r[0]), C.byref(r[1]), C.byref(r[2]))
+    return tuple(v.value for v in r)  # rote, rete, hte
+
+
+def indexx(arr: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(arr, np.float32)
+    out = np.zeros(max(1, a.size), np.int32)
+    rc = lib().orc_indexx(a.size, a if a.size else np.zeros(1, np.float32), out)
+    if rc < 0:
+        raise RuntimeError("NSTACK too small in indexx")
+    return out[: a.size]
+
+
+class KNN:
+    def __init__(self, x, y, z, L0, periodic):
+        self.x = np.ascontiguousarray(x, np.float32)
+        self.y = np.ascontiguousarray(y, np.float32)
+        self.z = np.ascontiguousarray(z, np.float32)
+        self._p = lib().orc_knn_new(self.x.size, self.x, self.y, self.z, float(L0), int(periodic))
+
+    def query(self, q, k):
+        idx = np.zeros(k, np.int64)
+        d2 = np.zeros(k, np.float32)
+        c = lib().orc_knn_query(self._p, float(q[0]), float(q[1]), float(q[2]), int(k), idx, d2)
+        return idx[:c], d2[:c]
+
+    def ball(self, q, r, cap=1 << 20):
+        idx = np.zeros(cap, np.int64)
+        d2 = np.zeros(cap, np.float32)
+        c = lib().orc_ball_query(self._p, float(q[0]), float(q[1]), float(q[2]), float(r), cap, idx, d2)
+        return idx[:c], d2[:c]
+
+    def __del__(self):
+        try:
+            lib().orc_knn_free(self._p)
+        except Exception:
+            pass
+
+
+@dataclass
+class VoidParams:
+    dens_thre: float = -0.6
+    dens_thre2: float = 10.0
+    grad_thre: float = 0.25
+    div_thre: float = 0.0
+    rmin: float = 1.0
+    rmin_sub: float = 1.0
+    nvoid_max: int = 1 << 20
+
+    def c(self):
+        return _VP(self.dens_thre, self.dens_thre2, self.grad_thre, self.div_thre, self.rmin,
+                   self.rmin_sub, self.nvoid_max)
+
+
+@dataclass
+class LevelResult:
+    grid: Grid
+    u1co: np.ndarray            # inner N^3 (1+delta)
+    div: np.ndarray             # inner N^3
+    meandens: float
+    nvoid: int = 0
+    cubes_i: np.ndarray = None  # [9, nvoid]: inix,iniy,iniz,finx,finy,finz,icx,icy,icz
+    cubes_f: np.ndarray = None  # [7, nvoid]: rinx,riny,rinz,rfix,rfiy,rfiz,vol
+    father: np.ndarray = None
+    indice: np.ndarray = None
+    uvoid: np.ndarray = None
+    marca: np.ndarray = None    # ext
+    vf: np.ndarray = None       # [13, nvoid]
+    ncellv: np.ndarray = None
+    nvoid2: int = 0
+    nvoidt: int = 0
+    volt_clean: float = 0.0
+    stats: np.ndarray = None
+    cand_lin: np.ndarray = None
+    cand_r: np.ndarray = None
+    extra: dict = field(default_factory=dict)
+
+
+def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.678,
+                          omega0=0.31, zeta=0.0, vel_interp=1, dens_interp=1):
+    """Field stage (avism.f90:1177-1257, 1371-1414). Inputs in file units
+    (reader.f90:447-461 conversion applied here).  Returns a dict of arrays."""
+    L = lib()
+    g = Grid(n, L0, periodic)
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    vx, vy, vz, m = (np.array(a, np.float32, copy=True) for a in (vx, vy, vz, m))
+    L.orc_convert_units(x.size, vx, vy, vz, m)
+    npart = x.size
+    ncell = n ** 3
+    u = np.zeros(ncell, np.float32); v = np.zeros(ncell, np.float32); w = np.zeros(ncell, np.float32)
+    hpart = np.zeros(npart, np.float32)
+    pdens = np.zeros(npart, np.float32)
+    hk = np.zeros(npart, np.float32)
+    knn = KNN(x, y, z, L0, periodic)
+    if vel_interp == 1:
+        L.orc_ppart_dens(knn._p, npart, x, y, z, m, k, pdens, hk)
+        L.orc_vvel_vw(knn._p, g._p, k, m, pdens, vx, vy, vz, None, hpart, u, v, w)
+        L.orc_fill_h_zeros(knn._p, npart, x, y, z, k, hpart)
+    else:
+        L.orc_vel_tsc(g._p, npart, x, y, z, vx, vy, vz, u, v, w)
+    mass = np.zeros(ncell, np.float32)
+    mass64 = np.zeros(ncell, np.float64)
+    if dens_interp == 1:
+        if vel_interp == 0:
+            L.orc_h_distance(knn._p, g._p, k, None, hpart)
+            L.orc_fill_h_zeros(knn._p, npart, x, y, z, k, hpart)
+        L.orc_ddens_sph(g._p, npart, x, y, z, hpart, m, mass, mass64)
+    else:
+        L.orc_dens_tsc(g._p, npart, x, y, z, m, mass)
+    u1co = np.zeros(ncell, np.float32)
+    s = C.c_double()
+    meandens = L.orc_normalise(g._p, ache, omega0, zeta, mass, u1co, C.byref(s))
+    div = np.zeros(ncell, np.float32)
+    L.orc_diver_uniform(n, g.dx, u, v, w, periodic, div)
+    return dict(grid=g, u=u, v=v, w=w, hpart=hpart, part_dens=pdens, hk=hk, mass=mass,
+                mass64=mass64, u1co=u1co, div=div, meandens=float(meandens), sum64=s.value)
+
+
+def fields_from_grid(delta, vx, vy, vz, n, L0, periodic, ache=0.678, omega0=0.31, zeta=0.0):
+    """Grid input (reader.f90:541-549; avism.f90:1342 and 1599-1600)."""
+    g = Grid(n, L0, periodic)
+    u1co = (np.asarray(delta, np.float32) + np.float32(1.0)).astype(np.float32)
+    u = (np.asarray(vx, np.float32) / np.float32(299792.458)).astype(np.float32)
+    v = (np.asarray(vy, np.float32) / np.float32(299792.458)).astype(np.float32)
+    w = (np.asarray(vz, np.float32) / np.float32(299792.458)).astype(np.float32)
+    div = np.zeros(n ** 3, np.float32)
+    lib().orc_diver_uniform(n, g.dx, u, v, w, periodic, div)
+    rote, rete, _ = cosmo(ache, omega0, zeta)
+    ul3 = np.float32(np.float32(10.98) * np.float32(10.98)) * np.float32(10.98)
+    meandens = np.float32(rote) * np.float32(np.float32(9.1717e18) / ul3)
+    return dict(grid=g, u=u, v=v, w=w, u1co=u1co, div=div, meandens=float(meandens))
+
+
+def void_stage(g: Grid, u1co, div, params: VoidParams, marcap=None, reqp=None, smask=None,
+               rete=None, meandens=1.0, stop_after=0, want_candidates=False) -> LevelResult:
+    """MARK_ALL .. catalogue (avism.f90:1633-1973) for one level."""
+    L = lib()
+    ne3 = g.next ** 3
+    u_ext = np.zeros(ne3, np.float32)
+    d_ext = np.zeros(ne3, np.float32)
+    L.orc_extend_f32(g._p, np.ascontiguousarray(u1co, np.float32), u_ext)
+    L.orc_extend_f32(g._p, np.ascontiguousarray(div, np.float32), d_ext)
+    mp = np.full(g.n ** 3, -1, np.int32) if marcap is None else np.ascontiguousarray(marcap, np.int32)
+    mp_ext = np.zeros(ne3, np.int32)
+    L.orc_extend_i32(g._p, mp, mp_ext)
+    sm = np.ones(ne3, np.int8) if smask is None else np.ascontiguousarray(smask, np.int8)
+    NV = params.nvoid_max
+    ci = np.zeros(9 * NV, np.int32)
+    cf = np.zeros(7 * NV, np.float32)
+    father = np.zeros(NV, np.int32)
+    stats = np.zeros(4, np.int64)
+    reqp_arr = None if reqp is None else np.ascontiguousarray(reqp, np.float32)
+    cap = ne3 if want_candidates else 0
+    cand_lin = np.zeros(max(cap, 1), np.int64)
+    cand_r = np.zeros(max(cap, 1), np.int32)
+    nvoid = L.orc_voidfind(g._p, C.byref(params.c()), u_ext, d_ext, mp_ext, sm,
+                           None if reqp_arr is None else reqp_arr.ctypes.data_as(C.c_void_p),
+                           ci, cf, father, stats, cap,
+                           cand_lin.ctypes.data_as(C.c_void_p) if cap else None,
+                           cand_r.ctypes.data_as(C.c_void_p) if cap else None)
+    if nvoid < 0:
+        raise RuntimeError(f"orc_voidfind failed: {nvoid}")
+    res = LevelResult(grid=g, u1co=np.asarray(u1co, np.float32), div=np.asarray(div, np.float32),
+                      meandens=meandens, nvoid=nvoid)
+    res.stats = stats
+    if want_candidates:
+        res.cand_lin = cand_lin[: stats[0]]
+        res.cand_r = cand_r[: stats[0]]
+    res.cubes_i = ci.reshape(9, NV)[:, :nvoid].copy()
+    res.cubes_f = cf.reshape(7, NV)[:, :nvoid].copy()
+    res.father = father[:nvoid].copy()
+    if stop_after < 0:
+        return res
+    if rete is None:
+        rete = cosmo(0.678, 0.31, 0.0)[1]
+    nv = max(nvoid, 1)
+    indice = np.zeros(nv, np.int32)
+    uvoid = np.zeros(nv, np.int32)
+    marca = np.zeros(ne3, np.int32)
+    vf = np.zeros(13 * nv, np.float32)
+    ncellv = np.zeros(nv, np.int32)
+    si = np.zeros(4, np.int32)
+    sd = np.zeros(2, np.float64)
+    rc = L.orc_merge_post(g._p, C.byref(params.c()), nvoid, ci, cf, father, u_ext, float(meandens),
+                          float(rete), indice, uvoid, marca, vf, ncellv, si, sd, int(stop_after))
+    if rc < 0:
+        raise RuntimeError(f"orc_merge_post failed: {rc}")
+    res.indice = indice[:nvoid]
+    res.uvoid = uvoid[:nvoid]
+    res.marca = marca
+    res.vf = vf.reshape(13, nv)[:, :nvoid] if nvoid else np.zeros((13, 0), np.float32)
+    res.ncellv = ncellv[:nvoid]
+    res.nvoid2 = int(si[0])
+    res.nvoidt = int(si[1])
+    res.volt_clean = float(sd[0])
+    return res
+
+
+# ----------------------------------------------------------------- reference
+REF_DIR = HERE / "_ref"
+
+
+def ref_available(periodic: int) -> bool:
+    return (REF_DIR / f"ref_harness_p{int(periodic)}.x").exists()
+
+
+def run_reference_indexx(jobdir: Path, keys: np.ndarray) -> np.ndarray:
+    """INDEXX (num.f90:2-84) run by the reference routine itself (1-based output)."""
+    jobdir = Path(jobdir)
+    jobdir.mkdir(parents=True, exist_ok=True)
+    keys = np.ascontiguousarray(keys, np.float32)
+    (jobdir / "keys.bin").write_bytes(keys.tobytes())
+    (jobdir / "job.in").write_text(f"3\n1 1.0 0\n0 0 0 0 0 0 1\n0 0 0\n0 0 0 {keys.size}\n")
+    subprocess.run([str(REF_DIR / "ref_harness_p0.x"), str(jobdir)], check=True, timeout=300,
+                   stdout=subprocess.DEVNULL)
+    return np.fromfile(jobdir / "out_indexx.bin", np.int32)
+
+
+def run_reference(jobdir: Path, mode: int, n: int, L0: float, periodic: int, params: VoidParams,
+                  ache=0.678, omega0=0.31, zeta=0.0, k=32, vel_interp=1, dens_interp=1,
+                  parts=None, fields=None, timeout=600) -> dict:
+    """Run the reference routines (oracle/_ref) on one level and read the outputs."""
+    import resource
+
+    jobdir = Path(jobdir)
+    jobdir.mkdir(parents=True, exist_ok=True)
+    npart = 0
+    if mode == 1:
+        arrs = [np.ascontiguousarray(a, np.float32) for a in parts]
+        npart = arrs[0].size
+        with open(jobdir / "parts.bin", "wb") as fh:
+            for a in arrs:
+                fh.write(a.tobytes())
+    else:
+        with open(jobdir / "fields.bin", "wb") as fh:
+            for a in fields:
+                fh.write(np.ascontiguousarray(a, np.float32).tobytes())
+    (jobdir / "job.in").write_text(
+        f"{mode}\n{n} {L0!r} {periodic}\n"
+        f"{params.dens_thre!r} {params.dens_thre2!r} {params.grad_thre!r} {params.div_thre!r} "
+        f"{params.rmin!r} {params.rmin_sub!r} {params.nvoid_max}\n"
+        f"{ache!r} {omega0!r} {zeta!r}\n{k} {vel_interp} {dens_interp} {npart}\n")
+
+    def unlimit():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"), OMP_STACKSIZE="512m")
+    exe = REF_DIR / f"ref_harness_p{int(periodic)}.x"
+    subprocess.run([str(exe), str(jobdir)], check=True, timeout=timeout, preexec_fn=unlimit, env=env,
+                   stdout=subprocess.DEVNULL)
+    head = (jobdir / "out_head.txt").read_text().split()
+    nvoid, nvoid2, nvoidt, low1, low2, nplus = (int(v) for v in head[:6])
+    volt, meandens = float(head[6]), float(head[7])
+    ne = low2 - low1 + 1
+    out = dict(nvoid=nvoid, nvoid2=nvoid2, nvoidt=nvoidt, volt_clean=volt, meandens=meandens)
+    fl = np.fromfile(jobdir / "out_fields.bin", np.float32)
+    out["u1co"], out["div"] = fl[: n ** 3], fl[n ** 3:]
+    out["marca"] = np.fromfile(jobdir / "out_marca.bin", np.int32)
+    assert out["marca"].size == ne ** 3
+    vel = np.fromfile(jobdir / "out_vel.bin", np.float32)
+    out["u"], out["v"], out["w"] = vel[: n ** 3], vel[n ** 3: 2 * n ** 3], vel[2 * n ** 3:]
+    if mode == 1:
+        out["mass"] = np.fromfile(jobdir / "out_mass.bin", np.float32)
+        pp = np.fromfile(jobdir / "out_parts.bin", np.float32)
+        out["part_dens"], out["hpart"] = pp[:npart], pp[npart:]
+    raw = open(jobdir / "out_cubes.bin", "rb").read()
+    if nvoid > 0:
+        ints = np.frombuffer(raw[: 13 * nvoid * 4], np.int32).reshape(13, nvoid)
+        flts = np.frombuffer(raw[13 * nvoid * 4:], np.float32).reshape(19, nvoid)
+        out["cubes_i"] = ints[:9].copy()
+        out["father"], out["indice"], out["uvoid"], out["ncellv"] = (ints[9].copy(), ints[10].copy(),
+                                                                     ints[11].copy(), ints[12].copy())
+        out["cubes_f"] = flts[:7].copy()
+        out["vf"] = flts[7:].copy()  # gxc..req in the order of orc_merge_post's vf[0:12]
+    return out

@@ -1,0 +1,88 @@
+"""Writes the golden fixtures under tests/golden/ by running the reference's own
+routines (compiled in place by ``make -C oracle ref`` from /root/reference/src and
+driven by oracle/ref_harness.f90).
+
+Run in the build container (needs oracle/_ref):  python tests/golden/make_golden.py
+
+Each fixture holds the inputs and the reference's outputs for one level:
+  grid_p{0,1}_n24 : grid input (reader.f90:523-552) -> divergence, marking,
+                    VOIDFIND, merge, post-processing, catalogue
+  part_p{0,1}     : particle input (reader.f90:410-465) -> kNN, PPART_DENS,
+                    VVEL_INTERP_SPH_VW, FILL_H_ZEROS, DDENS_INTERP_SPH, mean
+                    density, divergence and the same void stage
+"""
+from __future__ import annotations
+
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+import pyoracle as O  # noqa: E402
+from avism_amd import synth  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+PARAMS = O.VoidParams(dens_thre=-0.6, dens_thre2=10.0, grad_thre=0.25, div_thre=0.0, rmin=1.0,
+                      rmin_sub=1.0, nvoid_max=200000)
+
+
+def _save(name, inputs: dict, ref: dict, meta: dict):
+    keep = {k: v for k, v in ref.items() if isinstance(v, np.ndarray)}
+    scal = {k: np.asarray(v) for k, v in ref.items() if not isinstance(v, np.ndarray)}
+    np.savez_compressed(OUT / f"{name}.npz", **{f"in_{k}": v for k, v in inputs.items()},
+                        **{f"ref_{k}": v for k, v in keep.items()},
+                        **{f"ref_{k}": v for k, v in scal.items()},
+                        **{f"meta_{k}": np.asarray(v) for k, v in meta.items()})
+    print(name, {k: (v.shape if isinstance(v, np.ndarray) else v) for k, v in ref.items()
+                 if k in ("nvoid", "nvoid2", "nvoidt")})
+
+
+def main():
+    for per in (0, 1):
+        if not O.ref_available(per):
+            raise SystemExit("oracle/_ref not built: run `make -C oracle ref` first")
+    with tempfile.TemporaryDirectory(dir=ROOT / "_work" if (ROOT / "_work").exists() else None) as td:
+        td = Path(td)
+        for per, seed in ((0, 11), (1, 12)):
+            n, L0 = 24, 48.0
+            d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, seed))
+            ref = O.run_reference(td / f"g{per}", 2, n, L0, per, PARAMS, fields=(d, vx, vy, vz))
+            _save(f"grid_p{per}_n{n}", dict(delta=d, vx=vx, vy=vy, vz=vz), ref,
+                  dict(n=n, L0=L0, periodic=per, mode=2, seed=seed))
+        # particle mode: Zel'dovich (periodic) and mock voids (non-periodic)
+        p = synth.zeldovich(16, 40.0, seed=22, rms_disp=2.0)
+        arr = p.numpy()
+        ref = O.run_reference(td / "p1", 1, 20, 40.0, 1, PARAMS, k=16, parts=arr)
+        _save("part_p1", dict(zip(("x", "y", "z", "vx", "vy", "vz", "m"), arr)), ref,
+              dict(n=20, L0=40.0, periodic=1, mode=1, k=16, seed=22))
+        p, _, _ = synth.mock_voids(n0=5000, L0=60.0, nvoids=4, rmin=6.0, rmax=12.0, seed=23)
+        arr = p.numpy()
+        ref = O.run_reference(td / "p0", 1, 20, 60.0, 0, PARAMS, k=16, parts=arr)
+        _save("part_p0", dict(zip(("x", "y", "z", "vx", "vy", "vz", "m"), arr)), ref,
+              dict(n=20, L0=60.0, periodic=0, mode=1, k=16, seed=23))
+        # INDEXX permutations with many ties (the tie order decides cube and void
+        # ids), run by the reference routine itself
+        rng = np.random.default_rng(23)
+        sets = {
+            "ties_small": rng.integers(0, 40, size=3000).astype(np.float32),
+            "ties_few": rng.integers(0, 3, size=2000).astype(np.float32),
+            "distinct": rng.standard_normal(5000).astype(np.float32),
+            "sorted": np.arange(1000, dtype=np.float32),
+            "reversed": np.arange(1000, 0, -1).astype(np.float32),
+            "all_equal": np.ones(777, np.float32),
+            "tiny": np.array([3.0, 1.0, 2.0, 1.0, 3.0], np.float32),
+        }
+        out = {}
+        for name, keys in sets.items():
+            out[f"keys_{name}"] = keys
+            out[f"indx_{name}"] = O.run_reference_indexx(td / f"ix_{name}", keys)
+        np.savez_compressed(OUT / "indexx.npz", **out)
+
+
+if __name__ == "__main__":
+    main()
