@@ -1,0 +1,275 @@
+"""ctypes bindings to libavism_hip.so (C ABI in include/avism.h).
+
+The library is loaded from this package directory.  A missing library is an
+error: the package has no CPU fallback for any stage.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libavism_hip.so"
+
+AVISM_OK = 0
+AVISM_HOST, AVISM_DEVICE = 0, 1
+GROW_PLAIN, GROW_PROPOSAL = 0, 1
+
+ERRORS = {-1: "AVISM_ERR_ARG", -2: "AVISM_ERR_HIP", -3: "AVISM_ERR_INPUT", -4: "AVISM_ERR_LIMIT",
+          -5: "AVISM_ERR_STATE", -6: "AVISM_ERR_UNSUPPORTED"}
+
+
+class AvismError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class GridInfo(C.Structure):
+    _fields_ = [("n", C.c_int32), ("periodic", C.c_int32), ("nplus", C.c_int32), ("low1", C.c_int32),
+                ("low2", C.c_int32), ("next", C.c_int32), ("L0", C.c_float), ("lado0plus", C.c_float),
+                ("dx", C.c_float), ("rx1", C.c_float)]
+
+
+class VoidParams(C.Structure):
+    _fields_ = [("dens_thre", C.c_float), ("dens_thre2", C.c_float), ("grad_thre", C.c_float),
+                ("div_thre", C.c_float), ("rmin", C.c_float), ("rmin_sub", C.c_float),
+                ("nvoid_max", C.c_int32)]
+
+
+class LevelStats(C.Structure):
+    _fields_ = [("ncand", C.c_int64), ("ncubes", C.c_int32), ("nvoid2", C.c_int32),
+                ("nvoidt", C.c_int32), ("pad", C.c_int32), ("volt_clean", C.c_double),
+                ("meandens", C.c_float), ("ff", C.c_float)]
+
+
+EXPORTS = [
+    "avism_create", "avism_destroy", "avism_last_error", "avism_set_timing", "avism_get_timings",
+    "avism_set_grid", "avism_get_grid", "avism_set_cosmology", "avism_set_void_params",
+    "avism_set_parent_map", "avism_load_particles", "avism_particles_to_grid",
+    "avism_load_grid_fields", "avism_load_void_fields", "avism_get_fields", "avism_get_particle_fields", "avism_find_voids",
+    "avism_set_proposals", "avism_get_candidates", "avism_get_cubes", "avism_indexx",
+    "avism_merge_and_post", "avism_get_catalogue", "avism_get_marca", "avism_run_level",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m avism_amd.build` "
+                           "(there is no CPU fallback)")
+    L = C.CDLL(str(LIB_PATH))
+    vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
+    sig = {
+        "avism_create": ([C.POINTER(vp), i32], C.c_int),
+        "avism_destroy": ([vp], C.c_int),
+        "avism_last_error": ([vp], C.c_char_p),
+        "avism_set_timing": ([vp, i32], C.c_int),
+        "avism_get_timings": ([vp, C.c_char_p, i32, vp, C.POINTER(i32)], C.c_int),
+        "avism_set_grid": ([vp, i32, f32, i32], C.c_int),
+        "avism_get_grid": ([vp, C.POINTER(GridInfo), vp], C.c_int),
+        "avism_set_cosmology": ([vp, f32, f32, f32], C.c_int),
+        "avism_set_void_params": ([vp, C.POINTER(VoidParams)], C.c_int),
+        "avism_set_parent_map": ([vp, vp, i32], C.c_int),
+        "avism_load_particles": ([vp, i64, vp, vp, vp, vp, vp, vp, vp, i32], C.c_int),
+        "avism_particles_to_grid": ([vp, i32, i32, i32], C.c_int),
+        "avism_load_grid_fields": ([vp, vp, vp, vp, vp, i32], C.c_int),
+        "avism_load_void_fields": ([vp, vp, vp, f32, i32], C.c_int),
+        "avism_get_fields": ([vp, vp, vp, vp, vp, vp, i32], C.c_int),
+        "avism_get_particle_fields": ([vp, vp, vp], C.c_int),
+        "avism_find_voids": ([vp, i32, C.POINTER(i32)], C.c_int),
+        "avism_set_proposals": ([vp, vp, i64, i32], C.c_int),
+        "avism_get_candidates": ([vp, vp, vp, i64, C.POINTER(i64)], C.c_int),
+        "avism_get_cubes": ([vp, vp, vp, vp], C.c_int),
+        "avism_indexx": ([vp, vp, i64, vp, i32], C.c_int),
+        "avism_merge_and_post": ([vp, C.POINTER(i32)], C.c_int),
+        "avism_get_catalogue": ([vp, vp, vp, vp, vp, C.POINTER(LevelStats)], C.c_int),
+        "avism_get_marca": ([vp, vp, i32, i32], C.c_int),
+        "avism_run_level": ([vp, i32, i32, i32, i32, C.POINTER(LevelStats)], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data_as(C.c_void_p)
+    # torch tensor (device or host)
+    return C.c_void_p(a.data_ptr())
+
+
+class Context:
+    """One GPU context: owns device memory and a stream (avism_create)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        self._h = C.c_void_p()
+        rc = L.avism_create(C.byref(self._h), int(device))
+        if rc != AVISM_OK:
+            raise AvismError(rc, "avism_create failed (no HIP device?)")
+
+    def _chk(self, rc: int):
+        if rc != AVISM_OK:
+            raise AvismError(rc, lib().avism_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().avism_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- set-up
+    def set_grid(self, n: int, L0: float, periodic: int) -> GridInfo:
+        self._chk(lib().avism_set_grid(self._h, int(n), float(L0), int(periodic)))
+        return self.grid()
+
+    def grid(self, with_rad=False):
+        gi = GridInfo()
+        self._chk(lib().avism_get_grid(self._h, C.byref(gi), None))
+        if with_rad:
+            rad = np.zeros(gi.next, np.float32)
+            self._chk(lib().avism_get_grid(self._h, C.byref(gi), _ptr(rad)))
+            return gi, rad
+        return gi
+
+    def set_cosmology(self, ache=0.678, omega0=0.31, zeta=0.0):
+        self._chk(lib().avism_set_cosmology(self._h, float(ache), float(omega0), float(zeta)))
+
+    def set_void_params(self, dens_thre=-0.6, dens_thre2=10.0, grad_thre=0.25, div_thre=0.0,
+                        rmin=1.0, rmin_sub=1.0, nvoid_max=30000000):
+        p = VoidParams(dens_thre, dens_thre2, grad_thre, div_thre, rmin, rmin_sub, int(nvoid_max))
+        self._chk(lib().avism_set_void_params(self._h, C.byref(p)))
+
+    def set_timing(self, on: bool):
+        self._chk(lib().avism_set_timing(self._h, int(bool(on))))
+
+    def timings(self):
+        n = C.c_int32(256)
+        ms = np.zeros(256, np.float32)
+        names = C.create_string_buffer(1 << 14)
+        self._chk(lib().avism_get_timings(self._h, names, len(names), _ptr(ms), C.byref(n)))
+        keys = names.value.decode().split(";")[: n.value]
+        return list(zip(keys, ms[: n.value].tolist()))
+
+    # ---- field stage
+    def load_particles(self, x, y, z, vx, vy, vz, m, where=None):
+        arrs = [x, y, z, vx, vy, vz, m]
+        if where is None:
+            where = AVISM_HOST if isinstance(x, np.ndarray) else (
+                AVISM_DEVICE if getattr(x, "is_cuda", False) else AVISM_HOST)
+        if where == AVISM_HOST:
+            arrs = [np.ascontiguousarray(a if isinstance(a, np.ndarray) else a.numpy(), np.float32)
+                    for a in arrs]
+        n = int(arrs[0].shape[0]) if isinstance(arrs[0], np.ndarray) else int(arrs[0].numel())
+        self._keep = arrs
+        self._chk(lib().avism_load_particles(self._h, n, *[_ptr(a) for a in arrs], int(where)))
+
+    def particles_to_grid(self, k=32, vel_interp=1, dens_interp=1):
+        self._chk(lib().avism_particles_to_grid(self._h, int(k), int(vel_interp), int(dens_interp)))
+
+    def load_grid_fields(self, delta, vx, vy, vz):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (delta, vx, vy, vz)]
+        self._chk(lib().avism_load_grid_fields(self._h, *[_ptr(a) for a in arrs], AVISM_HOST))
+
+    def load_void_fields(self, u1co, div, meandens):
+        a = np.ascontiguousarray(u1co, np.float32)
+        b = np.ascontiguousarray(div, np.float32)
+        self._chk(lib().avism_load_void_fields(self._h, _ptr(a), _ptr(b), float(meandens), AVISM_HOST))
+
+    def fields(self):
+        n = self.grid().n
+        out = {k: np.zeros(n ** 3, np.float32) for k in ("u1co", "div", "u", "v", "w")}
+        self._chk(lib().avism_get_fields(self._h, _ptr(out["u1co"]), _ptr(out["div"]), _ptr(out["u"]),
+                                         _ptr(out["v"]), _ptr(out["w"]), AVISM_HOST))
+        return out
+
+    def particle_fields(self, np_total: int):
+        pd = np.zeros(np_total, np.float32)
+        hp = np.zeros(np_total, np.float32)
+        self._chk(lib().avism_get_particle_fields(self._h, _ptr(pd), _ptr(hp)))
+        return pd, hp
+
+    # ---- void stage
+    def find_voids(self, growth_mode=GROW_PLAIN) -> int:
+        n = C.c_int32()
+        self._chk(lib().avism_find_voids(self._h, int(growth_mode), C.byref(n)))
+        return n.value
+
+    def set_proposals(self, r_hat):
+        a = np.ascontiguousarray(r_hat, np.int8)
+        self._chk(lib().avism_set_proposals(self._h, _ptr(a), int(a.size), AVISM_HOST))
+
+    def candidates(self):
+        n = C.c_int64()
+        self._chk(lib().avism_get_candidates(self._h, None, None, 0, C.byref(n)))
+        lin = np.zeros(max(n.value, 1), np.int64)
+        r = np.zeros(max(n.value, 1), np.int32)
+        self._chk(lib().avism_get_candidates(self._h, _ptr(lin), _ptr(r), n.value, C.byref(n)))
+        return lin[: n.value], r[: n.value]
+
+    def cubes(self, ncubes: int):
+        ci = np.zeros(9 * max(ncubes, 1), np.int32)
+        cf = np.zeros(7 * max(ncubes, 1), np.float32)
+        fa = np.zeros(max(ncubes, 1), np.int32)
+        self._chk(lib().avism_get_cubes(self._h, _ptr(ci), _ptr(cf), _ptr(fa)))
+        return ci.reshape(9, -1)[:, :ncubes], cf.reshape(7, -1)[:, :ncubes], fa[:ncubes]
+
+    def indexx(self, keys):
+        a = np.ascontiguousarray(keys, np.float32)
+        out = np.zeros(max(a.size, 1), np.int32)
+        self._chk(lib().avism_indexx(self._h, _ptr(a), int(a.size), _ptr(out), AVISM_HOST))
+        return out[: a.size]
+
+    # ---- merge + post
+    def merge_and_post(self) -> int:
+        n = C.c_int32()
+        self._chk(lib().avism_merge_and_post(self._h, C.byref(n)))
+        return n.value
+
+    def catalogue(self, ncubes: int):
+        nv = max(ncubes, 1)
+        ind = np.zeros(nv, np.int32)
+        uv = np.zeros(nv, np.int32)
+        nc = np.zeros(nv, np.int32)
+        vf = np.zeros(12 * nv, np.float32)
+        st = LevelStats()
+        self._chk(lib().avism_get_catalogue(self._h, _ptr(ind), _ptr(uv), _ptr(nc), _ptr(vf), C.byref(st)))
+        return dict(indice=ind[:ncubes], uvoid=uv[:ncubes], ncellv=nc[:ncubes],
+                    vf=vf.reshape(12, nv)[:, :ncubes], stats=st)
+
+    def stats(self) -> LevelStats:
+        st = LevelStats()
+        self._chk(lib().avism_get_catalogue(self._h, None, None, None, None, C.byref(st)))
+        return st
+
+    def marca(self, inner_only=False):
+        gi = self.grid()
+        n = gi.n ** 3 if inner_only else gi.next ** 3
+        out = np.zeros(n, np.int32)
+        self._chk(lib().avism_get_marca(self._h, _ptr(out), int(bool(inner_only)), AVISM_HOST))
+        return out
+
+    def run_level(self, k=32, vel_interp=1, dens_interp=1, growth_mode=GROW_PLAIN) -> LevelStats:
+        st = LevelStats()
+        self._chk(lib().avism_run_level(self._h, int(k), int(vel_interp), int(dens_interp),
+                                        int(growth_mode), C.byref(st)))
+        return st

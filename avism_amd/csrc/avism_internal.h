@@ -1,0 +1,162 @@
+// avism_internal.h -- shared types and helpers of the HIP implementation.
+//
+// Layout conventions (same as the reference, column-major, x fastest):
+//   inner grid  : cell (i,j,k), 1..N      -> (i-1) + N*((j-1) + N*(k-1))
+//   ext grid    : cell (I,J,K), LOW1..LOW2 -> (I-LOW1) + NE*((J-LOW1) + NE*(K-LOW1))
+// Cube and void ids are 1-based, as in the reference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/avism.h"
+
+namespace avism {
+
+constexpr float kUM = 9.1717E18f;      // commondata.f90:17
+constexpr float kUV = 299792.458f;     // commondata.f90:16
+constexpr float kUL = 10.98f;          // commondata.f90:15
+constexpr int kNcellMin = 10;          // commondata.f90:10
+constexpr int kNsec = 1;               // voidfinding.f90:101
+constexpr int kNbuff = 1;              // voidfinding.f90:797
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// flag byte of an extended cell (voidfinding.f90:256-432 and MARK_ALL 13-55)
+enum : uint8_t {
+  FL_X1 = 1, FL_X2 = 2, FL_Y1 = 4, FL_Y2 = 8, FL_Z1 = 16, FL_Z2 = 32,
+  FL_SUB = 64,   // FLAG_SUB: may belong to a (sub)void
+  FL_V = 128,    // FLAGV: candidate centre
+};
+
+struct GridGeom {
+  int32_t n, periodic, nplus, low1, low2, next;
+  float L0, lado0plus, dx, rx1;
+  // RADX over reference indices rlo..rhi
+  int32_t rlo, rhi;
+  std::vector<float> rad;
+  float radx(int i) const { return rad[i - rlo]; }
+};
+
+GridGeom make_grid(int32_t n, float L0, int32_t periodic);
+
+// Device buffer that only grows.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DBuf();
+  void* ensure(size_t nbytes);
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+  void release();
+};
+
+struct Timing {
+  std::string name;
+  float ms;
+};
+
+}  // namespace avism
+
+// Context: owns all device memory and the stream.
+struct avism_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  avism::GridGeom g{};
+  bool grid_set = false;
+  float ache = 0.678f, omega0 = 0.31f, zeta = 0.0f;
+  float rote = 0, rete = 0, hte = 0;
+  avism_void_params vp{};
+  bool timing_on = false;
+  std::vector<avism::Timing> timings;
+
+  // particles (original order) and the binned copies
+  int64_t np = 0;
+  avism::DBuf px, py, pz, pvx, pvy, pvz, pm;   // converted units after upload
+  avism::DBuf sbin_key, sbin_val, sbin_key2, sbin_val2, sort_tmp;
+  avism::DBuf spos4, svel4, sorig;             // sorted by bin
+  avism::DBuf bin_start;                       // nb^3 + 1
+  avism::DBuf bin_reach;                       // nb^3, DDENS reach per bin
+  int32_t nb = 0;
+  float bsz = 0;
+  avism::DBuf pdens, hk, hpart, norm;          // sorted order
+  avism::DBuf scalars;                         // small device scalars
+  // inner fields
+  avism::DBuf u_mass;                          // mass -> density (before /MEANDENS)
+  avism::DBuf fu, fv, fw, u1co, div;
+  avism::DBuf partial64;
+  avism::DBuf marcap;                          // inner MARCAP (optional)
+  bool has_marcap = false;
+  float meandens = 0;                          // MEANDENS in Msun/Mpc^3
+  float meandens_raw = 0;                      // MEANDENS before unit scaling
+  double meansum = 0;
+  bool fields_ready = false;
+
+  // void stage
+  avism::DBuf flags;                           // ext flag bytes
+  avism::DBuf cand_key, cand_lin, cand_cnt, blk_cnt;
+  avism::DBuf sort_idx, sort_a, sort_b, seg_a, seg_b, seg_small, sort_misc, sort_gpos, sort_spos;
+  int64_t ncand = 0;
+  avism::DBuf cand_r, cand_state, cand_flag2;  // growth results per sorted candidate
+  avism::DBuf cover_acc, cover_und;            // ext grids for acceptance
+  avism::DBuf acc_rank, misc_i32;
+  avism::DBuf proposals;                       // learned half-width proposals (int8 per candidate)
+  int32_t nvoid = 0;                           // accepted cubes
+  avism::DBuf cube_i, cube_f, cube_father;     // [9][nvoid], [7][nvoid], [nvoid]
+  // merge and post
+  avism::DBuf indice, rank_of, paintP, paintK, jstar, pi_ptr, group, uvoid, marca;
+  avism::DBuf ev_key, ev_val, ev_key2, ev_val2, grp_start, vf, ncellv, removed;
+  avism::DBuf ccl, cellid, cellid2, cellkey, cellkey2, vstart, inert, per_tmp;
+  int32_t nvoid2 = 0, nvoidt = 0;
+  double volt_clean = 0;
+  int64_t growth_stats[8] = {0};
+};
+
+#define AV_CHECK_HIP(ctx, expr)                                                        \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(_e);                 \
+      return AVISM_ERR_HIP;                                                            \
+    }                                                                                  \
+  } while (0)
+
+#define AV_FAIL(ctx, code, msg) \
+  do {                          \
+    (ctx)->err = (msg);         \
+    return (code);              \
+  } while (0)
+
+namespace avism {
+// stage entry points (each returns an AVISM_* status)
+int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp);
+int fields_from_grid(avism_ctx* c, const float* delta, const float* vx, const float* vy,
+                     const float* vz, int where);
+int void_find(avism_ctx* c, int32_t growth_mode);
+int merge_and_post(avism_ctx* c, int32_t stop_after);
+int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx);
+void add_timing(avism_ctx* c, const char* name, hipEvent_t a, hipEvent_t b);
+int cnn_predict(avism_ctx* c, const void* weights, int32_t* d_out_r, int64_t ncand);
+
+// scoped stage timer on the context stream
+struct StageTimer {
+  avism_ctx* c;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(avism_ctx* c_, const char* n_) : c(c_), name(n_) {
+    if (c->timing_on) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~StageTimer() {
+    if (c->timing_on) {
+      (void)hipEventRecord(b, c->stream);
+      add_timing(c, name, a, b);
+    }
+  }
+};
+}  // namespace avism

@@ -1,0 +1,290 @@
+// capi.hip -- the extern "C" boundary declared in include/avism.h.
+#include <cstring>
+
+#include "avism_internal.h"
+
+namespace avism {
+int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, const float* z,
+                   const float* vx, const float* vy, const float* vz, const float* m, int where);
+int get_particle_fields(avism_ctx* c, float* part_dens, float* hpart);
+int set_proposals(avism_ctx* c, const int8_t* r, int64_t n, int where);
+int get_candidates(avism_ctx* c, int64_t* lin, int32_t* r, int64_t cap, int64_t* n);
+int get_cubes(avism_ctx* c, int32_t* ci, float* cf, int32_t* father);
+int get_catalogue(avism_ctx* c, int32_t* indice, int32_t* uvoid, int32_t* ncellv, float* vf,
+                  avism_level_stats* st);
+int get_marca(avism_ctx* c, int32_t* marca, int32_t inner_only, int32_t where);
+}  // namespace avism
+
+using namespace avism;
+
+#define CTX_GUARD(ctx)                      \
+  if (!(ctx)) return AVISM_ERR_ARG;         \
+  (ctx)->err.clear();                       \
+  if (hipSetDevice((ctx)->device) != hipSuccess) { (ctx)->err = "hipSetDevice failed"; return AVISM_ERR_HIP; }
+
+extern "C" {
+
+int avism_create(avism_ctx** out, int32_t device) {
+  if (!out) return AVISM_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AVISM_ERR_HIP;
+  if (device < 0 || device >= ndev) return AVISM_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return AVISM_ERR_HIP;
+  avism_ctx* c = new avism_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return AVISM_ERR_HIP;
+  }
+  c->vp = avism_void_params{-0.6f, 10.0f, 0.25f, 0.0f, 1.0f, 1.0f, 30000000};
+  avism_set_cosmology(c, 0.678f, 0.31f, 0.0f);
+  *out = c;
+  return AVISM_OK;
+}
+
+int avism_destroy(avism_ctx* c) {
+  if (!c) return AVISM_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AVISM_OK;
+}
+
+const char* avism_last_error(avism_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int avism_set_timing(avism_ctx* c, int32_t on) {
+  CTX_GUARD(c);
+  c->timing_on = on != 0;
+  c->timings.clear();
+  return AVISM_OK;
+}
+
+int avism_get_timings(avism_ctx* c, char* names, int32_t names_len, float* ms, int32_t* count) {
+  CTX_GUARD(c);
+  int32_t cap = count ? *count : 0;
+  int32_t nwr = 0;
+  std::string all;
+  for (auto& t : c->timings) {
+    if (nwr < cap && ms) ms[nwr] = t.ms;
+    all += t.name;
+    all += ';';
+    nwr++;
+  }
+  if (names && names_len > 0) {
+    size_t nb = std::min((size_t)names_len - 1, all.size());
+    memcpy(names, all.data(), nb);
+    names[nb] = 0;
+  }
+  if (count) *count = nwr;
+  c->timings.clear();
+  return AVISM_OK;
+}
+
+int avism_set_grid(avism_ctx* c, int32_t n, float L0, int32_t periodic) {
+  CTX_GUARD(c);
+  if (n < 4) AV_FAIL(c, AVISM_ERR_ARG, "grid size must be at least 4 cells per axis");
+  if (!(L0 > 0.0f)) AV_FAIL(c, AVISM_ERR_ARG, "box size must be positive");
+  if (periodic != 0 && periodic != 1) AV_FAIL(c, AVISM_ERR_ARG, "periodic must be 0 or 1");
+  int64_t next = 0;
+  c->g = make_grid(n, L0, periodic);
+  next = c->g.next;
+  if (next * next * next >= (int64_t)0x7fffffff)
+    AV_FAIL(c, AVISM_ERR_LIMIT, "extended grid above 2^31 cells is not supported by this build");
+  c->grid_set = true;
+  c->fields_ready = false;
+  c->np = 0;
+  c->has_marcap = false;
+  return AVISM_OK;
+}
+
+int avism_get_grid(avism_ctx* c, avism_grid_info* out, float* rad_ext) {
+  CTX_GUARD(c);
+  if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "no grid set");
+  if (out) {
+    out->n = c->g.n; out->periodic = c->g.periodic; out->nplus = c->g.nplus;
+    out->low1 = c->g.low1; out->low2 = c->g.low2; out->next = c->g.next;
+    out->L0 = c->g.L0; out->lado0plus = c->g.lado0plus; out->dx = c->g.dx; out->rx1 = c->g.rx1;
+  }
+  if (rad_ext)
+    for (int i = c->g.low1; i <= c->g.low2; i++) rad_ext[i - c->g.low1] = c->g.radx(i);
+  return AVISM_OK;
+}
+
+int avism_set_cosmology(avism_ctx* c, float ache, float omega0, float zeta) {
+  if (!c) return AVISM_ERR_ARG;
+  c->ache = ache; c->omega0 = omega0; c->zeta = zeta;
+  // avism.f90:469-479 and 599-606
+  float ache0 = (float)((double)ache * 3.66e-3);
+  float rodo = (float)(((double)omega0 * 3.0) * (double)(ache0 * ache0));
+  float re0 = 1.0f / 10.98f;
+  if (zeta < 0.0f) zeta = 0.0f;
+  float t = 1.0f + zeta;
+  c->rote = rodo * ((t * t) * t);
+  c->rete = re0 / t;
+  float r = re0 / c->rete;
+  float h = omega0 * (((r * r) * r) - 1.0f) + 1.0f;
+  c->hte = ache * sqrtf(h);
+  return AVISM_OK;
+}
+
+int avism_set_void_params(avism_ctx* c, const avism_void_params* p) {
+  CTX_GUARD(c);
+  if (!p) AV_FAIL(c, AVISM_ERR_ARG, "null params");
+  if (p->nvoid_max <= 0) AV_FAIL(c, AVISM_ERR_ARG, "nvoid_max must be positive");
+  c->vp = *p;
+  return AVISM_OK;
+}
+
+int avism_set_parent_map(avism_ctx* c, const int32_t* marcap, int32_t where) {
+  CTX_GUARD(c);
+  if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "no grid set");
+  if (!marcap) {
+    c->has_marcap = false;
+    return AVISM_OK;
+  }
+  size_t b = (size_t)c->g.n * c->g.n * c->g.n * 4;
+  if (!c->marcap.ensure(b)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+  AV_CHECK_HIP(c, hipMemcpy(c->marcap.p, marcap, b,
+                            where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  c->has_marcap = true;
+  return AVISM_OK;
+}
+
+int avism_load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, const float* z,
+                         const float* vx, const float* vy, const float* vz, const float* m,
+                         int32_t where) {
+  CTX_GUARD(c);
+  if (!x || !y || !z || !vx || !vy || !vz || !m) AV_FAIL(c, AVISM_ERR_ARG, "null particle array");
+  return load_particles(c, np, x, y, z, vx, vy, vz, m, where);
+}
+
+int avism_particles_to_grid(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp) {
+  CTX_GUARD(c);
+  return fields_from_particles(c, k, vel_interp, dens_interp);
+}
+
+int avism_load_grid_fields(avism_ctx* c, const float* delta, const float* vx, const float* vy,
+                           const float* vz, int32_t where) {
+  CTX_GUARD(c);
+  if (!delta || !vx || !vy || !vz) AV_FAIL(c, AVISM_ERR_ARG, "null field array");
+  return fields_from_grid(c, delta, vx, vy, vz, where);
+}
+
+int avism_load_void_fields(avism_ctx* c, const float* u1co, const float* div, float meandens,
+                           int32_t where) {
+  CTX_GUARD(c);
+  if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "avism_set_grid must be called first");
+  if (!u1co || !div) AV_FAIL(c, AVISM_ERR_ARG, "null field array");
+  size_t b = (size_t)c->g.n * c->g.n * c->g.n * 4;
+  if (!c->u1co.ensure(b) || !c->div.ensure(b) || !c->fu.ensure(b) || !c->fv.ensure(b) || !c->fw.ensure(b))
+    AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+  hipMemcpyKind kind = where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  AV_CHECK_HIP(c, hipMemcpy(c->u1co.p, u1co, b, kind));
+  AV_CHECK_HIP(c, hipMemcpy(c->div.p, div, b, kind));
+  AV_CHECK_HIP(c, hipMemset(c->fu.p, 0, b));
+  AV_CHECK_HIP(c, hipMemset(c->fv.p, 0, b));
+  AV_CHECK_HIP(c, hipMemset(c->fw.p, 0, b));
+  c->meandens = meandens;
+  c->np = 0;
+  c->fields_ready = true;
+  return AVISM_OK;
+}
+
+int avism_get_fields(avism_ctx* c, float* u1co, float* div, float* u, float* v, float* w,
+                     int32_t where) {
+  CTX_GUARD(c);
+  if (!c->fields_ready) AV_FAIL(c, AVISM_ERR_STATE, "fields not computed");
+  size_t b = (size_t)c->g.n * c->g.n * c->g.n * 4;
+  hipMemcpyKind kind = where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  float* dst[5] = {u1co, div, u, v, w};
+  void* src[5] = {c->u1co.p, c->div.p, c->fu.p, c->fv.p, c->fw.p};
+  for (int i = 0; i < 5; i++)
+    if (dst[i]) AV_CHECK_HIP(c, hipMemcpyAsync(dst[i], src[i], b, kind, c->stream));
+  AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  return AVISM_OK;
+}
+
+int avism_get_particle_fields(avism_ctx* c, float* part_dens, float* hpart) {
+  CTX_GUARD(c);
+  return get_particle_fields(c, part_dens, hpart);
+}
+
+int avism_find_voids(avism_ctx* c, int32_t growth_mode, int32_t* ncubes) {
+  CTX_GUARD(c);
+  int rc = void_find(c, growth_mode);
+  if (ncubes) *ncubes = c->nvoid;
+  return rc;
+}
+
+int avism_set_proposals(avism_ctx* c, const int8_t* r_hat, int64_t n, int32_t where) {
+  CTX_GUARD(c);
+  return set_proposals(c, r_hat, n, where);
+}
+
+int avism_get_candidates(avism_ctx* c, int64_t* lin, int32_t* r, int64_t cap, int64_t* n) {
+  CTX_GUARD(c);
+  return get_candidates(c, lin, r, cap, n);
+}
+
+int avism_get_cubes(avism_ctx* c, int32_t* ci, float* cf, int32_t* father) {
+  CTX_GUARD(c);
+  return get_cubes(c, ci, cf, father);
+}
+
+int avism_indexx(avism_ctx* c, const float* keys, int64_t n, int32_t* indx, int32_t where) {
+  CTX_GUARD(c);
+  if (n < 0) AV_FAIL(c, AVISM_ERR_ARG, "negative length");
+  if (n == 0) return AVISM_OK;
+  if (n >= (int64_t)0x7fffffff) AV_FAIL(c, AVISM_ERR_LIMIT, "INDEXX beyond 2^31-1 keys");
+  const float* dk = keys;
+  int32_t* di = indx;
+  if (where != AVISM_DEVICE) {
+    if (!c->sort_a.ensure((size_t)n * 4) || !c->sort_b.ensure((size_t)n * 4))
+      AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+    AV_CHECK_HIP(c, hipMemcpyAsync(c->sort_a.p, keys, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    dk = c->sort_a.as<float>();
+    di = c->sort_b.as<int32_t>();
+  }
+  int rc = indexx_device(c, dk, n, di);
+  if (rc) return rc;
+  if (where != AVISM_DEVICE) {
+    AV_CHECK_HIP(c, hipMemcpyAsync(indx, di, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return AVISM_OK;
+}
+
+int avism_merge_and_post(avism_ctx* c, int32_t* nvoids) {
+  CTX_GUARD(c);
+  int rc = merge_and_post(c, 0);
+  if (nvoids) *nvoids = c->nvoidt;
+  return rc;
+}
+
+int avism_get_catalogue(avism_ctx* c, int32_t* indice, int32_t* uvoid, int32_t* ncellv, float* vf,
+                        avism_level_stats* stats) {
+  CTX_GUARD(c);
+  return get_catalogue(c, indice, uvoid, ncellv, vf, stats);
+}
+
+int avism_get_marca(avism_ctx* c, int32_t* marca, int32_t inner_only, int32_t where) {
+  CTX_GUARD(c);
+  return get_marca(c, marca, inner_only, where);
+}
+
+int avism_run_level(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp,
+                    int32_t growth_mode, avism_level_stats* stats) {
+  CTX_GUARD(c);
+  int rc = fields_from_particles(c, k, vel_interp, dens_interp);
+  if (rc) return rc;
+  rc = void_find(c, growth_mode);
+  if (rc) return rc;
+  rc = merge_and_post(c, 0);
+  if (rc) return rc;
+  if (stats) return get_catalogue(c, nullptr, nullptr, nullptr, nullptr, stats);
+  return AVISM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1015 @@
+// merge.hip -- cube volume order, MERGE_VOID and post-processing on gfx950.
+//
+// Reference: avism.f90:1727-1973, voidfinding.f90:718-999 (MERGE_VOID),
+// 1007-1161 (SIMPLE_CONNECTION_FAST), 1367-1568 (VOID_PERIODIC),
+// shape.f90:140-225 (VOID_SHAPE).
+//
+// MERGE_VOID walks the cubes in descending volume (rank 0,1,...).  Its outcome
+// is reproduced without the sequential walk:
+//  * cube c is absorbed at step j*(c) = the lowest rank j < rank(c) whose cube,
+//    grown by NBUFF=1, overlaps c grown by 1 (the ball query of 918-929 is a
+//    superset of those cubes, since later cubes are never larger); with boxes
+//    grown by 2 painted as atomicMin(rank), j* is the minimum over c's box.
+//  * every cube paints its box once, at key 2*rank (if it is a root, i.e. has
+//    no such j) or 2*j*+1 (absorbed); fill-if-zero painting means MARCA(x) is
+//    the group of the cube with the smallest key over x.
+//  * a root looks for a "close brother" (829-860): the nearest cell of its box
+//    grown by 1 already painted before its own step (key < 2*rank), ties to the
+//    first cell in the reference's loop order.  Its parent pointer is that
+//    cell's painter (key>>1); an absorbed cube's is j*; a root without brother
+//    is a major.  Pointer jumping gives every cube its group (= major).
+//  * GXC/VOLNEW are the reference's FP32 running volume-weighted means, folded
+//    per group in the reference's event order (step, then ball order by
+//    distance, ties by id).
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "avism_internal.h"
+
+namespace avism {
+
+namespace {
+
+struct ExtG {
+  int n, low1, low2, ne, periodic;
+  __device__ __forceinline__ size_t ext(int i, int j, int k) const {
+    return (size_t)(i - low1) + (size_t)ne * ((size_t)(j - low1) + (size_t)ne * (k - low1));
+  }
+};
+
+struct Cubes {
+  const int32_t* ci;  // [9][nv]
+  const float* cf;    // [7][nv]
+  const int32_t* father;
+  int nv;
+  __device__ __forceinline__ int ini(int a, int v) const { return ci[a * nv + v]; }
+  __device__ __forceinline__ int fin(int a, int v) const { return ci[(3 + a) * nv + v]; }
+  __device__ __forceinline__ int ic(int a, int v) const { return ci[(6 + a) * nv + v]; }
+};
+
+inline unsigned nbk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+__global__ void k_fill(int64_t n, int32_t* a, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_vol_keys(int nv, Cubes C, float* vol) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nv) vol[v] = C.cf[6 * nv + v];
+}
+
+// INDICE (descending volume, avism.f90:1740-1745) and rank of each id
+__global__ void k_indice(int nv, const int32_t* __restrict__ asc, int32_t* indice, int32_t* rank) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  int id = asc[nv - 1 - i];
+  indice[i] = id;
+  rank[id - 1] = i;
+}
+
+__global__ void k_check_father(int nv, Cubes C, int32_t* flag) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nv && C.father[v] != C.father[0]) atomicOr(flag, 1);
+}
+
+// box of cube v grown by g, clipped to the grid, painted with atomicMin(val)
+__global__ void k_paint_box(int nv, ExtG G, Cubes C, const int32_t* __restrict__ indice,
+                            const int32_t* __restrict__ keyval, int grow, int32_t* grid) {
+  int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;  // rank
+  int lane = threadIdx.x % 64;
+  if (r >= nv) return;
+  int v = indice[r] - 1;
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = max(C.ini(a, v) - grow, G.low1);
+    hi[a] = min(C.fin(a, v) + grow, G.low2);
+  }
+  int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
+  int vol = sx * sy * sz;
+  int key = keyval ? keyval[r] : r;
+  for (int q = lane; q < vol; q += 64) {
+    int a = q % sx, b = (q / sx) % sy, c = q / (sx * sy);
+    atomicMin(&grid[G.ext(lo[0] + a, lo[1] + b, lo[2] + c)], key);
+  }
+}
+
+// j*(c): min of the +-2 painting over c's own box (rank order)
+__global__ void k_jstar(int nv, ExtG G, Cubes C, const int32_t* __restrict__ indice,
+                        const int32_t* __restrict__ P, int32_t* jstar, int32_t* key) {
+  int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  int lane = threadIdx.x % 64;
+  if (r >= nv) return;
+  int v = indice[r] - 1;
+  int lo[3], sz[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = C.ini(a, v);
+    sz[a] = C.fin(a, v) - lo[a] + 1;
+  }
+  int vol = sz[0] * sz[1] * sz[2];
+  int m = 0x7fffffff;
+  for (int q = lane; q < vol; q += 64) {
+    int a = q % sz[0], b = (q / sz[0]) % sz[1], c = q / (sz[0] * sz[1]);
+    m = min(m, P[G.ext(lo[0] + a, lo[1] + b, lo[2] + c)]);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
+  if (lane == 0) {
+    int js = m < r ? m : -1;
+    jstar[r] = js;
+    key[r] = js >= 0 ? 2 * js + 1 : 2 * r;
+  }
+}
+
+// parent pointer (rank space): absorber, close brother, or self (major)
+__global__ void k_parent(int nv, ExtG G, Cubes C, const int32_t* __restrict__ indice,
+                         const int32_t* __restrict__ jstar, const int32_t* __restrict__ K,
+                         int32_t* parent) {
+  int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  int lane = threadIdx.x % 64;
+  if (r >= nv) return;
+  if (jstar[r] >= 0) {
+    if (lane == 0) parent[r] = jstar[r];
+    return;
+  }
+  int v = indice[r] - 1;
+  int lo[3], sz[3], cc[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = max(C.ini(a, v) - 1, G.low1);
+    int hi = min(C.fin(a, v) + 1, G.low2);
+    sz[a] = hi - lo[a] + 1;
+    cc[a] = C.ic(a, v);
+  }
+  int vol = sz[0] * sz[1] * sz[2];
+  unsigned long long best = ~0ull;
+  int bestkey = -1;
+  for (int q = lane; q < vol; q += 64) {  // q follows the reference's loop order
+    int a = q % sz[0], b = (q / sz[0]) % sz[1], c = q / (sz[0] * sz[1]);
+    int x = lo[0] + a, y = lo[1] + b, z = lo[2] + c;
+    int kk = K[G.ext(x, y, z)];
+    if (kk < 2 * r) {
+      long long d2 = (long long)(cc[0] - x) * (cc[0] - x) + (long long)(cc[1] - y) * (cc[1] - y) +
+                     (long long)(cc[2] - z) * (cc[2] - z);
+      unsigned long long kq = ((unsigned long long)d2 << 32) | (unsigned)q;
+      if (kq < best) { best = kq; bestkey = kk; }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long ob = __shfl_xor(best, off);
+    int ok = __shfl_xor(bestkey, off);
+    if (ob < best) { best = ob; bestkey = ok; }
+  }
+  if (lane == 0) parent[r] = (bestkey >= 0) ? (bestkey >> 1) : r;
+}
+
+__global__ void k_jump(int nv, const int32_t* __restrict__ in, int32_t* out, int32_t* changed) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nv) return;
+  int p = in[r];
+  int pp = in[p];
+  out[r] = pp;
+  if (pp != p) atomicOr(changed, 1);
+}
+
+// UVOID (1-based ids; -1 for majors) and the major count
+__global__ void k_uvoid(int nv, const int32_t* __restrict__ indice, const int32_t* __restrict__ grp,
+                        int32_t* uvoid, int32_t* nmajor) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nv) return;
+  int id = indice[r];
+  int g = grp[r];
+  uvoid[id - 1] = (g == r) ? -1 : indice[g];
+  if (g == r) atomicAdd(nmajor, 1);
+}
+
+__global__ void k_marca(int64_t ne3, const int32_t* __restrict__ K, const int32_t* __restrict__ indice,
+                        const int32_t* __restrict__ grp, int32_t* marca) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne3) return;
+  int k = K[e];
+  marca[e] = (k == 0x7fffffff) ? 0 : indice[grp[k >> 1]];
+}
+
+// centres (voidfinding.f90:763-776): GXC = RADX(1)+(ICX-1)*DX, VOLNEW = cube volume
+__global__ void k_centres(int nv, Cubes C, float dx, float rx1, float* gxc, float* gyc, float* gzc,
+                          float* volnew) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  gxc[v] = rx1 + (float)(C.ic(0, v) - 1) * dx;
+  gyc[v] = rx1 + (float)(C.ic(1, v) - 1) * dx;
+  gzc[v] = rx1 + (float)(C.ic(2, v) - 1) * dx;
+  const float* cf = C.cf;
+  int nv_ = C.nv;
+  volnew[v] = ((cf[3 * nv_ + v] - cf[v]) * (cf[4 * nv_ + v] - cf[nv_ + v])) * (cf[5 * nv_ + v] - cf[2 * nv_ + v]);
+}
+
+__device__ __forceinline__ float kd_axis(float v, float L, int periodic) {
+  if (periodic) {
+    float a = fabsf(v), b = L - fabsf(v);
+    v = a < b ? a : b;
+  }
+  return v;
+}
+
+// absorption events: primary key (group, step, phase), secondary (ball distance, id)
+__global__ void k_events(int nv, const int32_t* __restrict__ indice, const int32_t* __restrict__ jstar,
+                         const int32_t* __restrict__ grp, const float* __restrict__ gxc,
+                         const float* __restrict__ gyc, const float* __restrict__ gzc, float L,
+                         int periodic, uint64_t* k1, uint64_t* k2, int32_t* val) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nv) return;
+  int id = indice[r];
+  int g = grp[r];
+  uint64_t step;
+  uint32_t dbits = 0;
+  if (g == r) {  // major: no event; sorts last
+    k1[r] = ~0ull;
+    k2[r] = 0;
+    val[r] = id - 1;
+    return;
+  }
+  int js = jstar[r];
+  if (js < 0) {
+    step = 2ull * (uint64_t)r;  // brother absorption at its own step (phase 0)
+  } else {
+    step = 2ull * (uint64_t)js + 1;  // ball absorption at step j*
+    int t = indice[js] - 1, a = id - 1;
+    // kd-tree metric over cube centres (voidfinding.f90:780-788, kdtree.f90:1441-1465)
+    float dx = kd_axis(gxc[a] - gxc[t], L, periodic);
+    float dy = kd_axis(gyc[a] - gyc[t], L, periodic);
+    float dz = kd_axis(gzc[a] - gzc[t], L, periodic);
+    float d = dx * dx;
+    d = d + dy * dy;
+    d = d + dz * dz;
+    dbits = __float_as_uint(d);
+  }
+  k1[r] = ((uint64_t)g << 40) | step;
+  k2[r] = ((uint64_t)dbits << 32) | (uint32_t)(id - 1);
+  val[r] = id - 1;
+}
+
+__global__ void k_group_ranges(int nv, const uint64_t* __restrict__ k1, int32_t* gstart, int32_t* gend) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  uint64_t key = k1[i];
+  if (key == ~0ull) return;
+  int g = (int)(key >> 40);
+  if (i == 0 || (k1[i - 1] >> 40) != (uint64_t)g || k1[i - 1] == ~0ull) gstart[g] = i;
+  if (i == nv - 1 || (k1[i + 1] >> 40) != (uint64_t)g || k1[i + 1] == ~0ull) gend[g] = i + 1;
+}
+
+// sequential FP32 fold per group (voidfinding.f90:865-869, 961-966)
+__global__ void k_fold(int nv, const int32_t* __restrict__ indice, const int32_t* __restrict__ grp,
+                       const int32_t* __restrict__ gstart, const int32_t* __restrict__ gend,
+                       const int32_t* __restrict__ ev, const float* __restrict__ cx,
+                       const float* __restrict__ cy, const float* __restrict__ cz,
+                       const float* __restrict__ cvol, float* gxc, float* gyc, float* gzc,
+                       float* volnew) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nv || grp[r] != r) return;
+  int v = indice[r] - 1;
+  float X = cx[v], Y = cy[v], Z = cz[v], V = cvol[v];
+  int s = gstart[r], e = gend[r];
+  for (int i = s; i < e; ++i) {
+    int a = ev[i];
+    float va = cvol[a];
+    float den = V + va;
+    X = (X * V + cx[a] * va) / den;
+    Y = (Y * V + cy[a] * va) / den;
+    Z = (Z * V + cz[a] * va) / den;
+    V = V + va;
+  }
+  gxc[v] = X;
+  gyc[v] = Y;
+  gzc[v] = Z;
+  volnew[v] = V;
+}
+
+__global__ void k_gather_u64(int n, const uint64_t* __restrict__ src, const int32_t* __restrict__ idx,
+                             uint64_t* dst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+// absorbed cubes keep their own centre and volume (the reference never updates them)
+__global__ void k_copy_nonleaders(int nv, const int32_t* __restrict__ indice, const int32_t* __restrict__ grp,
+                                  const float* __restrict__ cx, const float* __restrict__ cy,
+                                  const float* __restrict__ cz, const float* __restrict__ cvol, float* gxc,
+                                  float* gyc, float* gzc, float* volnew) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nv || grp[r] == r) return;
+  int v = indice[r] - 1;
+  gxc[v] = cx[v];
+  gyc[v] = cy[v];
+  gzc[v] = cz[v];
+  volnew[v] = cvol[v];
+}
+
+// ---------------------------------------------------------- post-processing
+__global__ void k_count_cells(int64_t ne3, ExtG G, const int32_t* __restrict__ marca, int inner_only,
+                              int32_t* ncell) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne3) return;
+  int id = marca[e];
+  if (id <= 0) return;
+  if (inner_only) {
+    int I = (int)(e % G.ne) + G.low1, J = (int)((e / G.ne) % G.ne) + G.low1,
+        K = (int)(e / ((int64_t)G.ne * G.ne)) + G.low1;
+    if (I < 1 || I > G.n || J < 1 || J > G.n || K < 1 || K > G.n) return;
+  }
+  atomicAdd(&ncell[id - 1], 1);
+}
+
+// NCELL_MIN / RMIN removals (avism.f90:1812-1827, 1845-1859): decided per main void
+__global__ void k_mark_removed(int nv, const int32_t* __restrict__ uvoid, const int32_t* __restrict__ ncell,
+                               int mode, float dx, double rmin3, int32_t* removed, float* volnew) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  removed[v] = 0;
+  if (uvoid[v] >= 0) return;
+  if (mode == 0) {
+    if (ncell[v] < kNcellMin) removed[v] = 1;
+  } else {
+    float vol = (float)((double)ncell[v] * (double)((dx * dx) * dx));
+    volnew[v] = vol;
+    if ((double)vol < rmin3) removed[v] = 1;
+  }
+}
+
+__global__ void k_apply_removed_cubes(int nv, const int32_t* __restrict__ removed, int32_t* uvoid,
+                                      int32_t* ncell, float* volnew) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  int u = uvoid[v];
+  if (removed[v]) {
+    uvoid[v] = 0;
+    ncell[v] = 0;
+    volnew[v] = 0.0f;
+  } else if (u > 0 && removed[u - 1]) {
+    uvoid[v] = 0;
+  }
+}
+
+__global__ void k_apply_removed_cells(int64_t ne3, const int32_t* __restrict__ removed, int32_t* marca) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne3) return;
+  int id = marca[e];
+  if (id > 0 && removed[id - 1]) marca[e] = 0;
+}
+
+// VOID_SHAPE (shape.f90:155-172): per-cell FP32 terms, FP64 sums
+__global__ void k_inertia(int64_t ne3, ExtG G, const float* __restrict__ rad, int rlo,
+                          const int32_t* __restrict__ marca, const float* __restrict__ gxc,
+                          const float* __restrict__ gyc, const float* __restrict__ gzc, double* inert) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int id = e < ne3 ? marca[e] : 0;
+  double t[6] = {0, 0, 0, 0, 0, 0};
+  if (id > 0) {
+    int I = (int)(e % G.ne) + G.low1, J = (int)((e / G.ne) % G.ne) + G.low1,
+        K = (int)(e / ((int64_t)G.ne * G.ne)) + G.low1;
+    float rk[3] = {rad[I - rlo] - gxc[id - 1], rad[J - rlo] - gyc[id - 1], rad[K - rlo] - gzc[id - 1]};
+    float rr = sqrtf((rk[0] * rk[0] + rk[1] * rk[1]) + rk[2] * rk[2]);
+    float d = (1.0f * rr) * rr;
+    t[0] = (double)(d - rk[0] * rk[0]);
+    t[1] = (double)(d - rk[1] * rk[1]);
+    t[2] = (double)(d - rk[2] * rk[2]);
+    t[3] = (double)(0.0f * rr * rr - rk[0] * rk[1]);
+    t[4] = (double)(0.0f * rr * rr - rk[0] * rk[2]);
+    t[5] = (double)(0.0f * rr * rr - rk[1] * rk[2]);
+  }
+  // waves whose void cells share one id add once
+  int lead = id;
+  for (int off = 32; off > 0; off >>= 1) lead = max(lead, __shfl_xor(lead, off));
+  bool uniform = __all(id == 0 || id == lead);
+  if (uniform) {
+    for (int c = 0; c < 6; ++c) {
+      double s = t[c];
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      if ((threadIdx.x % 64) == 0 && lead > 0) atomicAdd(&inert[(size_t)(lead - 1) * 6 + c], s);
+    }
+  } else if (id > 0) {
+    for (int c = 0; c < 6; ++c) atomicAdd(&inert[(size_t)(id - 1) * 6 + c], t[c]);
+  }
+}
+
+// JACOBI (shape.f90:46-134) in float with the reference's promotions
+__device__ void jacobi3(float A[3][3], float D[3]) {
+  float B[3], Z[3];
+  for (int ip = 0; ip < 3; ip++) { B[ip] = A[ip][ip]; D[ip] = B[ip]; Z[ip] = 0.0f; }
+  for (int i = 1; i <= 50; i++) {
+    float sm = 0.0f;
+    for (int ip = 0; ip < 2; ip++)
+      for (int iq = ip + 1; iq < 3; iq++) sm = sm + fabsf(A[ip][iq]);
+    if (sm == 0.0f) return;
+    float tresh = (i < 4) ? (float)(0.2 * (double)(sm * sm)) : 0.0f;
+    for (int ip = 0; ip < 2; ip++)
+      for (int iq = ip + 1; iq < 3; iq++) {
+        float g = (float)(100.0 * (double)fabsf(A[ip][iq]));
+        if (i > 4 && fabsf(D[ip]) + g == fabsf(D[ip]) && fabsf(D[iq]) + g == fabsf(D[iq])) {
+          A[ip][iq] = 0.0f;
+        } else if (fabsf(A[ip][iq]) > tresh) {
+          float h = D[iq] - D[ip], t;
+          if (fabsf(h) + g == fabsf(h)) t = A[ip][iq] / h;
+          else {
+            float theta = (float)(0.5 * (double)h / (double)A[ip][iq]);
+            t = (float)(1.0 / ((double)fabsf(theta) + (double)sqrtf((float)(1.0 + (double)(theta * theta)))));
+            if (theta < 0.0f) t = -t;
+          }
+          float c = (float)(1.0 / (double)sqrtf((float)(1.0 + (double)(t * t))));
+          float s = t * c;
+          float tau = (float)((double)s / (1.0 + (double)c));
+          h = t * A[ip][iq];
+          Z[ip] = Z[ip] - h; Z[iq] = Z[iq] + h;
+          D[ip] = D[ip] - h; D[iq] = D[iq] + h;
+          A[ip][iq] = 0.0f;
+          for (int j = 0; j < ip; j++) {
+            float gg = A[j][ip], hh = A[j][iq];
+            A[j][ip] = gg - s * (hh + gg * tau); A[j][iq] = hh + s * (gg - hh * tau);
+          }
+          for (int j = ip + 1; j < iq; j++) {
+            float gg = A[ip][j], hh = A[j][iq];
+            A[ip][j] = gg - s * (hh + gg * tau); A[j][iq] = hh + s * (gg - hh * tau);
+          }
+          for (int j = iq + 1; j < 3; j++) {
+            float gg = A[ip][j], hh = A[iq][j];
+            A[ip][j] = gg - s * (hh + gg * tau); A[iq][j] = hh + s * (gg - hh * tau);
+          }
+        }
+      }
+    for (int ip = 0; ip < 3; ip++) { B[ip] = B[ip] + Z[ip]; D[ip] = B[ip]; Z[ip] = 0.0f; }
+  }
+}
+
+__global__ void k_shape(int nv, const int32_t* __restrict__ uvoid, const int32_t* __restrict__ ncell,
+                        const double* __restrict__ inert, float dx, float* eps, float* ipr) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  eps[v] = 0.0f;
+  ipr[v] = 0.0f;
+  if (uvoid[v] != -1) return;
+  const double* T = inert + (size_t)v * 6;
+  float nc = (float)ncell[v];
+  float A[3][3];
+  A[0][0] = (float)T[0] / nc; A[1][1] = (float)T[1] / nc; A[2][2] = (float)T[2] / nc;
+  A[0][1] = A[1][0] = (float)T[3] / nc;
+  A[0][2] = A[2][0] = (float)T[4] / nc;
+  A[1][2] = A[2][1] = (float)T[5] / nc;
+  float D[3] = {0, 0, 0};
+  jacobi3(A, D);
+  float ax[3] = {sqrtf(2.5f * fabsf(D[1] + D[2] - D[0])), sqrtf(2.5f * fabsf(D[2] + D[0] - D[1])),
+                 sqrtf(2.5f * fabsf(D[0] + D[1] - D[2]))};
+  for (int i = 0; i < 2; i++) {  // SORT (shape.f90:2-29)
+    int kk = i;
+    float pp = ax[i];
+    for (int j = i + 1; j < 3; j++)
+      if (ax[j] >= pp) { kk = j; pp = ax[j]; }
+    if (kk != i) { ax[kk] = ax[i]; ax[i] = pp; }
+  }
+  float e = 1.0f - ax[2] / ax[0];
+  float volm = ((nc * dx) * dx) * dx;
+  float vell = (float)(((((double)(4.0f / 3.0f) * kPi) * (double)ax[0]) * (double)ax[1]) * (double)ax[2]);
+  float ip = volm / vell;
+  if (ip > 1.0f) { ip = -99.0f; e = -99.0f; }
+  eps[v] = e;
+  ipr[v] = ip;
+}
+
+__global__ void k_xc(int nv, Cubes C, const int32_t* __restrict__ uvoid, float dx, float rx1, float* xc,
+                     float* yc, float* zc) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  if (uvoid[v] >= 0) { xc[v] = yc[v] = zc[v] = 0.0f; return; }
+  xc[v] = rx1 + (float)(C.ic(0, v) - 1) * dx;
+  yc[v] = rx1 + (float)(C.ic(1, v) - 1) * dx;
+  zc[v] = rx1 + (float)(C.ic(2, v) - 1) * dx;
+}
+
+// ---- SIMPLE_CONNECTION_FAST as connected components of the empty cells
+__device__ __forceinline__ int cc_find(const int32_t* lab, int x) {
+  int p = lab[x];
+  while (p != x) {
+    x = p;
+    p = lab[x];
+  }
+  return x;
+}
+
+__global__ void k_cc_init(int64_t ne3, const int32_t* __restrict__ marca, int32_t* lab) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne3) lab[e] = marca[e] == 0 ? (int32_t)e : -1;
+}
+
+__device__ void cc_union(int32_t* lab, int a, int b) {
+  for (;;) {
+    a = cc_find(lab, a);
+    b = cc_find(lab, b);
+    if (a == b) return;
+    if (a < b) { int t = a; a = b; b = t; }  // link the larger root under the smaller
+    int old = atomicMin(&lab[a], b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+__global__ void k_cc_merge(int64_t ne3, ExtG G, int32_t* lab) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne3 || lab[e] < 0) return;
+  int i = (int)(e % G.ne);
+  int j = (int)((e / G.ne) % G.ne);
+  int k = (int)(e / ((int64_t)G.ne * G.ne));
+  if (i > 0 && lab[e - 1] >= 0) cc_union(lab, (int)e, (int)(e - 1));
+  if (j > 0 && lab[e - G.ne] >= 0) cc_union(lab, (int)e, (int)(e - G.ne));
+  if (k > 0 && lab[e - (int64_t)G.ne * G.ne] >= 0) cc_union(lab, (int)e, (int)(e - (int64_t)G.ne * G.ne));
+}
+
+__global__ void k_cc_flatten(int64_t ne3, int32_t* lab) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne3 || lab[e] < 0) return;
+  lab[e] = cc_find(lab, (int)e);
+}
+
+// holes (empty cells not connected to the corner, voidfinding.f90:1040-1135)
+// take the id of the first non-hole cell in +x (1140-1156)
+__global__ void k_fill_holes(int64_t ne3, ExtG G, const int32_t* __restrict__ lab, int32_t* marca,
+                             int32_t* nholes) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne3) return;
+  int outside = lab[0];
+  int l = lab[e];
+  if (l < 0 || l == outside) return;
+  atomicAdd(nholes, 1);
+  int i = (int)(e % G.ne);
+  int64_t f = e;
+  while (i < G.ne && lab[f] >= 0 && lab[f] != outside) {
+    ++i;
+    ++f;
+  }
+  marca[e] = (i < G.ne) ? marca[f] : 0;
+}
+
+// ---- VOID_PERIODIC (voidfinding.f90:1367-1568), exact sequential semantics
+__global__ void k_inner_presence(int64_t ne3, ExtG G, const int32_t* __restrict__ marca, int32_t* has) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne3) return;
+  int id = marca[e];
+  if (id <= 0) return;
+  int I = (int)(e % G.ne) + G.low1, J = (int)((e / G.ne) % G.ne) + G.low1,
+      K = (int)(e / ((int64_t)G.ne * G.ne)) + G.low1;
+  if (I >= 1 && I <= G.n && J >= 1 && J <= G.n && K >= 1 && K <= G.n) has[id - 1] = 1;
+}
+
+__global__ void k_absent_removed(int nv, const int32_t* __restrict__ uvoid, const int32_t* __restrict__ has,
+                                 int32_t* removed) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nv) removed[v] = (uvoid[v] == -1 && !has[v]) ? 1 : 0;
+}
+
+__global__ void k_cell_pairs(int64_t ne3, const int32_t* __restrict__ marca, uint32_t* key, uint32_t* val) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne3) return;
+  int id = marca[e];
+  key[e] = id > 0 ? (uint32_t)id : 0xffffffffu;
+  val[e] = (uint32_t)e;
+}
+
+__global__ void k_label_ranges(int64_t ne3, const uint32_t* __restrict__ key, int32_t* lstart, int32_t* lend) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne3) return;
+  uint32_t k = key[i];
+  if (k == 0xffffffffu) return;
+  if (i == 0 || key[i - 1] != k) lstart[k - 1] = (int32_t)i;
+  if (i == ne3 - 1 || key[i + 1] != k) lend[k - 1] = (int32_t)(i + 1);
+}
+
+__device__ __forceinline__ int resolve(const int32_t* lmap, int l) {
+  while (l > 0 && lmap[l - 1] != l) l = lmap[l - 1];
+  return l;
+}
+
+// one workgroup walks the main voids in INDICE order
+__global__ void __launch_bounds__(1024) k_void_periodic(
+    int nv, ExtG G, const int32_t* __restrict__ indice, const int32_t* __restrict__ uvoid,
+    const int32_t* __restrict__ marca_base, const float* __restrict__ gxc, const float* __restrict__ gyc,
+    const float* __restrict__ gzc, float half, float dx, float rx1, const uint32_t* __restrict__ cells,
+    const int32_t* __restrict__ lstart, const int32_t* __restrict__ lend, int32_t* lmap, int32_t* mnext,
+    int32_t* mtail, int32_t* selfm, int32_t* stats) {
+  __shared__ int s_cnt, s_cnt2, s_go, s_ind, s_ind2, s_sx, s_sy, s_sz;
+  const int tid = threadIdx.x;
+  for (int I = 0; I < nv; ++I) {
+    if (tid == 0) {
+      s_go = 0;
+      int id = indice[I];
+      if (uvoid[id - 1] == -1) {
+        float xc1 = gxc[id - 1], yc1 = gyc[id - 1], zc1 = gzc[id - 1];
+        int sx = 0, sy = 0, sz = 0;
+        if (xc1 < -half) sx = 1;
+        if (xc1 > half) sx = -1;
+        if (yc1 < -half) sy = 1;
+        if (yc1 > half) sy = -1;
+        if (zc1 < -half) sz = 1;
+        if (zc1 > half) sz = -1;
+        if (sx | sy | sz) {
+          int igx = (int)((gxc[id - 1] - rx1) / dx) + 1;
+          int igy = (int)((gyc[id - 1] - rx1) / dx) + 1;
+          int igz = (int)((gzc[id - 1] - rx1) / dx) + 1;
+          int ax = igx + sx * G.n, ay = igy + sy * G.n, az = igz + sz * G.n;
+          if (ax >= G.low1 && ax <= G.low2 && ay >= G.low1 && ay <= G.low2 && az >= G.low1 && az <= G.low2) {
+            int b2 = marca_base[G.ext(ax, ay, az)];
+            int ind2 = b2 > 0 ? resolve(lmap, b2) : 0;
+            if (ind2 != 0 && !selfm[id - 1]) {
+              s_go = 1;
+              s_ind = id;
+              s_ind2 = ind2;
+              s_sx = sx; s_sy = sy; s_sz = sz;
+            }
+          }
+        }
+      }
+      s_cnt = 0;
+      s_cnt2 = 0;
+    }
+    __syncthreads();
+    if (s_go) {
+      const int ind = s_ind, ind2 = s_ind2;
+      int c1 = 0, c2 = 0;
+      // cells labelled ind now: base labels of the member list of ind
+      for (int b = ind; b > 0; b = mnext[b - 1]) {
+        int s = lstart[b - 1], e = lend[b - 1];
+        for (int q = s + tid; q < e; q += blockDim.x) {
+          int64_t cell = cells[q];
+          int i = (int)(cell % G.ne) + G.low1, j = (int)((cell / G.ne) % G.ne) + G.low1,
+              k = (int)(cell / ((int64_t)G.ne * G.ne)) + G.low1;
+          int a = i + s_sx * G.n, bb = j + s_sy * G.n, cc = k + s_sz * G.n;
+          if (a >= G.low1 && a <= G.low2 && bb >= G.low1 && bb <= G.low2 && cc >= G.low1 && cc <= G.low2) {
+            c2++;
+            int l = marca_base[G.ext(a, bb, cc)];
+            if (l > 0 && resolve(lmap, l) == ind2) c1++;
+          }
+        }
+        if (b == mtail[ind - 1]) break;
+      }
+      atomicAdd(&s_cnt, c1);
+      atomicAdd(&s_cnt2, c2);
+      __syncthreads();
+      if (tid == 0) {
+        float inter = (s_cnt2 == 0) ? 0.0f : (float)s_cnt / (float)s_cnt2;
+        if (inter > 0.4f) {  // CHANGE_BELONG(IND2, IND)
+          stats[0]++;
+          if (ind2 == ind) {
+            selfm[ind - 1] = 1;
+          } else {
+            lmap[ind - 1] = ind2;
+            // append ind's member list to ind2's
+            mnext[mtail[ind2 - 1] - 1] = ind;
+            mtail[ind2 - 1] = mtail[ind - 1];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_apply_lmap(int nv, const int32_t* __restrict__ lmap, const int32_t* __restrict__ selfm,
+                             int32_t* uvoid, float* volnew) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  int u = uvoid[v];
+  if (u == -1) {
+    int r = resolve(lmap, v + 1);
+    if (r != v + 1) { uvoid[v] = r; volnew[v] = 0.0f; }
+    else if (selfm[v]) { uvoid[v] = v + 1; volnew[v] = 0.0f; }
+  } else if (u > 0) {
+    uvoid[v] = resolve(lmap, u);
+  }
+}
+
+__global__ void k_relabel_cells(int64_t ne3, const int32_t* __restrict__ lmap, int32_t* marca) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne3) return;
+  int id = marca[e];
+  if (id > 0) marca[e] = resolve(lmap, id);
+}
+
+__global__ void k_init_lmap(int nv, int32_t* lmap, int32_t* mnext, int32_t* mtail, int32_t* selfm) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  lmap[v] = v + 1;
+  mnext[v] = 0;
+  mtail[v] = v + 1;
+  selfm[v] = 0;
+}
+
+// final properties over the inner box (avism.f90:1918-1945)
+__global__ void k_final_props(int64_t ne3, ExtG G, const int32_t* __restrict__ marca,
+                              const int32_t* __restrict__ uvoid, const float* __restrict__ u1co,
+                              float mtot_scale_md, float dx, float rete3, int32_t* ncell, double* umean,
+                              double* mtot, unsigned long long* ncells_total) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne3) return;
+  int id = marca[e];
+  if (id <= 0 || uvoid[id - 1] != -1) return;
+  int I = (int)(e % G.ne) + G.low1, J = (int)((e / G.ne) % G.ne) + G.low1,
+      K = (int)(e / ((int64_t)G.ne * G.ne)) + G.low1;
+  if (I < 1 || I > G.n || J < 1 || J > G.n || K < 1 || K > G.n) return;
+  float u = u1co[(size_t)(I - 1) + (size_t)G.n * ((size_t)(J - 1) + (size_t)G.n * (K - 1))];
+  atomicAdd(&ncell[id - 1], 1);
+  atomicAdd(&umean[id - 1], (double)u);
+  float m = ((((u * mtot_scale_md) * dx) * dx) * dx) * rete3;
+  atomicAdd(&mtot[id - 1], (double)m);
+  atomicAdd(ncells_total, 1ull);
+}
+
+__global__ void k_finish_catalogue(int nv, const int32_t* __restrict__ ncell, const double* __restrict__ umean,
+                                   const double* __restrict__ mtot, float* fum, float* fmt) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  fum[v] = ncell[v] > 0 ? (float)umean[v] / (float)ncell[v] : (float)umean[v];
+  fmt[v] = (float)mtot[v];
+}
+
+__global__ void k_req(int nv, const int32_t* __restrict__ uvoid, const float* __restrict__ volnew,
+                      float* req, int32_t* nvoidt) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  if (uvoid[v] != -1) { req[v] = 0.0f; return; }
+  atomicAdd(nvoidt, 1);
+  double volm = (double)volnew[v];  // avism.f90:2074
+  req[v] = (float)pow((3.0 * volm) / (4.0 * kPi), (double)(1.0f / 3.0f));
+}
+
+}  // namespace
+
+// vf layout on the device: 12 blocks of nv floats (gxc gyc gzc xc yc zc volnew umean mtot eps ip req)
+int merge_and_post(avism_ctx* c, int32_t stop_after) {
+  const int nv = c->nvoid;
+  const GridGeom& g = c->g;
+  ExtG G{g.n, g.low1, g.low2, g.next, g.periodic};
+  const int64_t ne3 = (int64_t)g.next * g.next * g.next;
+  c->nvoid2 = 0;
+  c->nvoidt = 0;
+  c->volt_clean = 0.0;
+  if (!c->marca.ensure((size_t)ne3 * 4) || !c->vf.ensure((size_t)12 * (nv + 1) * 4) ||
+      !c->indice.ensure((size_t)(nv + 1) * 4) || !c->uvoid.ensure((size_t)(nv + 1) * 4) ||
+      !c->ncellv.ensure((size_t)(nv + 1) * 4))
+    AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (merge)");
+  if (nv == 0) {
+    AV_CHECK_HIP(c, hipMemsetAsync(c->marca.p, 0, (size_t)ne3 * 4, c->stream));
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    return AVISM_OK;
+  }
+  Cubes C{c->cube_i.as<int32_t>(), c->cube_f.as<float>(), c->cube_father.as<int32_t>(), nv};
+  float* vf = c->vf.as<float>();
+  float *GX = vf, *GY = vf + nv, *GZ = vf + 2 * nv, *XC = vf + 3 * nv, *YC = vf + 4 * nv, *ZC = vf + 5 * nv;
+  float *VOLNEW = vf + 6 * nv, *UMEAN = vf + 7 * nv, *MTOT = vf + 8 * nv, *EPS = vf + 9 * nv,
+        *IPR = vf + 10 * nv, *REQ = vf + 11 * nv;
+  int32_t* indice = c->indice.as<int32_t>();
+  int32_t* uvoid = c->uvoid.as<int32_t>();
+  int32_t* ncell = c->ncellv.as<int32_t>();
+  // scratch per cube (ranks)
+  if (!c->rank_of.ensure((size_t)(nv + 1) * 4 * 12) || !c->paintP.ensure((size_t)ne3 * 4) ||
+      !c->paintK.ensure((size_t)ne3 * 4) || !c->ev_key.ensure((size_t)(nv + 1) * 8) ||
+      !c->ev_key2.ensure((size_t)(nv + 1) * 8) || !c->ev_val.ensure((size_t)(nv + 1) * 4) ||
+      !c->ev_val2.ensure((size_t)(nv + 1) * 4) || !c->inert.ensure((size_t)(nv + 1) * 6 * 8 * 3))
+    AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (merge scratch)");
+  int32_t* R = c->rank_of.as<int32_t>();
+  int32_t *rank = R, *jstar = R + (nv + 1), *keyv = R + 2 * (nv + 1), *par = R + 3 * (nv + 1),
+          *par2 = R + 4 * (nv + 1), *gstart = R + 5 * (nv + 1), *gend = R + 6 * (nv + 1),
+          *removed = R + 7 * (nv + 1), *asc = R + 8 * (nv + 1), *lmap = R + 9 * (nv + 1),
+          *mnext = R + 10 * (nv + 1), *mtail = R + 11 * (nv + 1);
+  float* cvol = (float*)c->ev_val2.p;  // own cube volumes (reused below as key scratch)
+  int32_t* small = (int32_t*)((char*)c->scalars.ensure(4096) + 1024);  // counters
+  const unsigned W = 4;  // waves per block in wave-per-cube kernels
+  // ---- volume order
+  {
+    StageTimer t(c, "volume_sort");
+    float* volk = (float*)c->ev_key.p;
+    k_vol_keys<<<nbk(nv), 256, 0, c->stream>>>(nv, C, volk);
+    int rc = indexx_device(c, volk, nv, asc);
+    if (rc) return rc;
+    k_indice<<<nbk(nv), 256, 0, c->stream>>>(nv, asc, indice, rank);
+    AV_CHECK_HIP(c, hipMemsetAsync(small, 0, 64, c->stream));
+    k_check_father<<<nbk(nv), 256, 0, c->stream>>>(nv, C, small);
+    int32_t hf = 0;
+    AV_CHECK_HIP(c, hipMemcpyAsync(&hf, small, 4, hipMemcpyDeviceToHost, c->stream));
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    if (hf) AV_FAIL(c, AVISM_ERR_UNSUPPORTED, "cubes with different parent voids (void-in-void levels) are not supported by the GPU merge");
+  }
+  int32_t* P = c->paintP.as<int32_t>();
+  int32_t* K = c->paintK.as<int32_t>();
+  {
+    StageTimer t(c, "merge");
+    k_fill<<<nbk(ne3), 256, 0, c->stream>>>(ne3, P, 0x7fffffff);
+    k_fill<<<nbk(ne3), 256, 0, c->stream>>>(ne3, K, 0x7fffffff);
+    k_paint_box<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, nullptr, 2, P);
+    k_jstar<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, P, jstar, keyv);
+    k_paint_box<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, keyv, 0, K);
+    k_parent<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, jstar, K, par);
+    int32_t* a = par;
+    int32_t* b = par2;
+    for (int it = 0; it < 64; ++it) {
+      AV_CHECK_HIP(c, hipMemsetAsync(small, 0, 4, c->stream));
+      k_jump<<<nbk(nv), 256, 0, c->stream>>>(nv, a, b, small);
+      int32_t ch = 0;
+      AV_CHECK_HIP(c, hipMemcpyAsync(&ch, small, 4, hipMemcpyDeviceToHost, c->stream));
+      AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+      std::swap(a, b);
+      if (!ch) break;
+    }
+    int32_t* grp = a;
+    AV_CHECK_HIP(c, hipMemsetAsync(small + 1, 0, 4, c->stream));
+    k_uvoid<<<nbk(nv), 256, 0, c->stream>>>(nv, indice, grp, uvoid, small + 1);
+    k_marca<<<nbk(ne3), 256, 0, c->stream>>>(ne3, K, indice, grp, c->marca.as<int32_t>());
+    // GXC / VOLNEW fold in the reference's event order
+    float *cx = XC, *cy = YC, *cz = ZC;  // original centres (XC.. rewritten later)
+    k_centres<<<nbk(nv), 256, 0, c->stream>>>(nv, C, g.dx, g.rx1, cx, cy, cz, cvol);
+    uint64_t* k1 = (uint64_t*)c->ev_key.p;    // by rank
+    uint64_t* k2 = (uint64_t*)c->ev_key2.p;   // by rank
+    uint64_t* kb = (uint64_t*)c->inert.p;     // sorted keys scratch
+    uint64_t* kc = kb + (nv + 1);
+    int32_t* ranks = c->ev_val.as<int32_t>();
+    int32_t* rb = (int32_t*)(kc + (nv + 1));
+    int32_t* rs = rb + (nv + 1);
+    k_events<<<nbk(nv), 256, 0, c->stream>>>(nv, indice, jstar, grp, cx, cy, cz, g.lado0plus, g.periodic,
+                                             k1, k2, ranks);
+    size_t tmp = 0, tmp2 = 0;
+    AV_CHECK_HIP(c, rocprim::radix_sort_pairs(nullptr, tmp, k2, kb, ranks, rb, (size_t)nv, 0, 64, c->stream));
+    AV_CHECK_HIP(c, rocprim::radix_sort_pairs(nullptr, tmp2, kb, kc, rb, rs, (size_t)nv, 0, 64, c->stream));
+    if (!c->sort_tmp.ensure(std::max(tmp, tmp2) + 256)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+    AV_CHECK_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp.p, tmp, k2, kb, ranks, rb, (size_t)nv, 0, 64,
+                                              c->stream));
+    k_gather_u64<<<nbk(nv), 256, 0, c->stream>>>(nv, k1, rb, kb);
+    AV_CHECK_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp.p, tmp2, kb, kc, rb, rs, (size_t)nv, 0, 64,
+                                              c->stream));
+    AV_CHECK_HIP(c, hipMemsetAsync(gstart, 0, (size_t)(nv + 1) * 4, c->stream));
+    AV_CHECK_HIP(c, hipMemsetAsync(gend, 0, (size_t)(nv + 1) * 4, c->stream));
+    k_group_ranges<<<nbk(nv), 256, 0, c->stream>>>(nv, kc, gstart, gend);
+    k_fold<<<nbk(nv), 256, 0, c->stream>>>(nv, indice, grp, gstart, gend, rs, cx, cy, cz, cvol, GX, GY, GZ,
+                                           VOLNEW);
+    k_copy_nonleaders<<<nbk(nv), 256, 0, c->stream>>>(nv, indice, grp, cx, cy, cz, cvol, GX, GY, GZ, VOLNEW);
+    AV_CHECK_HIP(c, hipGetLastError());
+  }
+  int32_t hmaj = 0;
+  AV_CHECK_HIP(c, hipMemcpyAsync(&hmaj, small + 1, 4, hipMemcpyDeviceToHost, c->stream));
+  AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  c->nvoid2 = hmaj;
+  if (stop_after == 1) return AVISM_OK;
+
+  int32_t* marca = c->marca.as<int32_t>();
+  // ---- NCELL_MIN filter (avism.f90:1796-1827)
+  {
+    StageTimer t(c, "post_filters");
+    AV_CHECK_HIP(c, hipMemsetAsync(ncell, 0, (size_t)(nv + 1) * 4, c->stream));
+    k_count_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, marca, 0, ncell);
+    k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 0, g.dx, 0.0, removed, VOLNEW);
+    k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
+    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, marca);
+  }
+  // ---- VOID_SHAPE (avism.f90:1831)
+  float* d_rad = (float*)((char*)c->scalars.p + 4096);
+  {
+    StageTimer t(c, "shape");
+    AV_CHECK_HIP(c, hipMemcpyAsync(d_rad, g.rad.data(), g.rad.size() * 4, hipMemcpyHostToDevice, c->stream));
+    double* inert = (double*)c->inert.p;
+    AV_CHECK_HIP(c, hipMemsetAsync(inert, 0, (size_t)(nv + 1) * 6 * 8, c->stream));
+    k_inertia<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, d_rad, g.rlo, marca, GX, GY, GZ, inert);
+    k_shape<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, inert, g.dx, EPS, IPR);
+  }
+  // ---- VOLNEW from NCELLV + RMIN filter (1834-1859), centres XC (1863-1870)
+  double rmin3 = (double)(4.0f / 3.0f) * kPi * (double)((c->vp.rmin * c->vp.rmin) * c->vp.rmin);
+  {
+    StageTimer t(c, "post_filters");
+    AV_CHECK_HIP(c, hipMemsetAsync(VOLNEW, 0, (size_t)nv * 4, c->stream));
+    k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 1, g.dx, rmin3, removed, VOLNEW);
+    k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
+    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, marca);
+    k_xc<<<nbk(nv), 256, 0, c->stream>>>(nv, C, uvoid, g.dx, g.rx1, XC, YC, ZC);
+  }
+  if (stop_after == 2) {
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    return AVISM_OK;
+  }
+  // ---- SIMPLE_CONNECTION_FAST (1877)
+  {
+    StageTimer t(c, "hole_fill");
+    int32_t* lab = P;  // reuse
+    k_cc_init<<<nbk(ne3), 256, 0, c->stream>>>(ne3, marca, lab);
+    k_cc_merge<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, lab);
+    k_cc_flatten<<<nbk(ne3), 256, 0, c->stream>>>(ne3, lab);
+    AV_CHECK_HIP(c, hipMemsetAsync(small + 2, 0, 4, c->stream));
+    k_fill_holes<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, lab, marca, small + 2);
+    AV_CHECK_HIP(c, hipGetLastError());
+  }
+  if (stop_after == 3) {
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    return AVISM_OK;
+  }
+  // ---- VOID_PERIODIC (1892-1910)
+  if (g.periodic) {
+    StageTimer t(c, "void_periodic");
+    int32_t* has = gstart;
+    AV_CHECK_HIP(c, hipMemsetAsync(has, 0, (size_t)(nv + 1) * 4, c->stream));
+    k_inner_presence<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, marca, has);
+    k_absent_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, has, removed);
+    k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
+    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, marca);
+    // cell lists per label (sorted by label, then cell order)
+    if (!c->cellkey.ensure((size_t)ne3 * 4) || !c->cellkey2.ensure((size_t)ne3 * 4) ||
+        !c->cellid.ensure((size_t)ne3 * 4) || !c->cellid2.ensure((size_t)ne3 * 4))
+      AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (periodic)");
+    uint32_t* ka = c->cellkey.as<uint32_t>();
+    uint32_t* kb2 = c->cellkey2.as<uint32_t>();
+    uint32_t* va = c->cellid.as<uint32_t>();
+    uint32_t* vb = c->cellid2.as<uint32_t>();
+    k_cell_pairs<<<nbk(ne3), 256, 0, c->stream>>>(ne3, marca, ka, va);
+    size_t tmp = 0;
+    AV_CHECK_HIP(c, rocprim::radix_sort_pairs(nullptr, tmp, ka, kb2, va, vb, (size_t)ne3, 0, 32, c->stream));
+    if (!c->sort_tmp.ensure(tmp)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (periodic sort)");
+    AV_CHECK_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp.p, tmp, ka, kb2, va, vb, (size_t)ne3, 0, 32, c->stream));
+    int32_t* lstart = gend;
+    int32_t* lend = par2;
+    AV_CHECK_HIP(c, hipMemsetAsync(lstart, 0, (size_t)(nv + 1) * 4, c->stream));
+    AV_CHECK_HIP(c, hipMemsetAsync(lend, 0, (size_t)(nv + 1) * 4, c->stream));
+    k_label_ranges<<<nbk(ne3), 256, 0, c->stream>>>(ne3, kb2, lstart, lend);
+    int32_t* selfm = removed;
+    k_init_lmap<<<nbk(nv), 256, 0, c->stream>>>(nv, lmap, mnext, mtail, selfm);
+    AV_CHECK_HIP(c, hipMemsetAsync(small + 3, 0, 4, c->stream));
+    k_void_periodic<<<1, 1024, 0, c->stream>>>(nv, G, indice, uvoid, marca, GX, GY, GZ, g.L0 / 2.0f, g.dx, g.rx1,
+                                                vb, lstart, lend, lmap, mnext, mtail, selfm, small + 3);
+    k_apply_lmap<<<nbk(nv), 256, 0, c->stream>>>(nv, lmap, selfm, uvoid, VOLNEW);
+    k_relabel_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, lmap, marca);
+    AV_CHECK_HIP(c, hipGetLastError());
+  }
+  // ---- final properties over the inner box (1918-1973)
+  {
+    StageTimer t(c, "final_props");
+    double* um = (double*)c->inert.p;
+    double* mt = um + (nv + 1);
+    unsigned long long* tot = (unsigned long long*)((char*)c->scalars.p + 2048);
+    AV_CHECK_HIP(c, hipMemsetAsync(ncell, 0, (size_t)(nv + 1) * 4, c->stream));
+    AV_CHECK_HIP(c, hipMemsetAsync(um, 0, (size_t)(nv + 1) * 16, c->stream));
+    AV_CHECK_HIP(c, hipMemsetAsync(tot, 0, 8, c->stream));
+    float rete3 = (c->rete * c->rete) * c->rete;
+    k_final_props<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, marca, uvoid, c->u1co.as<float>(), c->meandens,
+                                                   g.dx, rete3, ncell, um, mt, tot);
+    k_finish_catalogue<<<nbk(nv), 256, 0, c->stream>>>(nv, ncell, um, mt, UMEAN, MTOT);
+    AV_CHECK_HIP(c, hipMemsetAsync(VOLNEW, 0, (size_t)nv * 4, c->stream));
+    k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 1, g.dx, rmin3, removed, VOLNEW);
+    k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
+    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, marca);
+    AV_CHECK_HIP(c, hipMemsetAsync(small + 4, 0, 4, c->stream));
+    k_req<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, VOLNEW, REQ, small + 4);
+    AV_CHECK_HIP(c, hipGetLastError());
+    unsigned long long ht = 0;
+    int32_t hn = 0;
+    AV_CHECK_HIP(c, hipMemcpyAsync(&ht, tot, 8, hipMemcpyDeviceToHost, c->stream));
+    AV_CHECK_HIP(c, hipMemcpyAsync(&hn, small + 4, 4, hipMemcpyDeviceToHost, c->stream));
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    c->nvoidt = hn;
+    c->volt_clean = (double)ht * (double)((g.dx * g.dx) * g.dx);
+  }
+  return AVISM_OK;
+}
+
+int get_catalogue(avism_ctx* c, int32_t* indice, int32_t* uvoid, int32_t* ncellv, float* vf,
+                  avism_level_stats* st) {
+  int nv = c->nvoid;
+  if (nv > 0) {
+    if (indice) AV_CHECK_HIP(c, hipMemcpy(indice, c->indice.p, (size_t)nv * 4, hipMemcpyDeviceToHost));
+    if (uvoid) AV_CHECK_HIP(c, hipMemcpy(uvoid, c->uvoid.p, (size_t)nv * 4, hipMemcpyDeviceToHost));
+    if (ncellv) AV_CHECK_HIP(c, hipMemcpy(ncellv, c->ncellv.p, (size_t)nv * 4, hipMemcpyDeviceToHost));
+    if (vf) AV_CHECK_HIP(c, hipMemcpy(vf, c->vf.p, (size_t)12 * nv * 4, hipMemcpyDeviceToHost));
+  }
+  if (st) {
+    st->ncand = c->ncand;
+    st->ncubes = c->nvoid;
+    st->nvoid2 = c->nvoid2;
+    st->nvoidt = c->nvoidt;
+    st->pad = 0;
+    st->volt_clean = c->volt_clean;
+    st->meandens = c->meandens;
+    float l3 = (c->g.L0 * c->g.L0) * c->g.L0;
+    st->ff = (float)(c->volt_clean / (double)l3);
+  }
+  return AVISM_OK;
+}
+
+int get_marca(avism_ctx* c, int32_t* marca, int32_t inner_only, int32_t where) {
+  const GridGeom& g = c->g;
+  const int64_t ne3 = (int64_t)g.next * g.next * g.next;
+  if (!c->marca.p) AV_FAIL(c, AVISM_ERR_STATE, "no MARCA computed");
+  hipMemcpyKind kind = where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (!inner_only) {
+    AV_CHECK_HIP(c, hipMemcpy(marca, c->marca.p, (size_t)ne3 * 4, kind));
+    return AVISM_OK;
+  }
+  // inner box = ext rows starting at (1,1,1)
+  const int n = g.n, off = 1 - g.low1;
+  for (int k = 0; k < n; ++k)
+    AV_CHECK_HIP(c, hipMemcpy2D(marca + (size_t)k * n * n, (size_t)n * 4,
+                                c->marca.as<int32_t>() + ((size_t)off + (size_t)g.next * ((size_t)off + (size_t)g.next * (k + off))),
+                                (size_t)g.next * 4, (size_t)n * 4, (size_t)n, kind));
+  return AVISM_OK;
+}
+
+}  // namespace avism

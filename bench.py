@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark of the AVISM grid-and-void path on one MI355X.
+
+Workload: 512^3 Zel'dovich particles (P(k) ~ k^-1.5 exp(-k^2/0.25), rms
+displacement 3 Mpc/h, f = 0.5) in a periodic 1000 Mpc/h box, SPH with k = 32
+onto a 512^3 grid (extended to 640^3 by the periodic shell), thresholds of
+config/voids.dat.  One step = particles resident in HBM -> fields -> marking,
+growth, acceptance -> merge -> post-processing -> catalogue and MARCA in HBM.
+
+Both growth modes run on the same particles: the plain method and the
+learned-proposal method (3D CNN half-width proposals, exactly re-verified);
+their catalogues must be identical.  `value` is the learned-proposal path's
+cells/s (N^3 inner cells per second of wall time); the plain path's figure and
+the ratio are reported beside it.
+
+Prints one JSON line (see the keys below).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512, help="particles and cells per axis")
+    ap.add_argument("--box", type=float, default=1000.0)
+    ap.add_argument("--k", type=int, default=32)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=40, help="side of the CPU-baseline sample")
+    ap.add_argument("--mode", choices=["both", "plain", "learned"], default="both")
+    return ap.parse_args()
+
+
+def stage_table(tim):
+    out = {}
+    for name, ms in tim:
+        out[name] = out.get(name, 0.0) + ms
+    return out
+
+
+def run_step(ctx, parts, k, growth_mode, proposer=None):
+    from avism_amd._lib import AVISM_DEVICE
+    ctx.load_particles(parts.x, parts.y, parts.z, parts.vx, parts.vy, parts.vz, parts.m, where=AVISM_DEVICE)
+    ctx.particles_to_grid(k=k)
+    if growth_mode == 1:
+        proposer.propose(ctx)
+    nv = ctx.find_voids(growth_mode)
+    ctx.merge_and_post()
+    return nv
+
+
+def cpu_baseline(n_side: int, box_per_cell: float, k: int):
+    """The oracle (single-threaded C port of the reference path) on a bounded
+    sample of the same workload: n_side^3 particles and cells, same cell size."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    from avism_amd import synth
+    L0 = n_side * box_per_cell
+    p = synth.zeldovich(n_side, L0, seed=104)
+    arr = p.numpy()
+    t0 = time.perf_counter()
+    f = O.fields_from_particles(*arr, n_side, L0, 1, k=k)
+    rete = O.cosmo(0.678, 0.31, 0.0)[1]
+    O.void_stage(f["grid"], f["u1co"], f["div"], O.VoidParams(nvoid_max=4_000_000), meandens=f["meandens"],
+                 rete=rete)
+    dt = time.perf_counter() - t0
+    return {"value": n_side ** 3 / dt, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (C, -O2, 1 thread) full level on {n_side}^3 Zel'dovich particles / "
+                      f"{n_side}^3 cells, cell size {box_per_cell:.4g} Mpc/h, k={k}: {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU layout is "
+              "described in DESIGN.md and is not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from avism_amd import synth
+    from avism_amd._lib import Context
+    from avism_amd import learned
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    t_gen = time.perf_counter()
+    parts = synth.zeldovich(n, args.box, seed=args.seed, device=dev)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+    ctx = Context(0)
+    ctx.set_grid(n, args.box, 1)
+    ctx.set_cosmology(0.678, 0.31, 0.0)
+    ctx.set_void_params(-0.6, 10.0, 0.25, 0.0, 1.0, 1.0, 30_000_000)
+    gi = ctx.grid()
+    proposer = learned.Proposer.load_default(dev) if args.mode != "plain" else None
+
+    results = {}
+    catalogues = {}
+    modes = {"both": [0, 1], "plain": [0], "learned": [1]}[args.mode]
+    for mode in modes:
+        if mode == 1 and proposer is None:
+            continue
+        for _ in range(args.warmup):
+            run_step(ctx, parts, args.k, mode, proposer)
+        torch.cuda.synchronize()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            nv = run_step(ctx, parts, args.k, mode, proposer)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        tim = stage_table(ctx.timings())
+        ctx.set_timing(False)
+        st = ctx.stats()
+        cat = ctx.catalogue(nv)
+        catalogues[mode] = (ctx.marca(), cat["uvoid"].copy(), cat["indice"].copy())
+        results[mode] = dict(ms=dt * 1e3, stages={k: v / args.steps for k, v in tim.items()},
+                             ncubes=int(st.ncubes), nvoids=int(st.nvoidt), ncand=int(st.ncand))
+    identical = None
+    if 0 in catalogues and 1 in catalogues:
+        import numpy as np
+        identical = all(np.array_equal(a, b) for a, b in zip(catalogues[0], catalogues[1]))
+
+    main_mode = 1 if 1 in results else 0
+    r = results[main_mode]
+    cells = n ** 3
+    value = cells / (r["ms"] / 1e3)
+
+    # roofline of the dominant kernel under the algorithmic byte model
+    k = args.k
+    byte_model = {
+        # gather model: k neighbour records (xyz, m, rho, velocity, HPART) + 3 outputs
+        "vvel_sph_vw": cells * (k * 32 + 12 + k * 8),
+        "ppart_dens": n ** 3 * (k * 16 + 4),
+        "ddens_gather": cells * 4,  # compulsory only (data-dependent gather)
+        "mark_flags": gi.next ** 3 * 9 + cells * 8,
+        "mean_div": cells * 24,
+    }
+    stages = r["stages"]
+    dom = max((s for s in stages if s in byte_model), key=lambda s: stages[s])
+    achieved = byte_model[dom] / (stages[dom] / 1e3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_model": byte_model[dom], "kernel_ms": round(stages[dom], 3)}
+    out = {
+        "metric": "cells/s end-to-end at 512^3: learned-proposal vs plain GPU, identical voids",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Zel'dovich particles generated on the GPU)",
+        "config": {"workload": f"zeldovich {n}^3 particles, {n}^3 grid (ext {gi.next}^3), periodic "
+                               f"{args.box:g} Mpc/h, SPH k={k}", "growth": "learned" if main_mode else "plain"},
+        "plain_ms_per_step": results.get(0, {}).get("ms"),
+        "learned_ms_per_step": results.get(1, {}).get("ms"),
+        "learned_over_plain": (results[0]["ms"] / results[1]["ms"]) if (0 in results and 1 in results) else None,
+        "identical_voids": identical,
+        "stages_ms": {m: {k2: round(v, 3) for k2, v in results[m]["stages"].items()} for m in results},
+        "ncand": r["ncand"], "ncubes": r["ncubes"], "nvoids": r["nvoids"],
+        "roofline": roofline,
+        "input_gen_s": round(t_gen, 2),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.box / n, k)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

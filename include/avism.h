@@ -1,0 +1,148 @@
+/*
+ * avism.h -- C ABI of libavism_hip.so, the MI355X (gfx950) implementation of
+ * AVISM's grid-and-void path (oscarmonllor99/AVISM).
+ *
+ * The reference has no plugin or FFI layer: the path is a chain of Fortran
+ * subroutines called from PROGRAM AVISM.  Each entry point below replaces one
+ * link of that chain; the comment names the reference code it stands for.
+ * A Fortran host binds these with ISO_C_BINDING (see host/avism_host.f90), a
+ * Python host with ctypes (avism_amd/_lib.py).
+ *
+ * Conventions
+ *  - All calls return 0 (AVISM_OK) or a negative AVISM_ERR_* code; the message
+ *    is available from avism_last_error().  Where the reference STOPs, the
+ *    library returns AVISM_ERR_INPUT / AVISM_ERR_LIMIT with the same message.
+ *  - Calls are synchronous and single-host-thread per context.  Pointer
+ *    arguments are host memory unless an argument named `where` says
+ *    AVISM_DEVICE (then they are device pointers on the context's GPU).
+ *  - 3-D arrays are column-major (x fastest).  "inner" arrays cover cells 1..N
+ *    of each axis; "ext" arrays cover the periodic-extended range LOW1..LOW2
+ *    (avism.f90:780-813), LOW1=1, LOW2=N without periodic boundaries.
+ *  - Cube and void ids are 1-based, as in the reference.
+ */
+#ifndef AVISM_H
+#define AVISM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AVISM_OK 0
+#define AVISM_ERR_ARG -1
+#define AVISM_ERR_HIP -2
+#define AVISM_ERR_INPUT -3
+#define AVISM_ERR_LIMIT -4
+#define AVISM_ERR_STATE -5
+#define AVISM_ERR_UNSUPPORTED -6
+
+#define AVISM_HOST 0
+#define AVISM_DEVICE 1
+
+/* growth_mode of avism_find_voids */
+#define AVISM_GROW_PLAIN 0     /* exact shell-by-shell growth of every candidate */
+#define AVISM_GROW_PROPOSAL 1  /* start from the learned half-width proposals */
+
+typedef struct avism_ctx avism_ctx;
+
+typedef struct {
+    int32_t n, periodic, nplus, low1, low2, next;
+    float L0, lado0plus, dx, rx1;
+} avism_grid_info;
+
+/* voids.dat void-finder block (config/voids.dat:20-33, avism.f90:201-213) */
+typedef struct {
+    float dens_thre, dens_thre2, grad_thre, div_thre, rmin, rmin_sub;
+    int32_t nvoid_max;
+} avism_void_params;
+
+typedef struct {
+    int64_t ncand;      /* NXYZ: candidate centres (voidfinding.f90:106-109) */
+    int32_t ncubes;     /* NVOID after VOIDFIND */
+    int32_t nvoid2;     /* voids after MERGE_VOID */
+    int32_t nvoidt;     /* voids in the catalogue (avism.f90:1973) */
+    int32_t pad;
+    double volt_clean;  /* total void volume (avism.f90:1928) */
+    float meandens;     /* MEANDENS in Msun/Mpc^3 (avism.f90:1409) */
+    float ff;           /* VOLT_CLEAN / L0^3 (catalogue header, avism.f90:2047) */
+} avism_level_stats;
+
+/* ---- context ---------------------------------------------------------- */
+int avism_create(avism_ctx** ctx, int32_t device);
+int avism_destroy(avism_ctx* ctx);
+const char* avism_last_error(avism_ctx* ctx);
+int avism_set_timing(avism_ctx* ctx, int32_t on);
+/* names and milliseconds of the stages timed since the last call */
+int avism_get_timings(avism_ctx* ctx, char* names, int32_t names_len, float* ms, int32_t* count);
+
+/* ---- set-up ------------------------------------------------------------ */
+/* MALLA (grid.f90:3-76) + the PBC extension (avism.f90:780-865) */
+int avism_set_grid(avism_ctx* ctx, int32_t n, float L0, int32_t periodic);
+int avism_get_grid(avism_ctx* ctx, avism_grid_info* out, float* rad_ext /* next or NULL */);
+/* cosmology of avism.f90:469-479, 599-606 (h, Omega_m, redshift) */
+int avism_set_cosmology(avism_ctx* ctx, float ache, float omega0, float zeta);
+int avism_set_void_params(avism_ctx* ctx, const avism_void_params* p);
+/* MARCAP of the previous level (avism.f90:2123-2161); NULL = -1 everywhere */
+int avism_set_parent_map(avism_ctx* ctx, const int32_t* marcap_inner, int32_t where);
+
+/* ---- field stage (avism.f90:1177-1257, 1371-1414) --------------------- */
+/* READ_BINARY_PART_1 data (reader.f90:410-465): positions Mpc/h, velocities
+ * km/s, masses Msun.  Units are converted as the reader does (m/UM, v/UV).
+ * Fails with AVISM_ERR_INPUT for particles outside [-L0/2, L0/2]
+ * (avism.f90:560-589). */
+int avism_load_particles(avism_ctx* ctx, int64_t np, const float* x, const float* y,
+                         const float* z, const float* vx, const float* vy, const float* vz,
+                         const float* m, int32_t where);
+/* PPART_DENS, VVEL_INTERP_SPH_VW / VEL_INTERP_TSC, FILL_H_ZEROS, H_DISTANCE,
+ * DDENS_INTERP_SPH / DENS_INTERP_TSC, mean density, DIVER_UNIFORM */
+int avism_particles_to_grid(avism_ctx* ctx, int32_t k, int32_t vel_interp, int32_t dens_interp);
+/* READ_BINARY_GRID (reader.f90:523-552): delta and velocities (km/s) on the
+ * inner grid; then DIVER_UNIFORM.  Used for grid input (FLAG_DATA=2). */
+int avism_load_grid_fields(avism_ctx* ctx, const float* delta, const float* vx, const float* vy,
+                           const float* vz, int32_t where);
+/* U1CO (1+delta) and DIVERCO already on the inner grid, as the reference holds
+ * them after avism.f90:1409 and 1633-1646 (the state MARK_ALL starts from);
+ * meandens in Msun/Mpc^3 is used for the void masses. */
+int avism_load_void_fields(avism_ctx* ctx, const float* u1co, const float* div, float meandens,
+                           int32_t where);
+/* inner fields: U1CO (1+delta), DIVERCO, and the velocity components; NULL skips */
+int avism_get_fields(avism_ctx* ctx, float* u1co, float* div, float* u, float* v, float* w,
+                     int32_t where);
+/* per-particle PART_DENS and HPART in the input order; NULL skips */
+int avism_get_particle_fields(avism_ctx* ctx, float* part_dens, float* hpart);
+
+/* ---- void stage --------------------------------------------------------- */
+/* MARK_ALL + VOIDFIND (voidfinding.f90:13-713): marking, flag stencil,
+ * candidate sort (INDEXX order), cube growth and acceptance. */
+int avism_find_voids(avism_ctx* ctx, int32_t growth_mode, int32_t* ncubes);
+/* learned half-width proposal per candidate (sorted candidate order, int8);
+ * used by AVISM_GROW_PROPOSAL.  n must equal the candidate count. */
+int avism_set_proposals(avism_ctx* ctx, const int8_t* r_hat, int64_t n, int32_t where);
+/* candidates in processing order: ext linear index, and half-width reached
+ * (-1 for centres skipped at the edge zone); NULL skips */
+int avism_get_candidates(avism_ctx* ctx, int64_t* lin, int32_t* r, int64_t cap, int64_t* n);
+/* cubes: ci[9*ncubes] = inicio x,y,z, final x,y,z, ic x,y,z (blocks of ncubes);
+ * cf[7*ncubes] = rini x,y,z, rfin x,y,z, vol; father[ncubes] */
+int avism_get_cubes(avism_ctx* ctx, int32_t* ci, float* cf, int32_t* father);
+/* INDEXX (num.f90:2-84) of a float array, reproducing the Numerical Recipes
+ * permutation including its tie order; indx is 1-based. */
+int avism_indexx(avism_ctx* ctx, const float* keys, int64_t n, int32_t* indx, int32_t where);
+
+/* ---- merge and post-processing (avism.f90:1727-1973) ------------------- */
+int avism_merge_and_post(avism_ctx* ctx, int32_t* nvoids);
+/* indice[ncubes] (descending volume), uvoid[ncubes], ncellv[ncubes],
+ * vf[12*ncubes] = gxc,gyc,gzc, xc,yc,zc, volnew, umean, mtot, eps, ip, req */
+int avism_get_catalogue(avism_ctx* ctx, int32_t* indice, int32_t* uvoid, int32_t* ncellv,
+                        float* vf, avism_level_stats* stats);
+/* MARCA on the ext grid, or on the inner grid (the map file, avism.f90:2036) */
+int avism_get_marca(avism_ctx* ctx, int32_t* marca, int32_t inner_only, int32_t where);
+
+/* ---- whole level ---------------------------------------------------------- */
+/* particles already loaded: fields, voids, merge and post in one call */
+int avism_run_level(avism_ctx* ctx, int32_t k, int32_t vel_interp, int32_t dens_interp,
+                    int32_t growth_mode, avism_level_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,153 @@
+"""GPU parity of the void stage: INDEXX, marking, growth, acceptance, merge and
+post-processing.  Given identical fields the catalogue must be bit-exact in
+cell membership (MARCA), cube extents, UVOID and ids; GXC (ball-order FP32
+rounding), EPS/IP (FP32 inertia sums) and UMEAN/MTOT (FP32 sums) carry
+tolerances, stated below.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = O.VoidParams(dens_thre=-0.6, dens_thre2=10.0, grad_thre=0.25, div_thre=0.0, rmin=1.0,
+                      rmin_sub=1.0, nvoid_max=2_000_000)
+VF_NAMES = ["gxc", "gyc", "gzc", "xc", "yc", "zc", "volnew", "umean", "mtot", "eps", "ip", "req"]
+VF_TOL = {"gxc": 1e-5, "gyc": 1e-5, "gzc": 1e-5, "eps": 1e-4, "ip": 1e-4, "umean": 1e-5, "mtot": 1e-5}
+
+
+def _ctx():
+    from avism_amd._lib import Context
+    return Context(0)
+
+
+def _setup(c, n, L0, per):
+    c.set_grid(n, L0, per)
+    c.set_cosmology(0.678, 0.31, 0.0)
+    c.set_void_params(PARAMS.dens_thre, PARAMS.dens_thre2, PARAMS.grad_thre, PARAMS.div_thre,
+                      PARAMS.rmin, PARAMS.rmin_sub, PARAMS.nvoid_max)
+
+
+def gpu_void_outputs(c, growth_mode=0):
+    nv = c.find_voids(growth_mode)
+    ci, cf, fa = c.cubes(nv)
+    c.merge_and_post()
+    cat = c.catalogue(nv)
+    out = dict(nvoid=nv, cubes_i=ci, cubes_f=cf, father=fa, indice=cat["indice"], uvoid=cat["uvoid"],
+               ncellv=cat["ncellv"], vf=cat["vf"], nvoid2=cat["stats"].nvoid2, nvoidt=cat["stats"].nvoidt,
+               volt_clean=cat["stats"].volt_clean, marca=c.marca())
+    return out
+
+
+def compare(gpu, ref_get, check_vf=True):
+    assert gpu["nvoid"] == int(ref_get("nvoid"))
+    if gpu["nvoid"]:
+        assert np.array_equal(gpu["cubes_i"], ref_get("cubes_i"))
+        assert np.array_equal(gpu["cubes_f"], ref_get("cubes_f"))
+        assert np.array_equal(gpu["indice"], ref_get("indice"))
+    assert gpu["nvoid2"] == int(ref_get("nvoid2"))
+    if gpu["nvoid"]:
+        assert np.array_equal(gpu["uvoid"], ref_get("uvoid"))
+    assert np.array_equal(gpu["marca"], ref_get("marca"))
+    assert gpu["nvoidt"] == int(ref_get("nvoidt"))
+    if gpu["nvoid"]:
+        assert np.array_equal(gpu["ncellv"], ref_get("ncellv"))
+    assert gpu["volt_clean"] == float(ref_get("volt_clean"))
+    if check_vf and gpu["nvoid"]:
+        rvf = ref_get("vf")
+        main = gpu["uvoid"] == -1
+        for i, nm in enumerate(VF_NAMES):
+            a = gpu["vf"][i][main].astype(np.float64)
+            b = rvf[i][main].astype(np.float64)
+            if nm in VF_TOL:
+                scale = np.maximum(np.abs(b), 1.0 if nm in ("gxc", "gyc", "gzc") else 1e-30)
+                assert np.max(np.abs(a - b) / scale, initial=0.0) <= VF_TOL[nm], nm
+            else:
+                assert np.array_equal(gpu["vf"][i][main], rvf[i][main]), nm
+
+
+@pytest.mark.parametrize("name", ["grid_p0_n24", "grid_p1_n24"])
+def test_grid_input_vs_reference(golden_dir, name):
+    g = dict(np.load(golden_dir / f"{name}.npz"))
+    n, L0, per = int(g["meta_n"]), float(g["meta_L0"]), int(g["meta_periodic"])
+    print(f"[voids fixture {name}]", flush=True)
+    c = _ctx()
+    _setup(c, n, L0, per)
+    c.load_grid_fields(g["in_delta"], g["in_vx"], g["in_vy"], g["in_vz"])
+    compare(gpu_void_outputs(c), lambda k: g[f"ref_{k}"])
+
+
+@pytest.mark.parametrize("name", ["part_p0", "part_p1"])
+def test_reference_particle_fields_vs_reference(golden_dir, name):
+    # the reference's own U1CO/DIVERCO fed to the GPU void stage
+    g = dict(np.load(golden_dir / f"{name}.npz"))
+    n, L0, per = int(g["meta_n"]), float(g["meta_L0"]), int(g["meta_periodic"])
+    print(f"[voids on reference fields {name}]", flush=True)
+    c = _ctx()
+    _setup(c, n, L0, per)
+    c.load_void_fields(g["ref_u1co"], g["ref_div"], float(g["ref_meandens"]))
+    compare(gpu_void_outputs(c), lambda k: g[f"ref_{k}"])
+
+
+@pytest.mark.parametrize("n,per,seed", [(40, 0, 201), (40, 1, 202), (64, 1, 203), (64, 0, 204)])
+def test_grid_input_vs_oracle(n, per, seed):
+    from avism_amd import synth
+    L0 = 2.0 * n
+    d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, seed))
+    f = O.fields_from_grid(d, vx, vy, vz, n, L0, per)
+    ref = O.void_stage(f["grid"], f["u1co"], f["div"], PARAMS, meandens=f["meandens"],
+                       want_candidates=True)
+    print(f"[voids oracle n={n} per={per}] nvoid={ref.nvoid} nvoidt={ref.nvoidt}", flush=True)
+    c = _ctx()
+    _setup(c, n, L0, per)
+    c.load_grid_fields(d, vx, vy, vz)
+    out = gpu_void_outputs(c)
+    lin, r = c.candidates()
+    assert np.array_equal(lin, ref.cand_lin)
+    grown = ref.cand_r >= 0
+    assert np.array_equal(r[grown], ref.cand_r[grown])
+    refd = dict(nvoid=ref.nvoid, cubes_i=ref.cubes_i, cubes_f=ref.cubes_f, indice=ref.indice,
+                nvoid2=ref.nvoid2, uvoid=ref.uvoid, marca=ref.marca, nvoidt=ref.nvoidt, ncellv=ref.ncellv,
+                volt_clean=ref.volt_clean, vf=ref.vf)
+    compare(out, lambda k: refd[k])
+
+
+def test_proposals_give_identical_catalogue():
+    from avism_amd import synth
+    n, L0, per = 48, 96.0, 1
+    d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, 211))
+    c = _ctx()
+    _setup(c, n, L0, per)
+    c.load_grid_fields(d, vx, vy, vz)
+    plain = gpu_void_outputs(c, 0)
+    lin, r = c.candidates()
+    rng = np.random.default_rng(0)
+    for kind in ("exact", "random", "zero", "high"):
+        if kind == "exact":
+            prop = np.maximum(r, 0)
+        elif kind == "random":
+            prop = rng.integers(0, 6, size=r.size)
+        elif kind == "zero":
+            prop = np.zeros(r.size)
+        else:
+            prop = np.full(r.size, 9)
+        c.set_proposals(prop.astype(np.int8))
+        learned = gpu_void_outputs(c, 1)
+        compare(learned, lambda k: plain[k])
+
+
+def test_deterministic_catalogue():
+    from avism_amd import synth
+    n, L0 = 48, 96.0
+    d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, 212))
+    outs = []
+    for _ in range(2):
+        c = _ctx()
+        _setup(c, n, L0, 1)
+        c.load_grid_fields(d, vx, vy, vz)
+        outs.append(gpu_void_outputs(c))
+    for k in ("marca", "uvoid", "indice", "cubes_i", "vf"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
