@@ -16,7 +16,8 @@ LIB_PATH = HERE / "libavism_hip.so"
 
 AVISM_OK = 0
 AVISM_HOST, AVISM_DEVICE = 0, 1
-GROW_PLAIN, GROW_PROPOSAL = 0, 1
+GROW_PLAIN, GROW_PROPOSAL, GROW_LEARNED = 0, 1, 2
+CNN_NPARAM, CNN_NCLASS, CNN_CACHE, CNN_PATCH = 19064, 8, 7880, 2187
 
 ERRORS = {-1: "AVISM_ERR_ARG", -2: "AVISM_ERR_HIP", -3: "AVISM_ERR_INPUT", -4: "AVISM_ERR_LIMIT",
           -5: "AVISM_ERR_STATE", -6: "AVISM_ERR_UNSUPPORTED"}
@@ -53,6 +54,7 @@ EXPORTS = [
     "avism_load_grid_fields", "avism_load_void_fields", "avism_get_fields", "avism_get_particle_fields", "avism_find_voids",
     "avism_set_proposals", "avism_get_candidates", "avism_get_cubes", "avism_indexx",
     "avism_merge_and_post", "avism_get_catalogue", "avism_get_marca", "avism_run_level",
+    "avism_cnn_set_params", "avism_cnn_patches", "avism_cnn_forward", "avism_cnn_backward", "avism_cnn_adam",
 ]
 
 _lib = None
@@ -93,6 +95,11 @@ def lib():
         "avism_get_catalogue": ([vp, vp, vp, vp, vp, C.POINTER(LevelStats)], C.c_int),
         "avism_get_marca": ([vp, vp, i32, i32], C.c_int),
         "avism_run_level": ([vp, i32, i32, i32, i32, C.POINTER(LevelStats)], C.c_int),
+        "avism_cnn_set_params": ([vp, vp, i32], C.c_int),
+        "avism_cnn_patches": ([vp, vp, i64, vp], C.c_int),
+        "avism_cnn_forward": ([vp, vp, i64, vp, vp, vp], C.c_int),
+        "avism_cnn_backward": ([vp, vp, i64, vp, vp, vp, vp, vp, vp], C.c_int),
+        "avism_cnn_adam": ([vp, vp, vp, vp, vp, f32, i32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -273,3 +280,73 @@ class Context:
         self._chk(lib().avism_run_level(self._h, int(k), int(vel_interp), int(dens_interp),
                                         int(growth_mode), C.byref(st)))
         return st
+
+    # ---- half-width proposal network (device tensors; shapes checked here
+    # before any launch)
+    @staticmethod
+    def _dev(t, dtype, shape, name):
+        import torch
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise ValueError(f"{name} must be a device tensor")
+        if t.dtype != dtype or tuple(t.shape) != tuple(shape) or not t.is_contiguous():
+            raise ValueError(f"{name}: expected contiguous {dtype} {tuple(shape)}, got {t.dtype} "
+                             f"{tuple(t.shape)}")
+        return C.c_void_p(t.data_ptr())
+
+    def cnn_set_params(self, params):
+        """Network used by GROW_LEARNED (numpy fp32 or device tensor, CNN_NPARAM values)."""
+        if isinstance(params, np.ndarray):
+            a = np.ascontiguousarray(params, np.float32)
+            if a.size != CNN_NPARAM:
+                raise ValueError("wrong parameter count")
+            self._chk(lib().avism_cnn_set_params(self._h, _ptr(a), AVISM_HOST))
+        else:
+            import torch
+            p = self._dev(params, torch.float32, (CNN_NPARAM,), "params")
+            self._chk(lib().avism_cnn_set_params(self._h, p, AVISM_DEVICE))
+
+    def cnn_patches(self, ranks):
+        """bf16 feature patches [n, 3, 9, 9, 9] (as int16) of candidates at processing ranks."""
+        import torch
+        ranks = ranks.to(torch.int64).contiguous()
+        n = int(ranks.numel())
+        nc = C.c_int64()
+        self._chk(lib().avism_get_candidates(self._h, None, None, 0, C.byref(nc)))
+        if n and (int(ranks.min()) < 0 or int(ranks.max()) >= nc.value):
+            raise ValueError("candidate rank out of range")
+        out = torch.empty((n, 3, 9, 9, 9), dtype=torch.int16, device=ranks.device)
+        if n:
+            self._chk(lib().avism_cnn_patches(self._h, self._dev(ranks, torch.int64, (n,), "ranks"), n,
+                                              C.c_void_p(out.data_ptr())))
+        return out
+
+    def cnn_forward(self, params, patches, want_cache=True):
+        import torch
+        n = int(patches.shape[0])
+        p = self._dev(params, torch.float32, (CNN_NPARAM,), "params")
+        x = self._dev(patches, torch.int16, (n, 3, 9, 9, 9), "patches")
+        logits = torch.empty((n, CNN_NCLASS), dtype=torch.float32, device=patches.device)
+        cache = torch.empty((n, CNN_CACHE), dtype=torch.int16, device=patches.device) if want_cache else None
+        self._chk(lib().avism_cnn_forward(self._h, p, n, x, C.c_void_p(logits.data_ptr()),
+                                          C.c_void_p(cache.data_ptr()) if cache is not None else None))
+        return logits, cache
+
+    def cnn_backward(self, params, patches, cache, logits, labels):
+        import torch
+        n = int(patches.shape[0])
+        p = self._dev(params, torch.float32, (CNN_NPARAM,), "params")
+        x = self._dev(patches, torch.int16, (n, 3, 9, 9, 9), "patches")
+        ca = self._dev(cache, torch.int16, (n, CNN_CACHE), "cache")
+        lg = self._dev(logits, torch.float32, (n, CNN_NCLASS), "logits")
+        lb = self._dev(labels, torch.int32, (n,), "labels")
+        grads = torch.empty(CNN_NPARAM, dtype=torch.float32, device=patches.device)
+        loss = torch.empty(1, dtype=torch.float32, device=patches.device)
+        self._chk(lib().avism_cnn_backward(self._h, p, n, x, ca, lg, lb, C.c_void_p(grads.data_ptr()),
+                                           C.c_void_p(loss.data_ptr())))
+        return grads, loss
+
+    def cnn_adam(self, params, grads, m, v, lr, step):
+        import torch
+        ptrs = [self._dev(t, torch.float32, (CNN_NPARAM,), nm)
+                for t, nm in ((params, "params"), (grads, "grads"), (m, "m"), (v, "v"))]
+        self._chk(lib().avism_cnn_adam(self._h, *ptrs, float(lr), int(step)))

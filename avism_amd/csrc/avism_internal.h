@@ -102,7 +102,9 @@ struct avism_ctx {
   avism::DBuf cand_r, cand_state, cand_flag2;  // growth results per sorted candidate
   avism::DBuf cover_acc, cover_und;            // ext grids for acceptance
   avism::DBuf acc_rank, misc_i32;
-  avism::DBuf proposals;                       // learned half-width proposals (int8 per candidate)
+  avism::DBuf proposals;                       // half-width proposals (int8 per candidate)
+  avism::DBuf cnn_params;                      // proposal network parameters (fp32)
+  bool cnn_ready = false;
   int32_t nvoid = 0;                           // accepted cubes
   avism::DBuf cube_i, cube_f, cube_father;     // [9][nvoid], [7][nvoid], [nvoid]
   // merge and post
@@ -138,7 +140,13 @@ int void_find(avism_ctx* c, int32_t growth_mode);
 int merge_and_post(avism_ctx* c, int32_t stop_after);
 int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx);
 void add_timing(avism_ctx* c, const char* name, hipEvent_t a, hipEvent_t b);
-int cnn_predict(avism_ctx* c, const void* weights, int32_t* d_out_r, int64_t ncand);
+int cnn_patches(avism_ctx* c, const int64_t* ranks, int64_t n, uint16_t* out);
+int cnn_forward(avism_ctx* c, const float* params, int64_t n, const uint16_t* patches, float* logits,
+                uint16_t* cache);
+int cnn_backward(avism_ctx* c, const float* params, int64_t n, const uint16_t* patches, const uint16_t* cache,
+                 const float* logits, const int32_t* labels, float* grads, float* loss);
+int cnn_adam(avism_ctx* c, float* params, const float* grads, float* m, float* v, float lr, int32_t step);
+int cnn_propose(avism_ctx* c);
 
 // scoped stage timer on the context stream
 struct StageTimer {

@@ -92,6 +92,10 @@ int avism_set_grid(avism_ctx* c, int32_t n, float L0, int32_t periodic) {
   next = c->g.next;
   if (next * next * next >= (int64_t)0x7fffffff)
     AV_FAIL(c, AVISM_ERR_LIMIT, "extended grid above 2^31 cells is not supported by this build");
+  // small device scalars: counters and sums in the first 4 KiB, the cell-centre
+  // table RADX from byte 4096 on (read by the field and shape kernels)
+  if (!c->scalars.ensure(4096 + c->g.rad.size() * 4))
+    AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (scalars)");
   c->grid_set = true;
   c->fields_ready = false;
   c->np = 0;
@@ -272,6 +276,46 @@ int avism_get_catalogue(avism_ctx* c, int32_t* indice, int32_t* uvoid, int32_t* 
 int avism_get_marca(avism_ctx* c, int32_t* marca, int32_t inner_only, int32_t where) {
   CTX_GUARD(c);
   return get_marca(c, marca, inner_only, where);
+}
+
+int avism_cnn_set_params(avism_ctx* c, const float* params, int32_t where) {
+  CTX_GUARD(c);
+  if (!params) AV_FAIL(c, AVISM_ERR_ARG, "null parameter array");
+  const size_t b = (size_t)AVISM_CNN_NPARAM * 4;
+  if (!c->cnn_params.ensure(b)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+  AV_CHECK_HIP(c, hipMemcpy(c->cnn_params.p, params, b,
+                            where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  c->cnn_ready = true;
+  return AVISM_OK;
+}
+
+int avism_cnn_patches(avism_ctx* c, const int64_t* ranks, int64_t n, uint16_t* patches) {
+  CTX_GUARD(c);
+  if (n < 0 || (n > 0 && (!ranks || !patches))) AV_FAIL(c, AVISM_ERR_ARG, "bad patch request");
+  return cnn_patches(c, ranks, n, patches);
+}
+
+int avism_cnn_forward(avism_ctx* c, const float* params, int64_t n, const uint16_t* patches, float* logits,
+                      uint16_t* cache) {
+  CTX_GUARD(c);
+  if (n < 0 || !params || (n > 0 && (!patches || !logits))) AV_FAIL(c, AVISM_ERR_ARG, "bad forward request");
+  return cnn_forward(c, params, n, patches, logits, cache);
+}
+
+int avism_cnn_backward(avism_ctx* c, const float* params, int64_t n, const uint16_t* patches,
+                       const uint16_t* cache, const float* logits, const int32_t* labels, float* grads,
+                       float* loss) {
+  CTX_GUARD(c);
+  if (n < 0 || !params || !grads || !loss || (n > 0 && (!patches || !cache || !logits || !labels)))
+    AV_FAIL(c, AVISM_ERR_ARG, "bad backward request");
+  return cnn_backward(c, params, n, patches, cache, logits, labels, grads, loss);
+}
+
+int avism_cnn_adam(avism_ctx* c, float* params, const float* grads, float* m, float* v, float lr,
+                   int32_t step) {
+  CTX_GUARD(c);
+  if (!params || !grads || !m || !v || step < 1) AV_FAIL(c, AVISM_ERR_ARG, "bad Adam request");
+  return cnn_adam(c, params, grads, m, v, lr, step);
 }
 
 int avism_run_level(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp,

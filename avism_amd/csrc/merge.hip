@@ -225,7 +225,7 @@ __global__ void k_events(int nv, const int32_t* __restrict__ indice, const int32
   if (g == r) {  // major: no event; sorts last
     k1[r] = ~0ull;
     k2[r] = 0;
-    val[r] = id - 1;
+    val[r] = r;
     return;
   }
   int js = jstar[r];
@@ -245,7 +245,7 @@ __global__ void k_events(int nv, const int32_t* __restrict__ indice, const int32
   }
   k1[r] = ((uint64_t)g << 40) | step;
   k2[r] = ((uint64_t)dbits << 32) | (uint32_t)(id - 1);
-  val[r] = id - 1;
+  val[r] = r;
 }
 
 __global__ void k_group_ranges(int nv, const uint64_t* __restrict__ k1, int32_t* gstart, int32_t* gend) {
@@ -271,7 +271,7 @@ __global__ void k_fold(int nv, const int32_t* __restrict__ indice, const int32_t
   float X = cx[v], Y = cy[v], Z = cz[v], V = cvol[v];
   int s = gstart[r], e = gend[r];
   for (int i = s; i < e; ++i) {
-    int a = ev[i];
+    int a = indice[ev[i]] - 1;
     float va = cvol[a];
     float den = V + va;
     X = (X * V + cx[a] * va) / den;

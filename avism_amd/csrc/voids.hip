@@ -261,6 +261,11 @@ __global__ void k_grow_lane(int64_t ncand, ExtGeom G, const int32_t* __restrict_
     // blocked shell found is exact because every lower shell was checked
     int rh = prop[t];
     if (rh < 0) rh = 0;
+    if (rh >= smax) {  // predicted large: the wave kernel grows it from shell 0
+      status[t] = ST_DEFER;
+      rr[t] = 0;
+      return;
+    }
     if (rh > szone - 1) rh = szone - 1;
     int first = -1;
     for (int s = 0; s <= rh && first < 0; ++s)
@@ -513,12 +518,18 @@ int void_find(avism_ctx* c, int32_t growth_mode) {
   int8_t* status = c->cand_state.as<int8_t>();
   int16_t* rr = c->cand_r.as<int16_t>();
   const int8_t* prop = nullptr;
-  if (growth_mode == AVISM_GROW_PROPOSAL) {
+  if (growth_mode == AVISM_GROW_LEARNED) {
+    rc = cnn_propose(c);
+    if (rc) return rc;
+    prop = c->proposals.as<int8_t>();
+  } else if (growth_mode == AVISM_GROW_PROPOSAL) {
     if (c->proposals.bytes < (size_t)nc) AV_FAIL(c, AVISM_ERR_STATE, "no proposals set for these candidates");
     prop = c->proposals.as<int8_t>();
+  } else if (growth_mode != AVISM_GROW_PLAIN) {
+    AV_FAIL(c, AVISM_ERR_ARG, "unknown growth mode");
   }
   {
-    StageTimer t(c, growth_mode == AVISM_GROW_PROPOSAL ? "grow_proposal" : "grow");
+    StageTimer t(c, prop ? "grow_proposal" : "grow");
     k_grow_lane<<<nbk(nc), 256, 0, c->stream>>>(nc, G, order, c->cand_lin.as<int32_t>(),
                                                 c->flags.as<uint8_t>(), marcap, prop, 4, status, rr);
     k_grow_wave<<<nbk(nc, 4), 256, 0, c->stream>>>(nc, G, order, c->cand_lin.as<int32_t>(),

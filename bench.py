@@ -56,8 +56,6 @@ def run_step(ctx, parts, k, growth_mode, proposer=None):
     from avism_amd._lib import AVISM_DEVICE
     ctx.load_particles(parts.x, parts.y, parts.z, parts.vx, parts.vy, parts.vz, parts.m, where=AVISM_DEVICE)
     ctx.particles_to_grid(k=k)
-    if growth_mode == 1:
-        proposer.propose(ctx)
     nv = ctx.find_voids(growth_mode)
     ctx.merge_and_post()
     return nv
@@ -106,12 +104,15 @@ def main():
     ctx.set_void_params(-0.6, 10.0, 0.25, 0.0, 1.0, 1.0, 30_000_000)
     gi = ctx.grid()
     proposer = learned.Proposer.load_default(dev) if args.mode != "plain" else None
+    if proposer is not None:
+        proposer.install(ctx)
 
     results = {}
     catalogues = {}
-    modes = {"both": [0, 1], "plain": [0], "learned": [1]}[args.mode]
+    GL = 2  # GROW_LEARNED
+    modes = {"both": [0, GL], "plain": [0], "learned": [GL]}[args.mode]
     for mode in modes:
-        if mode == 1 and proposer is None:
+        if mode == GL and proposer is None:
             continue
         for _ in range(args.warmup):
             run_step(ctx, parts, args.k, mode, proposer)
@@ -130,11 +131,11 @@ def main():
         results[mode] = dict(ms=dt * 1e3, stages={k: v / args.steps for k, v in tim.items()},
                              ncubes=int(st.ncubes), nvoids=int(st.nvoidt), ncand=int(st.ncand))
     identical = None
-    if 0 in catalogues and 1 in catalogues:
+    if 0 in catalogues and GL in catalogues:
         import numpy as np
-        identical = all(np.array_equal(a, b) for a, b in zip(catalogues[0], catalogues[1]))
+        identical = all(np.array_equal(a, b) for a, b in zip(catalogues[0], catalogues[GL]))
 
-    main_mode = 1 if 1 in results else 0
+    main_mode = GL if GL in results else 0
     r = results[main_mode]
     cells = n ** 3
     value = cells / (r["ms"] / 1e3)
@@ -171,8 +172,8 @@ def main():
         "config": {"workload": f"zeldovich {n}^3 particles, {n}^3 grid (ext {gi.next}^3), periodic "
                                f"{args.box:g} Mpc/h, SPH k={k}", "growth": "learned" if main_mode else "plain"},
         "plain_ms_per_step": results.get(0, {}).get("ms"),
-        "learned_ms_per_step": results.get(1, {}).get("ms"),
-        "learned_over_plain": (results[0]["ms"] / results[1]["ms"]) if (0 in results and 1 in results) else None,
+        "learned_ms_per_step": results.get(GL, {}).get("ms"),
+        "learned_over_plain": (results[0]["ms"] / results[GL]["ms"]) if (0 in results and GL in results) else None,
         "identical_voids": identical,
         "stages_ms": {m: {k2: round(v, 3) for k2, v in results[m]["stages"].items()} for m in results},
         "ncand": r["ncand"], "ncubes": r["ncubes"], "nvoids": r["nvoids"],

@@ -5,7 +5,7 @@
  * The reference has no plugin or FFI layer: the path is a chain of Fortran
  * subroutines called from PROGRAM AVISM.  Each entry point below replaces one
  * link of that chain; the comment names the reference code it stands for.
- * A Fortran host binds these with ISO_C_BINDING (see host/avism_host.f90), a
+ * A Fortran host binds these with ISO_C_BINDING (host/avism_host.f90), a
  * Python host with ctypes (avism_amd/_lib.py).
  *
  * Conventions
@@ -41,7 +41,14 @@ extern "C" {
 
 /* growth_mode of avism_find_voids */
 #define AVISM_GROW_PLAIN 0     /* exact shell-by-shell growth of every candidate */
-#define AVISM_GROW_PROPOSAL 1  /* start from the learned half-width proposals */
+#define AVISM_GROW_PROPOSAL 1  /* start from proposals given by avism_set_proposals */
+#define AVISM_GROW_LEARNED 2   /* proposals from the network of avism_cnn_set_params */
+
+/* proposal network: parameter count, classes, bf16 activations cached per patch */
+#define AVISM_CNN_NPARAM 19064
+#define AVISM_CNN_NCLASS 8
+#define AVISM_CNN_CACHE 7880
+#define AVISM_CNN_PATCH 2187 /* 3 x 9^3 bf16 features */
 
 typedef struct avism_ctx avism_ctx;
 
@@ -136,6 +143,27 @@ int avism_get_catalogue(avism_ctx* ctx, int32_t* indice, int32_t* uvoid, int32_t
                         float* vf, avism_level_stats* stats);
 /* MARCA on the ext grid, or on the inner grid (the map file, avism.f90:2036) */
 int avism_get_marca(avism_ctx* ctx, int32_t* marca, int32_t inner_only, int32_t where);
+
+/* ---- half-width proposal network (no counterpart in the reference) ------
+ * A 3-D CNN proposes each candidate's half-width; AVISM_GROW_LEARNED growth
+ * re-verifies every proposal with the exact criteria of voidfinding.f90:500-654,
+ * so the catalogue equals AVISM_GROW_PLAIN's.  The training calls take device
+ * pointers: params[AVISM_CNN_NPARAM] fp32, patches[n][3][9][9][9] bf16 (as
+ * uint16), logits[n][8] fp32, cache[n][AVISM_CNN_CACHE] bf16, labels[n]. */
+/* network used by AVISM_GROW_LEARNED (copied into the context) */
+int avism_cnn_set_params(avism_ctx* ctx, const float* params, int32_t where);
+/* feature patches of the current candidates at processing ranks ranks[n] */
+int avism_cnn_patches(avism_ctx* ctx, const int64_t* ranks, int64_t n, uint16_t* patches);
+/* forward: logits, and the activations the backward needs (cache may be NULL) */
+int avism_cnn_forward(avism_ctx* ctx, const float* params, int64_t n, const uint16_t* patches,
+                      float* logits, uint16_t* cache);
+/* backward of the mean softmax cross-entropy: grads[AVISM_CNN_NPARAM], loss[1] */
+int avism_cnn_backward(avism_ctx* ctx, const float* params, int64_t n, const uint16_t* patches,
+                       const uint16_t* cache, const float* logits, const int32_t* labels,
+                       float* grads, float* loss);
+/* one Adam step (beta 0.9, 0.999, eps 1e-8) on params with moments m, v; step >= 1 */
+int avism_cnn_adam(avism_ctx* ctx, float* params, const float* grads, float* m, float* v, float lr,
+                   int32_t step);
 
 /* ---- whole level ---------------------------------------------------------- */
 /* particles already loaded: fields, voids, merge and post in one call */
