@@ -17,7 +17,7 @@ LIB_PATH = HERE / "libavism_hip.so"
 AVISM_OK = 0
 AVISM_HOST, AVISM_DEVICE = 0, 1
 GROW_PLAIN, GROW_PROPOSAL, GROW_LEARNED = 0, 1, 2
-CNN_NPARAM, CNN_NCLASS, CNN_CACHE, CNN_PATCH = 19064, 8, 7880, 2187
+CNN_NPARAM, CNN_NCLASS, CNN_CACHE, CNN_PATCH = 19064, 8, 7920, 2187
 
 ERRORS = {-1: "AVISM_ERR_ARG", -2: "AVISM_ERR_HIP", -3: "AVISM_ERR_INPUT", -4: "AVISM_ERR_LIMIT",
           -5: "AVISM_ERR_STATE", -6: "AVISM_ERR_UNSUPPORTED"}

@@ -76,7 +76,7 @@ struct avism_ctx {
   int64_t np = 0;
   avism::DBuf px, py, pz, pvx, pvy, pvz, pm;   // converted units after upload
   avism::DBuf sbin_key, sbin_val, sbin_key2, sbin_val2, sort_tmp;
-  avism::DBuf spos4, svel4, sorig;             // sorted by bin
+  avism::DBuf spos4, svel4, sorig, sinv;       // sorted by bin; sinv: input index -> sorted
   avism::DBuf bin_start;                       // nb^3 + 1
   avism::DBuf bin_reach;                       // nb^3, DDENS reach per bin
   int32_t nb = 0;
@@ -89,6 +89,8 @@ struct avism_ctx {
   avism::DBuf partial64;
   avism::DBuf marcap;                          // inner MARCAP (optional)
   bool has_marcap = false;
+  float mass_max = 0;                          // max particle mass (code units)
+  avism::DBuf mass_fix;                        // DDENS cell sums, int64 fixed point
   float meandens = 0;                          // MEANDENS in Msun/Mpc^3
   float meandens_raw = 0;                      // MEANDENS before unit scaling
   double meansum = 0;

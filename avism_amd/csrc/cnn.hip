@@ -39,6 +39,8 @@ constexpr int OFF_W1 = 0, OFF_B1 = 1728, OFF_W2 = 1744, OFF_B2 = 8656, OFF_W3 = 
               OFF_WH = 15600, OFF_BH = 19056;
 constexpr int NCLS = 8;
 constexpr int CACHE_PER = 343 * 16 + 125 * 16 + 27 * 16;  // bf16 activations per patch
+static_assert(CACHE_PER == AVISM_CNN_CACHE, "activation cache size differs from the header");
+static_assert(OFF_BH + NCLS == AVISM_CNN_NPARAM, "parameter count differs from the header");
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -292,12 +294,26 @@ __global__ void __launch_bounds__(64) k_forward(const float* __restrict__ P, int
 
 // backward of the mean softmax cross-entropy over n patches; weight and bias
 // gradients are accumulated into g (zeroed by the host) with fp32 atomics.
+// Activations and weights enter the MFMAs as the bf16 values the forward used;
+// the fp32 gradients dZ enter as a bf16 pair hi + lo (hi = bf16(dZ),
+// lo = bf16(dZ - hi)), two MFMAs per tile, so the weight gradients keep
+// about 16 bits of dZ instead of 8.
 //
-// LDS (bytes): X0 [0,5840) | A1, later DZ1 in place [5840,17104) |
-// DZ2 [17104,21200) | region R [21200,43152): A2, A3, DZ3, DF, DA2 while
-// layers 3 and 2 are differentiated, then DA1 over all of them.
-constexpr int R_BASE = 21200;
+// LDS (bytes): X0 [0,5840) | A1 [5840,17104) | DZ2 fp32 [17104,25296) |
+// region R [25296,47248): A2, A3, DZ3, DA2 while layers 3 and 2 are
+// differentiated, then DA1 (dA1, masked in place into dZ1) over all of them.
+constexpr int R_BASE = 25296;
 constexpr int BWD_LDS = R_BASE + 343 * 16 * 4;
+
+__device__ __forceinline__ void split4(const float* x, v4s& hi, v4s& lo) {
+  float h[4], l[4];
+  for (int v = 0; v < 4; ++v) {
+    h[v] = bfr(x[v]);
+    l[v] = x[v] - h[v];
+  }
+  hi = pack4(h[0], h[1], h[2], h[3]);
+  lo = pack4(l[0], l[1], l[2], l[3]);
+}
 
 __global__ void __launch_bounds__(64) k_backward(const float* __restrict__ P, int64_t n,
                                                  const uint16_t* __restrict__ patches,
@@ -307,13 +323,12 @@ __global__ void __launch_bounds__(64) k_backward(const float* __restrict__ P, in
                                                  float* loss) {
   __shared__ __attribute__((aligned(16))) unsigned char sm[BWD_LDS];
   uint16_t* X0 = (uint16_t*)sm;
-  uint16_t* A1 = (uint16_t*)(sm + 5840);  // DZ1 in place
-  uint16_t* DZ2 = (uint16_t*)(sm + 17104);
+  uint16_t* A1 = (uint16_t*)(sm + 5840);
+  float* DZ2 = (float*)(sm + 17104);
   uint16_t* A2 = (uint16_t*)(sm + R_BASE);
   uint16_t* A3 = (uint16_t*)(sm + R_BASE + 4096);
-  uint16_t* DZ3 = (uint16_t*)(sm + R_BASE + 5120);
-  float* DF = (float*)(sm + R_BASE + 6144);
-  float* DA2 = (float*)(sm + R_BASE + 8192);
+  float* DZ3 = (float*)(sm + R_BASE + 5120);
+  float* DA2 = (float*)(sm + R_BASE + 7168);
   float* DA1 = (float*)(sm + R_BASE);
   const int lane = threadIdx.x;
   const int nn = lane & 15, kb = 4 * (lane >> 4);
@@ -344,20 +359,15 @@ __global__ void __launch_bounds__(64) k_backward(const float* __restrict__ P, in
         atomicAdd(&g[OFF_WH + k * NCLS + j], f * dl[j]);
         df += dl[j] * bfr(P[OFF_WH + k * NCLS + j]);
       }
-      const float dz = f > 0.0f ? df : 0.0f;
-      DF[k] = dz;
-      DZ3[k] = f2bf(dz);
+      DZ3[k] = f > 0.0f ? df : 0.0f;
     }
-    for (int k = 432 + lane; k < 512; k += 64) {
-      DZ3[k] = 0;
-      DF[k] = 0.0f;
-    }
+    for (int k = 432 + lane; k < 512; k += 64) DZ3[k] = 0.0f;
     if (lane < NCLS) atomicAdd(&g[OFF_BH + lane], dl[lane]);
     for (int q = lane; q < 125 * 16; q += 64) DA2[q] = 0.0f;
     __syncthreads();
     if (lane < 16) {
       float s = 0.0f;
-      for (int q = 0; q < 27; ++q) s += DF[q * 16 + lane];
+      for (int q = 0; q < 27; ++q) s += DZ3[q * 16 + lane];
       atomicAdd(&g[OFF_B3 + lane], s);
     }
     // dW3[(off,c)][n] = sum_o A2[inpos(o,off)][c] dZ3[o][n]: M = c, N = n, K = o
@@ -368,19 +378,26 @@ __global__ void __launch_bounds__(64) k_backward(const float* __restrict__ P, in
         for (int v = 0; v < 4; ++v) {
           const int o = ks * 16 + kb + v;
           av[v] = o < 27 ? bf2f(A2[inpos<3>(o, off) * 16 + nn]) : 0.0f;
-          bv[v] = bf2f(DZ3[o * 16 + nn]);
+          bv[v] = DZ3[o * 16 + nn];
         }
-        acc = mfma(pack4(av[0], av[1], av[2], av[3]), pack4(bv[0], bv[1], bv[2], bv[3]), acc);
+        v4s bh, bl;
+        split4(bv, bh, bl);
+        const v4s a = pack4(av[0], av[1], av[2], av[3]);
+        acc = mfma(a, bh, acc);
+        acc = mfma(a, bl, acc);
       }
       for (int v = 0; v < 4; ++v) atomicAdd(&g[OFF_W3 + (off * 16 + kb + v) * 16 + nn], acc[v]);
     }
     // dA2 = col2im(dZ3 W3^T): per offset, D[o][c] = sum_n dZ3[o][n] W3[(off,c)][n]
     for (int mt = 0; mt < 2; ++mt) {
-      const v4s a = *(const v4s*)&DZ3[(mt * 16 + nn) * 16 + kb];
+      v4s ah, al;
+      split4(&DZ3[(mt * 16 + nn) * 16 + kb], ah, al);
       for (int off = 0; off < 27; ++off) {
         const float* w = P + OFF_W3 + (off * 16 + nn) * 16 + kb;
+        const v4s b = pack4(w[0], w[1], w[2], w[3]);
         v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = mfma(a, pack4(w[0], w[1], w[2], w[3]), acc);
+        acc = mfma(ah, b, acc);
+        acc = mfma(al, b, acc);
         for (int v = 0; v < 4; ++v) {
           const int o = mt * 16 + kb + v;
           if (o < 27) DA2[inpos<3>(o, off) * 16 + nn] += acc[v];
@@ -388,22 +405,16 @@ __global__ void __launch_bounds__(64) k_backward(const float* __restrict__ P, in
         __syncthreads();
       }
     }
-    // dz2 = dA2 * (a2 > 0): fp32 in DA2 (bias sum), bf16 in DZ2 (MFMA operand)
-    for (int q = lane; q < 128 * 16; q += 64) {
-      float dz = 0.0f;
-      if (q < 125 * 16) {
-        dz = bf2f(A2[q]) > 0.0f ? DA2[q] : 0.0f;
-        DA2[q] = dz;
-      }
-      DZ2[q] = f2bf(dz);
-    }
+    // dz2 = dA2 * (a2 > 0), rows 125..127 zero
+    for (int q = lane; q < 128 * 16; q += 64)
+      DZ2[q] = (q < 125 * 16 && bf2f(A2[q]) > 0.0f) ? DA2[q] : 0.0f;
     __syncthreads();
     if (lane < 16) {
       float s = 0.0f;
-      for (int q = 0; q < 125; ++q) s += DA2[q * 16 + lane];
+      for (int q = 0; q < 125; ++q) s += DZ2[q * 16 + lane];
       atomicAdd(&g[OFF_B2 + lane], s);
     }
-    // dW2: K = 125 outputs (8 steps; rows 125..127 of DZ2 are zero)
+    // dW2: K = 125 outputs (8 steps)
     for (int off = 0; off < 27; ++off) {
       v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
       for (int ks = 0; ks < 8; ++ks) {
@@ -411,9 +422,13 @@ __global__ void __launch_bounds__(64) k_backward(const float* __restrict__ P, in
         for (int v = 0; v < 4; ++v) {
           const int o = ks * 16 + kb + v;
           av[v] = o < 125 ? bf2f(A1[inpos<5>(o, off) * 16 + nn]) : 0.0f;
-          bv[v] = bf2f(DZ2[o * 16 + nn]);
+          bv[v] = DZ2[o * 16 + nn];
         }
-        acc = mfma(pack4(av[0], av[1], av[2], av[3]), pack4(bv[0], bv[1], bv[2], bv[3]), acc);
+        v4s bh, bl;
+        split4(bv, bh, bl);
+        const v4s a = pack4(av[0], av[1], av[2], av[3]);
+        acc = mfma(a, bh, acc);
+        acc = mfma(a, bl, acc);
       }
       for (int v = 0; v < 4; ++v) atomicAdd(&g[OFF_W2 + (off * 16 + kb + v) * 16 + nn], acc[v]);
     }
@@ -422,11 +437,14 @@ __global__ void __launch_bounds__(64) k_backward(const float* __restrict__ P, in
     __syncthreads();
     // dA1 = col2im(dZ2 W2^T)
     for (int mt = 0; mt < 8; ++mt) {
-      const v4s a = *(const v4s*)&DZ2[(mt * 16 + nn) * 16 + kb];
+      v4s ah, al;
+      split4(&DZ2[(mt * 16 + nn) * 16 + kb], ah, al);
       for (int off = 0; off < 27; ++off) {
         const float* w = P + OFF_W2 + (off * 16 + nn) * 16 + kb;
+        const v4s b = pack4(w[0], w[1], w[2], w[3]);
         v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = mfma(a, pack4(w[0], w[1], w[2], w[3]), acc);
+        acc = mfma(ah, b, acc);
+        acc = mfma(al, b, acc);
         for (int v = 0; v < 4; ++v) {
           const int o = mt * 16 + kb + v;
           if (o < 125) DA1[inpos<5>(o, off) * 16 + nn] += acc[v];
@@ -434,15 +452,9 @@ __global__ void __launch_bounds__(64) k_backward(const float* __restrict__ P, in
         __syncthreads();
       }
     }
-    // dz1 = dA1 * (a1 > 0); bf16 copy over A1 (each element read, then written, by one lane)
-    for (int q = lane; q < 352 * 16; q += 64) {
-      float dz = 0.0f;
-      if (q < 343 * 16) {
-        dz = bf2f(A1[q]) > 0.0f ? DA1[q] : 0.0f;
-        DA1[q] = dz;
-      }
-      A1[q] = f2bf(dz);
-    }
+    // dz1 = dA1 * (a1 > 0), in place
+    for (int q = lane; q < 343 * 16; q += 64)
+      if (!(bf2f(A1[q]) > 0.0f)) DA1[q] = 0.0f;
     __syncthreads();
     if (lane < 16) {
       float s = 0.0f;
@@ -450,7 +462,6 @@ __global__ void __launch_bounds__(64) k_backward(const float* __restrict__ P, in
       atomicAdd(&g[OFF_B1 + lane], s);
     }
     // dW1[k = off*4 + c][n]: M = k (7 tiles of 16, 108 used), K = 343 outputs (22 steps)
-    const uint16_t* DZ1 = A1;
     for (int mt = 0; mt < 7; ++mt) {
       v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
       const int k = mt * 16 + nn;
@@ -460,9 +471,13 @@ __global__ void __launch_bounds__(64) k_backward(const float* __restrict__ P, in
         for (int v = 0; v < 4; ++v) {
           const int o = ks * 16 + kb + v;
           av[v] = (o < 343 && k < 108) ? bf2f(X0[inpos<7>(o, off) * 4 + c]) : 0.0f;
-          bv[v] = bf2f(DZ1[o * 16 + nn]);
+          bv[v] = o < 343 ? DA1[o * 16 + nn] : 0.0f;
         }
-        acc = mfma(pack4(av[0], av[1], av[2], av[3]), pack4(bv[0], bv[1], bv[2], bv[3]), acc);
+        v4s bh, bl;
+        split4(bv, bh, bl);
+        const v4s a = pack4(av[0], av[1], av[2], av[3]);
+        acc = mfma(a, bh, acc);
+        acc = mfma(a, bl, acc);
       }
       for (int v = 0; v < 4; ++v) {
         const int kk = mt * 16 + kb + v;

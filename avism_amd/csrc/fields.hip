@@ -174,183 +174,229 @@ struct Bins {
   }
 };
 
-// k best (squared distance, input index) pairs in registers; K >= runtime k
-template <int K>
-struct TopK {
-  float d[K];
-  uint32_t id[K];   // sorted position
-  int cnt;
-  float wd;         // worst distance
-  uint32_t wid;     // its sorted position
-  int ws;           // its slot
+// ---------------------------------------------------------- wave kNN
+// One wave answers one query: the 64 lanes split each shell's bins and the
+// candidates inside them, and the k best are kept as a key sorted across the
+// lanes (lane t holds the t-th smallest).  A key packs the squared distance
+// (non-negative floats order as their bits) over the original index, so equal
+// distances go to the lower index: the deterministic form of kdtree.f90's
+// strict-< admission, and the tie rule of the oracle.
+__device__ __forceinline__ uint64_t knn_key(float d2, uint32_t orig) {
+  return ((uint64_t)__float_as_uint(d2) << 32) | orig;
+}
+__device__ __forceinline__ float key_d2(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+constexpr uint64_t kNoKey = ~0ull;
 
-  __device__ __forceinline__ void init() {
-    cnt = 0;
-    wd = INFINITY;
-    wid = 0;
-    ws = 0;
-#pragma clang loop unroll(full)
-    for (int t = 0; t < K; ++t) { d[t] = INFINITY; id[t] = 0; }
+__device__ __forceinline__ uint64_t rdlane64(uint64_t v, int lane) {  // lane: wave-uniform
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+  uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m), hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) {
+  uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d), hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double wave_sum(double v) {  // fixed tree: deterministic
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// the k smallest keys of everything offered, sorted across the lanes
+struct WaveSel {
+  uint64_t best;
+  int k, lane;
+  __device__ __forceinline__ void init(int k_, int lane_) {
+    best = kNoKey;
+    k = k_;
+    lane = lane_;
   }
-  __device__ __forceinline__ void recompute(int k, const uint32_t* orig) {
-    wd = -1.0f;
-    ws = 0;
-    wid = 0;
-#pragma clang loop unroll(full)
-    for (int t = 0; t < K; ++t) {
-      if (t < k) {
-        bool take = d[t] > wd;
-        if (!take && d[t] == wd) take = orig[id[t]] > orig[wid];
-        wd = take ? d[t] : wd;
-        ws = take ? t : ws;
-        wid = take ? id[t] : wid;
+  __device__ __forceinline__ uint64_t kth() const { return rdlane64(best, k - 1); }
+  // every lane offers one key (kNoKey for none); called by the whole wave
+  __device__ __forceinline__ void offer(uint64_t key) {
+    const uint64_t kt = kth();
+    const bool in = key < kt;
+    const uint64_t m = __ballot(in);
+    if (m == 0) return;
+    if (__popcll(m) <= 4) {  // few improvements: sorted inserts
+      uint64_t mm = m;
+      while (mm) {
+        const int src = __ffsll((unsigned long long)mm) - 1;
+        mm &= mm - 1;
+        const uint64_t x = rdlane64(key, src);
+        const int pos = __popcll(__ballot(best < x));
+        const uint64_t up = shfl_up64(best, 1);
+        best = lane < pos ? best : (lane == pos ? x : up);
       }
-    }
-  }
-  __device__ __forceinline__ void consider(float v, uint32_t j, int k, const uint32_t* orig) {
-    if (cnt < k) {
-#pragma clang loop unroll(full)
-      for (int t = 0; t < K; ++t) {
-        // value selects (not conditional stores) keep the arrays in VGPRs
-        d[t] = (t == cnt) ? v : d[t];
-        id[t] = (t == cnt) ? j : id[t];
-      }
-      ++cnt;
-      if (cnt == k) recompute(k, orig);
       return;
     }
-    if (v > wd) return;
-    if (v == wd && orig[j] >= orig[wid]) return;
-#pragma clang loop unroll(full)
-    for (int t = 0; t < K; ++t) {
-      d[t] = (t == ws) ? v : d[t];
-      id[t] = (t == ws) ? j : id[t];
+    // many: bitonic sort of the offers (descending), keep the lane-wise
+    // minimum with the ascending best (a bitonic sequence holding the 64
+    // smallest of both), then a bitonic merge back to ascending
+    uint64_t a = in ? key : kNoKey;
+    for (int size = 2; size <= 64; size <<= 1)
+      for (int st = size >> 1; st > 0; st >>= 1) {
+        const uint64_t o = shfl_xor64(a, st);
+        const bool asc = (lane & size) != 0;
+        const bool lower = (lane & st) == 0;
+        a = (lower == asc) ? (a < o ? a : o) : (a < o ? o : a);
+      }
+    best = best < a ? best : a;
+    for (int st = 32; st > 0; st >>= 1) {
+      const uint64_t o = shfl_xor64(best, st);
+      best = (lane & st) == 0 ? (best < o ? best : o) : (best < o ? o : best);
     }
-    recompute(k, orig);
   }
 };
 
-// exact kNN of q into tk (k neighbours, or fewer if np < k)
-template <int K>
-__device__ __forceinline__ void knn_search(const Bins& B, float qx, float qy, float qz, int k, TopK<K>& tk) {
-  tk.init();
-  const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
-  const int nb = B.nb;
-  const int smax_p = (nb - 1) / 2;
-  for (int s = 0;; ++s) {
-    bool full_scan = B.periodic && s > smax_p;
-    if (full_scan) break;  // handled below
-    bool any = false;
-    for (int dz = -s; dz <= s; ++dz) {
-      int cz = bz + dz;
-      if (!B.periodic && (cz < 0 || cz >= nb)) continue;
-      float ddz = B.box_d(qz, cz);
-      int wz = B.periodic ? B.wrap(cz) : cz;
-      for (int dy = -s; dy <= s; ++dy) {
-        int cy = by + dy;
-        if (!B.periodic && (cy < 0 || cy >= nb)) continue;
-        float ddy = B.box_d(qy, cy);
-        int wy = B.periodic ? B.wrap(cy) : cy;
-        bool edge = (dz == -s || dz == s || dy == -s || dy == s);
-        int step = (edge || s == 0) ? 1 : 2 * s;
-        for (int dxo = -s; dxo <= s; dxo += step) {
-          int cx = bx + dxo;
-          if (!B.periodic && (cx < 0 || cx >= nb)) continue;
-          any = true;
-          float ddx = B.box_d(qx, cx);
-          float mind = ddx * ddx + ddy * ddy + ddz * ddz;
-          if (tk.cnt >= k && mind > tk.wd) continue;
-          int wx = B.periodic ? B.wrap(cx) : cx;
-          int bin = wx + nb * (wy + nb * wz);
-          int e = B.start[bin + 1];
-          for (int j = B.start[bin]; j < e; ++j) {
-            float4 p = B.pos[j];
-            float d = dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic);
-            tk.consider(d, (uint32_t)j, k, B.orig);
-          }
-          if (s == 0) break;
-        }
-      }
-    }
-    if (tk.cnt >= k) {
-      float b1 = B.outside(qx, bx, s), b2 = B.outside(qy, by, s), b3 = B.outside(qz, bz, s);
-      float bd = fminf(b1, fminf(b2, b3));
-      if (bd == INFINITY || (bd > 0.0f && tk.wd < bd * bd)) return;
-    }
-    if (!B.periodic && !any && s > 0) {
-      int far = max(max(max(bx, nb - 1 - bx), max(by, nb - 1 - by)), max(bz, nb - 1 - bz));
-      if (s > far) return;
-    }
-    if (B.periodic && s + 1 > smax_p) break;
+// Offers every particle of the bins given per lane (j0, cnt; cnt = 0 for
+// none) to f(valid, j) in batches of 64, one candidate per lane.
+template <class F>
+__device__ __forceinline__ void wave_candidates(int lane, int j0, int cnt, F&& f) {
+  int incl = cnt;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(incl, d);
+    if (lane >= d) incl += t;
   }
-  // periodic box smaller than the search: scan every bin once
-  tk.init();
-  int nbins = nb * nb * nb;
-  int e = B.start[nbins];
-  for (int j = 0; j < e; ++j) {
-    float4 p = B.pos[j];
-    float d = dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic);
-    tk.consider(d, (uint32_t)j, k, B.orig);
+  const int total = __builtin_amdgcn_readlane(incl, 63);
+  const int excl = incl - cnt;
+  for (int c0 = 0; c0 < total; c0 += 64) {
+    const int c = c0 + lane;
+    int src = 0;  // last lane whose range starts at or before c
+    for (int st = 32; st > 0; st >>= 1) {
+      const int e = __shfl(excl, src + st);
+      if (e <= c) src += st;
+    }
+    const int sj = __shfl(j0, src), se = __shfl(excl, src);
+    f(c < total, sj + (c - se));
   }
 }
 
-// visit every particle with d2 <= r2 (ball_search, kdtree.f90:1015-1072, 1126)
-template <class F>
-__device__ __forceinline__ void ball_visit(const Bins& B, float qx, float qy, float qz, float r, float r2, F&& f) {
+// exact kNN of q (k <= 64 and <= the number of particles), wave-uniform q.
+// Shells of bins around q's bin are searched until the k-th distance is below
+// the distance to everything outside them; a bin whose box is farther than
+// the current k-th distance is skipped.
+__device__ void wave_knn(const Bins& B, int64_t np, float qx, float qy, float qz, WaveSel& S) {
+  const int lane = S.lane;
   const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
   const int nb = B.nb;
-  int s = (int)ceilf((r + B.eps) / B.bsz) + 1;
-  if (B.periodic && 2 * s + 1 > nb) {
-    int e = B.start[nb * nb * nb];
-    for (int j = 0; j < e; ++j) {
-      float4 p = B.pos[j];
-      float d = dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic);
-      if (d <= r2) f(j, d);
+  const int smax_p = (nb - 1) / 2;
+  auto offer_j = [&](bool valid, int j) {
+    uint64_t key = kNoKey;
+    if (valid) {
+      const float4 p = B.pos[j];
+      key = knn_key(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), B.orig[j]);
     }
+    S.offer(key);
+  };
+  for (int s = 0; !(B.periodic && s > smax_p); ++s) {
+    const int side = 2 * s + 1, cube = side * side * side;
+    bool any = false;
+    for (int base = 0; base < cube; base += 64) {
+      const int f = base + lane;
+      const int dxo = f % side - s, dy = (f / side) % side - s, dz = f / (side * side) - s;
+      const int cx = bx + dxo, cy = by + dy, cz = bz + dz;
+      bool v = f < cube && max(abs(dxo), max(abs(dy), abs(dz))) == s;
+      if (!B.periodic) v = v && cx >= 0 && cx < nb && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
+      any = any || (__ballot(v) != 0);
+      const uint64_t kt = S.kth();
+      if (v && kt != kNoKey) {
+        const float ddx = B.box_d(qx, cx), ddy = B.box_d(qy, cy), ddz = B.box_d(qz, cz);
+        v = !((ddx * ddx + ddy * ddy) + ddz * ddz > key_d2(kt));
+      }
+      int j0 = 0, cnt = 0;
+      if (v) {
+        const int bin = (B.periodic ? B.wrap(cx) : cx) + nb * ((B.periodic ? B.wrap(cy) : cy) +
+                                                             nb * (B.periodic ? B.wrap(cz) : cz));
+        j0 = B.start[bin];
+        cnt = B.start[bin + 1] - j0;
+      }
+      wave_candidates(lane, j0, cnt, offer_j);
+    }
+    const uint64_t kt = S.kth();
+    if (kt != kNoKey) {
+      const float b1 = B.outside(qx, bx, s), b2 = B.outside(qy, by, s), b3 = B.outside(qz, bz, s);
+      const float bd = fminf(b1, fminf(b2, b3));
+      if (bd == INFINITY || (bd > 0.0f && key_d2(kt) < bd * bd)) return;
+    }
+    if (!B.periodic && !any && s > 0) {
+      const int far = max(max(max(bx, nb - 1 - bx), max(by, nb - 1 - by)), max(bz, nb - 1 - bz));
+      if (s > far) return;
+    }
+  }
+  // periodic box smaller than the search: every particle once
+  S.init(S.k, lane);
+  for (int64_t c0 = 0; c0 < np; c0 += 64) offer_j(c0 + lane < np, (int)(c0 + lane));
+}
+
+// every particle with d2 <= r2 (ball_search, kdtree.f90:1015-1072, 1126), to
+// f(valid, j, d2) in wave batches
+template <class F>
+__device__ void wave_ball(const Bins& B, int64_t np, int lane, float qx, float qy, float qz, float r, float r2,
+                          F&& f) {
+  const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
+  const int nb = B.nb;
+  const int R = (int)ceilf((r + B.eps) / B.bsz) + 1;
+  auto visit = [&](bool valid, int j) {
+    float d2 = 0.0f;
+    if (valid) {
+      const float4 p = B.pos[j];
+      d2 = dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic);
+    }
+    f(valid && d2 <= r2, j, d2);
+  };
+  if (B.periodic && 2 * R + 1 > nb) {
+    for (int64_t c0 = 0; c0 < np; c0 += 64) visit(c0 + lane < np, (int)(c0 + lane));
     return;
   }
-  for (int dz = -s; dz <= s; ++dz) {
-    int cz = bz + dz;
-    if (!B.periodic && (cz < 0 || cz >= nb)) continue;
-    float ddz = B.box_d(qz, cz);
-    if (ddz > r) continue;
-    int wz = B.periodic ? B.wrap(cz) : cz;
-    for (int dy = -s; dy <= s; ++dy) {
-      int cy = by + dy;
-      if (!B.periodic && (cy < 0 || cy >= nb)) continue;
-      float ddy = B.box_d(qy, cy);
-      if (ddy > r) continue;
-      int wy = B.periodic ? B.wrap(cy) : cy;
-      for (int dxo = -s; dxo <= s; ++dxo) {
-        int cx = bx + dxo;
-        if (!B.periodic && (cx < 0 || cx >= nb)) continue;
-        float ddx = B.box_d(qx, cx);
-        if (ddx * ddx + ddy * ddy + ddz * ddz > r2 * 1.0001f + 1e-6f) continue;
-        int wx = B.periodic ? B.wrap(cx) : cx;
-        int bin = wx + nb * (wy + nb * wz);
-        int e = B.start[bin + 1];
-        for (int j = B.start[bin]; j < e; ++j) {
-          float4 p = B.pos[j];
-          float d = dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic);
-          if (d <= r2) f(j, d);
-        }
-      }
+  const int side = 2 * R + 1, cube = side * side * side;
+  for (int base = 0; base < cube; base += 64) {
+    const int fi = base + lane;
+    const int cx = bx + fi % side - R, cy = by + (fi / side) % side - R, cz = bz + fi / (side * side) - R;
+    bool v = fi < cube;
+    if (!B.periodic) v = v && cx >= 0 && cx < nb && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
+    if (v) {
+      const float ddx = B.box_d(qx, cx), ddy = B.box_d(qy, cy), ddz = B.box_d(qz, cz);
+      v = !(ddx * ddx + ddy * ddy + ddz * ddz > r2 * 1.0001f + 1e-6f);
     }
+    int j0 = 0, cnt = 0;
+    if (v) {
+      const int bin = (B.periodic ? B.wrap(cx) : cx) + nb * ((B.periodic ? B.wrap(cy) : cy) +
+                                                           nb * (B.periodic ? B.wrap(cz) : cz));
+      j0 = B.start[bin];
+      cnt = B.start[bin + 1] - j0;
+    }
+    wave_candidates(lane, j0, cnt, visit);
   }
 }
 
 // ----------------------------------------------------------------- kernels
 __global__ void k_convert_check(int64_t np, float* vx, float* vy, float* vz, float* m,
                                 const float* x, const float* y, const float* z, float half,
-                                int* bad) {
+                                int* bad, unsigned* mmax) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  m[i] = m[i] / kUM;  // reader.f90:457-461
-  vx[i] = vx[i] / kUV;
-  vy[i] = vy[i] / kUV;
-  vz[i] = vz[i] / kUV;
-  float a = x[i], b = y[i], c = z[i];
-  if (a < -half || a > half || b < -half || b > half || c < -half || c > half) atomicOr(bad, 1);
+  unsigned mb = 0;
+  if (i < np) {
+    float mi = m[i] / kUM;  // reader.f90:457-461
+    m[i] = mi;
+    vx[i] = vx[i] / kUV;
+    vy[i] = vy[i] / kUV;
+    vz[i] = vz[i] / kUV;
+    float a = x[i], b = y[i], c = z[i];
+    if (a < -half || a > half || b < -half || b > half || c < -half || c > half) atomicOr(bad, 1);
+    mb = __float_as_uint(fabsf(mi));  // non-negative floats order as their bits
+  }
+  // one atomic per wave: a per-thread atomic on one address serialises
+  for (int o = 32; o > 0; o >>= 1) mb = max(mb, (unsigned)__shfl_xor((int)mb, o));
+  if ((threadIdx.x & 63) == 0 && mb) atomicMax(mmax, mb);
+}
+
+__global__ void k_fill_u64(int64_t n, unsigned long long* a, unsigned long long v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
 }
 
 __global__ void k_bin_keys(int64_t np, const float* x, const float* y, const float* z, Bins B,
@@ -365,105 +411,106 @@ __global__ void k_bin_keys(int64_t np, const float* x, const float* y, const flo
 
 __global__ void k_gather_sorted(int64_t np, const uint32_t* perm, const float* x, const float* y,
                                 const float* z, const float* m, const float* vx, const float* vy,
-                                const float* vz, float4* pos, float4* vel, uint32_t* orig) {
+                                const float* vz, float4* pos, float4* vel, uint32_t* orig, uint32_t* inv) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   uint32_t p = perm[i];
   pos[i] = make_float4(x[p], y[p], z[p], m[p]);
   vel[i] = make_float4(vx[p], vy[p], vz[p], 0.0f);
   orig[i] = p;
+  inv[p] = (uint32_t)i;
 }
 
-// PPART_DENS (particles.f90:742-802): one query per particle, in bin order.
-// hk = distance to the k-th neighbour, reused by FILL_H_ZEROS (same query).
-template <int K>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) k_ppart(Bins B, int64_t np, int k, int want_dens,
-                                               float* pdens, float* hk) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  float4 q = B.pos[i];
-  TopK<K> tk;
-  knn_search<K>(B, q.x, q.y, q.z, k, tk);
-  float h = sqrtf(tk.wd);
-  hk[i] = h;
-  if (!want_dens) return;
-  Kahan ms;
-#pragma clang loop unroll(full)
-  for (int t = 0; t < K; ++t)
-    if (t < tk.cnt) ms.add(B.pos[tk.id[t]].w);
-  float vol = 4.18879020478639098f * ((h * h) * h);  // (4/3) pi h^3
-  pdens[i] = ms.s / vol;
+// PPART_DENS (particles.f90:742-802): rho_p = (FP64 sum of the k nearest
+// masses, self included) / (4/3 pi d_k^3); d_k is kept for FILL_H_ZEROS.
+// One wave per particle, QPW consecutive (bin-sorted) particles per wave.
+constexpr int QPW = 4;
+
+__global__ void __launch_bounds__(256) k_ppart(Bins B, int64_t np, int k, const float* __restrict__ morig,
+                                              float* pdens, float* hk) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  for (int q = 0; q < QPW; ++q) {
+    const int64_t i = wave * QPW + q;
+    if (i >= np) return;
+    const float4 pq = B.pos[i];
+    WaveSel S;
+    S.init(k, lane);
+    wave_knn(B, np, pq.x, pq.y, pq.z, S);
+    const float dk = sqrtf(key_d2(S.kth()));  // kdtree.f90:539
+    const double m = lane < k ? (double)morig[(uint32_t)S.best] : 0.0;
+    const double bas8 = wave_sum(m);
+    if (lane == 0) {
+      hk[i] = dk;
+      const double vol = (4.0 / 3.0 * kPi) * (double)((dk * dk) * dk);
+      pdens[i] = (float)(bas8 / vol);  // particles.f90:790-794
+    }
+  }
 }
 
-// VVEL_INTERP_SPH_VW (particles.f90:806-930) and H_DISTANCE (276-351).
-// One cell per lane; a block covers an 8x8x4 tile of cells.
-template <int K>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) k_vvel(Bins B, int n, const float* __restrict__ rad,
-                                              int rlo, float dx, int k, int want_vel,
-                                              const float4* __restrict__ vel,
-                                              const float* __restrict__ pdens,
-                                              const int8_t* __restrict__ smask, uint32_t* hbits,
-                                              float* u, float* v, float* w) {
-  int ix = blockIdx.x * 8 + threadIdx.x % 8;
-  int iy = blockIdx.y * 8 + (threadIdx.x / 8) % 8;
-  int iz = blockIdx.z * 4 + threadIdx.x / 64;
-  if (ix >= n || iy >= n || iz >= n) return;
-  size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
-  if (smask && smask[c] == 0) return;
-  float qx = rad[ix + 1 - rlo], qy = rad[iy + 1 - rlo], qz = rad[iz + 1 - rlo];
-  TopK<K> tk;
-  knn_search<K>(B, qx, qy, qz, k, tk);
-  float dk = sqrtf(tk.wd);
-  if (dk > dx) {  // k neighbours (particles.f90:866-867)
-    float h = dk;
-    Kahan sw, sx, sy, sz;
-#pragma clang loop unroll(full)
-    for (int t = 0; t < K; ++t) {
-      if (t < tk.cnt) {
-        uint32_t j = tk.id[t];
-        float d = sqrtf(tk.d[t]);
-        atomicMax(&hbits[j], __float_as_uint(d));  // particles.f90:882-885
-        if (want_vel) {
-          float q = d / (h / 2.0f);
-          float wgt = m4(q) * (B.pos[j].w / pdens[j]);
-          float4 vv = vel[j];
-          sw.add(wgt);
-          sx.add(wgt * vv.x);
-          sy.add(wgt * vv.y);
-          sz.add(wgt * vv.z);
-        }
+// VVEL_INTERP_SPH_VW (particles.f90:806-930): per cell the k nearest (or,
+// where d_k <= DX, every particle within DX), HPART = max distance at which a
+// particle served a cell (atomic max on the float bits), and the FP64
+// weighted velocity average.  One wave per cell, QPW cells along x per wave.
+__global__ void __launch_bounds__(256) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad, int rlo,
+                                             float dx, int k, int want_vel, const float4* __restrict__ vel,
+                                             const float* __restrict__ pdens, const uint32_t* __restrict__ inv,
+                                             const int8_t* __restrict__ smask, uint32_t* hbits, float* u,
+                                             float* v, float* w) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int64_t ncell = (int64_t)n * n * n;
+  for (int q = 0; q < QPW; ++q) {
+    const int64_t c = wave * QPW + q;
+    if (c >= ncell) return;
+    if (smask && smask[c] == 0) continue;
+    const int ix = (int)(c % n), iy = (int)((c / n) % n), iz = (int)(c / ((int64_t)n * n));
+    const float qx = rad[ix + 1 - rlo], qy = rad[iy + 1 - rlo], qz = rad[iz + 1 - rlo];
+    WaveSel S;
+    S.init(k, lane);
+    wave_knn(B, np, qx, qy, qz, S);
+    const float dk = sqrtf(key_d2(S.kth()));
+    double sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    auto serve = [&](bool valid, uint32_t j, float d, float hh) {
+      if (!valid) return;
+      atomicMax(&hbits[j], __float_as_uint(d));  // particles.f90:882-885
+      if (want_vel) {
+        const float qq = d / (hh / 2.0f);                        // particles.f90:941
+        const float wgt = m4(qq) * (B.pos[j].w / pdens[j]);      // 897-900
+        const float4 vv = vel[j];
+        const double w8 = (double)wgt;
+        sw += w8;
+        sx += w8 * (double)vv.x;
+        sy += w8 * (double)vv.y;
+        sz += w8 * (double)vv.z;
+      }
+    };
+    if (dk > dx) {  // k neighbours (particles.f90:866-867)
+      const bool mine = lane < k;
+      serve(mine, mine ? inv[(uint32_t)S.best] : 0u, sqrtf(key_d2(S.best)), dk);
+    } else {        // ball of radius DX (particles.f90:868-878)
+      const float r2 = dx * dx;
+      float maxd2 = 0.0f;
+      wave_ball(B, np, lane, qx, qy, qz, dx, r2, [&](bool ok, int j, float d2) {
+        if (ok) maxd2 = fmaxf(maxd2, d2);
+      });
+      for (int o = 32; o > 0; o >>= 1) maxd2 = fmaxf(maxd2, __shfl_xor(maxd2, o));
+      const float hh = sqrtf(maxd2);
+      wave_ball(B, np, lane, qx, qy, qz, dx, r2, [&](bool ok, int j, float d2) {
+        serve(ok, (uint32_t)j, sqrtf(d2), hh);
+      });
+    }
+    if (want_vel) {
+      sw = wave_sum(sw);
+      sx = wave_sum(sx);
+      sy = wave_sum(sy);
+      sz = wave_sum(sz);
+      if (lane == 0) {
+        u[c] = (float)(sx / sw);
+        v[c] = (float)(sy / sw);
+        w[c] = (float)(sz / sw);
       }
     }
-    if (want_vel) {
-      u[c] = sx.s / sw.s;
-      v[c] = sy.s / sw.s;
-      w[c] = sz.s / sw.s;
-    }
-    return;
-  }
-  // ball of radius DX (particles.f90:868-878): every particle with d <= DX
-  float r2 = dx * dx;
-  float maxd2 = 0.0f;
-  ball_visit(B, qx, qy, qz, dx, r2, [&](int j, float d2) { maxd2 = fmaxf(maxd2, d2); });
-  float h = sqrtf(maxd2);
-  Kahan sw, sx, sy, sz;
-  ball_visit(B, qx, qy, qz, dx, r2, [&](int j, float d2) {
-    float d = sqrtf(d2);
-    atomicMax(&hbits[j], __float_as_uint(d));
-    if (want_vel) {
-      float q = d / (h / 2.0f);
-      float wgt = m4(q) * (B.pos[j].w / pdens[j]);
-      float4 vv = vel[j];
-      sw.add(wgt);
-      sx.add(wgt * vv.x);
-      sy.add(wgt * vv.y);
-      sz.add(wgt * vv.z);
-    }
-  });
-  if (want_vel) {
-    u[c] = sx.s / sw.s;
-    v[c] = sy.s / sw.s;
-    w[c] = sz.s / sw.s;
   }
 }
 
@@ -511,96 +558,94 @@ __device__ __forceinline__ float ddens_w(const DdensGeom& G, float x, float y, f
   return m4(q);
 }
 
-__global__ void k_ddens_norm(int64_t np, DdensGeom G, const float4* __restrict__ pos,
-                             const float* __restrict__ hpart, float* norm, int32_t* bin_reach,
-                             Bins B, int32_t* rmax) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  float4 p = pos[i];
-  int ix, iy, iz, inc, reach;
-  float inv_h, h2max;
-  ddens_setup(G, p.x, p.y, p.z, hpart[i], ix, iy, iz, inv_h, h2max, inc, reach);
-  float nrm = 0.0f;
-  for (int k2 = iz - reach; k2 <= iz + reach; ++k2) {
-    if (k2 < 1 || k2 > G.n) continue;
-    for (int j2 = iy - reach; j2 <= iy + reach; ++j2) {
-      if (j2 < 1 || j2 > G.n) continue;
-      for (int i2 = ix - reach; i2 <= ix + reach; ++i2) {
-        if (i2 < 1 || i2 > G.n) continue;
-        bool ok;
-        float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);
-        if (ok) nrm = nrm + wgt;
-      }
-    }
-  }
-  norm[i] = nrm;
-  // reach in bins (home cell and bin differ by at most one cell)
-  int rb = reach + 1;
-  int b = B.bin_of(p.x) + B.nb * (B.bin_of(p.y) + B.nb * B.bin_of(p.z));
-  atomicMax(&bin_reach[b], rb);
-  atomicMax(rmax, rb);
+// DDENS_INTERP_SPH (particles.f90:434-618), one thread per particle: the
+// particle's NORM over its cell box (first loop), then its deposits m*W/NORM
+// (second loop) added into int64 fixed-point cell sums.  Integer atomics make
+// the sums independent of the deposit order (bit-reproducible), and the fixed
+// point (scale = 2^62 / (np * max mass), so no sum can overflow) resolves
+// 1e-11 of a particle mass at 10^8 particles; the reference adds FP64.
+__device__ __forceinline__ void fix_add(unsigned long long* acc, size_t c, float v, double scale) {
+  atomicAdd(&acc[c], (unsigned long long)llrint((double)v * scale));
 }
 
-// DDENS pass 2: gather per cell.  Writes U1 = mass/(dx^3 rete^3)/rote*rote
-// (avism.f90:1249-1250, 1386) and the FP64 block sums of it (avism.f90:1395-1405).
-__global__ void __launch_bounds__(256) k_ddens_gather(DdensGeom G, Bins B,
-                                                      const float* __restrict__ hpart,
-                                                      const float* __restrict__ norm,
-                                                      const int32_t* __restrict__ bin_reach,
-                                                      int rsearch, float den, float rote,
-                                                      float* umass, double* partial) {
-  int ix = blockIdx.x * 8 + threadIdx.x % 8;
-  int iy = blockIdx.y * 8 + (threadIdx.x / 8) % 8;
-  int iz = blockIdx.z * 4 + threadIdx.x / 64;
-  double contrib = 0.0;
-  if (ix < G.n && iy < G.n && iz < G.n) {
-    int ci = ix + 1, cj = iy + 1, ck = iz + 1;  // reference indices
-    float qx = G.rad[ci - G.rlo], qy = G.rad[cj - G.rlo], qz = G.rad[ck - G.rlo];
-    int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
-    Kahan acc;
-    const int nb = B.nb;
-    for (int dz = -rsearch; dz <= rsearch; ++dz) {
-      int cz = bz + dz;
-      if (cz < 0 || cz >= nb) continue;  // DDENS never wraps (particles.f90:505-512)
-      for (int dy = -rsearch; dy <= rsearch; ++dy) {
-        int cy = by + dy;
-        if (cy < 0 || cy >= nb) continue;
-        for (int dxo = -rsearch; dxo <= rsearch; ++dxo) {
-          int cx = bx + dxo;
-          if (cx < 0 || cx >= nb) continue;
-          int bin = cx + nb * (cy + nb * cz);
-          int rb = bin_reach[bin];
-          int ad = max(abs(dxo), max(abs(dy), abs(dz)));
-          if (ad > rb) continue;
-          int e = B.start[bin + 1];
-          for (int j = B.start[bin]; j < e; ++j) {
-            float4 p = B.pos[j];
-            int hx, hy, hz, inc, reach;
-            float inv_h, h2max;
-            ddens_setup(G, p.x, p.y, p.z, hpart[j], hx, hy, hz, inv_h, h2max, inc, reach);
-            int ax = abs(ci - hx), ay = abs(cj - hy), az = abs(ck - hz);
-            float nrm = norm[j];
-            if (nrm == 0.0f) {  // particles.f90:549-557
-              if (ax == 0 && ay == 0 && az == 0) acc.add(p.w);
-              continue;
-            }
-            if (ax > reach || ay > reach || az > reach) continue;
-            bool ok;
-            float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, ci, cj, ck, ok);
-            if (!ok) continue;
-            acc.add((wgt * p.w) / nrm);  // particles.f90:599-600
+__global__ void __launch_bounds__(256) k_ddens_scatter(int64_t np, DdensGeom G, const float4* __restrict__ pos,
+                                                      const float* __restrict__ hpart, double scale,
+                                                      unsigned long long* acc, float* norm) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const float4 p = pos[i];
+  const float hh = hpart[i];
+  int ix, iy, iz, inc, reach;
+  float inv_h, h2max;
+  ddens_setup(G, p.x, p.y, p.z, hh, ix, iy, iz, inv_h, h2max, inc, reach);
+  const int n = G.n;
+  // the reference's cell box, clipped to the grid ...
+  int k0 = max(iz - reach, 1), k1 = min(iz + reach, n);
+  int j0 = max(iy - reach, 1), j1 = min(iy + reach, n);
+  const int i0 = max(ix - reach, 1), i1 = min(ix + reach, n);
+  // ... and to the cells whose centre can lie within h (beyond it M4(2d/h) is
+  // exactly 0, and adding +0 leaves the FP32 NORM unchanged).  The bounds keep
+  // a margin of one cell and 1e-5 h^2, far above the float rounding involved.
+  const float hm2 = hh * hh * 1.00001f;
+  const float rdx = 1.0f / G.dx;
+  k0 = max(k0, (int)floorf((p.z - hh - G.rx1) * rdx) + 1);
+  k1 = min(k1, (int)floorf((p.z + hh - G.rx1) * rdx) + 2);
+  j0 = max(j0, (int)floorf((p.y - hh - G.rx1) * rdx) + 1);
+  j1 = min(j1, (int)floorf((p.y + hh - G.rx1) * rdx) + 2);
+  float nrm = 0.0f;
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1) {
+      norm[i] = nrm;
+      if (nrm == 0.0f) {  // particles.f90:549-557: all the mass to the home cell
+        if (ix >= 1 && ix <= n && iy >= 1 && iy <= n && iz >= 1 && iz <= n)
+          fix_add(acc, (size_t)(ix - 1) + (size_t)n * ((size_t)(iy - 1) + (size_t)n * (iz - 1)), p.w, scale);
+        return;
+      }
+    }
+    for (int k2 = k0; k2 <= k1; ++k2) {
+      const float tz = p.z - G.rad[k2 - G.rlo];
+      for (int j2 = j0; j2 <= j1; ++j2) {
+        const float ty = p.y - G.rad[j2 - G.rlo];
+        const float rem = hm2 - (ty * ty + tz * tz);
+        if (rem < 0.0f) continue;  // no centre of this row within h
+        const float r = sqrtf(rem);
+        const int a = max(i0, (int)floorf((p.x - r - G.rx1) * rdx) + 1);
+        const int b = min(i1, (int)floorf((p.x + r - G.rx1) * rdx) + 2);
+        for (int i2 = a; i2 <= b; ++i2) {
+          bool ok;
+          const float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);
+          if (!ok) continue;
+          if (pass == 0) {
+            nrm = nrm + wgt;  // particles.f90:520-547, in the reference's loop order
+          } else {
+            const float v = (wgt * p.w) / nrm;  // particles.f90:599-600
+            if (v != 0.0f)
+              fix_add(acc, (size_t)(i2 - 1) + (size_t)n * ((size_t)(j2 - 1) + (size_t)n * (k2 - 1)), v, scale);
           }
         }
       }
     }
-    float u = acc.s / den;
+  }
+}
+
+// cell sums -> U1 = mass/(dx^3 rete^3)/rote*rote (avism.f90:1249-1250, 1386)
+// and the FP64 block sums of U1 (avism.f90:1395-1405), fixed reduction order
+__global__ void __launch_bounds__(256) k_ddens_finalize(int n, const unsigned long long* __restrict__ acc,
+                                                       double inv_scale, float den, float rote, float* umass,
+                                                       double* partial) {
+  int ix = blockIdx.x * 8 + threadIdx.x % 8;
+  int iy = blockIdx.y * 8 + (threadIdx.x / 8) % 8;
+  int iz = blockIdx.z * 4 + threadIdx.x / 64;
+  double contrib = 0.0;
+  if (ix < n && iy < n && iz < n) {
+    size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
+    float mass = (float)((double)acc[c] * inv_scale);
+    float u = mass / den;
     u = u / rote;
     u = u * rote;
-    size_t c = (size_t)ix + (size_t)G.n * ((size_t)iy + (size_t)G.n * iz);
     umass[c] = u;
     contrib = (double)u;
   }
-  // block FP64 sum (fixed reduction order -> deterministic)
   __shared__ double red[256];
   red[threadIdx.x] = contrib;
   __syncthreads();
@@ -712,16 +757,19 @@ int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, con
   }
   c->np = np;
   c->fields_ready = false;
-  int* bad = (int*)c->scalars.ensure(4096);
-  AV_CHECK_HIP(c, hipMemsetAsync(bad, 0, 4, c->stream));
+  int* bad = (int*)c->scalars.p;  // [0] out-of-box flag, [1] max |mass| bits
+  AV_CHECK_HIP(c, hipMemsetAsync(bad, 0, 8, c->stream));
   k_convert_check<<<nblk(np), 256, 0, c->stream>>>(np, c->pvx.as<float>(), c->pvy.as<float>(),
                                                    c->pvz.as<float>(), c->pm.as<float>(),
                                                    c->px.as<float>(), c->py.as<float>(),
-                                                   c->pz.as<float>(), c->g.L0 / 2.0f, bad);
-  int hbad = 0;
-  AV_CHECK_HIP(c, hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, c->stream));
+                                                   c->pz.as<float>(), c->g.L0 / 2.0f, bad,
+                                                   (unsigned*)bad + 1);
+  int hbad[2] = {0, 0};
+  AV_CHECK_HIP(c, hipMemcpyAsync(hbad, bad, 8, hipMemcpyDeviceToHost, c->stream));
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
-  if (hbad) AV_FAIL(c, AVISM_ERR_INPUT, "Particles outside the bounding box! Check L0! Particles must be inside [-L0/2, L0/2] in each direction.");
+  memcpy(&c->mass_max, &hbad[1], 4);
+  if (!std::isfinite(c->mass_max)) AV_FAIL(c, AVISM_ERR_INPUT, "non-finite particle mass");
+  if (hbad[0]) AV_FAIL(c, AVISM_ERR_INPUT, "Particles outside the bounding box! Check L0! Particles must be inside [-L0/2, L0/2] in each direction.");
   return AVISM_OK;
 }
 
@@ -736,7 +784,7 @@ static int build_bins(avism_ctx* c) {
   if (!c->bin_start.ensure((size_t)(nbins + 1) * 4) || !c->sbin_key.ensure((size_t)np * 4) ||
       !c->sbin_val.ensure((size_t)np * 4) || !c->sbin_key2.ensure((size_t)np * 4) ||
       !c->sbin_val2.ensure((size_t)np * 4) || !c->spos4.ensure((size_t)np * 16) ||
-      !c->svel4.ensure((size_t)np * 16) || !c->sorig.ensure((size_t)np * 4) ||
+      !c->svel4.ensure((size_t)np * 16) || !c->sorig.ensure((size_t)np * 4) || !c->sinv.ensure((size_t)np * 4) ||
       !c->bin_reach.ensure((size_t)nbins * 4))
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (bins)");
   int32_t* count = c->bin_reach.as<int32_t>();  // reused as the histogram
@@ -772,26 +820,10 @@ static int build_bins(avism_ctx* c) {
   k_gather_sorted<<<nblk(np), 256, 0, c->stream>>>(
       np, c->sbin_val2.as<uint32_t>(), c->px.as<float>(), c->py.as<float>(), c->pz.as<float>(),
       c->pm.as<float>(), c->pvx.as<float>(), c->pvy.as<float>(), c->pvz.as<float>(),
-      c->spos4.as<float4>(), c->svel4.as<float4>(), c->sorig.as<uint32_t>());
+      c->spos4.as<float4>(), c->svel4.as<float4>(), c->sorig.as<uint32_t>(), c->sinv.as<uint32_t>());
   AV_CHECK_HIP(c, hipGetLastError());
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   return AVISM_OK;
-}
-
-template <int K>
-static void launch_ppart(avism_ctx* c, Bins B, int k, int want_dens) {
-  k_ppart<K><<<nblk(c->np), 256, 0, c->stream>>>(B, c->np, k, want_dens, c->pdens.as<float>(),
-                                                 c->hk.as<float>());
-}
-
-template <int K>
-static void launch_vvel(avism_ctx* c, Bins B, const float* rad, int k, int want_vel) {
-  int n = c->g.n;
-  dim3 grid((n + 7) / 8, (n + 7) / 8, (n + 3) / 4);
-  k_vvel<K><<<grid, 256, 0, c->stream>>>(B, n, rad, c->g.rlo, c->g.dx, k, want_vel,
-                                         c->svel4.as<float4>(), c->pdens.as<float>(), nullptr,
-                                         c->hpart.as<uint32_t>(), c->fu.as<float>(),
-                                         c->fv.as<float>(), c->fw.as<float>());
 }
 
 int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp) {
@@ -819,45 +851,47 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   Bins B = make_bins(c);
   {
     StageTimer t(c, "ppart_dens");
-    if (k <= 16) launch_ppart<16>(c, B, k, 1);
-    else if (k <= 32) launch_ppart<32>(c, B, k, 1);
-    else launch_ppart<64>(c, B, k, 1);
+    k_ppart<<<nblk(np, 4 * QPW), 256, 0, c->stream>>>(B, np, k, c->pm.as<float>(), c->pdens.as<float>(),
+                                                     c->hk.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   AV_CHECK_HIP(c, hipMemsetAsync(c->hpart.p, 0, (size_t)np * 4, c->stream));
   {
     StageTimer t(c, "vvel_sph_vw");
-    if (k <= 16) launch_vvel<16>(c, B, rad, k, 1);
-    else if (k <= 32) launch_vvel<32>(c, B, rad, k, 1);
-    else launch_vvel<64>(c, B, rad, k, 1);
+    k_vvel<<<nblk(ncell, 4 * QPW), 256, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1,
+                                                        c->svel4.as<float4>(), c->pdens.as<float>(),
+                                                        c->sinv.as<uint32_t>(), nullptr, c->hpart.as<uint32_t>(),
+                                                        c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
+  }
+  {
+    StageTimer t(c, "fill_h");
     k_fill_h<<<nblk(np), 256, 0, c->stream>>>(np, c->hpart.as<float>(), c->hk.as<float>());
+    AV_CHECK_HIP(c, hipGetLastError());
   }
   DdensGeom G{rad, c->g.rlo, n, c->g.dx, c->g.rx1};
-  int32_t* rmax = (int32_t*)c->scalars.p + 8;
-  int64_t nbins = (int64_t)c->nb * c->nb * c->nb;
-  {
-    StageTimer t(c, "ddens_norm");
-    AV_CHECK_HIP(c, hipMemsetAsync(c->bin_reach.p, 0, (size_t)nbins * 4, c->stream));
-    AV_CHECK_HIP(c, hipMemsetAsync(rmax, 0, 4, c->stream));
-    k_ddens_norm<<<nblk(np), 256, 0, c->stream>>>(np, G, c->spos4.as<float4>(), c->hpart.as<float>(),
-                                                   c->norm.as<float>(), c->bin_reach.as<int32_t>(),
-                                                   B, rmax);
-    AV_CHECK_HIP(c, hipGetLastError());
-  }
-  int32_t hr = 0;
-  AV_CHECK_HIP(c, hipMemcpyAsync(&hr, rmax, 4, hipMemcpyDeviceToHost, c->stream));
-  AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   // cosmology constants (avism.f90:469-479, 599-606, 1249-1250)
   float den = ((c->g.dx * c->g.dx) * c->g.dx) * ((c->rete * c->rete) * c->rete);
   dim3 grid((n + 7) / 8, (n + 7) / 8, (n + 3) / 4);
   int64_t nparts = (int64_t)grid.x * grid.y * grid.z;
-  if (!c->partial64.ensure((size_t)nparts * 8)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+  if (!c->partial64.ensure((size_t)nparts * 8) || !c->mass_fix.ensure((size_t)ncell * 8))
+    AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+  // fixed point: every cell sum stays below np * max mass * scale <= 2^62
+  double bound = (double)np * (double)(c->mass_max > 0.0f ? c->mass_max : 1.0f);
+  int e2 = 0;
+  std::frexp(bound, &e2);  // bound < 2^e2
+  const double scale = std::ldexp(1.0, 62 - e2);
   {
-    StageTimer t(c, "ddens_gather");
-    k_ddens_gather<<<grid, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), c->norm.as<float>(),
-                                                c->bin_reach.as<int32_t>(), hr, den, c->rote,
-                                                c->u_mass.as<float>(), c->partial64.as<double>());
+    StageTimer t(c, "ddens_scatter");
+    k_fill_u64<<<nblk(ncell), 256, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 0ull);
+    k_ddens_scatter<<<nblk(np), 256, 0, c->stream>>>(np, G, c->spos4.as<float4>(), c->hpart.as<float>(), scale,
+                                                     c->mass_fix.as<unsigned long long>(), c->norm.as<float>());
+    AV_CHECK_HIP(c, hipGetLastError());
+  }
+  {
+    StageTimer t(c, "ddens_finalize");
+    k_ddens_finalize<<<grid, 256, 0, c->stream>>>(n, c->mass_fix.as<unsigned long long>(), 1.0 / scale, den,
+                                                  c->rote, c->u_mass.as<float>(), c->partial64.as<double>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   double* dsum = (double*)((char*)c->scalars.p + 256);

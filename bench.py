@@ -146,7 +146,8 @@ def main():
         # gather model: k neighbour records (xyz, m, rho, velocity, HPART) + 3 outputs
         "vvel_sph_vw": cells * (k * 32 + 12 + k * 8),
         "ppart_dens": n ** 3 * (k * 16 + 4),
-        "ddens_gather": cells * 4,  # compulsory only (data-dependent gather)
+        # compulsory: particle records (x,y,z,m,h) and NORM out, cell sums filled and read
+        "ddens_scatter": n ** 3 * 24 + cells * 16,
         "mark_flags": gi.next ** 3 * 9 + cells * 8,
         "mean_div": cells * 24,
     }

@@ -1,0 +1,279 @@
+!-----------------------------------------------------------------------
+! avism_host -- Fortran host of the MI355X grid-and-void path.
+!
+! Drop-in for the single-level path of PROGRAM AVISM (avism.f90): reads
+! config/voids.dat with the same positional READs (avism.f90:176-253), reads
+! input_data/bin_file_partNNNNN (reader.f90:410-450, FLAG_DATA=1) or
+! input_data/bin_file_gridNNNNN (reader.f90:521-540, FLAG_DATA=2), runs the
+! path through the C ABI (host/avism_c.f90) and writes output_files/voidsNNNNN,
+! mapNNNNN and, with FLAG_WRITE_CUBES=1, cubesNNNNN as avism.f90:707-722 and
+! 2028-2085 do.
+!
+! Usage:  avism_host.x [--learned weights.f32]
+!   --learned: growth with the half-width proposal network whose fp32
+!   parameters (19064 values, raw little-endian) are in the given file; the
+!   catalogue is the same as without it.
+!
+! Not covered (the library returns or this program stops with a message):
+! LEVMAX > LEVMIN (void-in-void levels), survey mode, MASCLET/GADGET input,
+! TSC interpolation, an input grid size different from NCOX.
+!-----------------------------------------------------------------------
+program avism_host
+  use iso_c_binding
+  use avism_c
+  implicit none
+
+  ! voids.dat
+  integer :: first, last, every, ncox, ncoy, ncoz, levmin, levmax, flag_period
+  real :: ache, omega0, lado0
+  integer :: nvoid_max
+  real :: dens_thre, dens_thre2, grad_thre, div_thre, rmin, rmin_sub
+  integer :: flag_data, nhyx, nhyy, nhyz, files_per_snap, parttypex, flag_gadget_reader, flag_survey
+  real :: massdm
+  integer(8) :: partired
+  integer :: flag_vel_available, flag_vel_interp, flag_dens_interp, kneighbours, flag_write_cubes
+  integer :: npalev, nlevels, namrx, namry, namrz, flag_div, flag_dens
+
+  type(c_ptr) :: ctx
+  type(avism_void_params) :: vp
+  type(avism_level_stats) :: st
+  integer(c_int) :: rc
+  integer(c_int32_t) :: ncubes, nvoids, growth
+  integer :: niter, ifi, iter, n, nlev, ir
+  real :: zeta
+  real(c_float), allocatable, target :: params(:)
+  character(len=512) :: arg, weights
+  character(len=:), allocatable :: dir
+
+  growth = AVISM_GROW_PLAIN
+  weights = ''
+  if (command_argument_count() >= 2) then
+    call get_command_argument(1, arg)
+    if (trim(arg) == '--learned') then
+      call get_command_argument(2, weights)
+      growth = AVISM_GROW_LEARNED
+    end if
+  end if
+
+  call read_voids_dat('config/voids.dat')
+  call check_supported()
+  dir = 'output_files'
+
+  rc = avism_create(ctx, 0_c_int32_t)
+  if (rc /= AVISM_OK) then
+    write(*,*) '     ERROR: no HIP device for the AVISM library (code', rc, ')'
+    stop 1
+  end if
+  call avism_check(ctx, avism_set_grid(ctx, int(ncox, c_int32_t), real(lado0, c_float), &
+                                    int(flag_period, c_int32_t)), 'avism_set_grid')
+  vp = avism_void_params(dens_thre, dens_thre2, grad_thre, div_thre, rmin, rmin_sub, nvoid_max)
+  call avism_check(ctx, avism_set_void_params(ctx, vp), 'avism_set_void_params')
+  if (growth == AVISM_GROW_LEARNED) then
+    allocate(params(AVISM_CNN_NPARAM))
+    open(unit=21, file=trim(weights), form='unformatted', access='stream', status='old', action='read')
+    read(21) params
+    close(21)
+    call avism_check(ctx, avism_cnn_set_params(ctx, params, AVISM_HOST), 'avism_cnn_set_params')
+  end if
+
+  niter = int((last - first) / every) + 1
+  do ifi = 1, niter
+    iter = first + every * (ifi - 1)
+    if (flag_data == 1) then
+      call run_particles(iter, zeta)
+    else
+      call run_grid(iter, zeta)
+    end if
+    call avism_check(ctx, avism_find_voids(ctx, growth, ncubes), 'avism_find_voids')
+    call avism_check(ctx, avism_merge_and_post(ctx, nvoids), 'avism_merge_and_post')
+    n = ncox
+    nlev = levmax - levmin + 1
+    ir = levmin
+    call write_outputs(iter, ir, nlev, n, ncubes)
+    write(*,*) 'ITER', iter, ': cubes', ncubes, ' voids', nvoids
+  end do
+  rc = avism_destroy(ctx)
+
+contains
+
+  subroutine skip(u, k)
+    integer, intent(in) :: u, k
+    integer :: i
+    do i = 1, k
+      read(u, *)
+    end do
+  end subroutine
+
+  ! positional read, one label (or banner) line before each value line
+  subroutine read_voids_dat(path)
+    character(len=*), intent(in) :: path
+    open(1, file=path, status='old', action='read')
+    call skip(1, 5); read(1, *) first, last, every
+    call skip(1, 1); read(1, *) ncox, ncoy, ncoz
+    call skip(1, 1); read(1, *) levmin, levmax
+    call skip(1, 1); read(1, *) ache, omega0
+    call skip(1, 1); read(1, *) lado0
+    call skip(1, 1); read(1, *) flag_period
+    call skip(1, 4); read(1, *) nvoid_max
+    call skip(1, 1); read(1, *) dens_thre
+    call skip(1, 1); read(1, *) dens_thre2
+    call skip(1, 1); read(1, *) grad_thre
+    call skip(1, 1); read(1, *) div_thre
+    call skip(1, 1); read(1, *) rmin
+    call skip(1, 1); read(1, *) rmin_sub
+    call skip(1, 4); read(1, *) flag_data
+    call skip(1, 1); read(1, *) nhyx, nhyy, nhyz
+    call skip(1, 1); read(1, *) files_per_snap, parttypex, massdm
+    call skip(1, 1); read(1, *) flag_gadget_reader
+    call skip(1, 1); read(1, *) flag_survey
+    call skip(1, 4); read(1, *) partired
+    call skip(1, 1); read(1, *) flag_vel_available
+    call skip(1, 1); read(1, *) flag_vel_interp
+    call skip(1, 1); read(1, *) flag_dens_interp
+    call skip(1, 1); read(1, *) kneighbours
+    call skip(1, 4); read(1, *) flag_write_cubes
+    call skip(1, 4); read(1, *) npalev, nlevels, namrx, namry, namrz
+    call skip(1, 1); read(1, *) flag_div
+    call skip(1, 1); read(1, *) flag_dens
+    close(1)
+  end subroutine
+
+  subroutine check_supported()
+    if (ncox /= ncoy .or. ncox /= ncoz) then
+      write(*,*) '     ERROR: NCOX, NCOY and NCOZ must be equal'
+      stop 1
+    end if
+    if (levmax /= levmin) then
+      write(*,*) '     ERROR: void-in-void levels (LEVMAX > LEVMIN) are not supported by this build'
+      stop 1
+    end if
+    if (flag_data /= 1 .and. flag_data /= 2) then
+      write(*,*) '     ERROR: only FLAG_DATA = 1 (BinPart) and 2 (BinGrid) are supported by this build'
+      stop 1
+    end if
+    if (flag_survey /= 0 .or. flag_vel_available /= 1) then
+      write(*,*) '     ERROR: survey mode and velocity-free input are not supported by this build'
+      stop 1
+    end if
+    if (flag_data == 2 .and. (nhyx /= ncox .or. nhyy /= ncoy .or. nhyz /= ncoz)) then
+      write(*,*) '     ERROR: the input grid must have NCOX^3 cells in this build'
+      stop 1
+    end if
+    if (kneighbours < 0) then
+      write(*,*) '     ERROR: KNEIGHBOURS must be positive'
+      stop 1
+    end if
+  end subroutine
+
+  function iter_tag(it) result(nom)
+    integer, intent(in) :: it
+    character(len=5) :: nom
+    write(nom, '(I5.5)') it
+  end function
+
+  subroutine run_particles(it, z)
+    integer, intent(in) :: it
+    real, intent(out) :: z
+    integer(8) :: np
+    real(c_float), allocatable :: x(:), y(:), zz(:), vx(:), vy(:), vz(:), m(:)
+    open(unit=41, file='input_data/bin_file_part'//iter_tag(it), form='unformatted', status='old', &
+         action='read')
+    read(41) np, z
+    if (np > partired) then
+      write(*,*) '     ERROR: NPARTT > PARTIRED'
+      stop 1
+    end if
+    allocate(x(np), y(np), zz(np), vx(np), vy(np), vz(np), m(np))
+    read(41) x
+    read(41) y
+    read(41) zz
+    read(41) vx
+    read(41) vy
+    read(41) vz
+    read(41) m
+    close(41)
+    write(*,*) '     TOTAL PARTICLES IN ITER=', np
+    call avism_check(ctx, avism_set_cosmology(ctx, ache, omega0, z), 'avism_set_cosmology')
+    call avism_check(ctx, avism_load_particles(ctx, int(np, c_int64_t), x, y, zz, vx, vy, vz, m, AVISM_HOST), &
+                  'avism_load_particles')
+    call avism_check(ctx, avism_particles_to_grid(ctx, int(kneighbours, c_int32_t), &
+                                               int(flag_vel_interp, c_int32_t), &
+                                               int(flag_dens_interp, c_int32_t)), 'avism_particles_to_grid')
+  end subroutine
+
+  subroutine run_grid(it, z)
+    integer, intent(in) :: it
+    real, intent(out) :: z
+    real(c_float), allocatable :: d(:,:,:), u(:,:,:), v(:,:,:), w(:,:,:)
+    allocate(d(nhyx, nhyy, nhyz), u(nhyx, nhyy, nhyz), v(nhyx, nhyy, nhyz), w(nhyx, nhyy, nhyz))
+    open(unit=41, file='input_data/bin_file_grid'//iter_tag(it), form='unformatted', status='old', &
+         action='read')
+    read(41) z
+    read(41) d
+    read(41) u
+    read(41) v
+    read(41) w
+    close(41)
+    call avism_check(ctx, avism_set_cosmology(ctx, ache, omega0, z), 'avism_set_cosmology')
+    call avism_check(ctx, avism_load_grid_fields(ctx, d, u, v, w, AVISM_HOST), 'avism_load_grid_fields')
+  end subroutine
+
+  subroutine write_outputs(it, lev, nl, nn, nv)
+    integer, intent(in) :: it, lev, nl, nn
+    integer(c_int32_t), intent(in) :: nv
+    integer(c_int32_t), allocatable, target :: indice(:), uvoid(:), ncellv(:), marca(:), ci(:,:)
+    real(c_float), allocatable, target :: vf(:,:), u1co(:), div(:), cf(:,:)
+    integer(c_int32_t), allocatable, target :: father(:)
+    integer :: iv, ind, nvp, nvs
+    integer(8) :: ncell
+    real :: volm, req, reqpp
+    integer :: indp
+
+    nvs = max(int(nv), 1)
+    allocate(indice(nvs), uvoid(nvs), ncellv(nvs), vf(nvs, 12))
+    call avism_check(ctx, avism_get_catalogue(ctx, c_loc(indice), c_loc(uvoid), c_loc(ncellv), c_loc(vf), st), &
+                  'avism_get_catalogue')
+    ncell = int(nn, 8)**3
+    allocate(marca(ncell), u1co(ncell), div(ncell))
+    call avism_check(ctx, avism_get_marca(ctx, c_loc(marca), 1_c_int32_t, AVISM_HOST), 'avism_get_marca')
+    call avism_check(ctx, avism_get_fields(ctx, c_loc(u1co), c_loc(div), c_null_ptr, c_null_ptr, c_null_ptr, &
+                                        AVISM_HOST), 'avism_get_fields')
+
+    ! voids catalogue: file header (avism.f90:713-714), level header (2047), one line per main void
+    open(unit=10, file=dir//'/voids'//iter_tag(it), status='replace')
+    write(10, *) nl, levmin, levmax, ncox, ncoy, ncoz, lado0
+    nvp = 0
+    write(10, *) lev, int(nv), int(st%nvoidt), nvp, real(st%volt_clean / (real(lado0, 8)**3)), st%meandens
+    if (flag_write_cubes == 1) then
+      allocate(ci(nvs, 9), cf(nvs, 7), father(nvs))
+      call avism_check(ctx, avism_get_cubes(ctx, c_loc(ci), c_loc(cf), c_loc(father)), 'avism_get_cubes')
+      open(unit=12, file=dir//'/cubes'//iter_tag(it), status='replace')
+      write(12, *) nl, levmin, levmax
+      write(12, *) lev, int(nv), int(st%nvoidt)
+    end if
+    do iv = 1, int(nv)
+      ind = indice(iv)
+      if (flag_write_cubes == 1) then
+        write(12, *) ind, uvoid(ind), ci(ind, 1), ci(ind, 4), ci(ind, 2), ci(ind, 5), ci(ind, 3), ci(ind, 6), &
+                     cf(ind, 1), cf(ind, 4), cf(ind, 2), cf(ind, 5), cf(ind, 3), cf(ind, 6)
+      end if
+      if (uvoid(ind) /= -1) cycle
+      volm = vf(ind, 7)
+      req = vf(ind, 12)
+      indp = 0
+      reqpp = 0.0
+      write(10, *) ind, vf(ind, 4), vf(ind, 5), vf(ind, 6), vf(ind, 1), vf(ind, 2), vf(ind, 3), volm, req, &
+                   vf(ind, 8), vf(ind, 10), vf(ind, 11), indp, reqpp, vf(ind, 9)
+    end do
+    close(10)
+    if (flag_write_cubes == 1) close(12)
+
+    ! map: MARCA, delta and divergence on the inner grid (avism.f90:2027-2037)
+    open(unit=11, file=dir//'/map'//iter_tag(it), form='unformatted', access='stream', status='replace')
+    write(11) marca
+    write(11) u1co - 1.0
+    write(11) div
+    close(11)
+  end subroutine
+
+end program avism_host

@@ -47,7 +47,7 @@ extern "C" {
 /* proposal network: parameter count, classes, bf16 activations cached per patch */
 #define AVISM_CNN_NPARAM 19064
 #define AVISM_CNN_NCLASS 8
-#define AVISM_CNN_CACHE 7880
+#define AVISM_CNN_CACHE 7920
 #define AVISM_CNN_PATCH 2187 /* 3 x 9^3 bf16 features */
 
 typedef struct avism_ctx avism_ctx;

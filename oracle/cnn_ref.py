@@ -94,24 +94,24 @@ def _col2im(dcols: np.ndarray, C: int, S: int) -> np.ndarray:
     return dx_
 
 
-def forward(params, x0, round_acts=True):
+def forward(params, x0, round_acts=True, acts=None):
     """x0: [B, 3, 9, 9, 9] (c, z, y, x).  Returns logits and a cache.
 
     round_acts=True rounds inputs, weights and activations to bf16 as the GPU
-    does; False is the plain FP64 network (for finite-difference checks)."""
+    does; False is the plain FP64 network (for finite-difference checks).
+    acts: optional given activations of the three layers ([B, 16, O, O, O]),
+    used instead of recomputed ones (a backward check given the same forward)."""
     rnd = bf16 if round_acts else (lambda a: np.asarray(a, np.float64))
     W = {k: rnd(v) if k.startswith("w") else np.asarray(v, np.float64) for k, v in params.items()}
     x = rnd(x0)
     x = np.concatenate([x, np.zeros((x.shape[0], C0P - x.shape[1]) + x.shape[2:])], axis=1)
     cache = {"x0": x}
-    acts = []
     for li, (w, b) in enumerate((("w1", "b1"), ("w2", "b2"), ("w3", "b3"))):
         cols = _im2col(x)
         z = cols @ W[w] + W[b]
         a = np.maximum(z, 0.0)
         a = np.moveaxis(a, -1, 1)  # [B, C1, O, O, O]
-        a = rnd(a)
-        acts.append(a)
+        a = rnd(a) if acts is None else np.asarray(acts[li], np.float64)
         cache[f"cols{li}"] = cols
         cache[f"a{li}"] = a
         x = a
@@ -122,9 +122,9 @@ def forward(params, x0, round_acts=True):
     return logits, cache
 
 
-def loss_and_grads(params, x0, labels, round_acts=True):
+def loss_and_grads(params, x0, labels, round_acts=True, acts=None):
     """Mean softmax cross-entropy and FP64 gradients w.r.t. every parameter."""
-    logits, cache = forward(params, x0, round_acts)
+    logits, cache = forward(params, x0, round_acts, acts)
     B = logits.shape[0]
     m = logits.max(1, keepdims=True)
     e = np.exp(logits - m)
