@@ -76,7 +76,7 @@ struct avism_ctx {
   int64_t np = 0;
   avism::DBuf px, py, pz, pvx, pvy, pvz, pm;   // converted units after upload
   avism::DBuf sbin_key, sbin_val, sbin_key2, sbin_val2, sort_tmp;
-  avism::DBuf spos4, svel4, sorig, sinv;       // sorted by bin; sinv: input index -> sorted
+  avism::DBuf spos4, svel4, sorig;             // sorted by bin
   avism::DBuf bin_start;                       // nb^3 + 1
   avism::DBuf bin_reach;                       // nb^3, DDENS reach per bin
   int32_t nb = 0;

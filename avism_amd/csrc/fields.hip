@@ -174,224 +174,268 @@ struct Bins {
   }
 };
 
-// ---------------------------------------------------------- wave kNN
-// One wave answers one query: the 64 lanes split each shell's bins and the
-// candidates inside them, and the k best are kept as a key sorted across the
-// lanes (lane t holds the t-th smallest).  A key packs the squared distance
-// (non-negative floats order as their bits) over the original index, so equal
-// distances go to the lower index: the deterministic form of kdtree.f90's
-// strict-< admission, and the tie rule of the oracle.
-__device__ __forceinline__ uint64_t knn_key(float d2, uint32_t orig) {
-  return ((uint64_t)__float_as_uint(d2) << 32) | orig;
-}
-__device__ __forceinline__ float key_d2(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
-constexpr uint64_t kNoKey = ~0ull;
-
-__device__ __forceinline__ uint64_t rdlane64(uint64_t v, int lane) {  // lane: wave-uniform
-  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
-  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
-  return ((uint64_t)hi << 32) | lo;
-}
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
-  uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m), hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
-  return ((uint64_t)hi << 32) | lo;
-}
-__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) {
-  uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d), hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d);
-  return ((uint64_t)hi << 32) | lo;
-}
-__device__ __forceinline__ double wave_sum(double v) {  // fixed tree: deterministic
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-// the k smallest keys of everything offered, sorted across the lanes
-struct WaveSel {
-  uint64_t best;
-  int k, lane;
-  __device__ __forceinline__ void init(int k_, int lane_) {
-    best = kNoKey;
+// ------------------------------------------------------------------ kNN
+// Lane-per-query search; the k best of a lane are a max-heap in LDS (element
+// t of lane l at [t * BS + l]: every lane hits its own bank).  Entries are
+// (squared distance, bin-sorted position) ordered by distance, then by the
+// particle's input index (looked up only when two distances are equal), so
+// equal distances go to the lower input index: the deterministic form of
+// kdtree.f90's strict-< admission, and the oracle's tie rule.  O(log k) LDS
+// steps per admission.
+template <int BS>
+struct Heap {
+  float* hd;
+  uint32_t* hj;
+  const uint32_t* orig;
+  int k, cnt;
+  float wd;     // root: worst kept distance (INFINITY while filling)
+  uint32_t wj;
+  __device__ __forceinline__ void init(float* d, uint32_t* j, const uint32_t* o, int k_) {
+    hd = d;
+    hj = j;
+    orig = o;
     k = k_;
-    lane = lane_;
+    cnt = 0;
+    wd = INFINITY;
+    wj = 0;
   }
-  __device__ __forceinline__ uint64_t kth() const { return rdlane64(best, k - 1); }
-  // every lane offers one key (kNoKey for none); called by the whole wave
-  __device__ __forceinline__ void offer(uint64_t key) {
-    const uint64_t kt = kth();
-    const bool in = key < kt;
-    const uint64_t m = __ballot(in);
-    if (m == 0) return;
-    if (__popcll(m) <= 4) {  // few improvements: sorted inserts
-      uint64_t mm = m;
-      while (mm) {
-        const int src = __ffsll((unsigned long long)mm) - 1;
-        mm &= mm - 1;
-        const uint64_t x = rdlane64(key, src);
-        const int pos = __popcll(__ballot(best < x));
-        const uint64_t up = shfl_up64(best, 1);
-        best = lane < pos ? best : (lane == pos ? x : up);
+  __device__ __forceinline__ bool full() const { return cnt >= k; }
+  __device__ __forceinline__ bool gt(float da, uint32_t ja, float db, uint32_t jb) const {
+    return da > db || (da == db && orig[ja] > orig[jb]);
+  }
+  __device__ __forceinline__ void consider(float v, uint32_t j) {
+    if (cnt < k) {
+      int i = cnt++;
+      while (i > 0) {  // sift up
+        const int p = (i - 1) >> 1;
+        const float pd = hd[p * BS];
+        const uint32_t pj = hj[p * BS];
+        if (!gt(v, j, pd, pj)) break;
+        hd[i * BS] = pd;
+        hj[i * BS] = pj;
+        i = p;
+      }
+      hd[i * BS] = v;
+      hj[i * BS] = j;
+      if (cnt == k) {
+        wd = hd[0];
+        wj = hj[0];
       }
       return;
     }
-    // many: bitonic sort of the offers (descending), keep the lane-wise
-    // minimum with the ascending best (a bitonic sequence holding the 64
-    // smallest of both), then a bitonic merge back to ascending
-    uint64_t a = in ? key : kNoKey;
-    for (int size = 2; size <= 64; size <<= 1)
-      for (int st = size >> 1; st > 0; st >>= 1) {
-        const uint64_t o = shfl_xor64(a, st);
-        const bool asc = (lane & size) != 0;
-        const bool lower = (lane & st) == 0;
-        a = (lower == asc) ? (a < o ? a : o) : (a < o ? o : a);
+    if (v > wd) return;
+    if (v == wd && orig[j] >= orig[wj]) return;
+    int i = 0;  // replace the root, sift down
+    for (;;) {
+      const int l = 2 * i + 1;
+      if (l >= k) break;
+      float cd = hd[l * BS];
+      uint32_t cj = hj[l * BS];
+      int c = l;
+      if (l + 1 < k) {
+        const float rd = hd[(l + 1) * BS];
+        const uint32_t rj = hj[(l + 1) * BS];
+        if (gt(rd, rj, cd, cj)) {
+          c = l + 1;
+          cd = rd;
+          cj = rj;
+        }
       }
-    best = best < a ? best : a;
-    for (int st = 32; st > 0; st >>= 1) {
-      const uint64_t o = shfl_xor64(best, st);
-      best = (lane & st) == 0 ? (best < o ? best : o) : (best < o ? o : best);
+      if (!gt(cd, cj, v, j)) break;
+      hd[i * BS] = cd;
+      hj[i * BS] = cj;
+      i = c;
     }
+    hd[i * BS] = v;
+    hj[i * BS] = j;
+    wd = hd[0];
+    wj = hj[0];
   }
+  __device__ __forceinline__ float dist(int t) const { return hd[t * BS]; }
+  __device__ __forceinline__ uint32_t pos(int t) const { return hj[t * BS]; }
 };
 
-// Offers every particle of the bins given per lane (j0, cnt; cnt = 0 for
-// none) to f(valid, j) in batches of 64, one candidate per lane.
-template <class F>
-__device__ __forceinline__ void wave_candidates(int lane, int j0, int cnt, F&& f) {
-  int incl = cnt;
-  for (int d = 1; d < 64; d <<= 1) {
-    const int t = __shfl_up(incl, d);
-    if (lane >= d) incl += t;
-  }
-  const int total = __builtin_amdgcn_readlane(incl, 63);
-  const int excl = incl - cnt;
-  for (int c0 = 0; c0 < total; c0 += 64) {
-    const int c = c0 + lane;
-    int src = 0;  // last lane whose range starts at or before c
-    for (int st = 32; st > 0; st >>= 1) {
-      const int e = __shfl(excl, src + st);
-      if (e <= c) src += st;
-    }
-    const int sj = __shfl(j0, src), se = __shfl(excl, src);
-    f(c < total, sj + (c - se));
-  }
-}
-
-// exact kNN of q (k <= 64 and <= the number of particles), wave-uniform q.
-// Shells of bins around q's bin are searched until the k-th distance is below
-// the distance to everything outside them; a bin whose box is farther than
-// the current k-th distance is skipped.
-__device__ void wave_knn(const Bins& B, int64_t np, float qx, float qy, float qz, WaveSel& S) {
-  const int lane = S.lane;
+// Shell search shared by both candidate sources.  scan(xa, xb, cy, cz)
+// offers the particles of bins xa..xb (unwrapped, xb - xa < nb) of the row at
+// unwrapped (cy, cz).  Shells of bins around q's bin are visited until the
+// k-th distance is below the distance to everything outside them; within a
+// shell each row is one contiguous range, clipped to the bins whose box can
+// still hold a closer particle.  Returns false if shell smax was reached
+// without an answer.
+template <class H, class Scan>
+__device__ __forceinline__ bool knn_shells(const Bins& B, float qx, float qy, float qz, int smax, H& h,
+                                           Scan&& scan) {
   const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
   const int nb = B.nb;
-  const int smax_p = (nb - 1) / 2;
-  auto offer_j = [&](bool valid, int j) {
-    uint64_t key = kNoKey;
-    if (valid) {
-      const float4 p = B.pos[j];
-      key = knn_key(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), B.orig[j]);
-    }
-    S.offer(key);
-  };
-  for (int s = 0; !(B.periodic && s > smax_p); ++s) {
-    const int side = 2 * s + 1, cube = side * side * side;
+  const float e2 = 2.0f * B.eps;
+  for (int s = 0; s <= smax; ++s) {
     bool any = false;
-    for (int base = 0; base < cube; base += 64) {
-      const int f = base + lane;
-      const int dxo = f % side - s, dy = (f / side) % side - s, dz = f / (side * side) - s;
-      const int cx = bx + dxo, cy = by + dy, cz = bz + dz;
-      bool v = f < cube && max(abs(dxo), max(abs(dy), abs(dz))) == s;
-      if (!B.periodic) v = v && cx >= 0 && cx < nb && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
-      any = any || (__ballot(v) != 0);
-      const uint64_t kt = S.kth();
-      if (v && kt != kNoKey) {
-        const float ddx = B.box_d(qx, cx), ddy = B.box_d(qy, cy), ddz = B.box_d(qz, cz);
-        v = !((ddx * ddx + ddy * ddy) + ddz * ddz > key_d2(kt));
+    for (int dz = -s; dz <= s; ++dz) {
+      const int cz = bz + dz;
+      if (!B.periodic && (cz < 0 || cz >= nb)) continue;
+      const float ddz = B.box_d(qz, cz);
+      for (int dy = -s; dy <= s; ++dy) {
+        const int cy = by + dy;
+        if (!B.periodic && (cy < 0 || cy >= nb)) continue;
+        const float ddy = B.box_d(qy, cy);
+        const bool edge = (dz == -s || dz == s || dy == -s || dy == s);
+        int lo_b = bx - s, hi_b = bx + s;
+        if (!B.periodic) {
+          lo_b = max(lo_b, 0);
+          hi_b = min(hi_b, nb - 1);
+        }
+        if (lo_b > hi_b) continue;
+        any = true;
+        if (h.full()) {  // bins whose box can hold d^2 <= worst
+          const float rem = h.wd - (ddy * ddy + ddz * ddz);
+          if (rem < 0.0f) continue;
+          const float r = sqrtf(rem);
+          lo_b = max(lo_b, (int)floorf((qx - r - e2 - B.lo) / B.bsz));
+          hi_b = min(hi_b, (int)floorf((qx + r + e2 - B.lo) / B.bsz));
+          if (lo_b > hi_b) continue;
+        }
+        if (edge || s == 0) {
+          scan(lo_b, hi_b, cy, cz);
+        } else {  // interior row of the shell: its two end bins
+          if (lo_b == bx - s) scan(lo_b, lo_b, cy, cz);
+          if (hi_b == bx + s) scan(hi_b, hi_b, cy, cz);
+        }
       }
-      int j0 = 0, cnt = 0;
-      if (v) {
-        const int bin = (B.periodic ? B.wrap(cx) : cx) + nb * ((B.periodic ? B.wrap(cy) : cy) +
-                                                             nb * (B.periodic ? B.wrap(cz) : cz));
-        j0 = B.start[bin];
-        cnt = B.start[bin + 1] - j0;
-      }
-      wave_candidates(lane, j0, cnt, offer_j);
     }
-    const uint64_t kt = S.kth();
-    if (kt != kNoKey) {
+    if (h.full()) {
       const float b1 = B.outside(qx, bx, s), b2 = B.outside(qy, by, s), b3 = B.outside(qz, bz, s);
       const float bd = fminf(b1, fminf(b2, b3));
-      if (bd == INFINITY || (bd > 0.0f && key_d2(kt) < bd * bd)) return;
+      if (bd == INFINITY || (bd > 0.0f && h.wd < bd * bd)) return true;
     }
     if (!B.periodic && !any && s > 0) {
       const int far = max(max(max(bx, nb - 1 - bx), max(by, nb - 1 - by)), max(bz, nb - 1 - bz));
-      if (s > far) return;
+      if (s > far) return true;
     }
   }
-  // periodic box smaller than the search: every particle once
-  S.init(S.k, lane);
-  for (int64_t c0 = 0; c0 < np; c0 += 64) offer_j(c0 + lane < np, (int)(c0 + lane));
+  return false;
 }
 
-// every particle with d2 <= r2 (ball_search, kdtree.f90:1015-1072, 1126), to
-// f(valid, j, d2) in wave batches
+// exact kNN over the bin-sorted particles in HBM (k <= number of particles)
+template <class H>
+__device__ void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz, H& h) {
+  const int nb = B.nb;
+  auto scan = [&](int xa, int xb, int cy, int cz) {
+    const int wy = B.periodic ? B.wrap(cy) : cy, wz = B.periodic ? B.wrap(cz) : cz;
+    const int rowb = nb * (wy + nb * wz);
+    int seg[3][2];
+    int ns = 0;
+    if (xa < 0) {  // periodic images on the low side: bins xa..min(xb,-1)
+      seg[ns][0] = xa + nb; seg[ns][1] = min(xb, -1) + nb; ++ns;
+      xa = 0;
+    }
+    if (xb >= nb) {  // and on the high side: bins max(xa,nb)..xb
+      seg[ns][0] = max(xa, nb) - nb; seg[ns][1] = xb - nb; ++ns;
+      xb = nb - 1;
+    }
+    if (xa <= xb) { seg[ns][0] = xa; seg[ns][1] = xb; ++ns; }
+    for (int g = 0; g < ns; ++g) {
+      const int j1 = B.start[rowb + seg[g][1] + 1];
+      int j = B.start[rowb + seg[g][0]];
+      for (; j + 1 < j1; j += 2) {  // two loads in flight per lane
+        const float4 p0 = B.pos[j], p1 = B.pos[j + 1];
+        const float d0 = dist2(p0.x, p0.y, p0.z, qx, qy, qz, B.L, B.periodic);
+        const float d1 = dist2(p1.x, p1.y, p1.z, qx, qy, qz, B.L, B.periodic);
+        h.consider(d0, (uint32_t)j);
+        h.consider(d1, (uint32_t)(j + 1));
+      }
+      if (j < j1) {
+        const float4 p = B.pos[j];
+        h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), (uint32_t)j);
+      }
+    }
+  };
+  if (knn_shells(B, qx, qy, qz, B.periodic ? (nb - 1) / 2 : nb, h, scan)) return;
+  // periodic box smaller than the search: every particle once
+  h.init(h.hd, h.hj, h.orig, h.k);
+  for (int64_t j = 0; j < np; ++j) {
+    const float4 p = B.pos[j];
+    h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), (uint32_t)j);
+  }
+}
+
+// every particle with d2 <= r2 (ball_search, kdtree.f90:1015-1072, 1126)
 template <class F>
-__device__ void wave_ball(const Bins& B, int64_t np, int lane, float qx, float qy, float qz, float r, float r2,
-                          F&& f) {
+__device__ __forceinline__ void ball_visit(const Bins& B, int64_t np, float qx, float qy, float qz, float r,
+                                           float r2, F&& f) {
   const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
   const int nb = B.nb;
-  const int R = (int)ceilf((r + B.eps) / B.bsz) + 1;
-  auto visit = [&](bool valid, int j) {
-    float d2 = 0.0f;
-    if (valid) {
+  const int s = (int)ceilf((r + B.eps) / B.bsz) + 1;
+  if (B.periodic && 2 * s + 1 > nb) {
+    for (int64_t j = 0; j < np; ++j) {
       const float4 p = B.pos[j];
-      d2 = dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic);
+      const float d = dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic);
+      if (d <= r2) f((int)j, d);
     }
-    f(valid && d2 <= r2, j, d2);
-  };
-  if (B.periodic && 2 * R + 1 > nb) {
-    for (int64_t c0 = 0; c0 < np; c0 += 64) visit(c0 + lane < np, (int)(c0 + lane));
     return;
   }
-  const int side = 2 * R + 1, cube = side * side * side;
-  for (int base = 0; base < cube; base += 64) {
-    const int fi = base + lane;
-    const int cx = bx + fi % side - R, cy = by + (fi / side) % side - R, cz = bz + fi / (side * side) - R;
-    bool v = fi < cube;
-    if (!B.periodic) v = v && cx >= 0 && cx < nb && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
-    if (v) {
-      const float ddx = B.box_d(qx, cx), ddy = B.box_d(qy, cy), ddz = B.box_d(qz, cz);
-      v = !(ddx * ddx + ddy * ddy + ddz * ddz > r2 * 1.0001f + 1e-6f);
+  for (int dz = -s; dz <= s; ++dz) {
+    const int cz = bz + dz;
+    if (!B.periodic && (cz < 0 || cz >= nb)) continue;
+    const float ddz = B.box_d(qz, cz);
+    if (ddz > r) continue;
+    const int wz = B.periodic ? B.wrap(cz) : cz;
+    for (int dy = -s; dy <= s; ++dy) {
+      const int cy = by + dy;
+      if (!B.periodic && (cy < 0 || cy >= nb)) continue;
+      const float ddy = B.box_d(qy, cy);
+      if (ddy > r) continue;
+      const int wy = B.periodic ? B.wrap(cy) : cy;
+      for (int dxo = -s; dxo <= s; ++dxo) {
+        const int cx = bx + dxo;
+        if (!B.periodic && (cx < 0 || cx >= nb)) continue;
+        const float ddx = B.box_d(qx, cx);
+        if (ddx * ddx + ddy * ddy + ddz * ddz > r2 * 1.0001f + 1e-6f) continue;
+        const int wx = B.periodic ? B.wrap(cx) : cx;
+        const int bin = wx + nb * (wy + nb * wz);
+        const int e = B.start[bin + 1];
+        for (int j = B.start[bin]; j < e; ++j) {
+          const float4 p = B.pos[j];
+          const float d = dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic);
+          if (d <= r2) f(j, d);
+        }
+      }
     }
-    int j0 = 0, cnt = 0;
-    if (v) {
-      const int bin = (B.periodic ? B.wrap(cx) : cx) + nb * ((B.periodic ? B.wrap(cy) : cy) +
-                                                           nb * (B.periodic ? B.wrap(cz) : cz));
-      j0 = B.start[bin];
-      cnt = B.start[bin + 1] - j0;
-    }
-    wave_candidates(lane, j0, cnt, visit);
   }
 }
 
 // ----------------------------------------------------------------- kernels
-__global__ void k_convert_check(int64_t np, float* vx, float* vy, float* vz, float* m,
-                                const float* x, const float* y, const float* z, float half,
-                                int* bad, unsigned* mmax) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// unit conversion (reader.f90:457-461), the box check (avism.f90:573-575) and
+// the largest |mass| (for the DDENS fixed point); grid-stride with one
+// atomic per block, since same-address atomics serialise
+__global__ void __launch_bounds__(256) k_convert_check(int64_t np, float* vx, float* vy, float* vz, float* m,
+                                                      const float* x, const float* y, const float* z, float half,
+                                                      int* bad, unsigned* mmax) {
   unsigned mb = 0;
-  if (i < np) {
-    float mi = m[i] / kUM;  // reader.f90:457-461
+  bool out = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+    const float mi = m[i] / kUM;
     m[i] = mi;
     vx[i] = vx[i] / kUV;
     vy[i] = vy[i] / kUV;
     vz[i] = vz[i] / kUV;
-    float a = x[i], b = y[i], c = z[i];
-    if (a < -half || a > half || b < -half || b > half || c < -half || c > half) atomicOr(bad, 1);
-    mb = __float_as_uint(fabsf(mi));  // non-negative floats order as their bits
+    const float a = x[i], b = y[i], c = z[i];
+    out = out || (a < -half || a > half || b < -half || b > half || c < -half || c > half);
+    mb = max(mb, __float_as_uint(fabsf(mi)));  // non-negative floats order as their bits
   }
-  // one atomic per wave: a per-thread atomic on one address serialises
   for (int o = 32; o > 0; o >>= 1) mb = max(mb, (unsigned)__shfl_xor((int)mb, o));
-  if ((threadIdx.x & 63) == 0 && mb) atomicMax(mmax, mb);
+  __shared__ unsigned wmax[4];
+  __shared__ int wbad[4];
+  const bool anyout = __ballot(out) != 0;
+  if ((threadIdx.x & 63) == 0) {
+    wmax[threadIdx.x >> 6] = mb;
+    wbad[threadIdx.x >> 6] = anyout ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned bm = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    if (bm) atomicMax(mmax, bm);
+    if (wbad[0] | wbad[1] | wbad[2] | wbad[3]) atomicOr(bad, 1);
+  }
 }
 
 __global__ void k_fill_u64(int64_t n, unsigned long long* a, unsigned long long v) {
@@ -411,106 +455,88 @@ __global__ void k_bin_keys(int64_t np, const float* x, const float* y, const flo
 
 __global__ void k_gather_sorted(int64_t np, const uint32_t* perm, const float* x, const float* y,
                                 const float* z, const float* m, const float* vx, const float* vy,
-                                const float* vz, float4* pos, float4* vel, uint32_t* orig, uint32_t* inv) {
+                                const float* vz, float4* pos, float4* vel, uint32_t* orig) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   uint32_t p = perm[i];
   pos[i] = make_float4(x[p], y[p], z[p], m[p]);
   vel[i] = make_float4(vx[p], vy[p], vz[p], 0.0f);
   orig[i] = p;
-  inv[p] = (uint32_t)i;
 }
 
 // PPART_DENS (particles.f90:742-802): rho_p = (FP64 sum of the k nearest
 // masses, self included) / (4/3 pi d_k^3); d_k is kept for FILL_H_ZEROS.
-// One wave per particle, QPW consecutive (bin-sorted) particles per wave.
-constexpr int QPW = 4;
+// One particle per lane, in bin-sorted order (a wave's queries are neighbours).
+constexpr int KQ = 128;  // threads per block of the kNN kernels
 
-__global__ void __launch_bounds__(256) k_ppart(Bins B, int64_t np, int k, const float* __restrict__ morig,
-                                              float* pdens, float* hk) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
-  for (int q = 0; q < QPW; ++q) {
-    const int64_t i = wave * QPW + q;
-    if (i >= np) return;
-    const float4 pq = B.pos[i];
-    WaveSel S;
-    S.init(k, lane);
-    wave_knn(B, np, pq.x, pq.y, pq.z, S);
-    const float dk = sqrtf(key_d2(S.kth()));  // kdtree.f90:539
-    const double m = lane < k ? (double)morig[(uint32_t)S.best] : 0.0;
-    const double bas8 = wave_sum(m);
-    if (lane == 0) {
-      hk[i] = dk;
-      const double vol = (4.0 / 3.0 * kPi) * (double)((dk * dk) * dk);
-      pdens[i] = (float)(bas8 / vol);  // particles.f90:790-794
-    }
-  }
+template <int K>
+__global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float* pdens, float* hk) {
+  __shared__ float hd_s[K * KQ];
+  __shared__ uint32_t hj_s[K * KQ];
+  const int64_t i = (int64_t)blockIdx.x * KQ + threadIdx.x;
+  if (i >= np) return;
+  const float4 pq = B.pos[i];
+  Heap<KQ> h;
+  h.init(hd_s + threadIdx.x, hj_s + threadIdx.x, B.orig, k);
+  knn_global(B, np, pq.x, pq.y, pq.z, h);
+  const float dk = sqrtf(h.wd);  // kdtree.f90:539
+  double bas8 = 0.0;
+  for (int t = 0; t < h.cnt; ++t) bas8 += (double)B.pos[h.pos(t)].w;
+  hk[i] = dk;
+  const double vol = (4.0 / 3.0 * kPi) * (double)((dk * dk) * dk);
+  pdens[i] = (float)(bas8 / vol);  // particles.f90:790-794
 }
 
 // VVEL_INTERP_SPH_VW (particles.f90:806-930): per cell the k nearest (or,
 // where d_k <= DX, every particle within DX), HPART = max distance at which a
 // particle served a cell (atomic max on the float bits), and the FP64
-// weighted velocity average.  One wave per cell, QPW cells along x per wave.
-__global__ void __launch_bounds__(256) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad, int rlo,
-                                             float dx, int k, int want_vel, const float4* __restrict__ vel,
-                                             const float* __restrict__ pdens, const uint32_t* __restrict__ inv,
-                                             const int8_t* __restrict__ smask, uint32_t* hbits, float* u,
-                                             float* v, float* w) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
-  const int64_t ncell = (int64_t)n * n * n;
-  for (int q = 0; q < QPW; ++q) {
-    const int64_t c = wave * QPW + q;
-    if (c >= ncell) return;
-    if (smask && smask[c] == 0) continue;
-    const int ix = (int)(c % n), iy = (int)((c / n) % n), iz = (int)(c / ((int64_t)n * n));
-    const float qx = rad[ix + 1 - rlo], qy = rad[iy + 1 - rlo], qz = rad[iz + 1 - rlo];
-    WaveSel S;
-    S.init(k, lane);
-    wave_knn(B, np, qx, qy, qz, S);
-    const float dk = sqrtf(key_d2(S.kth()));
-    double sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-    auto serve = [&](bool valid, uint32_t j, float d, float hh) {
-      if (!valid) return;
-      atomicMax(&hbits[j], __float_as_uint(d));  // particles.f90:882-885
-      if (want_vel) {
-        const float qq = d / (hh / 2.0f);                        // particles.f90:941
-        const float wgt = m4(qq) * (B.pos[j].w / pdens[j]);      // 897-900
-        const float4 vv = vel[j];
-        const double w8 = (double)wgt;
-        sw += w8;
-        sx += w8 * (double)vv.x;
-        sy += w8 * (double)vv.y;
-        sz += w8 * (double)vv.z;
-      }
-    };
-    if (dk > dx) {  // k neighbours (particles.f90:866-867)
-      const bool mine = lane < k;
-      serve(mine, mine ? inv[(uint32_t)S.best] : 0u, sqrtf(key_d2(S.best)), dk);
-    } else {        // ball of radius DX (particles.f90:868-878)
-      const float r2 = dx * dx;
-      float maxd2 = 0.0f;
-      wave_ball(B, np, lane, qx, qy, qz, dx, r2, [&](bool ok, int j, float d2) {
-        if (ok) maxd2 = fmaxf(maxd2, d2);
-      });
-      for (int o = 32; o > 0; o >>= 1) maxd2 = fmaxf(maxd2, __shfl_xor(maxd2, o));
-      const float hh = sqrtf(maxd2);
-      wave_ball(B, np, lane, qx, qy, qz, dx, r2, [&](bool ok, int j, float d2) {
-        serve(ok, (uint32_t)j, sqrtf(d2), hh);
-      });
-    }
+// weighted velocity average.  One cell per lane; a block covers 8 x 4 x 4 cells.
+template <int K>
+__global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad, int rlo,
+                                            float dx, int k, int want_vel, const float4* __restrict__ vel,
+                                            const float* __restrict__ pdens, const int8_t* __restrict__ smask,
+                                            uint32_t* hbits, float* u, float* v, float* w) {
+  __shared__ float hd_s[K * KQ];
+  __shared__ uint32_t hj_s[K * KQ];
+  const int ntx = (n + 7) / 8, nty = (n + 3) / 4;
+  const int tid = threadIdx.x;
+  const int ix = (blockIdx.x % ntx) * 8 + tid % 8, iy = ((blockIdx.x / ntx) % nty) * 4 + (tid / 8) % 4,
+            iz = (blockIdx.x / (ntx * nty)) * 4 + tid / 32;
+  if (ix >= n || iy >= n || iz >= n) return;
+  const size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
+  if (smask && smask[c] == 0) return;
+  const float qx = rad[ix + 1 - rlo], qy = rad[iy + 1 - rlo], qz = rad[iz + 1 - rlo];
+  Heap<KQ> h;
+  h.init(hd_s + tid, hj_s + tid, B.orig, k);
+  knn_global(B, np, qx, qy, qz, h);
+  const float dk = sqrtf(h.wd);
+  double sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+  auto serve = [&](uint32_t j, float d, float hh) {
+    atomicMax(&hbits[j], __float_as_uint(d));  // particles.f90:882-885
     if (want_vel) {
-      sw = wave_sum(sw);
-      sx = wave_sum(sx);
-      sy = wave_sum(sy);
-      sz = wave_sum(sz);
-      if (lane == 0) {
-        u[c] = (float)(sx / sw);
-        v[c] = (float)(sy / sw);
-        w[c] = (float)(sz / sw);
-      }
+      const float qq = d / (hh / 2.0f);                     // particles.f90:941
+      const float wgt = m4(qq) * (B.pos[j].w / pdens[j]);   // 897-900
+      const float4 vv = vel[j];
+      const double w8 = (double)wgt;
+      sw += w8;
+      sx += w8 * (double)vv.x;
+      sy += w8 * (double)vv.y;
+      sz += w8 * (double)vv.z;
     }
+  };
+  if (dk > dx) {  // k neighbours (particles.f90:866-867)
+    for (int t = 0; t < h.cnt; ++t) serve(h.pos(t), sqrtf(h.dist(t)), dk);
+  } else {        // ball of radius DX (particles.f90:868-878): every particle with d <= DX
+    const float r2 = dx * dx;
+    float maxd2 = 0.0f;
+    ball_visit(B, np, qx, qy, qz, dx, r2, [&](int j, float d2) { maxd2 = fmaxf(maxd2, d2); });
+    const float hh = sqrtf(maxd2);
+    ball_visit(B, np, qx, qy, qz, dx, r2, [&](int j, float d2) { serve((uint32_t)j, sqrtf(d2), hh); });
+  }
+  if (want_vel) {
+    u[c] = (float)(sx / sw);
+    v[c] = (float)(sy / sw);
+    w[c] = (float)(sz / sw);
   }
 }
 
@@ -759,7 +785,7 @@ int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, con
   c->fields_ready = false;
   int* bad = (int*)c->scalars.p;  // [0] out-of-box flag, [1] max |mass| bits
   AV_CHECK_HIP(c, hipMemsetAsync(bad, 0, 8, c->stream));
-  k_convert_check<<<nblk(np), 256, 0, c->stream>>>(np, c->pvx.as<float>(), c->pvy.as<float>(),
+  k_convert_check<<<(unsigned)std::min<int64_t>(nblk(np), 2048), 256, 0, c->stream>>>(np, c->pvx.as<float>(), c->pvy.as<float>(),
                                                    c->pvz.as<float>(), c->pm.as<float>(),
                                                    c->px.as<float>(), c->py.as<float>(),
                                                    c->pz.as<float>(), c->g.L0 / 2.0f, bad,
@@ -784,7 +810,7 @@ static int build_bins(avism_ctx* c) {
   if (!c->bin_start.ensure((size_t)(nbins + 1) * 4) || !c->sbin_key.ensure((size_t)np * 4) ||
       !c->sbin_val.ensure((size_t)np * 4) || !c->sbin_key2.ensure((size_t)np * 4) ||
       !c->sbin_val2.ensure((size_t)np * 4) || !c->spos4.ensure((size_t)np * 16) ||
-      !c->svel4.ensure((size_t)np * 16) || !c->sorig.ensure((size_t)np * 4) || !c->sinv.ensure((size_t)np * 4) ||
+      !c->svel4.ensure((size_t)np * 16) || !c->sorig.ensure((size_t)np * 4) ||
       !c->bin_reach.ensure((size_t)nbins * 4))
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (bins)");
   int32_t* count = c->bin_reach.as<int32_t>();  // reused as the histogram
@@ -820,7 +846,7 @@ static int build_bins(avism_ctx* c) {
   k_gather_sorted<<<nblk(np), 256, 0, c->stream>>>(
       np, c->sbin_val2.as<uint32_t>(), c->px.as<float>(), c->py.as<float>(), c->pz.as<float>(),
       c->pm.as<float>(), c->pvx.as<float>(), c->pvy.as<float>(), c->pvz.as<float>(),
-      c->spos4.as<float4>(), c->svel4.as<float4>(), c->sorig.as<uint32_t>(), c->sinv.as<uint32_t>());
+      c->spos4.as<float4>(), c->svel4.as<float4>(), c->sorig.as<uint32_t>());
   AV_CHECK_HIP(c, hipGetLastError());
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   return AVISM_OK;
@@ -851,17 +877,24 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   Bins B = make_bins(c);
   {
     StageTimer t(c, "ppart_dens");
-    k_ppart<<<nblk(np, 4 * QPW), 256, 0, c->stream>>>(B, np, k, c->pm.as<float>(), c->pdens.as<float>(),
-                                                     c->hk.as<float>());
+    if (k <= 32)
+      k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, c->pdens.as<float>(), c->hk.as<float>());
+    else
+      k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, c->pdens.as<float>(), c->hk.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   AV_CHECK_HIP(c, hipMemsetAsync(c->hpart.p, 0, (size_t)np * 4, c->stream));
   {
     StageTimer t(c, "vvel_sph_vw");
-    k_vvel<<<nblk(ncell, 4 * QPW), 256, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1,
-                                                        c->svel4.as<float4>(), c->pdens.as<float>(),
-                                                        c->sinv.as<uint32_t>(), nullptr, c->hpart.as<uint32_t>(),
-                                                        c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
+    const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
+    if (k <= 32)
+      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1, c->svel4.as<float4>(),
+                                               c->pdens.as<float>(), nullptr, c->hpart.as<uint32_t>(),
+                                               c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
+    else
+      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1, c->svel4.as<float4>(),
+                                               c->pdens.as<float>(), nullptr, c->hpart.as<uint32_t>(),
+                                               c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   {

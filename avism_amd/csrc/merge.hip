@@ -494,11 +494,6 @@ __device__ __forceinline__ int cc_find(const int32_t* lab, int x) {
   return x;
 }
 
-__global__ void k_cc_init(int64_t ne3, const int32_t* __restrict__ marca, int32_t* lab) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < ne3) lab[e] = marca[e] == 0 ? (int32_t)e : -1;
-}
-
 __device__ void cc_union(int32_t* lab, int a, int b) {
   for (;;) {
     a = cc_find(lab, a);
@@ -511,15 +506,75 @@ __device__ void cc_union(int32_t* lab, int a, int b) {
   }
 }
 
-__global__ void k_cc_merge(int64_t ne3, ExtG G, int32_t* lab) {
+// block-local components first: an 8^3 block labels its empty cells in LDS
+// (label = the component's smallest cell, which is also its smallest global
+// index), so the global union-find only joins components across block faces
+constexpr int CCB = 8;
+
+__global__ void __launch_bounds__(256) k_cc_local(ExtG G, const int32_t* __restrict__ marca, int32_t* lab) {
+  __shared__ int16_t L[CCB * CCB * CCB];
+  __shared__ int changed;
+  const int nbx = (G.ne + CCB - 1) / CCB;
+  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nbx, bz = blockIdx.x / (nbx * nbx);
+  const int tid = threadIdx.x;
+  int64_t eg[2];
+  for (int r = 0; r < 2; ++r) {
+    const int c = tid + 256 * r;
+    const int gx = bx * CCB + c % CCB, gy = by * CCB + (c / CCB) % CCB, gz = bz * CCB + c / (CCB * CCB);
+    const bool in = gx < G.ne && gy < G.ne && gz < G.ne;
+    eg[r] = in ? (int64_t)gx + (int64_t)G.ne * ((int64_t)gy + (int64_t)G.ne * gz) : -1;
+    L[c] = (in && marca[eg[r]] == 0) ? (int16_t)c : (int16_t)-1;
+  }
+  for (;;) {
+    if (tid == 0) changed = 0;
+    __syncthreads();
+    bool ch = false;
+    for (int r = 0; r < 2; ++r) {
+      const int c = tid + 256 * r;
+      int m = L[c];
+      if (m < 0) continue;
+      const int x = c % CCB, y = (c / CCB) % CCB, z = c / (CCB * CCB);
+      int t;
+      if (x > 0 && (t = L[c - 1]) >= 0) m = min(m, t);
+      if (x < CCB - 1 && (t = L[c + 1]) >= 0) m = min(m, t);
+      if (y > 0 && (t = L[c - CCB]) >= 0) m = min(m, t);
+      if (y < CCB - 1 && (t = L[c + CCB]) >= 0) m = min(m, t);
+      if (z > 0 && (t = L[c - CCB * CCB]) >= 0) m = min(m, t);
+      if (z < CCB - 1 && (t = L[c + CCB * CCB]) >= 0) m = min(m, t);
+      m = min(m, (int)L[m]);  // jump along the label chain
+      if (m < L[c]) {
+        L[c] = (int16_t)m;
+        ch = true;
+      }
+    }
+    if (ch) changed = 1;
+    __syncthreads();
+    const int any = changed;
+    __syncthreads();
+    if (!any) break;
+  }
+  for (int r = 0; r < 2; ++r) {
+    const int c = tid + 256 * r;
+    if (eg[r] < 0) continue;
+    const int m = L[c];
+    if (m < 0) {
+      lab[eg[r]] = -1;
+    } else {
+      const int gx = bx * CCB + m % CCB, gy = by * CCB + (m / CCB) % CCB, gz = bz * CCB + m / (CCB * CCB);
+      lab[eg[r]] = (int32_t)((int64_t)gx + (int64_t)G.ne * ((int64_t)gy + (int64_t)G.ne * gz));
+    }
+  }
+}
+
+// joins across the low faces of the blocks
+__global__ void k_cc_faces(int64_t ne3, ExtG G, int32_t* lab) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne3 || lab[e] < 0) return;
-  int i = (int)(e % G.ne);
-  int j = (int)((e / G.ne) % G.ne);
-  int k = (int)(e / ((int64_t)G.ne * G.ne));
-  if (i > 0 && lab[e - 1] >= 0) cc_union(lab, (int)e, (int)(e - 1));
-  if (j > 0 && lab[e - G.ne] >= 0) cc_union(lab, (int)e, (int)(e - G.ne));
-  if (k > 0 && lab[e - (int64_t)G.ne * G.ne] >= 0) cc_union(lab, (int)e, (int)(e - (int64_t)G.ne * G.ne));
+  const int i = (int)(e % G.ne), j = (int)((e / G.ne) % G.ne), k = (int)(e / ((int64_t)G.ne * G.ne));
+  if (i > 0 && i % CCB == 0 && lab[e - 1] >= 0) cc_union(lab, (int)e, (int)(e - 1));
+  if (j > 0 && j % CCB == 0 && lab[e - G.ne] >= 0) cc_union(lab, (int)e, (int)(e - G.ne));
+  if (k > 0 && k % CCB == 0 && lab[e - (int64_t)G.ne * G.ne] >= 0)
+    cc_union(lab, (int)e, (int)(e - (int64_t)G.ne * G.ne));
 }
 
 __global__ void k_cc_flatten(int64_t ne3, int32_t* lab) {
@@ -586,16 +641,50 @@ __device__ __forceinline__ int resolve(const int32_t* lmap, int l) {
   return l;
 }
 
-// one workgroup walks the main voids in INDICE order
+// the part of VOID_PERIODIC's per-void test that no earlier void can change
+// (voidfinding.f90:1440-1480): a main void whose centre lies outside the box
+// and whose image cell, inside the ext grid, belongs to a void.  flag[I] for
+// INDICE rank I.
+__global__ void k_vp_flag(int nv, ExtG G, const int32_t* __restrict__ indice, const int32_t* __restrict__ uvoid,
+                          const int32_t* __restrict__ marca_base, const float* __restrict__ gxc,
+                          const float* __restrict__ gyc, const float* __restrict__ gzc, float half, float dx,
+                          float rx1, int32_t* flag) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nv) return;
+  const int id = indice[I];
+  int f = 0;
+  if (uvoid[id - 1] == -1) {
+    const float xc1 = gxc[id - 1], yc1 = gyc[id - 1], zc1 = gzc[id - 1];
+    const int sx = xc1 < -half ? 1 : (xc1 > half ? -1 : 0);
+    const int sy = yc1 < -half ? 1 : (yc1 > half ? -1 : 0);
+    const int sz = zc1 < -half ? 1 : (zc1 > half ? -1 : 0);
+    if (sx | sy | sz) {
+      const int ax = (int)((xc1 - rx1) / dx) + 1 + sx * G.n, ay = (int)((yc1 - rx1) / dx) + 1 + sy * G.n,
+                az = (int)((zc1 - rx1) / dx) + 1 + sz * G.n;
+      if (ax >= G.low1 && ax <= G.low2 && ay >= G.low1 && ay <= G.low2 && az >= G.low1 && az <= G.low2)
+        f = marca_base[G.ext(ax, ay, az)] > 0;
+    }
+  }
+  flag[I] = f;
+}
+
+__global__ void k_vp_list(int nv, const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int32_t* list) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I < nv && flag[I]) list[pos[I]] = I;
+}
+
+// one workgroup walks the flagged main voids in INDICE order
 __global__ void __launch_bounds__(1024) k_void_periodic(
-    int nv, ExtG G, const int32_t* __restrict__ indice, const int32_t* __restrict__ uvoid,
+    const int32_t* __restrict__ list, const int32_t* __restrict__ nlist, ExtG G, const int32_t* __restrict__ indice, const int32_t* __restrict__ uvoid,
     const int32_t* __restrict__ marca_base, const float* __restrict__ gxc, const float* __restrict__ gyc,
     const float* __restrict__ gzc, float half, float dx, float rx1, const uint32_t* __restrict__ cells,
     const int32_t* __restrict__ lstart, const int32_t* __restrict__ lend, int32_t* lmap, int32_t* mnext,
     int32_t* mtail, int32_t* selfm, int32_t* stats) {
   __shared__ int s_cnt, s_cnt2, s_go, s_ind, s_ind2, s_sx, s_sy, s_sz;
   const int tid = threadIdx.x;
-  for (int I = 0; I < nv; ++I) {
+  const int nl = *nlist;
+  for (int L = 0; L < nl; ++L) {
+    const int I = list[L];
     if (tid == 0) {
       s_go = 0;
       int id = indice[I];
@@ -893,8 +982,9 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
   {
     StageTimer t(c, "hole_fill");
     int32_t* lab = P;  // reuse
-    k_cc_init<<<nbk(ne3), 256, 0, c->stream>>>(ne3, marca, lab);
-    k_cc_merge<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, lab);
+    const int nbc = (G.ne + CCB - 1) / CCB;
+    k_cc_local<<<(unsigned)((int64_t)nbc * nbc * nbc), 256, 0, c->stream>>>(G, marca, lab);
+    k_cc_faces<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, lab);
     k_cc_flatten<<<nbk(ne3), 256, 0, c->stream>>>(ne3, lab);
     AV_CHECK_HIP(c, hipMemsetAsync(small + 2, 0, 4, c->stream));
     k_fill_holes<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, lab, marca, small + 2);
@@ -934,8 +1024,23 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     int32_t* selfm = removed;
     k_init_lmap<<<nbk(nv), 256, 0, c->stream>>>(nv, lmap, mnext, mtail, selfm);
     AV_CHECK_HIP(c, hipMemsetAsync(small + 3, 0, 4, c->stream));
-    k_void_periodic<<<1, 1024, 0, c->stream>>>(nv, G, indice, uvoid, marca, GX, GY, GZ, g.L0 / 2.0f, g.dx, g.rx1,
-                                                vb, lstart, lend, lmap, mnext, mtail, selfm, small + 3);
+    // statically possible images, compacted in INDICE order (the cell lists
+    // vb are no longer needed as sort input, ka/va are free)
+    int32_t* vflag = (int32_t*)ka;
+    int32_t* vpos = (int32_t*)va;
+    int32_t* vlist = vflag + (nv + 2);
+    k_vp_flag<<<nbk(nv), 256, 0, c->stream>>>(nv, G, indice, uvoid, marca, GX, GY, GZ, g.L0 / 2.0f, g.dx, g.rx1,
+                                              vflag);
+    AV_CHECK_HIP(c, hipMemsetAsync(vflag + nv, 0, 4, c->stream));
+    size_t tmp2 = 0;
+    AV_CHECK_HIP(c, rocprim::exclusive_scan(nullptr, tmp2, vflag, vpos, 0, (size_t)nv + 1,
+                                            rocprim::plus<int32_t>(), c->stream));
+    if (!c->sort_tmp.ensure(std::max(tmp, tmp2))) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (periodic)");
+    AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmp2, vflag, vpos, 0, (size_t)nv + 1,
+                                            rocprim::plus<int32_t>(), c->stream));
+    k_vp_list<<<nbk(nv), 256, 0, c->stream>>>(nv, vflag, vpos, vlist);
+    k_void_periodic<<<1, 1024, 0, c->stream>>>(vlist, vpos + nv, G, indice, uvoid, marca, GX, GY, GZ, g.L0 / 2.0f,
+                                                g.dx, g.rx1, vb, lstart, lend, lmap, mnext, mtail, selfm, small + 3);
     k_apply_lmap<<<nbk(nv), 256, 0, c->stream>>>(nv, lmap, selfm, uvoid, VOLNEW);
     k_relabel_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, lmap, marca);
     AV_CHECK_HIP(c, hipGetLastError());

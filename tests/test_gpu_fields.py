@@ -137,3 +137,30 @@ def test_fields_lattice_ties(per):
     for c in ("u", "v", "w"):
         assert _rms_rel(f[c], ref[c]) <= REL, c
     assert _rel(f["u1co"], ref["u1co"]) <= REL
+
+
+@pytest.mark.parametrize("per", [0, 1])
+def test_fields_clustered(per):
+    """Dense blobs next to empty space: the blob tiles overflow the LDS
+    staging (HBM fallback path), blob cells take the DX ball path (d_k <= DX)
+    and the empty regions need wide searches."""
+    import pyoracle as O
+    n, L0 = 40, 80.0
+    rng = np.random.default_rng(55 + per)
+    nbg, nblob = 30000, 40000
+    bg = rng.uniform(-L0 / 2, L0 / 2, (nbg, 3))
+    centres = rng.uniform(-L0 / 3, L0 / 3, (4, 3))
+    blob = centres[rng.integers(0, 4, nblob)] + rng.standard_normal((nblob, 3)) * 1.2
+    xyz = np.clip(np.concatenate([bg, blob]), -L0 / 2, L0 / 2 - 1e-4).astype(np.float32)
+    npart = xyz.shape[0]
+    arr = [xyz[:, 0].copy(), xyz[:, 1].copy(), xyz[:, 2].copy()]
+    arr += [rng.standard_normal(npart).astype(np.float32) * 200 for _ in range(3)]
+    arr += [np.full(npart, 2e11, np.float32)]
+    print(f"[fields clustered periodic={per}] np={npart}", flush=True)
+    ref = O.fields_from_particles(*arr, n, L0, per, k=32)
+    _, f, pd, hp = _gpu_fields(arr, n, L0, per, 32)
+    assert np.array_equal(hp, ref["hpart"])
+    assert _rel(pd, ref["part_dens"]) <= REL
+    for c in ("u", "v", "w"):
+        assert _rms_rel(f[c], ref[c]) <= REL, c
+    assert _rel(f["u1co"], ref["u1co"]) <= REL
