@@ -67,6 +67,14 @@ def lib():
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m avism_amd.build` "
                            "(there is no CPU fallback)")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 (same
+    # soname as ROCm's).  Importing torch first lets the library bind to that
+    # copy; loading ROCm's copy first would leave torch with a second runtime
+    # that finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
     sig = {

@@ -22,6 +22,15 @@ using namespace avism;
   (ctx)->err.clear();                       \
   if (hipSetDevice((ctx)->device) != hipSuccess) { (ctx)->err = "hipSetDevice failed"; return AVISM_ERR_HIP; }
 
+// Device pointers handed in may still be written by work the caller queued on
+// other streams (the library's stream does not wait on them): wait for the
+// whole device before reading or writing them.
+#define DEVICE_ARGS(ctx, where)                                           \
+  if ((where) == AVISM_DEVICE && hipDeviceSynchronize() != hipSuccess) { \
+    (ctx)->err = "hipDeviceSynchronize failed";                          \
+    return AVISM_ERR_HIP;                                                \
+  }
+
 extern "C" {
 
 int avism_create(avism_ctx** out, int32_t device) {
@@ -143,6 +152,7 @@ int avism_set_void_params(avism_ctx* c, const avism_void_params* p) {
 
 int avism_set_parent_map(avism_ctx* c, const int32_t* marcap, int32_t where) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
   if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "no grid set");
   if (!marcap) {
     c->has_marcap = false;
@@ -160,6 +170,7 @@ int avism_load_particles(avism_ctx* c, int64_t np, const float* x, const float* 
                          const float* vx, const float* vy, const float* vz, const float* m,
                          int32_t where) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
   if (!x || !y || !z || !vx || !vy || !vz || !m) AV_FAIL(c, AVISM_ERR_ARG, "null particle array");
   return load_particles(c, np, x, y, z, vx, vy, vz, m, where);
 }
@@ -172,6 +183,7 @@ int avism_particles_to_grid(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t
 int avism_load_grid_fields(avism_ctx* c, const float* delta, const float* vx, const float* vy,
                            const float* vz, int32_t where) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
   if (!delta || !vx || !vy || !vz) AV_FAIL(c, AVISM_ERR_ARG, "null field array");
   return fields_from_grid(c, delta, vx, vy, vz, where);
 }
@@ -179,6 +191,7 @@ int avism_load_grid_fields(avism_ctx* c, const float* delta, const float* vx, co
 int avism_load_void_fields(avism_ctx* c, const float* u1co, const float* div, float meandens,
                            int32_t where) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
   if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "avism_set_grid must be called first");
   if (!u1co || !div) AV_FAIL(c, AVISM_ERR_ARG, "null field array");
   size_t b = (size_t)c->g.n * c->g.n * c->g.n * 4;
@@ -199,6 +212,7 @@ int avism_load_void_fields(avism_ctx* c, const float* u1co, const float* div, fl
 int avism_get_fields(avism_ctx* c, float* u1co, float* div, float* u, float* v, float* w,
                      int32_t where) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
   if (!c->fields_ready) AV_FAIL(c, AVISM_ERR_STATE, "fields not computed");
   size_t b = (size_t)c->g.n * c->g.n * c->g.n * 4;
   hipMemcpyKind kind = where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
@@ -224,6 +238,7 @@ int avism_find_voids(avism_ctx* c, int32_t growth_mode, int32_t* ncubes) {
 
 int avism_set_proposals(avism_ctx* c, const int8_t* r_hat, int64_t n, int32_t where) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
   return set_proposals(c, r_hat, n, where);
 }
 
@@ -239,6 +254,7 @@ int avism_get_cubes(avism_ctx* c, int32_t* ci, float* cf, int32_t* father) {
 
 int avism_indexx(avism_ctx* c, const float* keys, int64_t n, int32_t* indx, int32_t where) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
   if (n < 0) AV_FAIL(c, AVISM_ERR_ARG, "negative length");
   if (n == 0) return AVISM_OK;
   if (n >= (int64_t)0x7fffffff) AV_FAIL(c, AVISM_ERR_LIMIT, "INDEXX beyond 2^31-1 keys");
@@ -275,11 +291,13 @@ int avism_get_catalogue(avism_ctx* c, int32_t* indice, int32_t* uvoid, int32_t* 
 
 int avism_get_marca(avism_ctx* c, int32_t* marca, int32_t inner_only, int32_t where) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
   return get_marca(c, marca, inner_only, where);
 }
 
 int avism_cnn_set_params(avism_ctx* c, const float* params, int32_t where) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
   if (!params) AV_FAIL(c, AVISM_ERR_ARG, "null parameter array");
   const size_t b = (size_t)AVISM_CNN_NPARAM * 4;
   if (!c->cnn_params.ensure(b)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
@@ -291,6 +309,7 @@ int avism_cnn_set_params(avism_ctx* c, const float* params, int32_t where) {
 
 int avism_cnn_patches(avism_ctx* c, const int64_t* ranks, int64_t n, uint16_t* patches) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, AVISM_DEVICE);
   if (n < 0 || (n > 0 && (!ranks || !patches))) AV_FAIL(c, AVISM_ERR_ARG, "bad patch request");
   return cnn_patches(c, ranks, n, patches);
 }
@@ -298,6 +317,7 @@ int avism_cnn_patches(avism_ctx* c, const int64_t* ranks, int64_t n, uint16_t* p
 int avism_cnn_forward(avism_ctx* c, const float* params, int64_t n, const uint16_t* patches, float* logits,
                       uint16_t* cache) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, AVISM_DEVICE);
   if (n < 0 || !params || (n > 0 && (!patches || !logits))) AV_FAIL(c, AVISM_ERR_ARG, "bad forward request");
   return cnn_forward(c, params, n, patches, logits, cache);
 }
@@ -306,6 +326,7 @@ int avism_cnn_backward(avism_ctx* c, const float* params, int64_t n, const uint1
                        const uint16_t* cache, const float* logits, const int32_t* labels, float* grads,
                        float* loss) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, AVISM_DEVICE);
   if (n < 0 || !params || !grads || !loss || (n > 0 && (!patches || !cache || !logits || !labels)))
     AV_FAIL(c, AVISM_ERR_ARG, "bad backward request");
   return cnn_backward(c, params, n, patches, cache, logits, labels, grads, loss);
@@ -314,6 +335,7 @@ int avism_cnn_backward(avism_ctx* c, const float* params, int64_t n, const uint1
 int avism_cnn_adam(avism_ctx* c, float* params, const float* grads, float* m, float* v, float lr,
                    int32_t step) {
   CTX_GUARD(c);
+  DEVICE_ARGS(c, AVISM_DEVICE);
   if (!params || !grads || !m || !v || step < 1) AV_FAIL(c, AVISM_ERR_ARG, "bad Adam request");
   return cnn_adam(c, params, grads, m, v, lr, step);
 }

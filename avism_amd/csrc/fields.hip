@@ -25,6 +25,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "avism_internal.h"
+#include "knn_search.h"
 
 namespace avism {
 
@@ -97,27 +98,6 @@ void add_timing(avism_ctx* c, const char* name, hipEvent_t a, hipEvent_t b) {
 }
 
 // ------------------------------------------------------------ device helpers
-__device__ __forceinline__ float axis_d(float p, float q, float L, int periodic) {
-  float d = p - q;
-  if (periodic) {
-    float a = fabsf(d), b = L - fabsf(d);
-    d = a < b ? a : b;
-  }
-  return d;
-}
-
-// kdtree.f90:1441-1465 (squared distance, x, y, z accumulated in order)
-__device__ __forceinline__ float dist2(float px, float py, float pz, float qx, float qy, float qz,
-                                       float L, int periodic) {
-  float dx = axis_d(px, qx, L, periodic);
-  float dy = axis_d(py, qy, L, periodic);
-  float dz = axis_d(pz, qz, L, periodic);
-  float d = dx * dx;
-  d = d + dy * dy;
-  d = d + dz * dz;
-  return d;
-}
-
 // M4 spline (particles.f90:943-950 and 525-533)
 __device__ __forceinline__ float m4(float q) {
   if (q <= 1.0f) return 1.0f - (1.5f * (q * q)) * (1.0f - 0.5f * q);
@@ -137,226 +117,6 @@ struct Kahan {
     s = t;
   }
 };
-
-struct Bins {
-  const float4* __restrict__ pos;      // sorted (x,y,z,m)
-  const uint32_t* __restrict__ orig;   // sorted -> input index
-  const int32_t* __restrict__ start;   // nb^3 + 1
-  int32_t nb;
-  float lo, bsz, L, eps;
-  int periodic;
-  __device__ __forceinline__ int bin_of(float v) const {
-    int b = (int)floorf((v - lo) / bsz);
-    return b < 0 ? 0 : (b >= nb ? nb - 1 : b);
-  }
-  __device__ __forceinline__ int wrap(int b) const {
-    if (b < 0) b += nb;
-    else if (b >= nb) b -= nb;
-    return b;
-  }
-  // squared distance from q to the (unwrapped) bin box at index b, axis-wise
-  __device__ __forceinline__ float box_d(float q, int b) const {
-    float blo = lo + (float)b * bsz - eps, bhi = lo + (float)(b + 1) * bsz + eps;
-    float d = 0.0f;
-    if (q < blo) d = blo - q;
-    else if (q > bhi) d = q - bhi;
-    return d;
-  }
-  // lower bound of the distance from q (in bin b) to anything outside shells 0..s
-  __device__ __forceinline__ float outside(float q, int b, int s) const {
-    float dlo = q - (lo + (float)(b - s) * bsz) - eps;
-    float dhi = (lo + (float)(b + s + 1) * bsz) - q - eps;
-    if (!periodic) {
-      if (b - s <= 0) dlo = INFINITY;
-      if (b + s + 1 >= nb) dhi = INFINITY;
-    }
-    return dlo < dhi ? dlo : dhi;
-  }
-};
-
-// ------------------------------------------------------------------ kNN
-// Lane-per-query search; the k best of a lane are a max-heap in LDS (element
-// t of lane l at [t * BS + l]: every lane hits its own bank).  Entries are
-// (squared distance, bin-sorted position) ordered by distance, then by the
-// particle's input index (looked up only when two distances are equal), so
-// equal distances go to the lower input index: the deterministic form of
-// kdtree.f90's strict-< admission, and the oracle's tie rule.  O(log k) LDS
-// steps per admission.
-template <int BS>
-struct Heap {
-  float* hd;
-  uint32_t* hj;
-  const uint32_t* orig;
-  int k, cnt;
-  float wd;     // root: worst kept distance (INFINITY while filling)
-  uint32_t wj;
-  __device__ __forceinline__ void init(float* d, uint32_t* j, const uint32_t* o, int k_) {
-    hd = d;
-    hj = j;
-    orig = o;
-    k = k_;
-    cnt = 0;
-    wd = INFINITY;
-    wj = 0;
-  }
-  __device__ __forceinline__ bool full() const { return cnt >= k; }
-  __device__ __forceinline__ bool gt(float da, uint32_t ja, float db, uint32_t jb) const {
-    return da > db || (da == db && orig[ja] > orig[jb]);
-  }
-  __device__ __forceinline__ void consider(float v, uint32_t j) {
-    if (cnt < k) {
-      int i = cnt++;
-      while (i > 0) {  // sift up
-        const int p = (i - 1) >> 1;
-        const float pd = hd[p * BS];
-        const uint32_t pj = hj[p * BS];
-        if (!gt(v, j, pd, pj)) break;
-        hd[i * BS] = pd;
-        hj[i * BS] = pj;
-        i = p;
-      }
-      hd[i * BS] = v;
-      hj[i * BS] = j;
-      if (cnt == k) {
-        wd = hd[0];
-        wj = hj[0];
-      }
-      return;
-    }
-    if (v > wd) return;
-    if (v == wd && orig[j] >= orig[wj]) return;
-    int i = 0;  // replace the root, sift down
-    for (;;) {
-      const int l = 2 * i + 1;
-      if (l >= k) break;
-      float cd = hd[l * BS];
-      uint32_t cj = hj[l * BS];
-      int c = l;
-      if (l + 1 < k) {
-        const float rd = hd[(l + 1) * BS];
-        const uint32_t rj = hj[(l + 1) * BS];
-        if (gt(rd, rj, cd, cj)) {
-          c = l + 1;
-          cd = rd;
-          cj = rj;
-        }
-      }
-      if (!gt(cd, cj, v, j)) break;
-      hd[i * BS] = cd;
-      hj[i * BS] = cj;
-      i = c;
-    }
-    hd[i * BS] = v;
-    hj[i * BS] = j;
-    wd = hd[0];
-    wj = hj[0];
-  }
-  __device__ __forceinline__ float dist(int t) const { return hd[t * BS]; }
-  __device__ __forceinline__ uint32_t pos(int t) const { return hj[t * BS]; }
-};
-
-// Shell search shared by both candidate sources.  scan(xa, xb, cy, cz)
-// offers the particles of bins xa..xb (unwrapped, xb - xa < nb) of the row at
-// unwrapped (cy, cz).  Shells of bins around q's bin are visited until the
-// k-th distance is below the distance to everything outside them; within a
-// shell each row is one contiguous range, clipped to the bins whose box can
-// still hold a closer particle.  Returns false if shell smax was reached
-// without an answer.
-template <class H, class Scan>
-__device__ __forceinline__ bool knn_shells(const Bins& B, float qx, float qy, float qz, int smax, H& h,
-                                           Scan&& scan) {
-  const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
-  const int nb = B.nb;
-  const float e2 = 2.0f * B.eps;
-  for (int s = 0; s <= smax; ++s) {
-    bool any = false;
-    for (int dz = -s; dz <= s; ++dz) {
-      const int cz = bz + dz;
-      if (!B.periodic && (cz < 0 || cz >= nb)) continue;
-      const float ddz = B.box_d(qz, cz);
-      for (int dy = -s; dy <= s; ++dy) {
-        const int cy = by + dy;
-        if (!B.periodic && (cy < 0 || cy >= nb)) continue;
-        const float ddy = B.box_d(qy, cy);
-        const bool edge = (dz == -s || dz == s || dy == -s || dy == s);
-        int lo_b = bx - s, hi_b = bx + s;
-        if (!B.periodic) {
-          lo_b = max(lo_b, 0);
-          hi_b = min(hi_b, nb - 1);
-        }
-        if (lo_b > hi_b) continue;
-        any = true;
-        if (h.full()) {  // bins whose box can hold d^2 <= worst
-          const float rem = h.wd - (ddy * ddy + ddz * ddz);
-          if (rem < 0.0f) continue;
-          const float r = sqrtf(rem);
-          lo_b = max(lo_b, (int)floorf((qx - r - e2 - B.lo) / B.bsz));
-          hi_b = min(hi_b, (int)floorf((qx + r + e2 - B.lo) / B.bsz));
-          if (lo_b > hi_b) continue;
-        }
-        if (edge || s == 0) {
-          scan(lo_b, hi_b, cy, cz);
-        } else {  // interior row of the shell: its two end bins
-          if (lo_b == bx - s) scan(lo_b, lo_b, cy, cz);
-          if (hi_b == bx + s) scan(hi_b, hi_b, cy, cz);
-        }
-      }
-    }
-    if (h.full()) {
-      const float b1 = B.outside(qx, bx, s), b2 = B.outside(qy, by, s), b3 = B.outside(qz, bz, s);
-      const float bd = fminf(b1, fminf(b2, b3));
-      if (bd == INFINITY || (bd > 0.0f && h.wd < bd * bd)) return true;
-    }
-    if (!B.periodic && !any && s > 0) {
-      const int far = max(max(max(bx, nb - 1 - bx), max(by, nb - 1 - by)), max(bz, nb - 1 - bz));
-      if (s > far) return true;
-    }
-  }
-  return false;
-}
-
-// exact kNN over the bin-sorted particles in HBM (k <= number of particles)
-template <class H>
-__device__ void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz, H& h) {
-  const int nb = B.nb;
-  auto scan = [&](int xa, int xb, int cy, int cz) {
-    const int wy = B.periodic ? B.wrap(cy) : cy, wz = B.periodic ? B.wrap(cz) : cz;
-    const int rowb = nb * (wy + nb * wz);
-    int seg[3][2];
-    int ns = 0;
-    if (xa < 0) {  // periodic images on the low side: bins xa..min(xb,-1)
-      seg[ns][0] = xa + nb; seg[ns][1] = min(xb, -1) + nb; ++ns;
-      xa = 0;
-    }
-    if (xb >= nb) {  // and on the high side: bins max(xa,nb)..xb
-      seg[ns][0] = max(xa, nb) - nb; seg[ns][1] = xb - nb; ++ns;
-      xb = nb - 1;
-    }
-    if (xa <= xb) { seg[ns][0] = xa; seg[ns][1] = xb; ++ns; }
-    for (int g = 0; g < ns; ++g) {
-      const int j1 = B.start[rowb + seg[g][1] + 1];
-      int j = B.start[rowb + seg[g][0]];
-      for (; j + 1 < j1; j += 2) {  // two loads in flight per lane
-        const float4 p0 = B.pos[j], p1 = B.pos[j + 1];
-        const float d0 = dist2(p0.x, p0.y, p0.z, qx, qy, qz, B.L, B.periodic);
-        const float d1 = dist2(p1.x, p1.y, p1.z, qx, qy, qz, B.L, B.periodic);
-        h.consider(d0, (uint32_t)j);
-        h.consider(d1, (uint32_t)(j + 1));
-      }
-      if (j < j1) {
-        const float4 p = B.pos[j];
-        h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), (uint32_t)j);
-      }
-    }
-  };
-  if (knn_shells(B, qx, qy, qz, B.periodic ? (nb - 1) / 2 : nb, h, scan)) return;
-  // periodic box smaller than the search: every particle once
-  h.init(h.hd, h.hj, h.orig, h.k);
-  for (int64_t j = 0; j < np; ++j) {
-    const float4 p = B.pos[j];
-    h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), (uint32_t)j);
-  }
-}
 
 // every particle with d2 <= r2 (ball_search, kdtree.f90:1015-1072, 1126)
 template <class F>
@@ -469,11 +229,21 @@ __global__ void k_gather_sorted(int64_t np, const uint32_t* perm, const float* x
 // One particle per lane, in bin-sorted order (a wave's queries are neighbours).
 constexpr int KQ = 128;  // threads per block of the kNN kernels
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
+// and each XCD has its own L2.  Renumbering the blocks so that every XCD gets
+// one contiguous range of tiles keeps neighbouring tiles, which read the same
+// bins, on the same L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
+  constexpr int NX = 8;
+  const int64_t q = nblocks / NX, r = nblocks % NX, x = b % NX, y = b / NX;
+  return x * q + (x < r ? x : r) + y;
+}
+
 template <int K>
 __global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float* pdens, float* hk) {
   __shared__ float hd_s[K * KQ];
   __shared__ uint32_t hj_s[K * KQ];
-  const int64_t i = (int64_t)blockIdx.x * KQ + threadIdx.x;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * KQ + threadIdx.x;
   if (i >= np) return;
   const float4 pq = B.pos[i];
   Heap<KQ> h;
@@ -500,8 +270,9 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
   __shared__ uint32_t hj_s[K * KQ];
   const int ntx = (n + 7) / 8, nty = (n + 3) / 4;
   const int tid = threadIdx.x;
-  const int ix = (blockIdx.x % ntx) * 8 + tid % 8, iy = ((blockIdx.x / ntx) % nty) * 4 + (tid / 8) % 4,
-            iz = (blockIdx.x / (ntx * nty)) * 4 + tid / 32;
+  const int64_t bt = xcd_block(blockIdx.x, gridDim.x);
+  const int ix = (int)(bt % ntx) * 8 + tid % 8, iy = (int)((bt / ntx) % nty) * 4 + (tid / 8) % 4,
+            iz = (int)(bt / ((int64_t)ntx * nty)) * 4 + tid / 32;
   if (ix >= n || iy >= n || iz >= n) return;
   const size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
   if (smask && smask[c] == 0) return;
@@ -512,7 +283,11 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
   const float dk = sqrtf(h.wd);
   double sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
   auto serve = [&](uint32_t j, float d, float hh) {
-    atomicMax(&hbits[j], __float_as_uint(d));  // particles.f90:882-885
+    // particles.f90:882-885; HPART only grows, so a value already at least d
+    // (read without the atomic; a stale read is only smaller) makes the
+    // atomic unnecessary
+    const uint32_t db = __float_as_uint(d);
+    if (db > hbits[j]) atomicMax(&hbits[j], db);
     if (want_vel) {
       const float qq = d / (hh / 2.0f);                     // particles.f90:941
       const float wgt = m4(qq) * (B.pos[j].w / pdens[j]);   // 897-900
@@ -590,67 +365,126 @@ __device__ __forceinline__ float ddens_w(const DdensGeom& G, float x, float y, f
 // the sums independent of the deposit order (bit-reproducible), and the fixed
 // point (scale = 2^62 / (np * max mass), so no sum can overflow) resolves
 // 1e-11 of a particle mass at 10^8 particles; the reference adds FP64.
-__device__ __forceinline__ void fix_add(unsigned long long* acc, size_t c, float v, double scale) {
-  atomicAdd(&acc[c], (unsigned long long)llrint((double)v * scale));
-}
+//
+// A block takes the particles binned in one tile of DTX x DTY x DTZ cells and
+// adds their deposits into an LDS copy of the tile widened by DR cells (LDS
+// integer atomics), then adds that copy to HBM once per cell: at one particle
+// per cell a cell receives ~32 deposits but only a handful of tile flushes.
+// A particle whose cell box reaches past the widened tile (large h, in voids)
+// adds directly to HBM.
+constexpr int DTX = 8, DTY = 8, DTZ = 4, DR = 4;
+constexpr int DBX = DTX + 2 * DR, DBY = DTY + 2 * DR, DBZ = DTZ + 2 * DR;
 
-__global__ void __launch_bounds__(256) k_ddens_scatter(int64_t np, DdensGeom G, const float4* __restrict__ pos,
-                                                      const float* __restrict__ hpart, double scale,
-                                                      unsigned long long* acc, float* norm) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  const float4 p = pos[i];
-  const float hh = hpart[i];
-  int ix, iy, iz, inc, reach;
-  float inv_h, h2max;
-  ddens_setup(G, p.x, p.y, p.z, hh, ix, iy, iz, inv_h, h2max, inc, reach);
+__global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, const float* __restrict__ hpart,
+                                                      double scale, unsigned long long* acc, float* norm) {
+  __shared__ unsigned long long box[DBX * DBY * DBZ];
+  __shared__ int qrow[DTY * DTZ + 1], qbeg[DTY * DTZ];
   const int n = G.n;
-  // the reference's cell box, clipped to the grid ...
-  int k0 = max(iz - reach, 1), k1 = min(iz + reach, n);
-  int j0 = max(iy - reach, 1), j1 = min(iy + reach, n);
-  const int i0 = max(ix - reach, 1), i1 = min(ix + reach, n);
-  // ... and to the cells whose centre can lie within h (beyond it M4(2d/h) is
-  // exactly 0, and adding +0 leaves the FP32 NORM unchanged).  The bounds keep
-  // a margin of one cell and 1e-5 h^2, far above the float rounding involved.
-  const float hm2 = hh * hh * 1.00001f;
-  const float rdx = 1.0f / G.dx;
-  k0 = max(k0, (int)floorf((p.z - hh - G.rx1) * rdx) + 1);
-  k1 = min(k1, (int)floorf((p.z + hh - G.rx1) * rdx) + 2);
-  j0 = max(j0, (int)floorf((p.y - hh - G.rx1) * rdx) + 1);
-  j1 = min(j1, (int)floorf((p.y + hh - G.rx1) * rdx) + 2);
-  float nrm = 0.0f;
-  for (int pass = 0; pass < 2; ++pass) {
-    if (pass == 1) {
-      norm[i] = nrm;
-      if (nrm == 0.0f) {  // particles.f90:549-557: all the mass to the home cell
-        if (ix >= 1 && ix <= n && iy >= 1 && iy <= n && iz >= 1 && iz <= n)
-          fix_add(acc, (size_t)(ix - 1) + (size_t)n * ((size_t)(iy - 1) + (size_t)n * (iz - 1)), p.w, scale);
-        return;
+  const int ntx = (n + DTX - 1) / DTX, nty = (n + DTY - 1) / DTY;
+  const int64_t bt = xcd_block(blockIdx.x, gridDim.x);
+  const int tx0 = (int)(bt % ntx) * DTX, ty0 = (int)((bt / ntx) % nty) * DTY, tz0 = (int)(bt / ((int64_t)ntx * nty)) * DTZ;
+  const int tid = threadIdx.x;
+  for (int c = tid; c < DBX * DBY * DBZ; c += 256) box[c] = 0ull;
+  if (tid == 0) {  // the tile's particles: one contiguous range per tile row (bins = cells)
+    int accn = 0;
+    for (int r = 0; r < DTY * DTZ; ++r) {
+      const int y = ty0 + r % DTY, z = tz0 + r / DTY;
+      int cnt = 0, b0 = 0;
+      if (y < n && z < n) {
+        const int rowb = n * (y + n * z);
+        b0 = B.start[rowb + tx0];
+        cnt = B.start[rowb + min(tx0 + DTX, n)] - b0;
       }
+      qrow[r] = accn;
+      qbeg[r] = b0;
+      accn += cnt;
     }
-    for (int k2 = k0; k2 <= k1; ++k2) {
-      const float tz = p.z - G.rad[k2 - G.rlo];
-      for (int j2 = j0; j2 <= j1; ++j2) {
-        const float ty = p.y - G.rad[j2 - G.rlo];
-        const float rem = hm2 - (ty * ty + tz * tz);
-        if (rem < 0.0f) continue;  // no centre of this row within h
-        const float r = sqrtf(rem);
-        const int a = max(i0, (int)floorf((p.x - r - G.rx1) * rdx) + 1);
-        const int b = min(i1, (int)floorf((p.x + r - G.rx1) * rdx) + 2);
-        for (int i2 = a; i2 <= b; ++i2) {
-          bool ok;
-          const float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);
-          if (!ok) continue;
-          if (pass == 0) {
-            nrm = nrm + wgt;  // particles.f90:520-547, in the reference's loop order
-          } else {
-            const float v = (wgt * p.w) / nrm;  // particles.f90:599-600
-            if (v != 0.0f)
-              fix_add(acc, (size_t)(i2 - 1) + (size_t)n * ((size_t)(j2 - 1) + (size_t)n * (k2 - 1)), v, scale);
+    qrow[DTY * DTZ] = accn;
+  }
+  __syncthreads();
+  // widened tile in 1-based cells
+  const int bx0 = tx0 + 1 - DR, by0 = ty0 + 1 - DR, bz0 = tz0 + 1 - DR;
+  const int nq = qrow[DTY * DTZ];
+  for (int q = tid; q < nq; q += 256) {
+    int r = 0;
+    while (qrow[r + 1] <= q) ++r;
+    const int64_t i = qbeg[r] + (q - qrow[r]);
+    const float4 p = B.pos[i];
+    const float hh = hpart[i];
+    int ix, iy, iz, inc, reach;
+    float inv_h, h2max;
+    ddens_setup(G, p.x, p.y, p.z, hh, ix, iy, iz, inv_h, h2max, inc, reach);
+    // the reference's cell box, clipped to the grid ...
+    int k0 = max(iz - reach, 1), k1 = min(iz + reach, n);
+    int j0 = max(iy - reach, 1), j1 = min(iy + reach, n);
+    const int i0 = max(ix - reach, 1), i1 = min(ix + reach, n);
+    // ... and to the cells whose centre can lie within h (beyond it M4(2d/h)
+    // is exactly 0, and adding +0 leaves the FP32 NORM unchanged); the bounds
+    // keep a margin of one cell and 1e-5 h^2, far above the rounding involved
+    const float hm2 = hh * hh * 1.00001f;
+    const float rdx = 1.0f / G.dx;
+    k0 = max(k0, (int)floorf((p.z - hh - G.rx1) * rdx) + 1);
+    k1 = min(k1, (int)floorf((p.z + hh - G.rx1) * rdx) + 2);
+    j0 = max(j0, (int)floorf((p.y - hh - G.rx1) * rdx) + 1);
+    j1 = min(j1, (int)floorf((p.y + hh - G.rx1) * rdx) + 2);
+    const float ra = sqrtf(hm2);  // bounds every row's half-width below
+    const int ia = max(i0, (int)floorf((p.x - ra - G.rx1) * rdx) + 1);
+    const int ib = min(i1, (int)floorf((p.x + ra - G.rx1) * rdx) + 2);
+    const bool local = ia >= bx0 && ib < bx0 + DBX && j0 >= by0 && j1 < by0 + DBY && k0 >= bz0 && k1 < bz0 + DBZ;
+    auto deposit = [&](int ci, int cj, int ck, float v) {
+      const unsigned long long f = (unsigned long long)llrint((double)v * scale);
+      if (local)
+        atomicAdd(&box[(ci - bx0) + DBX * ((cj - by0) + DBY * (ck - bz0))], f);
+      else
+        atomicAdd(&acc[(size_t)(ci - 1) + (size_t)n * ((size_t)(cj - 1) + (size_t)n * (ck - 1))], f);
+    };
+    float nrm = 0.0f;
+    for (int pass = 0; pass < 2; ++pass) {
+      if (pass == 1) {
+        norm[i] = nrm;
+        if (nrm == 0.0f) {  // particles.f90:549-557: all the mass to the home cell
+          if (ix >= 1 && ix <= n && iy >= 1 && iy <= n && iz >= 1 && iz <= n) {
+            const bool home_local = ix >= bx0 && ix < bx0 + DBX && iy >= by0 && iy < by0 + DBY && iz >= bz0 &&
+                                    iz < bz0 + DBZ;
+            const unsigned long long f = (unsigned long long)llrint((double)p.w * scale);
+            if (home_local)
+              atomicAdd(&box[(ix - bx0) + DBX * ((iy - by0) + DBY * (iz - bz0))], f);
+            else
+              atomicAdd(&acc[(size_t)(ix - 1) + (size_t)n * ((size_t)(iy - 1) + (size_t)n * (iz - 1))], f);
+          }
+          break;
+        }
+      }
+      for (int k2 = k0; k2 <= k1; ++k2) {
+        const float tz = p.z - G.rad[k2 - G.rlo];
+        for (int j2 = j0; j2 <= j1; ++j2) {
+          const float ty = p.y - G.rad[j2 - G.rlo];
+          const float rem = hm2 - (ty * ty + tz * tz);
+          if (rem < 0.0f) continue;  // no centre of this row within h
+          const float rr = sqrtf(rem);
+          const int a = max(i0, (int)floorf((p.x - rr - G.rx1) * rdx) + 1);
+          const int b = min(i1, (int)floorf((p.x + rr - G.rx1) * rdx) + 2);
+          for (int i2 = a; i2 <= b; ++i2) {
+            bool ok;
+            const float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);
+            if (!ok) continue;
+            if (pass == 0) {
+              nrm = nrm + wgt;  // particles.f90:520-547, in the reference's loop order
+            } else {
+              const float v = (wgt * p.w) / nrm;  // particles.f90:599-600
+              if (v != 0.0f) deposit(i2, j2, k2, v);
+            }
           }
         }
       }
     }
+  }
+  __syncthreads();
+  for (int c = tid; c < DBX * DBY * DBZ; c += 256) {
+    const unsigned long long v = box[c];
+    if (v == 0ull) continue;
+    const int ci = bx0 + c % DBX, cj = by0 + (c / DBX) % DBY, ck = bz0 + c / (DBX * DBY);
+    atomicAdd(&acc[(size_t)(ci - 1) + (size_t)n * ((size_t)(cj - 1) + (size_t)n * (ck - 1))], v);
   }
 }
 
@@ -917,8 +751,9 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   {
     StageTimer t(c, "ddens_scatter");
     k_fill_u64<<<nblk(ncell), 256, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 0ull);
-    k_ddens_scatter<<<nblk(np), 256, 0, c->stream>>>(np, G, c->spos4.as<float4>(), c->hpart.as<float>(), scale,
-                                                     c->mass_fix.as<unsigned long long>(), c->norm.as<float>());
+    const unsigned ntiles = (unsigned)(((n + DTX - 1) / DTX) * ((n + DTY - 1) / DTY) * ((n + DTZ - 1) / DTZ));
+    k_ddens_scatter<<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
+                                                   c->mass_fix.as<unsigned long long>(), c->norm.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   {

@@ -1,0 +1,302 @@
+// knn_search.h -- the exact k-nearest-neighbour search over bin-sorted
+// particles (kdtree.f90:504-675 semantics), shared by the field kernels.
+//
+// Written to compile both as HIP device code (fields.hip) and as plain host
+// C++, so that a host test can check it against brute force.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define AVK_DEV __device__ __forceinline__
+#else
+#include <algorithm>
+#define AVK_DEV inline
+struct float4 {
+  float x, y, z, w;
+};
+using std::max;
+using std::min;
+#endif
+
+namespace avism {
+
+// per-axis separation, minimum image under periodic boundaries
+AVK_DEV float axis_d(float p, float q, float L, int periodic) {
+  float d = p - q;
+  if (periodic) {
+    float a = fabsf(d), b = L - fabsf(d);
+    d = a < b ? a : b;
+  }
+  return d;
+}
+
+// kdtree.f90:1441-1465 (squared distance, x, y, z accumulated in order)
+AVK_DEV float dist2(float px, float py, float pz, float qx, float qy, float qz, float L, int periodic) {
+  float dx = axis_d(px, qx, L, periodic);
+  float dy = axis_d(py, qy, L, periodic);
+  float dz = axis_d(pz, qz, L, periodic);
+  float d = dx * dx;
+  d = d + dy * dy;
+  d = d + dz * dz;
+  return d;
+}
+
+// uniform bins over the box; particles sorted by bin, start[] = bin offsets
+struct Bins {
+  const float4* __restrict__ pos;      // sorted (x,y,z,m)
+  const uint32_t* __restrict__ orig;   // sorted -> input index
+  const int32_t* __restrict__ start;   // nb^3 + 1
+  int32_t nb;
+  float lo, bsz, L, eps;
+  int periodic;
+  AVK_DEV int bin_of(float v) const {
+    int b = (int)floorf((v - lo) / bsz);
+    return b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+  }
+  AVK_DEV int wrap(int b) const {
+    if (b < 0) b += nb;
+    else if (b >= nb) b -= nb;
+    return b;
+  }
+  // squared distance from q to the (unwrapped) bin box at index b, axis-wise
+  AVK_DEV float box_d(float q, int b) const {
+    float blo = lo + (float)b * bsz - eps, bhi = lo + (float)(b + 1) * bsz + eps;
+    float d = 0.0f;
+    if (q < blo) d = blo - q;
+    else if (q > bhi) d = q - bhi;
+    return d;
+  }
+  // lower bound of the distance from q (in bin b) to anything outside shells 0..s
+  AVK_DEV float outside(float q, int b, int s) const {
+    float dlo = q - (lo + (float)(b - s) * bsz) - eps;
+    float dhi = (lo + (float)(b + s + 1) * bsz) - q - eps;
+    if (!periodic) {
+      if (b - s <= 0) dlo = INFINITY;
+      if (b + s + 1 >= nb) dhi = INFINITY;
+    }
+    return dlo < dhi ? dlo : dhi;
+  }
+};
+
+// ------------------------------------------------------------------ kNN
+// Lane-per-query search; the k best of a lane are a max-heap in LDS (element
+// t of lane l at [t * BS + l]: every lane hits its own bank).  Entries are
+// (squared distance, bin-sorted position) ordered by distance, then by the
+// particle's input index (looked up only when two distances are equal), so
+// equal distances go to the lower input index: the deterministic form of
+// kdtree.f90's strict-< admission, and the oracle's tie rule.  O(log k) LDS
+// steps per admission.
+template <int BS>
+struct Heap {
+  float* hd;
+  uint32_t* hj;
+  const uint32_t* orig;
+  int k, cnt;
+  float wd;     // root: worst kept distance (INFINITY while filling)
+  uint32_t wj;
+  AVK_DEV void init(float* d, uint32_t* j, const uint32_t* o, int k_) {
+    hd = d;
+    hj = j;
+    orig = o;
+    k = k_;
+    cnt = 0;
+    wd = INFINITY;
+    wj = 0;
+  }
+  AVK_DEV bool full() const { return cnt >= k; }
+  AVK_DEV bool gt(float da, uint32_t ja, float db, uint32_t jb) const {
+    return da > db || (da == db && orig[ja] > orig[jb]);
+  }
+  AVK_DEV void consider(float v, uint32_t j) {
+    if (cnt < k) {
+      int i = cnt++;
+      while (i > 0) {  // sift up
+        const int p = (i - 1) >> 1;
+        const float pd = hd[p * BS];
+        const uint32_t pj = hj[p * BS];
+        if (!gt(v, j, pd, pj)) break;
+        hd[i * BS] = pd;
+        hj[i * BS] = pj;
+        i = p;
+      }
+      hd[i * BS] = v;
+      hj[i * BS] = j;
+      if (cnt == k) {
+        wd = hd[0];
+        wj = hj[0];
+      }
+      return;
+    }
+    if (v > wd) return;
+    if (v == wd && orig[j] >= orig[wj]) return;
+    int i = 0;  // replace the root, sift down
+    for (;;) {
+      const int l = 2 * i + 1;
+      if (l >= k) break;
+      float cd = hd[l * BS];
+      uint32_t cj = hj[l * BS];
+      int c = l;
+      if (l + 1 < k) {
+        const float rd = hd[(l + 1) * BS];
+        const uint32_t rj = hj[(l + 1) * BS];
+        if (gt(rd, rj, cd, cj)) {
+          c = l + 1;
+          cd = rd;
+          cj = rj;
+        }
+      }
+      if (!gt(cd, cj, v, j)) break;
+      hd[i * BS] = cd;
+      hj[i * BS] = cj;
+      i = c;
+    }
+    hd[i * BS] = v;
+    hj[i * BS] = j;
+    wd = hd[0];
+    wj = hj[0];
+  }
+  AVK_DEV float dist(int t) const { return hd[t * BS]; }
+  AVK_DEV uint32_t pos(int t) const { return hj[t * BS]; }
+};
+
+// exact kNN of q over the bin-sorted particles (k <= number of particles).
+// Shells of bins around q's bin are searched until the k-th distance is below
+// the distance to everything outside them.  Within a shell, every row of bins
+// is one contiguous range of the sorted particles (two under the periodic
+// wrap), clipped to the bins whose box can still hold a closer particle; the
+// rows are taken RB at a time so that their 4*RB bin-start loads are in
+// flight together before the candidates are read, and their candidates are
+// then read in one flat pass.
+constexpr int RB = 4;  // rows per batch (2 ranges each: pick8 below)
+
+// the sorted-particle range of bins xa..xb (unwrapped, xb - xa < nb) of the row
+// starting at rowb, as up to two [start index, end index) slots into B.start
+AVK_DEV void row_slots(const Bins& B, int rowb, int xa, int xb, bool on, int& a0, int& b0,
+                                          bool& v0, int& a1, int& b1, bool& v1) {
+  const int nb = B.nb;
+  v0 = v1 = false;
+  if (!on) return;
+  if (xa < 0) {  // periodic images on the low side: bins xa..min(xb,-1)
+    a0 = rowb + xa + nb;
+    b0 = rowb + min(xb, -1) + nb + 1;
+    v0 = true;
+    xa = 0;
+  } else if (xb >= nb) {  // and on the high side: bins max(xa,nb)..xb
+    a0 = rowb + max(xa, nb) - nb;
+    b0 = rowb + xb - nb + 1;
+    v0 = true;
+    xb = nb - 1;
+  }
+  if (xa <= xb) {
+    a1 = rowb + xa;
+    b1 = rowb + xb + 1;
+    v1 = true;
+  }
+}
+
+// a[g] for a runtime g in 0..7 without dynamic register indexing
+AVK_DEV int pick8(int a0, int a1, int a2, int a3, int a4, int a5, int a6, int a7, int g) {
+  const int lo = g & 1 ? ((g & 2) ? a3 : a1) : ((g & 2) ? a2 : a0);
+  const int hi = g & 1 ? ((g & 2) ? a7 : a5) : ((g & 2) ? a6 : a4);
+  return (g & 4) ? hi : lo;
+}
+
+template <class H>
+AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz, H& h) {
+  const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
+  const int nb = B.nb;
+  const float e2 = 2.0f * B.eps;
+  const int smax = B.periodic ? (nb - 1) / 2 : nb;
+  for (int s = 0; s <= smax; ++s) {
+    bool any = false;
+    const int side = 2 * s + 1, nrow = side * side;
+    for (int r0 = 0; r0 < nrow; r0 += RB) {
+      int sa[2 * RB], sb[2 * RB];
+      bool sv[2 * RB];
+#pragma unroll
+      for (int t = 0; t < RB; ++t) {
+        const int r = r0 + t;
+        const int dy = r % side - s, dz = r / side - s;
+        const int cy = by + dy, cz = bz + dz;
+        bool on = r < nrow;
+        if (!B.periodic) on = on && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
+        const bool edge = (dz == -s || dz == s || dy == -s || dy == s);
+        int lo_b = bx - s, hi_b = bx + s;
+        if (!B.periodic) {
+          lo_b = max(lo_b, 0);
+          hi_b = min(hi_b, nb - 1);
+        }
+        on = on && lo_b <= hi_b;
+        any = any || on;
+        if (on && h.full()) {  // bins whose box can hold d^2 <= worst
+          const float ddy = B.box_d(qy, cy), ddz = B.box_d(qz, cz);
+          const float rem = h.wd - (ddy * ddy + ddz * ddz);
+          if (rem < 0.0f) {
+            on = false;
+          } else {
+            const float rr = sqrtf(rem);
+            lo_b = max(lo_b, (int)floorf((qx - rr - e2 - B.lo) / B.bsz));
+            hi_b = min(hi_b, (int)floorf((qx + rr + e2 - B.lo) / B.bsz));
+            on = lo_b <= hi_b;
+          }
+        }
+        const int wy = B.periodic ? B.wrap(cy) : cy, wz = B.periodic ? B.wrap(cz) : cz;
+        const int rowb = nb * (wy + nb * wz);
+        if (edge || s == 0) {
+          row_slots(B, rowb, lo_b, hi_b, on, sa[2 * t], sb[2 * t], sv[2 * t], sa[2 * t + 1], sb[2 * t + 1],
+                    sv[2 * t + 1]);
+        } else {  // interior row of the shell: its two end bins
+          const bool l = on && lo_b == bx - s, u = on && hi_b == bx + s;
+          const int wl = B.periodic ? B.wrap(bx - s) : bx - s, wu = B.periodic ? B.wrap(bx + s) : bx + s;
+          sa[2 * t] = rowb + wl;
+          sb[2 * t] = rowb + wl + 1;
+          sv[2 * t] = l;
+          sa[2 * t + 1] = rowb + wu;
+          sb[2 * t + 1] = rowb + wu + 1;
+          sv[2 * t + 1] = u;
+        }
+      }
+      int j0[2 * RB], j1[2 * RB];
+#pragma unroll
+      for (int g = 0; g < 2 * RB; ++g) {
+        j0[g] = sv[g] ? B.start[sa[g]] : 0;
+        j1[g] = sv[g] ? B.start[sb[g]] : 0;
+      }
+      // one flat pass over the batch's candidates: a lane moves on to its next
+      // range as soon as one ends, so a wave pays for its longest lane's total
+      // rather than for the longest lane of every range
+      int g = -1, j = 0, je = 0;
+      for (;;) {
+        while (j >= je && g < 2 * RB - 1) {
+          ++g;
+          j = pick8(j0[0], j0[1], j0[2], j0[3], j0[4], j0[5], j0[6], j0[7], g);
+          je = pick8(j1[0], j1[1], j1[2], j1[3], j1[4], j1[5], j1[6], j1[7], g);
+        }
+        if (j >= je) break;
+        const float4 p = B.pos[j];
+        h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), (uint32_t)j);
+        ++j;
+      }
+    }
+    if (h.full()) {
+      const float b1 = B.outside(qx, bx, s), b2 = B.outside(qy, by, s), b3 = B.outside(qz, bz, s);
+      const float bd = fminf(b1, fminf(b2, b3));
+      if (bd == INFINITY || (bd > 0.0f && h.wd < bd * bd)) return;
+    }
+    if (!B.periodic && !any && s > 0) {
+      const int far = max(max(max(bx, nb - 1 - bx), max(by, nb - 1 - by)), max(bz, nb - 1 - bz));
+      if (s > far) return;
+    }
+  }
+  // periodic box smaller than the search: every particle once
+  h.init(h.hd, h.hj, h.orig, h.k);
+  for (int64_t j = 0; j < np; ++j) {
+    const float4 p = B.pos[j];
+    h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), (uint32_t)j);
+  }
+}
+
+}  // namespace avism

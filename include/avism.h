@@ -14,7 +14,11 @@
  *    library returns AVISM_ERR_INPUT / AVISM_ERR_LIMIT with the same message.
  *  - Calls are synchronous and single-host-thread per context.  Pointer
  *    arguments are host memory unless an argument named `where` says
- *    AVISM_DEVICE (then they are device pointers on the context's GPU).
+ *    AVISM_DEVICE (then they are device pointers on the context's GPU; the
+ *    avism_cnn_* training calls always take device pointers).  A call given
+ *    device pointers first waits for all work queued on the device, so data
+ *    produced on any stream is complete; its device outputs are complete
+ *    when it returns.
  *  - 3-D arrays are column-major (x fastest).  "inner" arrays cover cells 1..N
  *    of each axis; "ext" arrays cover the periodic-extended range LOW1..LOW2
  *    (avism.f90:780-813), LOW1=1, LOW2=N without periodic boundaries.

@@ -360,8 +360,12 @@ def run_reference(jobdir: Path, mode: int, n: int, L0: float, periodic: int, par
         f"{params.rmin!r} {params.rmin_sub!r} {params.nvoid_max}\n"
         f"{ache!r} {omega0!r} {zeta!r}\n{k} {vel_interp} {dens_interp} {npart}\n")
 
-    def unlimit():
-        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    def unlimit():  # the reference needs a large stack (run.sh:16); raise the soft limit to the hard one
+        soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+        try:
+            resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+        except (ValueError, OSError):
+            pass
 
     env = dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"), OMP_STACKSIZE="512m")
     exe = REF_DIR / f"ref_harness_p{int(periodic)}.x"

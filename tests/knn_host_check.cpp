@@ -1,0 +1,97 @@
+// Host build of the library's kNN search (avism_amd/csrc/knn_search.h)
+// checked against brute force: random and lattice (exact ties) particle sets,
+// periodic and not, k = 16, 32, 64, and boxes smaller than the search.
+// Prints one line per case; exit status = number of failing cases.
+#include <cstdio>
+#include <random>
+#include <set>
+#include <vector>
+
+#include "../avism_amd/csrc/knn_search.h"
+
+using namespace avism;
+
+static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, int nq) {
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<float> U(-L / 2, L / 2);
+  const int np = lattice ? n * n * n : n * n * n + 7;
+  std::vector<float4> raw(np);
+  for (int i = 0; i < np; ++i) {
+    float4& p = raw[i];
+    if (lattice) {
+      const float d = L / n;
+      const int a = i % n, b = (i / n) % n, c = i / (n * n);
+      p.x = -L / 2 + (a + 0.5f) * d;
+      p.y = -L / 2 + (b + 0.5f) * d;
+      p.z = -L / 2 + (c + 0.5f) * d;
+    } else {
+      p.x = U(rng);
+      p.y = U(rng);
+      p.z = U(rng);
+    }
+    p.w = 1.0f;
+  }
+  Bins B;
+  B.nb = n;
+  B.lo = -L / 2;
+  B.bsz = L / n;
+  B.L = L;
+  B.eps = L * 2e-6f + B.bsz * 1e-5f;
+  B.periodic = periodic;
+  std::vector<int> key(np), idx(np);
+  for (int i = 0; i < np; ++i) {
+    key[i] = B.bin_of(raw[i].x) + n * (B.bin_of(raw[i].y) + n * B.bin_of(raw[i].z));
+    idx[i] = i;
+  }
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return key[a] < key[b]; });
+  std::vector<float4> pos(np);
+  std::vector<uint32_t> orig(np);
+  std::vector<int32_t> start(n * n * n + 1, 0);
+  for (int i = 0; i < np; ++i) {
+    pos[i] = raw[idx[i]];
+    orig[i] = idx[i];
+    start[key[idx[i]] + 1]++;
+  }
+  for (int b = 0; b < n * n * n; ++b) start[b + 1] += start[b];
+  B.pos = pos.data();
+  B.orig = orig.data();
+  B.start = start.data();
+  std::vector<float> hd(64);
+  std::vector<uint32_t> hj(64);
+  int bad = 0;
+  for (int qi = 0; qi < nq; ++qi) {
+    float qx, qy, qz;
+    if (qi % 2) {
+      const float4 q = pos[(qi * 7919) % np];
+      qx = q.x; qy = q.y; qz = q.z;
+    } else {
+      qx = U(rng); qy = U(rng); qz = U(rng);
+    }
+    Heap<1> h;
+    h.init(hd.data(), hj.data(), B.orig, k);
+    knn_global(B, np, qx, qy, qz, h);
+    std::vector<std::pair<float, uint32_t>> all;
+    for (int j = 0; j < np; ++j)
+      all.push_back({dist2(pos[j].x, pos[j].y, pos[j].z, qx, qy, qz, L, periodic), orig[j]});
+    std::sort(all.begin(), all.end());
+    std::set<uint32_t> want, got;
+    for (int t = 0; t < k; ++t) want.insert(all[t].second);
+    for (int t = 0; t < h.cnt; ++t) got.insert(orig[h.pos(t)]);
+    if (want != got || h.cnt != k || h.wd != all[k - 1].first) ++bad;
+  }
+  printf("n=%d k=%d periodic=%d lattice=%d: %d/%d wrong\n", n, k, periodic, lattice, bad, nq);
+  return bad ? 1 : 0;
+}
+
+int main() {
+  int fails = 0;
+  for (int per = 0; per < 2; ++per)
+    for (int lat = 0; lat < 2; ++lat) {
+      fails += run(20, 16, 40.0f, per, lat, 1, 600);
+      fails += run(16, 32, 32.0f, per, lat, 2, 600);
+      fails += run(12, 64, 30.0f, per, lat, 3, 300);
+      fails += run(5, 32, 10.0f, per, lat, 4, 200);   // box smaller than the search
+      fails += run(4, 16, 8.0f, per, lat, 5, 100);
+    }
+  return fails;
+}

@@ -164,3 +164,33 @@ def test_fields_clustered(per):
     for c in ("u", "v", "w"):
         assert _rms_rel(f[c], ref[c]) <= REL, c
     assert _rel(f["u1co"], ref["u1co"]) <= REL
+
+
+def test_device_inputs_from_another_stream():
+    """Particles written on a torch side stream and handed over as device
+    pointers without a synchronize: the library waits for them (the fields
+    must equal those of a synchronized load)."""
+    import torch
+    from avism_amd import synth
+    from avism_amd._lib import AVISM_DEVICE
+    p = synth.zeldovich(24, 48.0, seed=35)
+    ref_c = _ctx()
+    ref_c.set_grid(24, 48.0, 1)
+    ref_c.load_particles(*p.numpy())
+    ref_c.particles_to_grid(k=32)
+    ref = ref_c.fields()
+    side = torch.cuda.Stream()
+    host = [torch.from_numpy(a) for a in p.numpy()]
+    with torch.cuda.stream(side):
+        slow = torch.ones(1 << 26, device="cuda")
+        for _ in range(20):
+            slow = slow * 1.0000001  # keeps the side stream busy before the copies
+        dev = [h.to("cuda", non_blocking=True) for h in host]
+    c = _ctx()
+    c.set_grid(24, 48.0, 1)
+    c.load_particles(*dev, where=AVISM_DEVICE)
+    c.particles_to_grid(k=32)
+    f = c.fields()
+    torch.cuda.synchronize()
+    for key in ("u1co", "div", "u"):
+        assert np.array_equal(f[key], ref[key]), key
