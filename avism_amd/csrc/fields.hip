@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float* 
 template <int K>
 __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad, int rlo,
                                             float dx, int k, int want_vel, const float4* __restrict__ vel,
-                                            const float* __restrict__ pdens, const int8_t* __restrict__ smask,
+                                            const int8_t* __restrict__ smask,
                                             uint32_t* hbits, float* u, float* v, float* w) {
   __shared__ float hd_s[K * KQ];
   __shared__ uint32_t hj_s[K * KQ];
@@ -289,9 +289,9 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
     const uint32_t db = __float_as_uint(d);
     if (db > hbits[j]) atomicMax(&hbits[j], db);
     if (want_vel) {
+      const float4 vv = vel[j];                             // vx, vy, vz, m/rho
       const float qq = d / (hh / 2.0f);                     // particles.f90:941
-      const float wgt = m4(qq) * (B.pos[j].w / pdens[j]);   // 897-900
-      const float4 vv = vel[j];
+      const float wgt = m4(qq) * vv.w;                      // 897-900
       const double w8 = (double)wgt;
       sw += w8;
       sx += w8 * (double)vv.x;
@@ -313,6 +313,13 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
     v[c] = (float)(sy / sw);
     w[c] = (float)(sz / sw);
   }
+}
+
+// m/rho of every particle into the w slot of its velocity record (the SPH
+// weight factor of particles.f90:897-900, one FP32 division as there)
+__global__ void k_mrho(int64_t np, const float4* __restrict__ pos, const float* __restrict__ pdens, float4* vel) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) vel[i].w = pos[i].w / pdens[i];
 }
 
 // FILL_H_ZEROS (particles.f90:398-431): h = distance to the k-th neighbour
@@ -719,15 +726,16 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   }
   AV_CHECK_HIP(c, hipMemsetAsync(c->hpart.p, 0, (size_t)np * 4, c->stream));
   {
+    k_mrho<<<nblk(np), 256, 0, c->stream>>>(np, c->spos4.as<float4>(), c->pdens.as<float>(), c->svel4.as<float4>());
     StageTimer t(c, "vvel_sph_vw");
     const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
     if (k <= 32)
       k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1, c->svel4.as<float4>(),
-                                               c->pdens.as<float>(), nullptr, c->hpart.as<uint32_t>(),
+                                               nullptr, c->hpart.as<uint32_t>(),
                                                c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
     else
       k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1, c->svel4.as<float4>(),
-                                               c->pdens.as<float>(), nullptr, c->hpart.as<uint32_t>(),
+                                               nullptr, c->hpart.as<uint32_t>(),
                                                c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }

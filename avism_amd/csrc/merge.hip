@@ -648,7 +648,7 @@ __device__ __forceinline__ int resolve(const int32_t* lmap, int l) {
 __global__ void k_vp_flag(int nv, ExtG G, const int32_t* __restrict__ indice, const int32_t* __restrict__ uvoid,
                           const int32_t* __restrict__ marca_base, const float* __restrict__ gxc,
                           const float* __restrict__ gyc, const float* __restrict__ gzc, float half, float dx,
-                          float rx1, int32_t* flag) {
+                          float rx1, int32_t* flag, int4* info) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nv) return;
   const int id = indice[I];
@@ -661,58 +661,44 @@ __global__ void k_vp_flag(int nv, ExtG G, const int32_t* __restrict__ indice, co
     if (sx | sy | sz) {
       const int ax = (int)((xc1 - rx1) / dx) + 1 + sx * G.n, ay = (int)((yc1 - rx1) / dx) + 1 + sy * G.n,
                 az = (int)((zc1 - rx1) / dx) + 1 + sz * G.n;
-      if (ax >= G.low1 && ax <= G.low2 && ay >= G.low1 && ay <= G.low2 && az >= G.low1 && az <= G.low2)
-        f = marca_base[G.ext(ax, ay, az)] > 0;
+      if (ax >= G.low1 && ax <= G.low2 && ay >= G.low1 && ay <= G.low2 && az >= G.low1 && az <= G.low2) {
+        const int b2 = marca_base[G.ext(ax, ay, az)];
+        f = b2 > 0;
+        // id, base label at the image of the centre, image shifts (+1 so they fit 2 bits)
+        info[I] = make_int4(id, b2, (sx + 1) | ((sy + 1) << 2) | ((sz + 1) << 4), 0);
+      }
     }
   }
   flag[I] = f;
 }
 
-__global__ void k_vp_list(int nv, const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int32_t* list) {
+__global__ void k_vp_list(int nv, const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                          const int4* __restrict__ info, int4* list) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I < nv && flag[I]) list[pos[I]] = I;
+  if (I < nv && flag[I]) list[pos[I]] = info[I];
 }
 
 // one workgroup walks the flagged main voids in INDICE order
 __global__ void __launch_bounds__(1024) k_void_periodic(
-    const int32_t* __restrict__ list, const int32_t* __restrict__ nlist, ExtG G, const int32_t* __restrict__ indice, const int32_t* __restrict__ uvoid,
-    const int32_t* __restrict__ marca_base, const float* __restrict__ gxc, const float* __restrict__ gyc,
-    const float* __restrict__ gzc, float half, float dx, float rx1, const uint32_t* __restrict__ cells,
+    const int4* __restrict__ list, const int32_t* __restrict__ nlist, ExtG G,
+    const int32_t* __restrict__ marca_base, const uint32_t* __restrict__ cells,
     const int32_t* __restrict__ lstart, const int32_t* __restrict__ lend, int32_t* lmap, int32_t* mnext,
     int32_t* mtail, int32_t* selfm, int32_t* stats) {
   __shared__ int s_cnt, s_cnt2, s_go, s_ind, s_ind2, s_sx, s_sy, s_sz;
   const int tid = threadIdx.x;
   const int nl = *nlist;
   for (int L = 0; L < nl; ++L) {
-    const int I = list[L];
-    if (tid == 0) {
+    if (tid == 0) {  // the part of the test that earlier voids can change (voidfinding.f90:1476-1480)
+      const int4 in = list[L];
+      const int id = in.x;
       s_go = 0;
-      int id = indice[I];
-      if (uvoid[id - 1] == -1) {
-        float xc1 = gxc[id - 1], yc1 = gyc[id - 1], zc1 = gzc[id - 1];
-        int sx = 0, sy = 0, sz = 0;
-        if (xc1 < -half) sx = 1;
-        if (xc1 > half) sx = -1;
-        if (yc1 < -half) sy = 1;
-        if (yc1 > half) sy = -1;
-        if (zc1 < -half) sz = 1;
-        if (zc1 > half) sz = -1;
-        if (sx | sy | sz) {
-          int igx = (int)((gxc[id - 1] - rx1) / dx) + 1;
-          int igy = (int)((gyc[id - 1] - rx1) / dx) + 1;
-          int igz = (int)((gzc[id - 1] - rx1) / dx) + 1;
-          int ax = igx + sx * G.n, ay = igy + sy * G.n, az = igz + sz * G.n;
-          if (ax >= G.low1 && ax <= G.low2 && ay >= G.low1 && ay <= G.low2 && az >= G.low1 && az <= G.low2) {
-            int b2 = marca_base[G.ext(ax, ay, az)];
-            int ind2 = b2 > 0 ? resolve(lmap, b2) : 0;
-            if (ind2 != 0 && !selfm[id - 1]) {
-              s_go = 1;
-              s_ind = id;
-              s_ind2 = ind2;
-              s_sx = sx; s_sy = sy; s_sz = sz;
-            }
-          }
-        }
+      if (!selfm[id - 1]) {
+        s_go = 1;
+        s_ind = id;
+        s_ind2 = resolve(lmap, in.y);
+        s_sx = (in.z & 3) - 1;
+        s_sy = ((in.z >> 2) & 3) - 1;
+        s_sz = ((in.z >> 4) & 3) - 1;
       }
       s_cnt = 0;
       s_cnt2 = 0;
@@ -1028,9 +1014,10 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     // vb are no longer needed as sort input, ka/va are free)
     int32_t* vflag = (int32_t*)ka;
     int32_t* vpos = (int32_t*)va;
-    int32_t* vlist = vflag + (nv + 2);
+    int4* vinfo = (int4*)(vflag + ((nv + 2 + 3) & ~3));
+    int4* vlist = vinfo + (nv + 1);
     k_vp_flag<<<nbk(nv), 256, 0, c->stream>>>(nv, G, indice, uvoid, marca, GX, GY, GZ, g.L0 / 2.0f, g.dx, g.rx1,
-                                              vflag);
+                                              vflag, vinfo);
     AV_CHECK_HIP(c, hipMemsetAsync(vflag + nv, 0, 4, c->stream));
     size_t tmp2 = 0;
     AV_CHECK_HIP(c, rocprim::exclusive_scan(nullptr, tmp2, vflag, vpos, 0, (size_t)nv + 1,
@@ -1038,9 +1025,9 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     if (!c->sort_tmp.ensure(std::max(tmp, tmp2))) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (periodic)");
     AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmp2, vflag, vpos, 0, (size_t)nv + 1,
                                             rocprim::plus<int32_t>(), c->stream));
-    k_vp_list<<<nbk(nv), 256, 0, c->stream>>>(nv, vflag, vpos, vlist);
-    k_void_periodic<<<1, 1024, 0, c->stream>>>(vlist, vpos + nv, G, indice, uvoid, marca, GX, GY, GZ, g.L0 / 2.0f,
-                                                g.dx, g.rx1, vb, lstart, lend, lmap, mnext, mtail, selfm, small + 3);
+    k_vp_list<<<nbk(nv), 256, 0, c->stream>>>(nv, vflag, vpos, vinfo, vlist);
+    k_void_periodic<<<1, 1024, 0, c->stream>>>(vlist, vpos + nv, G, marca, vb, lstart, lend, lmap, mnext, mtail,
+                                                selfm, small + 3);
     k_apply_lmap<<<nbk(nv), 256, 0, c->stream>>>(nv, lmap, selfm, uvoid, VOLNEW);
     k_relabel_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, lmap, marca);
     AV_CHECK_HIP(c, hipGetLastError());
