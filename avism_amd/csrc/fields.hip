@@ -214,19 +214,18 @@ __global__ void k_bin_keys(int64_t np, const float* x, const float* y, const flo
 }
 
 __global__ void k_gather_sorted(int64_t np, const uint32_t* perm, const float* x, const float* y,
-                                const float* z, const float* m, const float* vx, const float* vy,
-                                const float* vz, float4* pos, float4* vel, uint32_t* orig) {
+                                const float* z, const float* m, float4* pos, uint32_t* orig) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   uint32_t p = perm[i];
   pos[i] = make_float4(x[p], y[p], z[p], m[p]);
-  vel[i] = make_float4(vx[p], vy[p], vz[p], 0.0f);
   orig[i] = p;
 }
 
 // PPART_DENS (particles.f90:742-802): rho_p = (FP64 sum of the k nearest
 // masses, self included) / (4/3 pi d_k^3); d_k is kept for FILL_H_ZEROS.
 // One particle per lane, in bin-sorted order (a wave's queries are neighbours).
+// The search returns input indices, so the masses are read in input order.
 constexpr int KQ = 128;  // threads per block of the kNN kernels
 
 // Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
@@ -240,18 +239,18 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
 }
 
 template <int K>
-__global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float* pdens, float* hk) {
-  __shared__ float hd_s[K * KQ];
-  __shared__ uint32_t hj_s[K * KQ];
+__global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, const float* __restrict__ mass,
+                                             float* pdens, float* hk) {
+  __shared__ uint64_t keys_s[K * KQ];
   const int64_t i = xcd_block(blockIdx.x, gridDim.x) * KQ + threadIdx.x;
   if (i >= np) return;
   const float4 pq = B.pos[i];
-  Heap<KQ> h;
-  h.init(hd_s + threadIdx.x, hj_s + threadIdx.x, B.orig, k);
+  Sel<KQ, K / 4> h;
+  h.init(keys_s + threadIdx.x, k);
   knn_global(B, np, pq.x, pq.y, pq.z, h);
-  const float dk = sqrtf(h.wd);  // kdtree.f90:539
+  const float dk = sqrtf(h.wd());  // kdtree.f90:539
   double bas8 = 0.0;
-  for (int t = 0; t < h.cnt; ++t) bas8 += (double)B.pos[h.pos(t)].w;
+  for (int t = 0; t < h.cnt; ++t) bas8 += (double)mass[h.id(t)];
   hk[i] = dk;
   const double vol = (4.0 / 3.0 * kPi) * (double)((dk * dk) * dk);
   pdens[i] = (float)(bas8 / vol);  // particles.f90:790-794
@@ -261,13 +260,13 @@ __global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float* 
 // where d_k <= DX, every particle within DX), HPART = max distance at which a
 // particle served a cell (atomic max on the float bits), and the FP64
 // weighted velocity average.  One cell per lane; a block covers 8 x 4 x 4 cells.
+// HPART bits and the velocity records are in input order.
 template <int K>
 __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad, int rlo,
                                             float dx, int k, int want_vel, const float4* __restrict__ vel,
                                             const int8_t* __restrict__ smask,
                                             uint32_t* hbits, float* u, float* v, float* w) {
-  __shared__ float hd_s[K * KQ];
-  __shared__ uint32_t hj_s[K * KQ];
+  __shared__ uint64_t keys_s[K * KQ];
   const int ntx = (n + 7) / 8, nty = (n + 3) / 4;
   const int tid = threadIdx.x;
   const int64_t bt = xcd_block(blockIdx.x, gridDim.x);
@@ -277,10 +276,10 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
   const size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
   if (smask && smask[c] == 0) return;
   const float qx = rad[ix + 1 - rlo], qy = rad[iy + 1 - rlo], qz = rad[iz + 1 - rlo];
-  Heap<KQ> h;
-  h.init(hd_s + tid, hj_s + tid, B.orig, k);
+  Sel<KQ, K / 4> h;
+  h.init(keys_s + tid, k);
   knn_global(B, np, qx, qy, qz, h);
-  const float dk = sqrtf(h.wd);
+  const float dk = sqrtf(h.wd());
   double sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
   auto serve = [&](uint32_t j, float d, float hh) {
     // particles.f90:882-885; HPART only grows, so a value already at least d
@@ -300,13 +299,13 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
     }
   };
   if (dk > dx) {  // k neighbours (particles.f90:866-867)
-    for (int t = 0; t < h.cnt; ++t) serve(h.pos(t), sqrtf(h.dist(t)), dk);
+    for (int t = 0; t < h.cnt; ++t) serve(h.id(t), sqrtf(h.dist(t)), dk);
   } else {        // ball of radius DX (particles.f90:868-878): every particle with d <= DX
     const float r2 = dx * dx;
     float maxd2 = 0.0f;
     ball_visit(B, np, qx, qy, qz, dx, r2, [&](int j, float d2) { maxd2 = fmaxf(maxd2, d2); });
     const float hh = sqrtf(maxd2);
-    ball_visit(B, np, qx, qy, qz, dx, r2, [&](int j, float d2) { serve((uint32_t)j, sqrtf(d2), hh); });
+    ball_visit(B, np, qx, qy, qz, dx, r2, [&](int j, float d2) { serve(B.orig[j], sqrtf(d2), hh); });
   }
   if (want_vel) {
     u[c] = (float)(sx / sw);
@@ -315,18 +314,31 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
   }
 }
 
-// m/rho of every particle into the w slot of its velocity record (the SPH
-// weight factor of particles.f90:897-900, one FP32 division as there)
-__global__ void k_mrho(int64_t np, const float4* __restrict__ pos, const float* __restrict__ pdens, float4* vel) {
+// the velocity record of every particle, in input order: (vx, vy, vz, m/rho),
+// m/rho being the SPH weight factor of particles.f90:897-900 (one FP32
+// division as there); pdens is in bin-sorted order
+__global__ void k_velrec(int64_t np, const uint32_t* __restrict__ orig, const float4* __restrict__ pos,
+                         const float* __restrict__ pdens, const float* __restrict__ vx,
+                         const float* __restrict__ vy, const float* __restrict__ vz, float4* vel) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < np) vel[i].w = pos[i].w / pdens[i];
+  if (i >= np) return;
+  const uint32_t p = orig[i];
+  vel[p] = make_float4(vx[p], vy[p], vz[p], pos[i].w / pdens[i]);
 }
 
 // FILL_H_ZEROS (particles.f90:398-431): h = distance to the k-th neighbour
-__global__ void k_fill_h(int64_t np, float* hpart, const float* hk) {
+// where no cell was served.  hpart is in input order; hk (bin-sorted) is
+// overwritten with the final h for the DDENS pass.
+__global__ void k_fill_h(int64_t np, const uint32_t* __restrict__ orig, float* hpart, float* hk) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  if (!(hpart[i] > 0.0f)) hpart[i] = hk[i];
+  const uint32_t p = orig[i];
+  const float h = hpart[p];
+  if (h > 0.0f) {
+    hk[i] = h;
+  } else {
+    hpart[p] = hk[i];
+  }
 }
 
 // DDENS_INTERP_SPH pass 1 (particles.f90:479-547): per particle the NORM, in
@@ -382,7 +394,7 @@ __device__ __forceinline__ float ddens_w(const DdensGeom& G, float x, float y, f
 constexpr int DTX = 8, DTY = 8, DTZ = 4, DR = 4;
 constexpr int DBX = DTX + 2 * DR, DBY = DTY + 2 * DR, DBZ = DTZ + 2 * DR;
 
-__global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, const float* __restrict__ hpart,
+__global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, const float* __restrict__ hsort,
                                                       double scale, unsigned long long* acc, float* norm) {
   __shared__ unsigned long long box[DBX * DBY * DBZ];
   __shared__ int qrow[DTY * DTZ + 1], qbeg[DTY * DTZ];
@@ -417,7 +429,7 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
     while (qrow[r + 1] <= q) ++r;
     const int64_t i = qbeg[r] + (q - qrow[r]);
     const float4 p = B.pos[i];
-    const float hh = hpart[i];
+    const float hh = hsort[i];  // HPART, bin-sorted
     int ix, iy, iz, inc, reach;
     float inv_h, h2max;
     ddens_setup(G, p.x, p.y, p.z, hh, ix, iy, iz, inv_h, h2max, inc, reach);
@@ -651,7 +663,7 @@ static int build_bins(avism_ctx* c) {
   if (!c->bin_start.ensure((size_t)(nbins + 1) * 4) || !c->sbin_key.ensure((size_t)np * 4) ||
       !c->sbin_val.ensure((size_t)np * 4) || !c->sbin_key2.ensure((size_t)np * 4) ||
       !c->sbin_val2.ensure((size_t)np * 4) || !c->spos4.ensure((size_t)np * 16) ||
-      !c->svel4.ensure((size_t)np * 16) || !c->sorig.ensure((size_t)np * 4) ||
+      !c->vel4.ensure((size_t)np * 16) || !c->sorig.ensure((size_t)np * 4) ||
       !c->bin_reach.ensure((size_t)nbins * 4))
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (bins)");
   int32_t* count = c->bin_reach.as<int32_t>();  // reused as the histogram
@@ -686,8 +698,7 @@ static int build_bins(avism_ctx* c) {
                                  hipMemcpyHostToDevice, c->stream));
   k_gather_sorted<<<nblk(np), 256, 0, c->stream>>>(
       np, c->sbin_val2.as<uint32_t>(), c->px.as<float>(), c->py.as<float>(), c->pz.as<float>(),
-      c->pm.as<float>(), c->pvx.as<float>(), c->pvy.as<float>(), c->pvz.as<float>(),
-      c->spos4.as<float4>(), c->svel4.as<float4>(), c->sorig.as<uint32_t>());
+      c->pm.as<float>(), c->spos4.as<float4>(), c->sorig.as<uint32_t>());
   AV_CHECK_HIP(c, hipGetLastError());
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   return AVISM_OK;
@@ -719,29 +730,34 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   {
     StageTimer t(c, "ppart_dens");
     if (k <= 32)
-      k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, c->pdens.as<float>(), c->hk.as<float>());
+      k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, c->pm.as<float>(), c->pdens.as<float>(),
+                                                       c->hk.as<float>());
     else
-      k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, c->pdens.as<float>(), c->hk.as<float>());
+      k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, c->pm.as<float>(), c->pdens.as<float>(),
+                                                       c->hk.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   AV_CHECK_HIP(c, hipMemsetAsync(c->hpart.p, 0, (size_t)np * 4, c->stream));
   {
-    k_mrho<<<nblk(np), 256, 0, c->stream>>>(np, c->spos4.as<float4>(), c->pdens.as<float>(), c->svel4.as<float4>());
+    k_velrec<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), c->spos4.as<float4>(),
+                                              c->pdens.as<float>(), c->pvx.as<float>(), c->pvy.as<float>(),
+                                              c->pvz.as<float>(), c->vel4.as<float4>());
     StageTimer t(c, "vvel_sph_vw");
     const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
     if (k <= 32)
-      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1, c->svel4.as<float4>(),
+      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1, c->vel4.as<float4>(),
                                                nullptr, c->hpart.as<uint32_t>(),
                                                c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
     else
-      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1, c->svel4.as<float4>(),
+      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1, c->vel4.as<float4>(),
                                                nullptr, c->hpart.as<uint32_t>(),
                                                c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   {
     StageTimer t(c, "fill_h");
-    k_fill_h<<<nblk(np), 256, 0, c->stream>>>(np, c->hpart.as<float>(), c->hk.as<float>());
+    k_fill_h<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), c->hpart.as<float>(),
+                                              c->hk.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   DdensGeom G{rad, c->g.rlo, n, c->g.dx, c->g.rx1};
@@ -760,7 +776,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     StageTimer t(c, "ddens_scatter");
     k_fill_u64<<<nblk(ncell), 256, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 0ull);
     const unsigned ntiles = (unsigned)(((n + DTX - 1) / DTX) * ((n + DTY - 1) / DTY) * ((n + DTZ - 1) / DTZ));
-    k_ddens_scatter<<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
+    k_ddens_scatter<<<ntiles, 256, 0, c->stream>>>(G, B, c->hk.as<float>(), scale,
                                                    c->mass_fix.as<unsigned long long>(), c->norm.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
@@ -833,12 +849,13 @@ int get_particle_fields(avism_ctx* c, float* part_dens, float* hpart) {
   if (c->np <= 0 || !c->fields_ready) AV_FAIL(c, AVISM_ERR_STATE, "no particle fields computed");
   const int64_t np = c->np;
   float* tmp = (float*)c->sbin_key.ensure((size_t)np * 4);
-  float* srcs[2] = {c->pdens.as<float>(), c->hpart.as<float>()};
-  float* dsts[2] = {part_dens, hpart};
-  for (int i = 0; i < 2; i++) {
-    if (!dsts[i]) continue;
-    k_scatter_orig<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), srcs[i], tmp);
-    AV_CHECK_HIP(c, hipMemcpyAsync(dsts[i], tmp, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream));
+  if (part_dens) {  // bin-sorted on the device
+    k_scatter_orig<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), c->pdens.as<float>(), tmp);
+    AV_CHECK_HIP(c, hipMemcpyAsync(part_dens, tmp, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream));
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  if (hpart) {  // input order on the device
+    AV_CHECK_HIP(c, hipMemcpyAsync(hpart, c->hpart.p, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream));
     AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   }
   return AVISM_OK;

@@ -82,84 +82,119 @@ struct Bins {
 };
 
 // ------------------------------------------------------------------ kNN
-// Lane-per-query search; the k best of a lane are a max-heap in LDS (element
-// t of lane l at [t * BS + l]: every lane hits its own bank).  Entries are
-// (squared distance, bin-sorted position) ordered by distance, then by the
-// particle's input index (looked up only when two distances are equal), so
-// equal distances go to the lower input index: the deterministic form of
-// kdtree.f90's strict-< admission, and the oracle's tie rule.  O(log k) LDS
-// steps per admission.
-template <int BS>
-struct Heap {
-  float* hd;
-  uint32_t* hj;
-  const uint32_t* orig;
-  int k, cnt;
-  float wd;     // root: worst kept distance (INFINITY while filling)
-  uint32_t wj;
-  AVK_DEV void init(float* d, uint32_t* j, const uint32_t* o, int k_) {
-    hd = d;
-    hj = j;
-    orig = o;
+AVK_DEV uint32_t f2u(float f) {
+  union {
+    float f;
+    uint32_t u;
+  } c;
+  c.f = f;
+  return c.u;
+}
+AVK_DEV float u2f(uint32_t u) {
+  union {
+    uint32_t u;
+    float f;
+  } c;
+  c.u = u;
+  return c.f;
+}
+
+// The k best candidates of one query (one lane), as 64-bit keys: the squared
+// distance (non-negative floats order as their bit patterns) over the
+// particle's input index.  Keys are unique, and the smaller key is the closer
+// particle, on equal distances the one with the lower input index: the
+// deterministic form of kdtree.f90's strict-< admission.
+//
+// The 4*NG slots live in LDS (slot t of lane l at [t * BS + l], so every lane
+// hits its own bank) in NG groups of 4 whose maxima stay in registers.
+// Replacing the worst key costs one LDS store, a reload of its group's 4 keys
+// and register selects, with no data-dependent branches; a candidate farther
+// than the worst key costs one compare.  Slots at or beyond k hold key 0 and
+// never become a maximum.
+template <int BS, int NG>
+struct Sel {
+  uint64_t* keys;     // this lane's slots in LDS
+  uint64_t gmax[NG];  // largest key of each group
+  int gpos[NG];       // its slot within the group
+  uint64_t worst;     // largest key held once full
+  int wg, k, cnt;
+
+  AVK_DEV void init(uint64_t* ks, int k_) {
+    keys = ks;
     k = k_;
+    reset();
+  }
+  AVK_DEV void reset() {
     cnt = 0;
-    wd = INFINITY;
-    wj = 0;
+    worst = ~0ull;
+    wg = 0;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      gmax[g] = 0ull;
+      gpos[g] = 0;
+    }
+#pragma unroll
+    for (int t = 0; t < 4 * NG; ++t) keys[t * BS] = 0ull;
   }
   AVK_DEV bool full() const { return cnt >= k; }
-  AVK_DEV bool gt(float da, uint32_t ja, float db, uint32_t jb) const {
-    return da > db || (da == db && orig[ja] > orig[jb]);
+  // squared distance of the worst key held (INFINITY while filling)
+  AVK_DEV float wd() const { return cnt >= k ? u2f((uint32_t)(worst >> 32)) : INFINITY; }
+  AVK_DEV void find_worst() {
+    uint64_t w = gmax[0];
+    int g0 = 0;
+#pragma unroll
+    for (int g = 1; g < NG; ++g) {
+      const bool b = gmax[g] > w;
+      w = b ? gmax[g] : w;
+      g0 = b ? g : g0;
+    }
+    worst = w;
+    wg = g0;
   }
-  AVK_DEV void consider(float v, uint32_t j) {
+  // candidate at squared distance v with input index o
+  AVK_DEV void consider(float v, uint32_t o) {
+    const uint32_t vb = f2u(v);
     if (cnt < k) {
-      int i = cnt++;
-      while (i > 0) {  // sift up
-        const int p = (i - 1) >> 1;
-        const float pd = hd[p * BS];
-        const uint32_t pj = hj[p * BS];
-        if (!gt(v, j, pd, pj)) break;
-        hd[i * BS] = pd;
-        hj[i * BS] = pj;
-        i = p;
+      const uint64_t key = ((uint64_t)vb << 32) | o;
+      const int slot = cnt++;
+      keys[slot * BS] = key;
+      const int g = slot >> 2;
+#pragma unroll
+      for (int t = 0; t < NG; ++t) {
+        const bool b = (t == g) && key > gmax[t];
+        gmax[t] = b ? key : gmax[t];
+        gpos[t] = b ? (slot & 3) : gpos[t];
       }
-      hd[i * BS] = v;
-      hj[i * BS] = j;
-      if (cnt == k) {
-        wd = hd[0];
-        wj = hj[0];
-      }
+      if (cnt == k) find_worst();
       return;
     }
-    if (v > wd) return;
-    if (v == wd && orig[j] >= orig[wj]) return;
-    int i = 0;  // replace the root, sift down
-    for (;;) {
-      const int l = 2 * i + 1;
-      if (l >= k) break;
-      float cd = hd[l * BS];
-      uint32_t cj = hj[l * BS];
-      int c = l;
-      if (l + 1 < k) {
-        const float rd = hd[(l + 1) * BS];
-        const uint32_t rj = hj[(l + 1) * BS];
-        if (gt(rd, rj, cd, cj)) {
-          c = l + 1;
-          cd = rd;
-          cj = rj;
-        }
-      }
-      if (!gt(cd, cj, v, j)) break;
-      hd[i * BS] = cd;
-      hj[i * BS] = cj;
-      i = c;
+    if (vb > (uint32_t)(worst >> 32)) return;
+    const uint64_t key = ((uint64_t)vb << 32) | o;
+    if (key >= worst) return;
+    int p = gpos[0];
+#pragma unroll
+    for (int t = 1; t < NG; ++t) p = (t == wg) ? gpos[t] : p;
+    const int base = 4 * wg;
+    keys[(base + p) * BS] = key;
+    // the group's new maximum
+    uint64_t m = keys[base * BS];
+    int mp = 0;
+#pragma unroll
+    for (int s = 1; s < 4; ++s) {
+      const uint64_t x = keys[(base + s) * BS];
+      const bool b = x > m;
+      m = b ? x : m;
+      mp = b ? s : mp;
     }
-    hd[i * BS] = v;
-    hj[i * BS] = j;
-    wd = hd[0];
-    wj = hj[0];
+#pragma unroll
+    for (int t = 0; t < NG; ++t) {
+      gmax[t] = (t == wg) ? m : gmax[t];
+      gpos[t] = (t == wg) ? mp : gpos[t];
+    }
+    find_worst();
   }
-  AVK_DEV float dist(int t) const { return hd[t * BS]; }
-  AVK_DEV uint32_t pos(int t) const { return hj[t * BS]; }
+  AVK_DEV float dist(int t) const { return u2f((uint32_t)(keys[t * BS] >> 32)); }
+  AVK_DEV uint32_t id(int t) const { return (uint32_t)keys[t * BS]; }
 };
 
 // exact kNN of q over the bin-sorted particles (k <= number of particles).
@@ -233,7 +268,7 @@ AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz,
         any = any || on;
         if (on && h.full()) {  // bins whose box can hold d^2 <= worst
           const float ddy = B.box_d(qy, cy), ddz = B.box_d(qz, cz);
-          const float rem = h.wd - (ddy * ddy + ddz * ddz);
+          const float rem = h.wd() - (ddy * ddy + ddz * ddz);
           if (rem < 0.0f) {
             on = false;
           } else {
@@ -277,14 +312,15 @@ AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz,
         }
         if (j >= je) break;
         const float4 p = B.pos[j];
-        h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), (uint32_t)j);
+        const uint32_t o = B.orig[j];
+        h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), o);
         ++j;
       }
     }
     if (h.full()) {
       const float b1 = B.outside(qx, bx, s), b2 = B.outside(qy, by, s), b3 = B.outside(qz, bz, s);
       const float bd = fminf(b1, fminf(b2, b3));
-      if (bd == INFINITY || (bd > 0.0f && h.wd < bd * bd)) return;
+      if (bd == INFINITY || (bd > 0.0f && h.wd() < bd * bd)) return;
     }
     if (!B.periodic && !any && s > 0) {
       const int far = max(max(max(bx, nb - 1 - bx), max(by, nb - 1 - by)), max(bz, nb - 1 - bz));
@@ -292,10 +328,10 @@ AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz,
     }
   }
   // periodic box smaller than the search: every particle once
-  h.init(h.hd, h.hj, h.orig, h.k);
+  h.reset();
   for (int64_t j = 0; j < np; ++j) {
     const float4 p = B.pos[j];
-    h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), (uint32_t)j);
+    h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), B.orig[j]);
   }
 }
 

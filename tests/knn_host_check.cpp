@@ -1,6 +1,6 @@
 // Host build of the library's kNN search (avism_amd/csrc/knn_search.h)
 // checked against brute force: random and lattice (exact ties) particle sets,
-// periodic and not, k = 16, 32, 64, and boxes smaller than the search.
+// periodic and not, k = 16, 21, 32, 47, 64, and boxes smaller than the search.
 // Prints one line per case; exit status = number of failing cases.
 #include <cstdio>
 #include <random>
@@ -56,8 +56,7 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
   B.pos = pos.data();
   B.orig = orig.data();
   B.start = start.data();
-  std::vector<float> hd(64);
-  std::vector<uint32_t> hj(64);
+  std::vector<uint64_t> keys(64);
   int bad = 0;
   for (int qi = 0; qi < nq; ++qi) {
     float qx, qy, qz;
@@ -67,8 +66,8 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
     } else {
       qx = U(rng); qy = U(rng); qz = U(rng);
     }
-    Heap<1> h;
-    h.init(hd.data(), hj.data(), B.orig, k);
+    Sel<1, 16> h;
+    h.init(keys.data(), k);
     knn_global(B, np, qx, qy, qz, h);
     std::vector<std::pair<float, uint32_t>> all;
     for (int j = 0; j < np; ++j)
@@ -76,8 +75,8 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
     std::sort(all.begin(), all.end());
     std::set<uint32_t> want, got;
     for (int t = 0; t < k; ++t) want.insert(all[t].second);
-    for (int t = 0; t < h.cnt; ++t) got.insert(orig[h.pos(t)]);
-    if (want != got || h.cnt != k || h.wd != all[k - 1].first) ++bad;
+    for (int t = 0; t < h.cnt; ++t) got.insert(h.id(t));
+    if (want != got || h.cnt != k || h.wd() != all[k - 1].first) ++bad;
   }
   printf("n=%d k=%d periodic=%d lattice=%d: %d/%d wrong\n", n, k, periodic, lattice, bad, nq);
   return bad ? 1 : 0;
@@ -92,6 +91,8 @@ int main() {
       fails += run(12, 64, 30.0f, per, lat, 3, 300);
       fails += run(5, 32, 10.0f, per, lat, 4, 200);   // box smaller than the search
       fails += run(4, 16, 8.0f, per, lat, 5, 100);
+      fails += run(14, 21, 28.0f, per, lat, 6, 300);  // k not a multiple of the group size
+      fails += run(10, 47, 20.0f, per, lat, 7, 200);
     }
   return fails;
 }
