@@ -241,13 +241,16 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
 template <int K>
 __global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, const float* __restrict__ mass,
                                              float* pdens, float* hk) {
-  __shared__ uint64_t keys_s[K * KQ];
+  __shared__ uint64_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
   const int64_t i = xcd_block(blockIdx.x, gridDim.x) * KQ + threadIdx.x;
   if (i >= np) return;
   const float4 pq = B.pos[i];
   Sel<KQ, K / 4> h;
-  h.init(keys_s + threadIdx.x, k);
-  knn_global(B, np, pq.x, pq.y, pq.z, h);
+  h.init(keys_s + threadIdx.x, gmx_s + threadIdx.x, k);
+  if (B.periodic)
+    knn_global<1>(B, np, pq.x, pq.y, pq.z, h);
+  else
+    knn_global<0>(B, np, pq.x, pq.y, pq.z, h);
   const float dk = sqrtf(h.wd());  // kdtree.f90:539
   double bas8 = 0.0;
   for (int t = 0; t < h.cnt; ++t) bas8 += (double)mass[h.id(t)];
@@ -266,7 +269,7 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
                                             float dx, int k, int want_vel, const float4* __restrict__ vel,
                                             const int8_t* __restrict__ smask,
                                             uint32_t* hbits, float* u, float* v, float* w) {
-  __shared__ uint64_t keys_s[K * KQ];
+  __shared__ uint64_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
   const int ntx = (n + 7) / 8, nty = (n + 3) / 4;
   const int tid = threadIdx.x;
   const int64_t bt = xcd_block(blockIdx.x, gridDim.x);
@@ -277,8 +280,11 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
   if (smask && smask[c] == 0) return;
   const float qx = rad[ix + 1 - rlo], qy = rad[iy + 1 - rlo], qz = rad[iz + 1 - rlo];
   Sel<KQ, K / 4> h;
-  h.init(keys_s + tid, k);
-  knn_global(B, np, qx, qy, qz, h);
+  h.init(keys_s + tid, gmx_s + tid, k);
+  if (B.periodic)
+    knn_global<1>(B, np, qx, qy, qz, h);
+  else
+    knn_global<0>(B, np, qx, qy, qz, h);
   const float dk = sqrtf(h.wd());
   double sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
   auto serve = [&](uint32_t j, float d, float hh) {

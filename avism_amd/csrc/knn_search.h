@@ -17,6 +17,7 @@
 struct float4 {
   float x, y, z, w;
 };
+inline float4 make_float4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
 using std::max;
 using std::min;
 #endif
@@ -26,9 +27,9 @@ namespace avism {
 // per-axis separation, minimum image under periodic boundaries
 AVK_DEV float axis_d(float p, float q, float L, int periodic) {
   float d = p - q;
-  if (periodic) {
-    float a = fabsf(d), b = L - fabsf(d);
-    d = a < b ? a : b;
+  if (periodic) {  // the smaller of |d| and L - |d| (fminf: both are finite and non-negative)
+    const float a = fabsf(d);
+    d = fminf(a, L - a);
   }
   return d;
 }
@@ -105,22 +106,22 @@ AVK_DEV float u2f(uint32_t u) {
 // particle, on equal distances the one with the lower input index: the
 // deterministic form of kdtree.f90's strict-< admission.
 //
-// The 4*NG slots live in LDS (slot t of lane l at [t * BS + l], so every lane
-// hits its own bank) in NG groups of 4 whose maxima stay in registers.
-// Replacing the worst key costs one LDS store, a reload of its group's 4 keys
-// and register selects, with no data-dependent branches; a candidate farther
-// than the worst key costs one compare.  Slots at or beyond k hold key 0 and
-// never become a maximum.
+// The 4*NG slots and the NG group maxima live in LDS (entry t of lane l at
+// [t * BS + l], so every lane hits its own bank); only the worst key and its
+// group stay in registers.  A candidate farther than the worst key costs one
+// compare.  Replacing the worst key reads its group's 4 keys and the NG group
+// maxima and writes two entries, with no data-dependent branches.  Slots at or
+// beyond k hold key 0 and never become a maximum.
 template <int BS, int NG>
 struct Sel {
-  uint64_t* keys;     // this lane's slots in LDS
-  uint64_t gmax[NG];  // largest key of each group
-  int gpos[NG];       // its slot within the group
-  uint64_t worst;     // largest key held once full
+  uint64_t* keys;  // this lane's 4*NG slots in LDS
+  uint64_t* gmx;   // this lane's NG group maxima in LDS
+  uint64_t worst;  // largest key held once full (~0 while filling)
   int wg, k, cnt;
 
-  AVK_DEV void init(uint64_t* ks, int k_) {
+  AVK_DEV void init(uint64_t* ks, uint64_t* gm, int k_) {
     keys = ks;
+    gmx = gm;
     k = k_;
     reset();
   }
@@ -129,23 +130,20 @@ struct Sel {
     worst = ~0ull;
     wg = 0;
 #pragma unroll
-    for (int g = 0; g < NG; ++g) {
-      gmax[g] = 0ull;
-      gpos[g] = 0;
-    }
-#pragma unroll
     for (int t = 0; t < 4 * NG; ++t) keys[t * BS] = 0ull;
   }
   AVK_DEV bool full() const { return cnt >= k; }
   // squared distance of the worst key held (INFINITY while filling)
   AVK_DEV float wd() const { return cnt >= k ? u2f((uint32_t)(worst >> 32)) : INFINITY; }
+  // the largest group maximum and its group
   AVK_DEV void find_worst() {
-    uint64_t w = gmax[0];
+    uint64_t w = gmx[0];
     int g0 = 0;
 #pragma unroll
     for (int g = 1; g < NG; ++g) {
-      const bool b = gmax[g] > w;
-      w = b ? gmax[g] : w;
+      const uint64_t x = gmx[g * BS];
+      const bool b = x > w;
+      w = b ? x : w;
       g0 = b ? g : g0;
     }
     worst = w;
@@ -154,43 +152,41 @@ struct Sel {
   // candidate at squared distance v with input index o
   AVK_DEV void consider(float v, uint32_t o) {
     const uint32_t vb = f2u(v);
+    if (vb > (uint32_t)(worst >> 32)) return;  // also false for every candidate while filling
+    const uint64_t key = ((uint64_t)vb << 32) | o;
     if (cnt < k) {
-      const uint64_t key = ((uint64_t)vb << 32) | o;
-      const int slot = cnt++;
-      keys[slot * BS] = key;
-      const int g = slot >> 2;
+      keys[cnt * BS] = key;
+      if (++cnt == k) {  // full: the group maxima, then the worst
 #pragma unroll
-      for (int t = 0; t < NG; ++t) {
-        const bool b = (t == g) && key > gmax[t];
-        gmax[t] = b ? key : gmax[t];
-        gpos[t] = b ? (slot & 3) : gpos[t];
+        for (int g = 0; g < NG; ++g) {
+          uint64_t m = keys[(4 * g) * BS];
+#pragma unroll
+          for (int s = 1; s < 4; ++s) {
+            const uint64_t x = keys[(4 * g + s) * BS];
+            m = x > m ? x : m;
+          }
+          gmx[g * BS] = m;
+        }
+        find_worst();
       }
-      if (cnt == k) find_worst();
       return;
     }
-    if (vb > (uint32_t)(worst >> 32)) return;
-    const uint64_t key = ((uint64_t)vb << 32) | o;
     if (key >= worst) return;
-    int p = gpos[0];
+    // the worst key sits in group wg: replace it and recompute that group's maximum
+    uint64_t* grp = keys + (4 * wg) * BS;
+    uint64_t x[4];
 #pragma unroll
-    for (int t = 1; t < NG; ++t) p = (t == wg) ? gpos[t] : p;
-    const int base = 4 * wg;
-    keys[(base + p) * BS] = key;
-    // the group's new maximum
-    uint64_t m = keys[base * BS];
-    int mp = 0;
+    for (int s = 0; s < 4; ++s) x[s] = grp[s * BS];
+    int p = 0;
 #pragma unroll
-    for (int s = 1; s < 4; ++s) {
-      const uint64_t x = keys[(base + s) * BS];
-      const bool b = x > m;
-      m = b ? x : m;
-      mp = b ? s : mp;
-    }
+    for (int s = 1; s < 4; ++s) p = x[s] == worst ? s : p;
 #pragma unroll
-    for (int t = 0; t < NG; ++t) {
-      gmax[t] = (t == wg) ? m : gmax[t];
-      gpos[t] = (t == wg) ? mp : gpos[t];
-    }
+    for (int s = 0; s < 4; ++s) x[s] = s == p ? key : x[s];
+    grp[p * BS] = key;
+    uint64_t m = x[0];
+#pragma unroll
+    for (int s = 1; s < 4; ++s) m = x[s] > m ? x[s] : m;
+    gmx[wg * BS] = m;
     find_worst();
   }
   AVK_DEV float dist(int t) const { return u2f((uint32_t)(keys[t * BS] >> 32)); }
@@ -239,12 +235,13 @@ AVK_DEV int pick8(int a0, int a1, int a2, int a3, int a4, int a5, int a6, int a7
   return (g & 4) ? hi : lo;
 }
 
-template <class H>
+// PER: B.periodic as a compile-time constant
+template <int PER, class H>
 AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz, H& h) {
   const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
   const int nb = B.nb;
   const float e2 = 2.0f * B.eps;
-  const int smax = B.periodic ? (nb - 1) / 2 : nb;
+  const int smax = PER ? (nb - 1) / 2 : nb;
   for (int s = 0; s <= smax; ++s) {
     bool any = false;
     const int side = 2 * s + 1, nrow = side * side;
@@ -257,10 +254,10 @@ AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz,
         const int dy = r % side - s, dz = r / side - s;
         const int cy = by + dy, cz = bz + dz;
         bool on = r < nrow;
-        if (!B.periodic) on = on && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
+        if (!PER) on = on && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
         const bool edge = (dz == -s || dz == s || dy == -s || dy == s);
         int lo_b = bx - s, hi_b = bx + s;
-        if (!B.periodic) {
+        if (!PER) {
           lo_b = max(lo_b, 0);
           hi_b = min(hi_b, nb - 1);
         }
@@ -278,14 +275,14 @@ AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz,
             on = lo_b <= hi_b;
           }
         }
-        const int wy = B.periodic ? B.wrap(cy) : cy, wz = B.periodic ? B.wrap(cz) : cz;
+        const int wy = PER ? B.wrap(cy) : cy, wz = PER ? B.wrap(cz) : cz;
         const int rowb = nb * (wy + nb * wz);
         if (edge || s == 0) {
           row_slots(B, rowb, lo_b, hi_b, on, sa[2 * t], sb[2 * t], sv[2 * t], sa[2 * t + 1], sb[2 * t + 1],
                     sv[2 * t + 1]);
         } else {  // interior row of the shell: its two end bins
           const bool l = on && lo_b == bx - s, u = on && hi_b == bx + s;
-          const int wl = B.periodic ? B.wrap(bx - s) : bx - s, wu = B.periodic ? B.wrap(bx + s) : bx + s;
+          const int wl = PER ? B.wrap(bx - s) : bx - s, wu = PER ? B.wrap(bx + s) : bx + s;
           sa[2 * t] = rowb + wl;
           sb[2 * t] = rowb + wl + 1;
           sv[2 * t] = l;
@@ -303,18 +300,35 @@ AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz,
       // one flat pass over the batch's candidates: a lane moves on to its next
       // range as soon as one ends, so a wave pays for its longest lane's total
       // rather than for the longest lane of every range
+      unsigned nz = 0;  // the batch's non-empty ranges
+#pragma unroll
+      for (int t = 0; t < 2 * RB; ++t) nz |= (j1[t] > j0[t] ? 1u : 0u) << t;
       int g = -1, j = 0, je = 0;
-      for (;;) {
-        while (j >= je && g < 2 * RB - 1) {
-          ++g;
-          j = pick8(j0[0], j0[1], j0[2], j0[3], j0[4], j0[5], j0[6], j0[7], g);
-          je = pick8(j1[0], j1[1], j1[2], j1[3], j1[4], j1[5], j1[6], j1[7], g);
-        }
-        if (j >= je) break;
-        const float4 p = B.pos[j];
-        const uint32_t o = B.orig[j];
-        h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), o);
+      auto adv = [&]() {  // on to the next non-empty range once this one is done
+        if (j < je) return;
+        const unsigned rest = nz & (~0u << (g + 1));
+        if (rest == 0u) return;
+        g = __builtin_ctz(rest);
+        j = pick8(j0[0], j0[1], j0[2], j0[3], j0[4], j0[5], j0[6], j0[7], g);
+        je = pick8(j1[0], j1[1], j1[2], j1[3], j1[4], j1[5], j1[6], j1[7], g);
+      };
+      adv();
+      float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+      uint32_t o = 0;
+      if (j < je) {
+        p = B.pos[j];
+        o = B.orig[j];
+      }
+      while (j < je) {
+        const float4 pc = p;
+        const uint32_t oc = o;
         ++j;
+        adv();
+        if (j < je) {  // the next candidate's load is in flight while this one is considered
+          p = B.pos[j];
+          o = B.orig[j];
+        }
+        h.consider(dist2(pc.x, pc.y, pc.z, qx, qy, qz, B.L, PER), oc);
       }
     }
     if (h.full()) {
@@ -322,7 +336,7 @@ AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz,
       const float bd = fminf(b1, fminf(b2, b3));
       if (bd == INFINITY || (bd > 0.0f && h.wd() < bd * bd)) return;
     }
-    if (!B.periodic && !any && s > 0) {
+    if (!PER && !any && s > 0) {
       const int far = max(max(max(bx, nb - 1 - bx), max(by, nb - 1 - by)), max(bz, nb - 1 - bz));
       if (s > far) return;
     }
@@ -331,7 +345,7 @@ AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz,
   h.reset();
   for (int64_t j = 0; j < np; ++j) {
     const float4 p = B.pos[j];
-    h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic), B.orig[j]);
+    h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, PER), B.orig[j]);
   }
 }
 

@@ -56,7 +56,7 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
   B.pos = pos.data();
   B.orig = orig.data();
   B.start = start.data();
-  std::vector<uint64_t> keys(64);
+  std::vector<uint64_t> keys(64), gmx(16);
   int bad = 0;
   for (int qi = 0; qi < nq; ++qi) {
     float qx, qy, qz;
@@ -67,8 +67,11 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
       qx = U(rng); qy = U(rng); qz = U(rng);
     }
     Sel<1, 16> h;
-    h.init(keys.data(), k);
-    knn_global(B, np, qx, qy, qz, h);
+    h.init(keys.data(), gmx.data(), k);
+    if (periodic)
+      knn_global<1>(B, np, qx, qy, qz, h);
+    else
+      knn_global<0>(B, np, qx, qy, qz, h);
     std::vector<std::pair<float, uint32_t>> all;
     for (int j = 0; j < np; ++j)
       all.push_back({dist2(pos[j].x, pos[j].y, pos[j].z, qx, qy, qz, L, periodic), orig[j]});
