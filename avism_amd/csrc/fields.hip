@@ -239,7 +239,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
 }
 
 template <int K>
-__global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, const float* __restrict__ mass,
+__global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float r0, const float* __restrict__ mass,
                                              float* pdens, float* hk) {
   __shared__ uint64_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
   const int64_t i = xcd_block(blockIdx.x, gridDim.x) * KQ + threadIdx.x;
@@ -248,9 +248,9 @@ __global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, const f
   Sel<KQ, K / 4> h;
   h.init(keys_s + threadIdx.x, gmx_s + threadIdx.x, k);
   if (B.periodic)
-    knn_global<1>(B, np, pq.x, pq.y, pq.z, h);
+    knn_global<1>(B, np, pq.x, pq.y, pq.z, r0, h);
   else
-    knn_global<0>(B, np, pq.x, pq.y, pq.z, h);
+    knn_global<0>(B, np, pq.x, pq.y, pq.z, r0, h);
   const float dk = sqrtf(h.wd());  // kdtree.f90:539
   double bas8 = 0.0;
   for (int t = 0; t < h.cnt; ++t) bas8 += (double)mass[h.id(t)];
@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, const f
 // HPART bits and the velocity records are in input order.
 template <int K>
 __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad, int rlo,
-                                            float dx, int k, int want_vel, const float4* __restrict__ vel,
+                                            float dx, int k, float r0, int want_vel, const float4* __restrict__ vel,
                                             const int8_t* __restrict__ smask,
                                             uint32_t* hbits, float* u, float* v, float* w) {
   __shared__ uint64_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
@@ -282,9 +282,9 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
   Sel<KQ, K / 4> h;
   h.init(keys_s + tid, gmx_s + tid, k);
   if (B.periodic)
-    knn_global<1>(B, np, qx, qy, qz, h);
+    knn_global<1>(B, np, qx, qy, qz, r0, h);
   else
-    knn_global<0>(B, np, qx, qy, qz, h);
+    knn_global<0>(B, np, qx, qy, qz, r0, h);
   const float dk = sqrtf(h.wd());
   double sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
   auto serve = [&](uint32_t j, float d, float hh) {
@@ -733,13 +733,16 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   AV_CHECK_HIP(c, hipMemcpyAsync(rad, c->g.rad.data(), c->g.rad.size() * 4, hipMemcpyHostToDevice,
                                  c->stream));
   Bins B = make_bins(c);
+  // first search radius: 1.3 x the k-th neighbour distance at the mean density
+  const double rho = (double)np / ((double)c->g.L0 * c->g.L0 * c->g.L0);
+  const float r0 = (float)(1.3 * std::cbrt(3.0 * k / (4.0 * kPi * rho)));
   {
     StageTimer t(c, "ppart_dens");
     if (k <= 32)
-      k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, c->pm.as<float>(), c->pdens.as<float>(),
+      k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->pm.as<float>(), c->pdens.as<float>(),
                                                        c->hk.as<float>());
     else
-      k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, c->pm.as<float>(), c->pdens.as<float>(),
+      k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->pm.as<float>(), c->pdens.as<float>(),
                                                        c->hk.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
@@ -751,11 +754,11 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     StageTimer t(c, "vvel_sph_vw");
     const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
     if (k <= 32)
-      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1, c->vel4.as<float4>(),
+      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, 1, c->vel4.as<float4>(),
                                                nullptr, c->hpart.as<uint32_t>(),
                                                c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
     else
-      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, 1, c->vel4.as<float4>(),
+      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, 1, c->vel4.as<float4>(),
                                                nullptr, c->hpart.as<uint32_t>(),
                                                c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());

@@ -194,13 +194,20 @@ struct Sel {
 };
 
 // exact kNN of q over the bin-sorted particles (k <= number of particles).
-// Shells of bins around q's bin are searched until the k-th distance is below
-// the distance to everything outside them.  Within a shell, every row of bins
-// is one contiguous range of the sorted particles (two under the periodic
-// wrap), clipped to the bins whose box can still hold a closer particle; the
-// rows are taken RB at a time so that their 4*RB bin-start loads are in
-// flight together before the candidates are read, and their candidates are
-// then read in one flat pass.
+//
+// The rows of bins (fixed y and z bin, a run of x bins) around q's bin are
+// visited in square rings of increasing size in y-z, centre first.  A pass
+// takes every particle within a radius R of q: a row's x run is clipped to the
+// bins that can hold a particle within min(R, current k-th distance), and
+// candidates farther than R are skipped, so a pass considers each particle
+// within R exactly once.  If k particles were found the answer is exact (all
+// particles within R were seen); otherwise the next pass takes the shell
+// between R and 2R.  R starts at the caller's estimate of the k-th distance.
+// Each particle in a row run is one contiguous range of the sorted particles
+// (two under the periodic wrap); the rows are taken RB at a time so that their
+// 4*RB bin-start loads are in flight together, and their candidates are then
+// read in one flat pass, the next candidate's load in flight while the current
+// one is considered.
 constexpr int RB = 4;  // rows per batch (2 ranges each: pick8 below)
 
 // the sorted-particle range of bins xa..xb (unwrapped, xb - xa < nb) of the row
@@ -235,111 +242,132 @@ AVK_DEV int pick8(int a0, int a1, int a2, int a3, int a4, int a5, int a6, int a7
   return (g & 4) ? hi : lo;
 }
 
-// PER: B.periodic as a compile-time constant
+// rows of ring r >= 1 of the (2r+1)^2 square of rows, t in 0..8r-1: the
+// two lines at dz = -r and dz = r, then the columns at dy = -r and dy = r
+AVK_DEV void ring_row(int r, int t, int& dy, int& dz) {
+  const int side = 2 * r + 1;
+  if (t < side) {
+    dy = t - r;
+    dz = -r;
+  } else if (t < 2 * side) {
+    dy = t - side - r;
+    dz = r;
+  } else {
+    const int u = t - 2 * side;
+    dy = (u & 1) ? r : -r;
+    dz = (u >> 1) - r + 1;
+  }
+}
+
+// PER: B.periodic as a compile-time constant; r0: first search radius
 template <int PER, class H>
-AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz, H& h) {
+AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz, float r0, H& h) {
   const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
   const int nb = B.nb;
   const float e2 = 2.0f * B.eps;
-  const int smax = PER ? (nb - 1) / 2 : nb;
-  for (int s = 0; s <= smax; ++s) {
-    bool any = false;
-    const int side = 2 * s + 1, nrow = side * side;
-    for (int r0 = 0; r0 < nrow; r0 += RB) {
-      int sa[2 * RB], sb[2 * RB];
-      bool sv[2 * RB];
-#pragma unroll
-      for (int t = 0; t < RB; ++t) {
-        const int r = r0 + t;
-        const int dy = r % side - s, dz = r / side - s;
-        const int cy = by + dy, cz = bz + dz;
-        bool on = r < nrow;
-        if (!PER) on = on && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
-        const bool edge = (dz == -s || dz == s || dy == -s || dy == s);
-        int lo_b = bx - s, hi_b = bx + s;
-        if (!PER) {
-          lo_b = max(lo_b, 0);
-          hi_b = min(hi_b, nb - 1);
+  const float ib = 1.0f / B.bsz;
+  // rows and x runs reach at most rmax bins from q's bin (periodic: no bin twice)
+  const int rmax = PER ? (nb - 1) / 2 : nb;
+  float R2 = r0 * r0, lo2 = -1.0f;
+  for (;;) {
+    bool exhausted = true;
+    // ring 1 carries the centre row as well
+    for (int r = rmax < 1 ? 0 : 1; r <= rmax; ++r) {
+      if (r > 1) {  // every row of ring r is farther than the bound in y or z
+        const float lb = fminf(B.outside(qy, by, r - 1), B.outside(qz, bz, r - 1));
+        if (lb == INFINITY || (lb > 0.0f && lb * lb > fminf(R2, h.wd()))) {
+          exhausted = false;
+          break;
         }
-        on = on && lo_b <= hi_b;
-        any = any || on;
-        if (on && h.full()) {  // bins whose box can hold d^2 <= worst
+      }
+      const int nrow = r == 0 ? 1 : (r == 1 ? 9 : 8 * r);
+      for (int r0w = 0; r0w < nrow; r0w += RB) {
+        const float T2 = fminf(R2, h.wd());
+        int sa[2 * RB], sb[2 * RB];
+        bool sv[2 * RB];
+#pragma unroll
+        for (int t = 0; t < RB; ++t) {
+          const int row = r0w + t;
+          int dy = 0, dz = 0;
+          if (r > 1)
+            ring_row(r, row, dy, dz);
+          else if (r == 1 && row > 0)
+            ring_row(1, row - 1, dy, dz);
+          const int cy = by + dy, cz = bz + dz;
+          bool on = row < nrow;
+          if (!PER) on = on && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
+          int lo_b = bx - rmax, hi_b = bx + rmax;
+          if (!PER) {
+            lo_b = max(lo_b, 0);
+            hi_b = min(hi_b, nb - 1);
+          }
           const float ddy = B.box_d(qy, cy), ddz = B.box_d(qz, cz);
-          const float rem = h.wd() - (ddy * ddy + ddz * ddz);
+          const float rem = T2 - (ddy * ddy + ddz * ddz);
           if (rem < 0.0f) {
             on = false;
-          } else {
+          } else if (rem < INFINITY) {  // bins whose box can hold d^2 <= T2
             const float rr = sqrtf(rem);
-            lo_b = max(lo_b, (int)floorf((qx - rr - e2 - B.lo) / B.bsz));
-            hi_b = min(hi_b, (int)floorf((qx + rr + e2 - B.lo) / B.bsz));
-            on = lo_b <= hi_b;
+            lo_b = max(lo_b, (int)floorf((qx - rr - e2 - B.lo) * ib));
+            hi_b = min(hi_b, (int)floorf((qx + rr + e2 - B.lo) * ib));
           }
+          on = on && lo_b <= hi_b;
+          const int wy = PER ? B.wrap(cy) : cy, wz = PER ? B.wrap(cz) : cz;
+          row_slots(B, nb * (wy + nb * wz), lo_b, hi_b, on, sa[2 * t], sb[2 * t], sv[2 * t], sa[2 * t + 1],
+                    sb[2 * t + 1], sv[2 * t + 1]);
         }
-        const int wy = PER ? B.wrap(cy) : cy, wz = PER ? B.wrap(cz) : cz;
-        const int rowb = nb * (wy + nb * wz);
-        if (edge || s == 0) {
-          row_slots(B, rowb, lo_b, hi_b, on, sa[2 * t], sb[2 * t], sv[2 * t], sa[2 * t + 1], sb[2 * t + 1],
-                    sv[2 * t + 1]);
-        } else {  // interior row of the shell: its two end bins
-          const bool l = on && lo_b == bx - s, u = on && hi_b == bx + s;
-          const int wl = PER ? B.wrap(bx - s) : bx - s, wu = PER ? B.wrap(bx + s) : bx + s;
-          sa[2 * t] = rowb + wl;
-          sb[2 * t] = rowb + wl + 1;
-          sv[2 * t] = l;
-          sa[2 * t + 1] = rowb + wu;
-          sb[2 * t + 1] = rowb + wu + 1;
-          sv[2 * t + 1] = u;
+        int j0[2 * RB], j1[2 * RB];
+#pragma unroll
+        for (int g = 0; g < 2 * RB; ++g) {
+          j0[g] = sv[g] ? B.start[sa[g]] : 0;
+          j1[g] = sv[g] ? B.start[sb[g]] : 0;
         }
-      }
-      int j0[2 * RB], j1[2 * RB];
+        // one flat pass over the batch's candidates: a lane moves on to its next
+        // range as soon as one ends, so a wave pays for its longest lane's total
+        // rather than for the longest lane of every range
+        unsigned nz = 0;  // the batch's non-empty ranges
 #pragma unroll
-      for (int g = 0; g < 2 * RB; ++g) {
-        j0[g] = sv[g] ? B.start[sa[g]] : 0;
-        j1[g] = sv[g] ? B.start[sb[g]] : 0;
-      }
-      // one flat pass over the batch's candidates: a lane moves on to its next
-      // range as soon as one ends, so a wave pays for its longest lane's total
-      // rather than for the longest lane of every range
-      unsigned nz = 0;  // the batch's non-empty ranges
-#pragma unroll
-      for (int t = 0; t < 2 * RB; ++t) nz |= (j1[t] > j0[t] ? 1u : 0u) << t;
-      int g = -1, j = 0, je = 0;
-      auto adv = [&]() {  // on to the next non-empty range once this one is done
-        if (j < je) return;
-        const unsigned rest = nz & (~0u << (g + 1));
-        if (rest == 0u) return;
-        g = __builtin_ctz(rest);
-        j = pick8(j0[0], j0[1], j0[2], j0[3], j0[4], j0[5], j0[6], j0[7], g);
-        je = pick8(j1[0], j1[1], j1[2], j1[3], j1[4], j1[5], j1[6], j1[7], g);
-      };
-      adv();
-      float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-      uint32_t o = 0;
-      if (j < je) {
-        p = B.pos[j];
-        o = B.orig[j];
-      }
-      while (j < je) {
-        const float4 pc = p;
-        const uint32_t oc = o;
-        ++j;
+        for (int t = 0; t < 2 * RB; ++t) nz |= (j1[t] > j0[t] ? 1u : 0u) << t;
+        int g = -1, j = 0, je = 0;
+        auto adv = [&]() {  // on to the next non-empty range once this one is done
+          if (j < je) return;
+          const unsigned rest = nz & (~0u << (g + 1));
+          if (rest == 0u) return;
+          g = __builtin_ctz(rest);
+          j = pick8(j0[0], j0[1], j0[2], j0[3], j0[4], j0[5], j0[6], j0[7], g);
+          je = pick8(j1[0], j1[1], j1[2], j1[3], j1[4], j1[5], j1[6], j1[7], g);
+        };
         adv();
-        if (j < je) {  // the next candidate's load is in flight while this one is considered
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t o = 0;
+        if (j < je) {
           p = B.pos[j];
           o = B.orig[j];
         }
-        h.consider(dist2(pc.x, pc.y, pc.z, qx, qy, qz, B.L, PER), oc);
+        while (j < je) {
+          const float4 pc = p;
+          const uint32_t oc = o;
+          ++j;
+          adv();
+          if (j < je) {  // the next candidate's load is in flight while this one is considered
+            p = B.pos[j];
+            o = B.orig[j];
+          }
+          const float d2 = dist2(pc.x, pc.y, pc.z, qx, qy, qz, B.L, PER);
+          if (d2 <= R2 && d2 > lo2) h.consider(d2, oc);
+        }
       }
     }
     if (h.full()) {
-      const float b1 = B.outside(qx, bx, s), b2 = B.outside(qy, by, s), b3 = B.outside(qz, bz, s);
-      const float bd = fminf(b1, fminf(b2, b3));
+      // exact unless the +-rmax clip (periodic boxes) cut into the ball of the
+      // k-th distance: in x always, in y and z when the rings ran out
+      float bd = B.outside(qx, bx, rmax);
+      if (exhausted) bd = fminf(bd, fminf(B.outside(qy, by, rmax), B.outside(qz, bz, rmax)));
       if (bd == INFINITY || (bd > 0.0f && h.wd() < bd * bd)) return;
+      break;
     }
-    if (!PER && !any && s > 0) {
-      const int far = max(max(max(bx, nb - 1 - bx), max(by, nb - 1 - by)), max(bz, nb - 1 - bz));
-      if (s > far) return;
-    }
+    if (exhausted) break;  // the rings reached rmax without k particles
+    lo2 = R2;
+    R2 *= 4.0f;
   }
   // periodic box smaller than the search: every particle once
   h.reset();

@@ -2,6 +2,7 @@
 // checked against brute force: random and lattice (exact ties) particle sets,
 // periodic and not, k = 16, 21, 32, 47, 64, and boxes smaller than the search.
 // Prints one line per case; exit status = number of failing cases.
+#include <cmath>
 #include <cstdio>
 #include <random>
 #include <set>
@@ -11,7 +12,7 @@
 
 using namespace avism;
 
-static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, int nq) {
+static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, int nq, float rfac) {
   std::mt19937 rng(seed);
   std::uniform_real_distribution<float> U(-L / 2, L / 2);
   const int np = lattice ? n * n * n : n * n * n + 7;
@@ -57,6 +58,8 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
   B.orig = orig.data();
   B.start = start.data();
   std::vector<uint64_t> keys(64), gmx(16);
+  // first search radius: rfac x the k-th neighbour distance at the mean density
+  const float r0 = rfac * std::cbrt(3.0f * k / (4.0f * 3.14159265f * np / (L * L * L)));
   int bad = 0;
   for (int qi = 0; qi < nq; ++qi) {
     float qx, qy, qz;
@@ -69,9 +72,9 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
     Sel<1, 16> h;
     h.init(keys.data(), gmx.data(), k);
     if (periodic)
-      knn_global<1>(B, np, qx, qy, qz, h);
+      knn_global<1>(B, np, qx, qy, qz, r0, h);
     else
-      knn_global<0>(B, np, qx, qy, qz, h);
+      knn_global<0>(B, np, qx, qy, qz, r0, h);
     std::vector<std::pair<float, uint32_t>> all;
     for (int j = 0; j < np; ++j)
       all.push_back({dist2(pos[j].x, pos[j].y, pos[j].z, qx, qy, qz, L, periodic), orig[j]});
@@ -81,21 +84,24 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
     for (int t = 0; t < h.cnt; ++t) got.insert(h.id(t));
     if (want != got || h.cnt != k || h.wd() != all[k - 1].first) ++bad;
   }
-  printf("n=%d k=%d periodic=%d lattice=%d: %d/%d wrong\n", n, k, periodic, lattice, bad, nq);
+  printf("n=%d k=%d periodic=%d lattice=%d r0=%.3g: %d/%d wrong\n", n, k, periodic, lattice, r0, bad, nq);
   return bad ? 1 : 0;
 }
 
 int main() {
   int fails = 0;
-  for (int per = 0; per < 2; ++per)
-    for (int lat = 0; lat < 2; ++lat) {
-      fails += run(20, 16, 40.0f, per, lat, 1, 600);
-      fails += run(16, 32, 32.0f, per, lat, 2, 600);
-      fails += run(12, 64, 30.0f, per, lat, 3, 300);
-      fails += run(5, 32, 10.0f, per, lat, 4, 200);   // box smaller than the search
-      fails += run(4, 16, 8.0f, per, lat, 5, 100);
-      fails += run(14, 21, 28.0f, per, lat, 6, 300);  // k not a multiple of the group size
-      fails += run(10, 47, 20.0f, per, lat, 7, 200);
-    }
+  // first radius well below, near and above the k-th distance (retry passes,
+  // the usual single pass, pruning by the k-th distance alone)
+  for (float rf : {0.3f, 1.3f, 3.0f})
+    for (int per = 0; per < 2; ++per)
+      for (int lat = 0; lat < 2; ++lat) {
+        fails += run(20, 16, 40.0f, per, lat, 1, 400, rf);
+        fails += run(16, 32, 32.0f, per, lat, 2, 400, rf);
+        fails += run(12, 64, 30.0f, per, lat, 3, 200, rf);
+        fails += run(5, 32, 10.0f, per, lat, 4, 200, rf);   // box smaller than the search
+        fails += run(4, 16, 8.0f, per, lat, 5, 100, rf);
+        fails += run(14, 21, 28.0f, per, lat, 6, 200, rf);  // k not a multiple of the group size
+        fails += run(10, 47, 20.0f, per, lat, 7, 200, rf);
+      }
   return fails;
 }
