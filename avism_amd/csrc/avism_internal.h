@@ -77,13 +77,12 @@ struct avism_ctx {
   avism::DBuf px, py, pz, pvx, pvy, pvz, pm;   // converted units after upload
   avism::DBuf sbin_key, sbin_val, sbin_key2, sbin_val2, sort_tmp;
   avism::DBuf spos4, sorig;                    // sorted by bin
-  avism::DBuf vel4;                            // (vx, vy, vz, m/rho), input order
+  avism::DBuf vel4;                            // (vx, vy, vz, m/rho), sorted by bin
   avism::DBuf bin_start;                       // nb^3 + 1
   avism::DBuf bin_reach;                       // nb^3, DDENS reach per bin
   int32_t nb = 0;
   float bsz = 0;
-  avism::DBuf pdens, hk, norm;                 // sorted order
-  avism::DBuf hpart;                           // input order
+  avism::DBuf pdens, hk, hpart, norm;          // sorted order
   avism::DBuf scalars;                         // small device scalars
   // inner fields
   avism::DBuf u_mass;                          // mass -> density (before /MEANDENS)

@@ -118,51 +118,6 @@ struct Kahan {
   }
 };
 
-// every particle with d2 <= r2 (ball_search, kdtree.f90:1015-1072, 1126)
-template <class F>
-__device__ __forceinline__ void ball_visit(const Bins& B, int64_t np, float qx, float qy, float qz, float r,
-                                           float r2, F&& f) {
-  const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
-  const int nb = B.nb;
-  const int s = (int)ceilf((r + B.eps) / B.bsz) + 1;
-  if (B.periodic && 2 * s + 1 > nb) {
-    for (int64_t j = 0; j < np; ++j) {
-      const float4 p = B.pos[j];
-      const float d = dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic);
-      if (d <= r2) f((int)j, d);
-    }
-    return;
-  }
-  for (int dz = -s; dz <= s; ++dz) {
-    const int cz = bz + dz;
-    if (!B.periodic && (cz < 0 || cz >= nb)) continue;
-    const float ddz = B.box_d(qz, cz);
-    if (ddz > r) continue;
-    const int wz = B.periodic ? B.wrap(cz) : cz;
-    for (int dy = -s; dy <= s; ++dy) {
-      const int cy = by + dy;
-      if (!B.periodic && (cy < 0 || cy >= nb)) continue;
-      const float ddy = B.box_d(qy, cy);
-      if (ddy > r) continue;
-      const int wy = B.periodic ? B.wrap(cy) : cy;
-      for (int dxo = -s; dxo <= s; ++dxo) {
-        const int cx = bx + dxo;
-        if (!B.periodic && (cx < 0 || cx >= nb)) continue;
-        const float ddx = B.box_d(qx, cx);
-        if (ddx * ddx + ddy * ddy + ddz * ddz > r2 * 1.0001f + 1e-6f) continue;
-        const int wx = B.periodic ? B.wrap(cx) : cx;
-        const int bin = wx + nb * (wy + nb * wz);
-        const int e = B.start[bin + 1];
-        for (int j = B.start[bin]; j < e; ++j) {
-          const float4 p = B.pos[j];
-          const float d = dist2(p.x, p.y, p.z, qx, qy, qz, B.L, B.periodic);
-          if (d <= r2) f(j, d);
-        }
-      }
-    }
-  }
-}
-
 // ----------------------------------------------------------------- kernels
 // unit conversion (reader.f90:457-461), the box check (avism.f90:573-575) and
 // the largest |mass| (for the DDENS fixed point); grid-stride with one
@@ -238,63 +193,62 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
   return x * q + (x < r ? x : r) + y;
 }
 
-template <int K>
-__global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float r0, const float* __restrict__ mass,
-                                             float* pdens, float* hk) {
-  __shared__ uint64_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
-  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * KQ + threadIdx.x;
-  if (i >= np) return;
+template <int PER, int K>
+__device__ __forceinline__ void ppart_one(const Bins& B, int64_t np, int k, float r0, int64_t i, uint32_t* keys,
+                                          uint32_t* gmx, float* pdens, float* hk) {
   const float4 pq = B.pos[i];
   Sel<KQ, K / 4> h;
-  h.init(keys_s + threadIdx.x, gmx_s + threadIdx.x, k);
-  if (B.periodic)
-    knn_global<1>(B, np, pq.x, pq.y, pq.z, r0, h);
-  else
-    knn_global<0>(B, np, pq.x, pq.y, pq.z, r0, h);
-  const float dk = sqrtf(h.wd());  // kdtree.f90:539
-  double bas8 = 0.0;
-  for (int t = 0; t < h.cnt; ++t) bas8 += (double)mass[h.id(t)];
+  h.init(keys, gmx, k);
+  bool scanned;
+  const float dk2 = knn_kth<PER>(B, np, pq.x, pq.y, pq.z, r0, k, h, scanned);
+  double bas8 = 0.0;  // particles.f90:779-788
+  knn_each<PER>(B, np, pq.x, pq.y, pq.z, k, dk2, scanned, h,
+                [&](int, float, const float4& p) { bas8 += (double)p.w; });
+  const float dk = sqrtf(dk2);  // kdtree.f90:539
   hk[i] = dk;
   const double vol = (4.0 / 3.0 * kPi) * (double)((dk * dk) * dk);
   pdens[i] = (float)(bas8 / vol);  // particles.f90:790-794
+}
+
+template <int K>
+__global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float r0, float* pdens, float* hk) {
+  __shared__ uint32_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * KQ + threadIdx.x;
+  if (i >= np) return;
+  if (B.periodic)
+    ppart_one<1, K>(B, np, k, r0, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
+  else
+    ppart_one<0, K>(B, np, k, r0, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
 }
 
 // VVEL_INTERP_SPH_VW (particles.f90:806-930): per cell the k nearest (or,
 // where d_k <= DX, every particle within DX), HPART = max distance at which a
 // particle served a cell (atomic max on the float bits), and the FP64
 // weighted velocity average.  One cell per lane; a block covers 8 x 4 x 4 cells.
-// HPART bits and the velocity records are in input order.
-template <int K>
-__global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad, int rlo,
-                                            float dx, int k, float r0, int want_vel, const float4* __restrict__ vel,
-                                            const int8_t* __restrict__ smask,
-                                            uint32_t* hbits, float* u, float* v, float* w) {
-  __shared__ uint64_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
-  const int ntx = (n + 7) / 8, nty = (n + 3) / 4;
-  const int tid = threadIdx.x;
-  const int64_t bt = xcd_block(blockIdx.x, gridDim.x);
-  const int ix = (int)(bt % ntx) * 8 + tid % 8, iy = (int)((bt / ntx) % nty) * 4 + (tid / 8) % 4,
-            iz = (int)(bt / ((int64_t)ntx * nty)) * 4 + tid / 32;
-  if (ix >= n || iy >= n || iz >= n) return;
-  const size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
-  if (smask && smask[c] == 0) return;
-  const float qx = rad[ix + 1 - rlo], qy = rad[iy + 1 - rlo], qz = rad[iz + 1 - rlo];
+struct VvelOut {
+  const float4* __restrict__ vel;  // (vx, vy, vz, m/rho), bin-sorted
+  uint32_t* hbits;                 // HPART bits, bin-sorted
+  float *u, *v, *w;
+  int want_vel;
+};
+
+template <int PER, int K>
+__device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float r0, float dx, float qx, float qy,
+                                         float qz, size_t c, uint32_t* keys, uint32_t* gmx, const VvelOut& O) {
   Sel<KQ, K / 4> h;
-  h.init(keys_s + tid, gmx_s + tid, k);
-  if (B.periodic)
-    knn_global<1>(B, np, qx, qy, qz, r0, h);
-  else
-    knn_global<0>(B, np, qx, qy, qz, r0, h);
-  const float dk = sqrtf(h.wd());
+  h.init(keys, gmx, k);
+  bool scanned;
+  const float dk2 = knn_kth<PER>(B, np, qx, qy, qz, r0, k, h, scanned);
+  const float dk = sqrtf(dk2);
   double sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-  auto serve = [&](uint32_t j, float d, float hh) {
+  auto serve = [&](int j, float d, float hh) {
     // particles.f90:882-885; HPART only grows, so a value already at least d
     // (read without the atomic; a stale read is only smaller) makes the
     // atomic unnecessary
     const uint32_t db = __float_as_uint(d);
-    if (db > hbits[j]) atomicMax(&hbits[j], db);
-    if (want_vel) {
-      const float4 vv = vel[j];                             // vx, vy, vz, m/rho
+    if (db > O.hbits[j]) atomicMax(&O.hbits[j], db);
+    if (O.want_vel) {
+      const float4 vv = O.vel[j];                           // vx, vy, vz, m/rho
       const float qq = d / (hh / 2.0f);                     // particles.f90:941
       const float wgt = m4(qq) * vv.w;                      // 897-900
       const double w8 = (double)wgt;
@@ -305,46 +259,59 @@ __global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const fl
     }
   };
   if (dk > dx) {  // k neighbours (particles.f90:866-867)
-    for (int t = 0; t < h.cnt; ++t) serve(h.id(t), sqrtf(h.dist(t)), dk);
-  } else {        // ball of radius DX (particles.f90:868-878): every particle with d <= DX
+    knn_each<PER>(B, np, qx, qy, qz, k, dk2, scanned, h,
+                  [&](int j, float d2, const float4&) { serve(j, sqrtf(d2), dk); });
+  } else {  // ball of radius DX (particles.f90:868-878): every particle with d <= DX
     const float r2 = dx * dx;
     float maxd2 = 0.0f;
-    ball_visit(B, np, qx, qy, qz, dx, r2, [&](int j, float d2) { maxd2 = fmaxf(maxd2, d2); });
+    ball_each<PER>(B, np, qx, qy, qz, r2, [&](int, float d2, const float4&) { maxd2 = fmaxf(maxd2, d2); });
     const float hh = sqrtf(maxd2);
-    ball_visit(B, np, qx, qy, qz, dx, r2, [&](int j, float d2) { serve(B.orig[j], sqrtf(d2), hh); });
+    ball_each<PER>(B, np, qx, qy, qz, r2, [&](int j, float d2, const float4&) { serve(j, sqrtf(d2), hh); });
   }
-  if (want_vel) {
-    u[c] = (float)(sx / sw);
-    v[c] = (float)(sy / sw);
-    w[c] = (float)(sz / sw);
+  if (O.want_vel) {
+    O.u[c] = (float)(sx / sw);
+    O.v[c] = (float)(sy / sw);
+    O.w[c] = (float)(sz / sw);
   }
 }
 
-// the velocity record of every particle, in input order: (vx, vy, vz, m/rho),
-// m/rho being the SPH weight factor of particles.f90:897-900 (one FP32
-// division as there); pdens is in bin-sorted order
+template <int K>
+__global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad, int rlo,
+                                            float dx, int k, float r0, const int8_t* __restrict__ smask, VvelOut O) {
+  __shared__ uint32_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
+  const int ntx = (n + 7) / 8, nty = (n + 3) / 4;
+  const int tid = threadIdx.x;
+  const int64_t bt = xcd_block(blockIdx.x, gridDim.x);
+  const int ix = (int)(bt % ntx) * 8 + tid % 8, iy = (int)((bt / ntx) % nty) * 4 + (tid / 8) % 4,
+            iz = (int)(bt / ((int64_t)ntx * nty)) * 4 + tid / 32;
+  if (ix >= n || iy >= n || iz >= n) return;
+  const size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
+  if (smask && smask[c] == 0) return;
+  const float qx = rad[ix + 1 - rlo], qy = rad[iy + 1 - rlo], qz = rad[iz + 1 - rlo];
+  if (B.periodic)
+    vvel_one<1, K>(B, np, k, r0, dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
+  else
+    vvel_one<0, K>(B, np, k, r0, dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
+}
+
+// the velocity record of every particle, bin-sorted: (vx, vy, vz, m/rho), m/rho
+// being the SPH weight factor of particles.f90:897-900 (one FP32 division as
+// there)
 __global__ void k_velrec(int64_t np, const uint32_t* __restrict__ orig, const float4* __restrict__ pos,
                          const float* __restrict__ pdens, const float* __restrict__ vx,
                          const float* __restrict__ vy, const float* __restrict__ vz, float4* vel) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint32_t p = orig[i];
-  vel[p] = make_float4(vx[p], vy[p], vz[p], pos[i].w / pdens[i]);
+  vel[i] = make_float4(vx[p], vy[p], vz[p], pos[i].w / pdens[i]);
 }
 
 // FILL_H_ZEROS (particles.f90:398-431): h = distance to the k-th neighbour
-// where no cell was served.  hpart is in input order; hk (bin-sorted) is
-// overwritten with the final h for the DDENS pass.
-__global__ void k_fill_h(int64_t np, const uint32_t* __restrict__ orig, float* hpart, float* hk) {
+// where no cell was served (bin-sorted)
+__global__ void k_fill_h(int64_t np, float* hpart, const float* hk) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  const uint32_t p = orig[i];
-  const float h = hpart[p];
-  if (h > 0.0f) {
-    hk[i] = h;
-  } else {
-    hpart[p] = hk[i];
-  }
+  if (!(hpart[i] > 0.0f)) hpart[i] = hk[i];
 }
 
 // DDENS_INTERP_SPH pass 1 (particles.f90:479-547): per particle the NORM, in
@@ -400,7 +367,7 @@ __device__ __forceinline__ float ddens_w(const DdensGeom& G, float x, float y, f
 constexpr int DTX = 8, DTY = 8, DTZ = 4, DR = 4;
 constexpr int DBX = DTX + 2 * DR, DBY = DTY + 2 * DR, DBZ = DTZ + 2 * DR;
 
-__global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, const float* __restrict__ hsort,
+__global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, const float* __restrict__ hpart,
                                                       double scale, unsigned long long* acc, float* norm) {
   __shared__ unsigned long long box[DBX * DBY * DBZ];
   __shared__ int qrow[DTY * DTZ + 1], qbeg[DTY * DTZ];
@@ -435,7 +402,7 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
     while (qrow[r + 1] <= q) ++r;
     const int64_t i = qbeg[r] + (q - qrow[r]);
     const float4 p = B.pos[i];
-    const float hh = hsort[i];  // HPART, bin-sorted
+    const float hh = hpart[i];
     int ix, iy, iz, inc, reach;
     float inv_h, h2max;
     ddens_setup(G, p.x, p.y, p.z, hh, ix, iy, iz, inv_h, h2max, inc, reach);
@@ -739,11 +706,9 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   {
     StageTimer t(c, "ppart_dens");
     if (k <= 32)
-      k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->pm.as<float>(), c->pdens.as<float>(),
-                                                       c->hk.as<float>());
+      k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->pdens.as<float>(), c->hk.as<float>());
     else
-      k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->pm.as<float>(), c->pdens.as<float>(),
-                                                       c->hk.as<float>());
+      k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->pdens.as<float>(), c->hk.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   AV_CHECK_HIP(c, hipMemsetAsync(c->hpart.p, 0, (size_t)np * 4, c->stream));
@@ -753,20 +718,17 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
                                               c->pvz.as<float>(), c->vel4.as<float4>());
     StageTimer t(c, "vvel_sph_vw");
     const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
+    const VvelOut O{c->vel4.as<float4>(), c->hpart.as<uint32_t>(), c->fu.as<float>(), c->fv.as<float>(),
+                    c->fw.as<float>(), 1};
     if (k <= 32)
-      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, 1, c->vel4.as<float4>(),
-                                               nullptr, c->hpart.as<uint32_t>(),
-                                               c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
+      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
     else
-      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, 1, c->vel4.as<float4>(),
-                                               nullptr, c->hpart.as<uint32_t>(),
-                                               c->fu.as<float>(), c->fv.as<float>(), c->fw.as<float>());
+      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
     AV_CHECK_HIP(c, hipGetLastError());
   }
   {
     StageTimer t(c, "fill_h");
-    k_fill_h<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), c->hpart.as<float>(),
-                                              c->hk.as<float>());
+    k_fill_h<<<nblk(np), 256, 0, c->stream>>>(np, c->hpart.as<float>(), c->hk.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   DdensGeom G{rad, c->g.rlo, n, c->g.dx, c->g.rx1};
@@ -785,7 +747,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     StageTimer t(c, "ddens_scatter");
     k_fill_u64<<<nblk(ncell), 256, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 0ull);
     const unsigned ntiles = (unsigned)(((n + DTX - 1) / DTX) * ((n + DTY - 1) / DTY) * ((n + DTZ - 1) / DTZ));
-    k_ddens_scatter<<<ntiles, 256, 0, c->stream>>>(G, B, c->hk.as<float>(), scale,
+    k_ddens_scatter<<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
                                                    c->mass_fix.as<unsigned long long>(), c->norm.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
@@ -858,13 +820,12 @@ int get_particle_fields(avism_ctx* c, float* part_dens, float* hpart) {
   if (c->np <= 0 || !c->fields_ready) AV_FAIL(c, AVISM_ERR_STATE, "no particle fields computed");
   const int64_t np = c->np;
   float* tmp = (float*)c->sbin_key.ensure((size_t)np * 4);
-  if (part_dens) {  // bin-sorted on the device
-    k_scatter_orig<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), c->pdens.as<float>(), tmp);
-    AV_CHECK_HIP(c, hipMemcpyAsync(part_dens, tmp, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream));
-    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
-  }
-  if (hpart) {  // input order on the device
-    AV_CHECK_HIP(c, hipMemcpyAsync(hpart, c->hpart.p, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream));
+  float* srcs[2] = {c->pdens.as<float>(), c->hpart.as<float>()};  // bin-sorted on the device
+  float* dsts[2] = {part_dens, hpart};
+  for (int i = 0; i < 2; i++) {
+    if (!dsts[i]) continue;
+    k_scatter_orig<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), srcs[i], tmp);
+    AV_CHECK_HIP(c, hipMemcpyAsync(dsts[i], tmp, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream));
     AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   }
   return AVISM_OK;

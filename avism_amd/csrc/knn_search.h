@@ -100,48 +100,44 @@ AVK_DEV float u2f(uint32_t u) {
   return c.f;
 }
 
-// The k best candidates of one query (one lane), as 64-bit keys: the squared
-// distance (non-negative floats order as their bit patterns) over the
-// particle's input index.  Keys are unique, and the smaller key is the closer
-// particle, on equal distances the one with the lower input index: the
-// deterministic form of kdtree.f90's strict-< admission.
-//
+// The k smallest squared distances seen by one query (one lane), as their
+// bit patterns (non-negative floats order as their bits), with multiplicity.
 // The 4*NG slots and the NG group maxima live in LDS (entry t of lane l at
-// [t * BS + l], so every lane hits its own bank); only the worst key and its
-// group stay in registers.  A candidate farther than the worst key costs one
-// compare.  Replacing the worst key reads its group's 4 keys and the NG group
-// maxima and writes two entries, with no data-dependent branches.  Slots at or
-// beyond k hold key 0 and never become a maximum.
+// [t * BS + l], so every lane hits its own bank); only the worst value and its
+// group stay in registers.  A value not below the worst costs one compare;
+// replacing the worst reads its group's 4 slots and the NG group maxima and
+// writes two entries, with no data-dependent branches.  Slots at or beyond k
+// hold 0 and never become a maximum.  The same structure selects the m
+// smallest input indices among tied candidates.
 template <int BS, int NG>
 struct Sel {
-  uint64_t* keys;  // this lane's 4*NG slots in LDS
-  uint64_t* gmx;   // this lane's NG group maxima in LDS
-  uint64_t worst;  // largest key held once full (~0 while filling)
+  uint32_t* keys;  // this lane's 4*NG slots in LDS
+  uint32_t* gmx;   // this lane's NG group maxima in LDS
+  uint32_t worst;  // largest value held once full (~0 while filling)
   int wg, k, cnt;
 
-  AVK_DEV void init(uint64_t* ks, uint64_t* gm, int k_) {
+  AVK_DEV void init(uint32_t* ks, uint32_t* gm, int k_) {
     keys = ks;
     gmx = gm;
-    k = k_;
-    reset();
+    reset(k_);
   }
-  AVK_DEV void reset() {
+  AVK_DEV void reset(int k_) {
+    k = k_;
     cnt = 0;
-    worst = ~0ull;
+    worst = ~0u;
     wg = 0;
 #pragma unroll
-    for (int t = 0; t < 4 * NG; ++t) keys[t * BS] = 0ull;
+    for (int t = 0; t < 4 * NG; ++t) keys[t * BS] = 0u;
   }
   AVK_DEV bool full() const { return cnt >= k; }
-  // squared distance of the worst key held (INFINITY while filling)
-  AVK_DEV float wd() const { return cnt >= k ? u2f((uint32_t)(worst >> 32)) : INFINITY; }
-  // the largest group maximum and its group
+  // the k-th smallest value as a float (INFINITY while filling)
+  AVK_DEV float wd() const { return cnt >= k ? u2f(worst) : INFINITY; }
   AVK_DEV void find_worst() {
-    uint64_t w = gmx[0];
+    uint32_t w = gmx[0];
     int g0 = 0;
 #pragma unroll
     for (int g = 1; g < NG; ++g) {
-      const uint64_t x = gmx[g * BS];
+      const uint32_t x = gmx[g * BS];
       const bool b = x > w;
       w = b ? x : w;
       g0 = b ? g : g0;
@@ -149,20 +145,17 @@ struct Sel {
     worst = w;
     wg = g0;
   }
-  // candidate at squared distance v with input index o
-  AVK_DEV void consider(float v, uint32_t o) {
-    const uint32_t vb = f2u(v);
-    if (vb > (uint32_t)(worst >> 32)) return;  // also false for every candidate while filling
-    const uint64_t key = ((uint64_t)vb << 32) | o;
+  AVK_DEV void consider(uint32_t key) {
+    if (key >= worst) return;  // never true while filling
     if (cnt < k) {
       keys[cnt * BS] = key;
       if (++cnt == k) {  // full: the group maxima, then the worst
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-          uint64_t m = keys[(4 * g) * BS];
+          uint32_t m = keys[(4 * g) * BS];
 #pragma unroll
           for (int s = 1; s < 4; ++s) {
-            const uint64_t x = keys[(4 * g + s) * BS];
+            const uint32_t x = keys[(4 * g + s) * BS];
             m = x > m ? x : m;
           }
           gmx[g * BS] = m;
@@ -171,10 +164,9 @@ struct Sel {
       }
       return;
     }
-    if (key >= worst) return;
-    // the worst key sits in group wg: replace it and recompute that group's maximum
-    uint64_t* grp = keys + (4 * wg) * BS;
-    uint64_t x[4];
+    // a copy of the worst value sits in group wg: replace it, recompute the group's maximum
+    uint32_t* grp = keys + (4 * wg) * BS;
+    uint32_t x[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) x[s] = grp[s * BS];
     int p = 0;
@@ -183,31 +175,29 @@ struct Sel {
 #pragma unroll
     for (int s = 0; s < 4; ++s) x[s] = s == p ? key : x[s];
     grp[p * BS] = key;
-    uint64_t m = x[0];
+    uint32_t m = x[0];
 #pragma unroll
     for (int s = 1; s < 4; ++s) m = x[s] > m ? x[s] : m;
     gmx[wg * BS] = m;
     find_worst();
   }
-  AVK_DEV float dist(int t) const { return u2f((uint32_t)(keys[t * BS] >> 32)); }
-  AVK_DEV uint32_t id(int t) const { return (uint32_t)keys[t * BS]; }
 };
 
-// exact kNN of q over the bin-sorted particles (k <= number of particles).
+// Exact kNN of q over the bin-sorted particles (k <= number of particles), in
+// two steps: the k-th smallest squared distance dk2 (knn_kth), then a sweep of
+// the ball d^2 <= dk2 that hands the k nearest to the caller (knn_each).  The
+// neighbours are those with d^2 < dk2 and, of those with d^2 == dk2, the ones
+// with the lowest input indices: the deterministic form of kdtree.f90's
+// strict-< admission.
 //
-// The rows of bins (fixed y and z bin, a run of x bins) around q's bin are
-// visited in square rings of increasing size in y-z, centre first.  A pass
-// takes every particle within a radius R of q: a row's x run is clipped to the
-// bins that can hold a particle within min(R, current k-th distance), and
-// candidates farther than R are skipped, so a pass considers each particle
-// within R exactly once.  If k particles were found the answer is exact (all
-// particles within R were seen); otherwise the next pass takes the shell
-// between R and 2R.  R starts at the caller's estimate of the k-th distance.
-// Each particle in a row run is one contiguous range of the sorted particles
-// (two under the periodic wrap); the rows are taken RB at a time so that their
-// 4*RB bin-start loads are in flight together, and their candidates are then
-// read in one flat pass, the next candidate's load in flight while the current
-// one is considered.
+// Sweeps visit the rows of bins (fixed y and z bin, a run of x bins) around
+// q's bin in square rings of increasing size in y-z, centre first, each row
+// clipped to the x bins that can hold a particle within the visitor's current
+// bound.  Each row run is one contiguous range of the sorted particles (two
+// under the periodic wrap); the rows are taken RB at a time so that their 4*RB
+// bin-start loads are in flight together, and their candidates are read in
+// one flat pass, the next candidate's load in flight while the current one is
+// handled.
 constexpr int RB = 4;  // rows per batch (2 ranges each: pick8 below)
 
 // the sorted-particle range of bins xa..xb (unwrapped, xb - xa < nb) of the row
@@ -259,122 +249,247 @@ AVK_DEV void ring_row(int r, int t, int& dy, int& dz) {
   }
 }
 
-// PER: B.periodic as a compile-time constant; r0: first search radius
-template <int PER, class H>
-AVK_DEV void knn_global(const Bins& B, int64_t np, float qx, float qy, float qz, float r0, H& h) {
+// rows and x runs reach at most rmax bins from q's bin (periodic: no bin twice)
+template <int PER>
+AVK_DEV int sweep_rmax(const Bins& B) {
+  return PER ? (B.nb - 1) / 2 : B.nb;
+}
+
+// One sweep: every candidate in the visited rows goes to v.cand(d2, j, p) (j =
+// sorted position, p = its record), rows pruned by v.bound() (re-read per
+// batch, so it may shrink).  Returns true if the rings ran out at rmax, false
+// if they stopped because every further row is beyond the bound.
+template <int PER, class V>
+AVK_DEV bool ring_sweep(const Bins& B, float qx, float qy, float qz, V& v) {
   const int bx = B.bin_of(qx), by = B.bin_of(qy), bz = B.bin_of(qz);
   const int nb = B.nb;
   const float e2 = 2.0f * B.eps;
   const float ib = 1.0f / B.bsz;
-  // rows and x runs reach at most rmax bins from q's bin (periodic: no bin twice)
-  const int rmax = PER ? (nb - 1) / 2 : nb;
-  float R2 = r0 * r0, lo2 = -1.0f;
-  for (;;) {
-    bool exhausted = true;
-    // ring 1 carries the centre row as well
-    for (int r = rmax < 1 ? 0 : 1; r <= rmax; ++r) {
-      if (r > 1) {  // every row of ring r is farther than the bound in y or z
-        const float lb = fminf(B.outside(qy, by, r - 1), B.outside(qz, bz, r - 1));
-        if (lb == INFINITY || (lb > 0.0f && lb * lb > fminf(R2, h.wd()))) {
-          exhausted = false;
-          break;
+  const int rmax = sweep_rmax<PER>(B);
+  // ring 1 carries the centre row as well
+  for (int r = rmax < 1 ? 0 : 1; r <= rmax; ++r) {
+    if (r > 1) {  // every row of ring r is beyond the bound in y or z
+      const float lb = fminf(B.outside(qy, by, r - 1), B.outside(qz, bz, r - 1));
+      if (lb == INFINITY || (lb > 0.0f && lb * lb > v.bound())) return false;
+    }
+    const int nrow = r == 0 ? 1 : (r == 1 ? 9 : 8 * r);
+    for (int r0w = 0; r0w < nrow; r0w += RB) {
+      const float T2 = v.bound();
+      int sa[2 * RB], sb[2 * RB];
+      bool sv[2 * RB];
+#pragma unroll
+      for (int t = 0; t < RB; ++t) {
+        const int row = r0w + t;
+        int dy = 0, dz = 0;
+        if (r > 1)
+          ring_row(r, row, dy, dz);
+        else if (r == 1 && row > 0)
+          ring_row(1, row - 1, dy, dz);
+        const int cy = by + dy, cz = bz + dz;
+        bool on = row < nrow;
+        if (!PER) on = on && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
+        int lo_b = bx - rmax, hi_b = bx + rmax;
+        if (!PER) {
+          lo_b = max(lo_b, 0);
+          hi_b = min(hi_b, nb - 1);
         }
+        const float ddy = B.box_d(qy, cy), ddz = B.box_d(qz, cz);
+        const float rem = T2 - (ddy * ddy + ddz * ddz);
+        if (rem < 0.0f) {
+          on = false;
+        } else if (rem < INFINITY) {  // bins whose box can hold d^2 <= T2
+          const float rr = sqrtf(rem);
+          lo_b = max(lo_b, (int)floorf((qx - rr - e2 - B.lo) * ib));
+          hi_b = min(hi_b, (int)floorf((qx + rr + e2 - B.lo) * ib));
+        }
+        on = on && lo_b <= hi_b;
+        const int wy = PER ? B.wrap(cy) : cy, wz = PER ? B.wrap(cz) : cz;
+        row_slots(B, nb * (wy + nb * wz), lo_b, hi_b, on, sa[2 * t], sb[2 * t], sv[2 * t], sa[2 * t + 1],
+                  sb[2 * t + 1], sv[2 * t + 1]);
       }
-      const int nrow = r == 0 ? 1 : (r == 1 ? 9 : 8 * r);
-      for (int r0w = 0; r0w < nrow; r0w += RB) {
-        const float T2 = fminf(R2, h.wd());
-        int sa[2 * RB], sb[2 * RB];
-        bool sv[2 * RB];
+      int j0[2 * RB], j1[2 * RB];
 #pragma unroll
-        for (int t = 0; t < RB; ++t) {
-          const int row = r0w + t;
-          int dy = 0, dz = 0;
-          if (r > 1)
-            ring_row(r, row, dy, dz);
-          else if (r == 1 && row > 0)
-            ring_row(1, row - 1, dy, dz);
-          const int cy = by + dy, cz = bz + dz;
-          bool on = row < nrow;
-          if (!PER) on = on && cy >= 0 && cy < nb && cz >= 0 && cz < nb;
-          int lo_b = bx - rmax, hi_b = bx + rmax;
-          if (!PER) {
-            lo_b = max(lo_b, 0);
-            hi_b = min(hi_b, nb - 1);
-          }
-          const float ddy = B.box_d(qy, cy), ddz = B.box_d(qz, cz);
-          const float rem = T2 - (ddy * ddy + ddz * ddz);
-          if (rem < 0.0f) {
-            on = false;
-          } else if (rem < INFINITY) {  // bins whose box can hold d^2 <= T2
-            const float rr = sqrtf(rem);
-            lo_b = max(lo_b, (int)floorf((qx - rr - e2 - B.lo) * ib));
-            hi_b = min(hi_b, (int)floorf((qx + rr + e2 - B.lo) * ib));
-          }
-          on = on && lo_b <= hi_b;
-          const int wy = PER ? B.wrap(cy) : cy, wz = PER ? B.wrap(cz) : cz;
-          row_slots(B, nb * (wy + nb * wz), lo_b, hi_b, on, sa[2 * t], sb[2 * t], sv[2 * t], sa[2 * t + 1],
-                    sb[2 * t + 1], sv[2 * t + 1]);
-        }
-        int j0[2 * RB], j1[2 * RB];
+      for (int g = 0; g < 2 * RB; ++g) {
+        j0[g] = sv[g] ? B.start[sa[g]] : 0;
+        j1[g] = sv[g] ? B.start[sb[g]] : 0;
+      }
+      // one flat pass over the batch's candidates: a lane moves on to its next
+      // range as soon as one ends, so a wave pays for its longest lane's total
+      // rather than for the longest lane of every range
+      unsigned nz = 0;  // the batch's non-empty ranges
 #pragma unroll
-        for (int g = 0; g < 2 * RB; ++g) {
-          j0[g] = sv[g] ? B.start[sa[g]] : 0;
-          j1[g] = sv[g] ? B.start[sb[g]] : 0;
-        }
-        // one flat pass over the batch's candidates: a lane moves on to its next
-        // range as soon as one ends, so a wave pays for its longest lane's total
-        // rather than for the longest lane of every range
-        unsigned nz = 0;  // the batch's non-empty ranges
-#pragma unroll
-        for (int t = 0; t < 2 * RB; ++t) nz |= (j1[t] > j0[t] ? 1u : 0u) << t;
-        int g = -1, j = 0, je = 0;
-        auto adv = [&]() {  // on to the next non-empty range once this one is done
-          if (j < je) return;
-          const unsigned rest = nz & (~0u << (g + 1));
-          if (rest == 0u) return;
-          g = __builtin_ctz(rest);
-          j = pick8(j0[0], j0[1], j0[2], j0[3], j0[4], j0[5], j0[6], j0[7], g);
-          je = pick8(j1[0], j1[1], j1[2], j1[3], j1[4], j1[5], j1[6], j1[7], g);
-        };
+      for (int t = 0; t < 2 * RB; ++t) nz |= (j1[t] > j0[t] ? 1u : 0u) << t;
+      int g = -1, j = 0, je = 0;
+      auto adv = [&]() {  // on to the next non-empty range once this one is done
+        if (j < je) return;
+        const unsigned rest = nz & (~0u << (g + 1));
+        if (rest == 0u) return;
+        g = __builtin_ctz(rest);
+        j = pick8(j0[0], j0[1], j0[2], j0[3], j0[4], j0[5], j0[6], j0[7], g);
+        je = pick8(j1[0], j1[1], j1[2], j1[3], j1[4], j1[5], j1[6], j1[7], g);
+      };
+      adv();
+      float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (j < je) p = B.pos[j];
+      while (j < je) {
+        const float4 pc = p;
+        const int jc = j;
+        ++j;
         adv();
-        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        uint32_t o = 0;
-        if (j < je) {
-          p = B.pos[j];
-          o = B.orig[j];
-        }
-        while (j < je) {
-          const float4 pc = p;
-          const uint32_t oc = o;
-          ++j;
-          adv();
-          if (j < je) {  // the next candidate's load is in flight while this one is considered
-            p = B.pos[j];
-            o = B.orig[j];
-          }
-          const float d2 = dist2(pc.x, pc.y, pc.z, qx, qy, qz, B.L, PER);
-          if (d2 <= R2 && d2 > lo2) h.consider(d2, oc);
-        }
+        if (j < je) p = B.pos[j];  // the next candidate's load is in flight while this one is handled
+        v.cand(dist2(pc.x, pc.y, pc.z, qx, qy, qz, B.L, PER), jc, pc);
       }
     }
-    if (h.full()) {
-      // exact unless the +-rmax clip (periodic boxes) cut into the ball of the
-      // k-th distance: in x always, in y and z when the rings ran out
-      float bd = B.outside(qx, bx, rmax);
-      if (exhausted) bd = fminf(bd, fminf(B.outside(qy, by, rmax), B.outside(qz, bz, rmax)));
-      if (bd == INFINITY || (bd > 0.0f && h.wd() < bd * bd)) return;
+  }
+  return true;
+}
+
+// after a sweep bounded by w: did it see every particle with d^2 <= w?  Only
+// the +-rmax clip of periodic boxes can cut into that ball: in x always, in y
+// and z when the rings ran out.
+template <int PER>
+AVK_DEV bool sweep_complete(const Bins& B, float qx, float qy, float qz, bool exhausted, float w) {
+  const int rmax = sweep_rmax<PER>(B);
+  float bd = B.outside(qx, B.bin_of(qx), rmax);
+  if (exhausted) bd = fminf(bd, fminf(B.outside(qy, B.bin_of(qy), rmax), B.outside(qz, B.bin_of(qz), rmax)));
+  return bd == INFINITY || (bd > 0.0f && w < bd * bd);
+}
+
+// the k-th smallest squared distance from q.  A pass takes every particle
+// within a radius R of q, the rows clipped to min(R, current k-th distance):
+// if k were found the result is exact, otherwise the next pass takes the
+// shell between R and 2R.  R starts at r0, the caller's estimate of the k-th
+// distance.  scanned: the ring sweeps could not cover the ball (a periodic box
+// smaller than the search) and every particle was scanned instead.
+template <int PER, class S>
+struct KthVisitor {
+  S& sel;
+  float R2, lo2;
+  AVK_DEV float bound() const { return fminf(R2, sel.wd()); }
+  AVK_DEV void cand(float d2, int, const float4&) {
+    if (d2 <= R2 && d2 > lo2) sel.consider(f2u(d2));
+  }
+};
+
+template <int PER, class S>
+AVK_DEV float knn_kth(const Bins& B, int64_t np, float qx, float qy, float qz, float r0, int k, S& sel,
+                      bool& scanned) {
+  sel.reset(k);
+  scanned = false;
+  KthVisitor<PER, S> v{sel, r0 * r0, -1.0f};
+  for (;;) {
+    const bool ex = ring_sweep<PER>(B, qx, qy, qz, v);
+    if (sel.full()) {
+      if (sweep_complete<PER>(B, qx, qy, qz, ex, sel.wd())) return sel.wd();
       break;
     }
-    if (exhausted) break;  // the rings reached rmax without k particles
-    lo2 = R2;
-    R2 *= 4.0f;
+    if (ex) break;  // the rings reached rmax without k particles
+    v.lo2 = v.R2;
+    v.R2 *= 4.0f;
   }
-  // periodic box smaller than the search: every particle once
-  h.reset();
+  scanned = true;
+  sel.reset(k);
   for (int64_t j = 0; j < np; ++j) {
     const float4 p = B.pos[j];
-    h.consider(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, PER), B.orig[j]);
+    sel.consider(f2u(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, PER)));
   }
+  return sel.wd();
+}
+
+// visits every particle with d^2 <= w once (a ring sweep, or all particles)
+template <int PER, class V>
+AVK_DEV void ball_sweep(const Bins& B, int64_t np, float qx, float qy, float qz, bool scanned, V& v) {
+  if (!scanned) {
+    ring_sweep<PER>(B, qx, qy, qz, v);
+    return;
+  }
+  for (int64_t j = 0; j < np; ++j) {
+    const float4 p = B.pos[j];
+    v.cand(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, PER), (int)j, p);
+  }
+}
+
+template <class F>
+struct LessVisitor {  // d^2 < dk2: neighbours; d^2 == dk2: counted
+  float dk2;
+  F& f;
+  int nlt, neq, jeq;
+  float4 peq;
+  AVK_DEV float bound() const { return dk2; }
+  AVK_DEV void cand(float d2, int j, const float4& p) {
+    if (d2 < dk2) {
+      ++nlt;
+      f(j, d2, p);
+    } else if (d2 == dk2) {
+      ++neq;
+      jeq = j;
+      peq = p;
+    }
+  }
+};
+template <class S>
+struct TieSelVisitor {  // the m lowest input indices among d^2 == dk2
+  float dk2;
+  const uint32_t* orig;
+  S& sel;
+  AVK_DEV float bound() const { return dk2; }
+  AVK_DEV void cand(float d2, int j, const float4&) {
+    if (d2 == dk2) sel.consider(orig[j]);
+  }
+};
+template <class F>
+struct TieVisitor {  // d^2 == dk2 and input index <= omax
+  float dk2;
+  const uint32_t* orig;
+  uint32_t omax;
+  F& f;
+  AVK_DEV float bound() const { return dk2; }
+  AVK_DEV void cand(float d2, int j, const float4& p) {
+    if (d2 == dk2 && orig[j] <= omax) f(j, d2, p);
+  }
+};
+
+// f(j, d2, p) for each of the k nearest particles (j = sorted position, p = its
+// record): those with d^2 < dk2 in sweep order, then the ones chosen among the
+// ties at dk2.  dk2 and scanned come from knn_kth; sel is free for reuse.
+template <int PER, class S, class F>
+AVK_DEV void knn_each(const Bins& B, int64_t np, float qx, float qy, float qz, int k, float dk2, bool scanned,
+                      S& sel, F&& f) {
+  LessVisitor<F> lv{dk2, f, 0, 0, 0, make_float4(0.f, 0.f, 0.f, 0.f)};
+  ball_sweep<PER>(B, np, qx, qy, qz, scanned, lv);
+  const int m = k - lv.nlt;  // ties to take, 1 <= m <= neq
+  if (lv.neq == 1) {  // the usual case: the k-th neighbour alone at dk2
+    f(lv.jeq, dk2, lv.peq);
+    return;
+  }
+  uint32_t omax = ~0u;
+  if (m < lv.neq) {
+    sel.reset(m);
+    TieSelVisitor<S> sv{dk2, B.orig, sel};
+    ball_sweep<PER>(B, np, qx, qy, qz, scanned, sv);
+    omax = sel.worst;
+  }
+  TieVisitor<F> tv{dk2, B.orig, omax, f};
+  ball_sweep<PER>(B, np, qx, qy, qz, scanned, tv);
+}
+
+template <class F>
+struct BallVisitor {  // every particle with d^2 <= r2
+  float r2;
+  F& f;
+  AVK_DEV float bound() const { return r2; }
+  AVK_DEV void cand(float d2, int j, const float4& p) {
+    if (d2 <= r2) f(j, d2, p);
+  }
+};
+
+// f(j, d2, p) for every particle with d^2 <= r2 (ball_search, kdtree.f90:
+// 1015-1072, 1126), in sweep order
+template <int PER, class F>
+AVK_DEV void ball_each(const Bins& B, int64_t np, float qx, float qy, float qz, float r2, F&& f) {
+  BallVisitor<F> v{r2, f};
+  // a ring sweep covers the ball unless the +-rmax clip cuts into it
+  ball_sweep<PER>(B, np, qx, qy, qz, !sweep_complete<PER>(B, qx, qy, qz, true, r2), v);
 }
 
 }  // namespace avism

@@ -1,10 +1,13 @@
-// Host build of the library's kNN search (avism_amd/csrc/knn_search.h)
-// checked against brute force: random and lattice (exact ties) particle sets,
+// Host build of the library's kNN search (avism_amd/csrc/knn_search.h): the
+// k nearest (k-th distance and neighbour set, each neighbour once) and the
+// fixed-radius ball, checked against brute force: random and lattice (exact
+// ties) particle sets,
 // periodic and not, k = 16, 21, 32, 47, 64, and boxes smaller than the search.
 // Prints one line per case; exit status = number of failing cases.
 #include <cmath>
 #include <cstdio>
 #include <random>
+#include <algorithm>
 #include <set>
 #include <vector>
 
@@ -57,7 +60,7 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
   B.pos = pos.data();
   B.orig = orig.data();
   B.start = start.data();
-  std::vector<uint64_t> keys(64), gmx(16);
+  std::vector<uint32_t> keys(64), gmx(16);
   // first search radius: rfac x the k-th neighbour distance at the mean density
   const float r0 = rfac * std::cbrt(3.0f * k / (4.0f * 3.14159265f * np / (L * L * L)));
   int bad = 0;
@@ -71,18 +74,32 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
     }
     Sel<1, 16> h;
     h.init(keys.data(), gmx.data(), k);
-    if (periodic)
-      knn_global<1>(B, np, qx, qy, qz, r0, h);
-    else
-      knn_global<0>(B, np, qx, qy, qz, r0, h);
+    bool scanned = false;
+    std::multiset<uint32_t> got, ball;
+    auto take = [&](int j, float, const float4&) { got.insert(orig[j]); };
+    const float r2 = 0.6f * B.bsz * B.bsz * (1 + qi % 5);
+    auto inball = [&](int j, float, const float4&) { ball.insert(orig[j]); };
+    float dk2;
+    if (periodic) {
+      dk2 = knn_kth<1>(B, np, qx, qy, qz, r0, k, h, scanned);
+      knn_each<1>(B, np, qx, qy, qz, k, dk2, scanned, h, take);
+      ball_each<1>(B, np, qx, qy, qz, r2, inball);
+    } else {
+      dk2 = knn_kth<0>(B, np, qx, qy, qz, r0, k, h, scanned);
+      knn_each<0>(B, np, qx, qy, qz, k, dk2, scanned, h, take);
+      ball_each<0>(B, np, qx, qy, qz, r2, inball);
+    }
     std::vector<std::pair<float, uint32_t>> all;
-    for (int j = 0; j < np; ++j)
-      all.push_back({dist2(pos[j].x, pos[j].y, pos[j].z, qx, qy, qz, L, periodic), orig[j]});
+    std::multiset<uint32_t> want_ball;
+    for (int j = 0; j < np; ++j) {
+      const float d = dist2(pos[j].x, pos[j].y, pos[j].z, qx, qy, qz, L, periodic);
+      all.push_back({d, orig[j]});
+      if (d <= r2) want_ball.insert(orig[j]);
+    }
     std::sort(all.begin(), all.end());
-    std::set<uint32_t> want, got;
+    std::multiset<uint32_t> want;
     for (int t = 0; t < k; ++t) want.insert(all[t].second);
-    for (int t = 0; t < h.cnt; ++t) got.insert(h.id(t));
-    if (want != got || h.cnt != k || h.wd() != all[k - 1].first) ++bad;
+    if (want != got || dk2 != all[k - 1].first || ball != want_ball) ++bad;
   }
   printf("n=%d k=%d periodic=%d lattice=%d r0=%.3g: %d/%d wrong\n", n, k, periodic, lattice, r0, bad, nq);
   return bad ? 1 : 0;
