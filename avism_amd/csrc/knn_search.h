@@ -24,6 +24,16 @@ using std::min;
 
 namespace avism {
 
+// square root for search bounds only: the hardware approximation (within 1
+// ulp), far inside the eps margins the bounds carry
+AVK_DEV float bound_sqrt(float x) {
+#if defined(__HIPCC__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+
 // per-axis separation, minimum image under periodic boundaries
 AVK_DEV float axis_d(float p, float q, float L, int periodic) {
   float d = p - q;
@@ -62,13 +72,10 @@ struct Bins {
     else if (b >= nb) b -= nb;
     return b;
   }
-  // squared distance from q to the (unwrapped) bin box at index b, axis-wise
+  // distance from q to the (unwrapped) bin box at index b along one axis
   AVK_DEV float box_d(float q, int b) const {
-    float blo = lo + (float)b * bsz - eps, bhi = lo + (float)(b + 1) * bsz + eps;
-    float d = 0.0f;
-    if (q < blo) d = blo - q;
-    else if (q > bhi) d = q - bhi;
-    return d;
+    const float blo = lo + (float)b * bsz - eps, bhi = lo + (float)(b + 1) * bsz + eps;
+    return fmaxf(fmaxf(blo - q, q - bhi), 0.0f);  // at most one of the two is positive
   }
   // lower bound of the distance from q (in bin b) to anything outside shells 0..s
   AVK_DEV float outside(float q, int b, int s) const {
@@ -298,7 +305,7 @@ AVK_DEV bool ring_sweep(const Bins& B, float qx, float qy, float qz, V& v) {
         if (rem < 0.0f) {
           on = false;
         } else if (rem < INFINITY) {  // bins whose box can hold d^2 <= T2
-          const float rr = sqrtf(rem);
+          const float rr = bound_sqrt(rem);
           lo_b = max(lo_b, (int)floorf((qx - rr - e2 - B.lo) * ib));
           hi_b = min(hi_b, (int)floorf((qx + rr + e2 - B.lo) * ib));
         }
