@@ -52,9 +52,11 @@ struct DBuf {
   void release();
 };
 
+// a timed stage: two events on the context stream, read when the timings are
+// fetched (so timing adds no synchronisation to the timed work)
 struct Timing {
   std::string name;
-  float ms;
+  hipEvent_t a, b;
 };
 
 }  // namespace avism

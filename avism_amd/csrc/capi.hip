@@ -52,10 +52,19 @@ int avism_create(avism_ctx** out, int32_t device) {
   return AVISM_OK;
 }
 
+static void drop_timings(avism_ctx* c) {
+  for (auto& t : c->timings) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  c->timings.clear();
+}
+
 int avism_destroy(avism_ctx* c) {
   if (!c) return AVISM_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  drop_timings(c);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return AVISM_OK;
@@ -66,7 +75,7 @@ const char* avism_last_error(avism_ctx* c) { return c ? c->err.c_str() : "null c
 int avism_set_timing(avism_ctx* c, int32_t on) {
   CTX_GUARD(c);
   c->timing_on = on != 0;
-  c->timings.clear();
+  drop_timings(c);
   return AVISM_OK;
 }
 
@@ -76,7 +85,10 @@ int avism_get_timings(avism_ctx* c, char* names, int32_t names_len, float* ms, i
   int32_t nwr = 0;
   std::string all;
   for (auto& t : c->timings) {
-    if (nwr < cap && ms) ms[nwr] = t.ms;
+    float tm = 0.0f;
+    (void)hipEventSynchronize(t.b);
+    (void)hipEventElapsedTime(&tm, t.a, t.b);
+    if (nwr < cap && ms) ms[nwr] = tm;
     all += t.name;
     all += ';';
     nwr++;
@@ -87,7 +99,7 @@ int avism_get_timings(avism_ctx* c, char* names, int32_t names_len, float* ms, i
     names[nb] = 0;
   }
   if (count) *count = nwr;
-  c->timings.clear();
+  drop_timings(c);
   return AVISM_OK;
 }
 

@@ -1,0 +1,109 @@
+"""Summaries of rocprofv3 runs of bench.py (kernel times and HBM traffic).
+
+    python -m avism_amd.profile_summary stats  <kernel_stats.csv> <out.md> <title> <command>
+    python -m avism_amd.profile_summary traffic <fetch_dir> <write_dir> <n> <k> <out.json> <command...>
+
+`stats` condenses a `rocprofv3 --kernel-trace --stats` kernel_stats CSV into a
+markdown table.  `traffic` reads the counter_collection CSVs of two `--pmc`
+runs (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch) and scales each
+counter by what it reads for a copy of known size: the n^3 x 4-byte
+device-to-device copies that `load_particles` makes (`__amd_rocclr_copyBuffer`
+dispatches), which move exactly n^3 x 4 bytes each way.  The result is the
+HBM bytes per launch of each field kernel, which bench.py reports as
+`roofline.traffic`.
+"""
+from __future__ import annotations
+
+import csv
+import json
+import re
+import sys
+from pathlib import Path
+
+# library kernel -> the stage name bench.py reports it under
+STAGE_OF = {
+    "k_ppart": "ppart_dens",
+    "k_vvel": "vvel_sph_vw",
+    "k_ddens_scatter": "ddens_scatter",
+    "k_ddens_finalize": "ddens_finalize",
+    "k_div_finalize": "mean_div",
+    "k_mark": "mark_flags",
+}
+
+
+def short(name: str) -> str:
+    n = name.replace("avism::(anonymous namespace)::", "").replace("avism::", "")
+    n = re.sub(r"\(.*", "", n)
+    if "rocprim" in n:
+        n = "rocprim::" + ("radix_sort" if "radix" in n else "scan" if "scan" in n else "other")
+    if n.startswith("void at::") or n.startswith("at::"):
+        n = "torch elementwise (input generation)"
+    if n.startswith("fft_"):
+        n = "rocFFT (input generation)"
+    return n.replace("void ", "")
+
+
+def stats(src: str, dst: str, title: str, cmd: str) -> None:
+    rows = list(csv.DictReader(open(src)))
+    agg: dict[str, list] = {}
+    for r in rows:
+        a = agg.setdefault(short(r["Name"]), [0, 0.0])
+        a[0] += int(r["Calls"])
+        a[1] += float(r["TotalDurationNs"])
+    tot = sum(v[1] for v in agg.values())
+    out = [f"# {title}", "", "Command (run from /tmp, TMPDIR=/tmp, repository at $R):", "", "```", cmd, "```", "",
+           "| kernel | calls | total ms | avg ms | % |", "|---|---:|---:|---:|---:|"]
+    for name, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:25]:
+        out.append(f"| {name} | {c} | {t / 1e6:.3f} | {t / 1e6 / c:.3f} | {100 * t / tot:.1f} |")
+    out.append(f"| (all kernels) | | {tot / 1e6:.3f} | | 100 |")
+    Path(dst).write_text("\n".join(out) + "\n")
+
+
+def _counters(d: str, counter: str) -> list[tuple[int, str, float]]:
+    f = next(Path(d).glob("**/*counter_collection.csv"))
+    return [(int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(open(f))
+            if r["Counter_Name"] == counter]
+
+
+def traffic(fetch_dir: str, write_dir: str, n: int, k: int, dst: str, cmd: str) -> dict:
+    fetch = _counters(fetch_dir, "FETCH_SIZE")
+    write = _counters(write_dir, "WRITE_SIZE")
+    copy_kib = n ** 3 * 4 / 1024.0
+    # the particle copies: copyBuffer dispatches that write exactly n^3 x 4 bytes;
+    # the two runs execute the same program, so the same dispatch ids are the
+    # same copies in the fetch run
+    ids = {i for i, name, v in write if "copyBuffer" in name and v == copy_kib}
+    fv_copy = [v for i, name, v in fetch if i in ids and "copyBuffer" in name]
+    if not ids or len(fv_copy) != len(ids):
+        raise SystemExit("no matching copies of n^3 x 4 bytes found for the calibration")
+    wc = copy_kib
+    fc = sum(fv_copy) / len(fv_copy)
+    ffac, wfac = copy_kib / fc, copy_kib / wc
+    kernels = {}
+    for base, stage in STAGE_OF.items():
+        fv = [v for _, name, v in fetch if short(name).split("<")[0] == base]
+        wv = [v for _, name, v in write if short(name).split("<")[0] == base]
+        if not fv or not wv:
+            continue
+        fk, wk = sum(fv) / len(fv), sum(wv) / len(wv)
+        kernels[stage] = {"kernel": base, "n": n, "k": k, "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+                          "bytes": int(round((fk * ffac + wk * wfac) * 1024))}
+    out = {"command": cmd,
+           "calibration": {"copy_bytes": n ** 3 * 4, "fetch_kib": round(fc, 1), "write_kib": round(wc, 1),
+                           "fetch_factor": round(ffac, 4), "write_factor": round(wfac, 4)},
+           "kernels": kernels}
+    Path(dst).write_text(json.dumps(out, indent=1) + "\n")
+    return out
+
+
+def main(argv: list[str]) -> None:
+    if argv[0] == "stats":
+        stats(argv[1], argv[2], argv[3], argv[4])
+    elif argv[0] == "traffic":
+        print(json.dumps(traffic(argv[1], argv[2], int(argv[3]), int(argv[4]), argv[5], " ".join(argv[6:]))))
+    else:
+        raise SystemExit(__doc__)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

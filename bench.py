@@ -157,22 +157,35 @@ def main():
     value = cells / (r["ms"] / 1e3)
 
     # roofline of the dominant kernel under the algorithmic byte model
+    # (per-unit figures: neighbour records gathered per query, cell records
+    # streamed per cell)
     k = args.k
     byte_model = {
         # gather model: k neighbour records (xyz, m, rho, velocity, HPART) + 3 outputs
         "vvel_sph_vw": cells * (k * 32 + 12 + k * 8),
+        # gather model: k neighbour records (xyz, m) + 1 output
         "ppart_dens": n ** 3 * (k * 16 + 4),
         # compulsory: particle records (x,y,z,m,h) and NORM out, cell sums filled and read
         "ddens_scatter": n ** 3 * 24 + cells * 16,
+        "ddens_finalize": cells * 8,
         "mark_flags": gi.next ** 3 * 9 + cells * 8,
         "mean_div": cells * 24,
     }
     stages = r["stages"]
     dom = max((s for s in stages if s in byte_model), key=lambda s: stages[s])
     achieved = byte_model[dom] / (stages[dom] / 1e3) / 1e9
+    traffic = None  # HBM bytes per launch of that kernel, from the committed counter profile
+    tfile = ROOT / "profiles" / "hbm_traffic.json"
+    if tfile.exists():
+        t = json.loads(tfile.read_text()).get("kernels", {}).get(dom)
+        if t and t.get("n") == n and t.get("k") == k:
+            traffic = t["bytes"]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_model": byte_model[dom], "kernel_ms": round(stages[dom], 3)}
+    # every modelled stage's algorithmic bytes over the whole step's wall time
+    e2e_bytes = sum(byte_model[s2] for s2 in stages if s2 in byte_model)
+    roofline["end_to_end_achieved"] = round(e2e_bytes / (r["ms"] / 1e3) / 1e9, 1)
     out = {
         "metric": "cells/s end-to-end at 512^3: learned-proposal vs plain GPU, identical voids",
         "value": value,
