@@ -6,18 +6,18 @@
 // density block avism.f90:1249-1250, 1371-1414.
 //
 // Design (MI355X):
-//  * particles are binned into a uniform grid of cells (stable radix sort by
-//    bin, then SoA float4 copies), so every kNN query walks Chebyshev shells of
-//    bins nearest-first with bounding-box pruning; the k best (squared FP32
-//    distance, particle index) pairs live in registers (one query per lane).
+//  * particles are binned into a uniform grid of cells (radix sort by bin, then
+//    float4 records in bin order), so a row of bins is one contiguous range;
+//    each kNN query (one lane) finds its k-th distance with a ring sweep of
+//    rows and a 32-bit selection in LDS, then sweeps that exact ball to hand
+//    its neighbours to the consumer (knn_search.h).
 //  * distances, kernel weights and the DDENS normalisation are evaluated in the
 //    reference's FP32 operation order (compiled without FMA contraction), so
 //    neighbour sets, HPART and NORM are bit-identical to the reference; the
-//    accumulations are FP32 with Kahan compensation instead of FP64.
-//  * DDENS is a gather: each cell sums the contributions of the particles
-//    whose support box covers it, in a fixed order -- deterministic, no float
-//    atomics.  HPART uses integer atomicMax on the float bits (order-free).
-//  * only the mean-density reduction runs in FP64.
+//    PPART and VVEL sums are FP64 as there.
+//  * DDENS deposits go into int64 fixed-point cell sums through an LDS copy of
+//    each block's tile (integer atomics: the sums do not depend on the order).
+//    HPART uses integer atomicMax on the float bits (order-free).
 #include <cstring>
 #include <algorithm>
 #include <cmath>
