@@ -197,7 +197,7 @@ __device__ __forceinline__ void ppart_one(const Bins& B, int64_t np, int k, floa
   bool scanned;
   const float dk2 = knn_kth<PER>(B, np, pq.x, pq.y, pq.z, r0, k, h, scanned);
   double bas8 = 0.0;  // particles.f90:779-788
-  knn_each<PER>(B, np, pq.x, pq.y, pq.z, k, dk2, scanned, h,
+  knn_each<PER>(B, np, pq.x, pq.y, pq.z, k, dk2, scanned, h, PosLoad{B.pos},
                 [&](int, float, const float4& p) { bas8 += (double)p.w; });
   const float dk = sqrtf(dk2);  // kdtree.f90:539
   hk[i] = dk;
@@ -237,13 +237,12 @@ __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float
   const float dk = sqrtf(dk2);
   double sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
   auto serve = [&](int j, float d, float hh) {
-    // particles.f90:882-885; HPART only grows, so a value already at least d
-    // (read without the atomic; a stale read is only smaller) makes the
-    // atomic unnecessary
-    const uint32_t db = __float_as_uint(d);
-    if (db > O.hbits[j]) atomicMax(&O.hbits[j], db);
+    // particles.f90:882-885.  The atomic returns nothing, so the lane does
+    // not wait for it (reading HPART first to skip it costs a load latency
+    // per neighbour, which measured slower).
+    const float4 vv = O.want_vel ? O.vel[j] : make_float4(0.f, 0.f, 0.f, 0.f);  // vx, vy, vz, m/rho
+    atomicMax(&O.hbits[j], __float_as_uint(d));
     if (O.want_vel) {
-      const float4 vv = O.vel[j];                           // vx, vy, vz, m/rho
       const float qq = d / (hh / 2.0f);                     // particles.f90:941
       const float wgt = m4(qq) * vv.w;                      // 897-900
       const double w8 = (double)wgt;
@@ -254,14 +253,16 @@ __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float
     }
   };
   if (dk > dx) {  // k neighbours (particles.f90:866-867)
-    knn_each<PER>(B, np, qx, qy, qz, k, dk2, scanned, h,
+    knn_each<PER>(B, np, qx, qy, qz, k, dk2, scanned, h, PosLoad{B.pos},
                   [&](int j, float d2, const float4&) { serve(j, sqrtf(d2), dk); });
   } else {  // ball of radius DX (particles.f90:868-878): every particle with d <= DX
     const float r2 = dx * dx;
     float maxd2 = 0.0f;
-    ball_each<PER>(B, np, qx, qy, qz, r2, [&](int, float d2, const float4&) { maxd2 = fmaxf(maxd2, d2); });
+    ball_each<PER>(B, np, qx, qy, qz, r2, PosLoad{B.pos},
+                   [&](int, float d2, const float4&) { maxd2 = fmaxf(maxd2, d2); });
     const float hh = sqrtf(maxd2);
-    ball_each<PER>(B, np, qx, qy, qz, r2, [&](int j, float d2, const float4&) { serve(j, sqrtf(d2), hh); });
+    ball_each<PER>(B, np, qx, qy, qz, r2, PosLoad{B.pos},
+                   [&](int j, float d2, const float4&) { serve(j, sqrtf(d2), hh); });
   }
   if (O.want_vel) {
     O.u[c] = (float)(sx / sw);

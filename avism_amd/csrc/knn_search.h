@@ -262,6 +262,17 @@ AVK_DEV int sweep_rmax(const Bins& B) {
   return PER ? (B.nb - 1) / 2 : B.nb;
 }
 
+// A candidate's record as a sweep loads it: by default the bin-sorted
+// (x, y, z, m); a consumer that needs more per neighbour (velocities, HPART)
+// supplies a loader whose record also carries those, so that they are loaded
+// ahead with the position.
+struct PosLoad {
+  using Rec = float4;
+  const float4* pos;
+  AVK_DEV Rec load(int j) const { return pos[j]; }
+  AVK_DEV static const float4& xyz(const Rec& r) { return r; }
+};
+
 // One sweep: every candidate in the visited rows goes to v.cand(d2, j, p) (j =
 // sorted position, p = its record), rows pruned by v.bound() (re-read per
 // batch, so it may shrink).  Returns true if the rings ran out at rmax, false
@@ -336,15 +347,16 @@ AVK_DEV bool ring_sweep(const Bins& B, float qx, float qy, float qz, V& v) {
         je = pick8(j1[0], j1[1], j1[2], j1[3], j1[4], j1[5], j1[6], j1[7], g);
       };
       adv();
-      float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (j < je) p = B.pos[j];
+      typename V::Rec p{};
+      if (j < je) p = v.ld.load(j);
       while (j < je) {
-        const float4 pc = p;
+        const typename V::Rec pc = p;
         const int jc = j;
         ++j;
         adv();
-        if (j < je) p = B.pos[j];  // the next candidate's load is in flight while this one is handled
-        v.cand(dist2(pc.x, pc.y, pc.z, qx, qy, qz, B.L, PER), jc, pc);
+        if (j < je) p = v.ld.load(j);  // the next candidate's load is in flight while this one is handled
+        const float4& x = V::Load::xyz(pc);
+        v.cand(dist2(x.x, x.y, x.z, qx, qy, qz, B.L, PER), jc, pc);
       }
     }
   }
@@ -370,10 +382,13 @@ AVK_DEV bool sweep_complete(const Bins& B, float qx, float qy, float qz, bool ex
 // smaller than the search) and every particle was scanned instead.
 template <int PER, class S>
 struct KthVisitor {
+  using Load = PosLoad;
+  using Rec = PosLoad::Rec;
+  PosLoad ld;
   S& sel;
   float R2, lo2;
   AVK_DEV float bound() const { return fminf(R2, sel.wd()); }
-  AVK_DEV void cand(float d2, int, const float4&) {
+  AVK_DEV void cand(float d2, int, const Rec&) {
     if (d2 <= R2 && d2 > lo2) sel.consider(f2u(d2));
   }
 };
@@ -383,7 +398,7 @@ AVK_DEV float knn_kth(const Bins& B, int64_t np, float qx, float qy, float qz, f
                       bool& scanned) {
   sel.reset(k);
   scanned = false;
-  KthVisitor<PER, S> v{sel, r0 * r0, -1.0f};
+  KthVisitor<PER, S> v{PosLoad{B.pos}, sel, r0 * r0, -1.0f};
   for (;;) {
     const bool ex = ring_sweep<PER>(B, qx, qy, qz, v);
     if (sel.full()) {
@@ -411,19 +426,23 @@ AVK_DEV void ball_sweep(const Bins& B, int64_t np, float qx, float qy, float qz,
     return;
   }
   for (int64_t j = 0; j < np; ++j) {
-    const float4 p = B.pos[j];
-    v.cand(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, PER), (int)j, p);
+    const typename V::Rec r = v.ld.load((int)j);
+    const float4& p = V::Load::xyz(r);
+    v.cand(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, PER), (int)j, r);
   }
 }
 
-template <class F>
+template <class L, class F>
 struct LessVisitor {  // d^2 < dk2: neighbours; d^2 == dk2: counted
+  using Load = L;
+  using Rec = typename L::Rec;
+  L ld;
   float dk2;
   F& f;
   int nlt, neq, jeq;
-  float4 peq;
+  Rec peq;
   AVK_DEV float bound() const { return dk2; }
-  AVK_DEV void cand(float d2, int j, const float4& p) {
+  AVK_DEV void cand(float d2, int j, const Rec& p) {
     if (d2 < dk2) {
       ++nlt;
       f(j, d2, p);
@@ -436,33 +455,40 @@ struct LessVisitor {  // d^2 < dk2: neighbours; d^2 == dk2: counted
 };
 template <class S>
 struct TieSelVisitor {  // the m lowest input indices among d^2 == dk2
+  using Load = PosLoad;
+  using Rec = PosLoad::Rec;
+  PosLoad ld;
   float dk2;
   const uint32_t* orig;
   S& sel;
   AVK_DEV float bound() const { return dk2; }
-  AVK_DEV void cand(float d2, int j, const float4&) {
+  AVK_DEV void cand(float d2, int j, const Rec&) {
     if (d2 == dk2) sel.consider(orig[j]);
   }
 };
-template <class F>
+template <class L, class F>
 struct TieVisitor {  // d^2 == dk2 and input index <= omax
+  using Load = L;
+  using Rec = typename L::Rec;
+  L ld;
   float dk2;
   const uint32_t* orig;
   uint32_t omax;
   F& f;
   AVK_DEV float bound() const { return dk2; }
-  AVK_DEV void cand(float d2, int j, const float4& p) {
+  AVK_DEV void cand(float d2, int j, const Rec& p) {
     if (d2 == dk2 && orig[j] <= omax) f(j, d2, p);
   }
 };
 
-// f(j, d2, p) for each of the k nearest particles (j = sorted position, p = its
-// record): those with d^2 < dk2 in sweep order, then the ones chosen among the
-// ties at dk2.  dk2 and scanned come from knn_kth; sel is free for reuse.
-template <int PER, class S, class F>
+// f(j, d2, r) for each of the k nearest particles (j = sorted position, r = its
+// record as ld loads it): those with d^2 < dk2 in sweep order, then the ones
+// chosen among the ties at dk2.  dk2 and scanned come from knn_kth; sel is
+// free for reuse.
+template <int PER, class S, class L, class F>
 AVK_DEV void knn_each(const Bins& B, int64_t np, float qx, float qy, float qz, int k, float dk2, bool scanned,
-                      S& sel, F&& f) {
-  LessVisitor<F> lv{dk2, f, 0, 0, 0, make_float4(0.f, 0.f, 0.f, 0.f)};
+                      S& sel, const L& ld, F&& f) {
+  LessVisitor<L, F> lv{ld, dk2, f, 0, 0, 0, typename L::Rec{}};
   ball_sweep<PER>(B, np, qx, qy, qz, scanned, lv);
   const int m = k - lv.nlt;  // ties to take, 1 <= m <= neq
   if (lv.neq == 1) {  // the usual case: the k-th neighbour alone at dk2
@@ -472,29 +498,32 @@ AVK_DEV void knn_each(const Bins& B, int64_t np, float qx, float qy, float qz, i
   uint32_t omax = ~0u;
   if (m < lv.neq) {
     sel.reset(m);
-    TieSelVisitor<S> sv{dk2, B.orig, sel};
+    TieSelVisitor<S> sv{PosLoad{B.pos}, dk2, B.orig, sel};
     ball_sweep<PER>(B, np, qx, qy, qz, scanned, sv);
     omax = sel.worst;
   }
-  TieVisitor<F> tv{dk2, B.orig, omax, f};
+  TieVisitor<L, F> tv{ld, dk2, B.orig, omax, f};
   ball_sweep<PER>(B, np, qx, qy, qz, scanned, tv);
 }
 
-template <class F>
+template <class L, class F>
 struct BallVisitor {  // every particle with d^2 <= r2
+  using Load = L;
+  using Rec = typename L::Rec;
+  L ld;
   float r2;
   F& f;
   AVK_DEV float bound() const { return r2; }
-  AVK_DEV void cand(float d2, int j, const float4& p) {
+  AVK_DEV void cand(float d2, int j, const Rec& p) {
     if (d2 <= r2) f(j, d2, p);
   }
 };
 
-// f(j, d2, p) for every particle with d^2 <= r2 (ball_search, kdtree.f90:
+// f(j, d2, r) for every particle with d^2 <= r2 (ball_search, kdtree.f90:
 // 1015-1072, 1126), in sweep order
-template <int PER, class F>
-AVK_DEV void ball_each(const Bins& B, int64_t np, float qx, float qy, float qz, float r2, F&& f) {
-  BallVisitor<F> v{r2, f};
+template <int PER, class L, class F>
+AVK_DEV void ball_each(const Bins& B, int64_t np, float qx, float qy, float qz, float r2, const L& ld, F&& f) {
+  BallVisitor<L, F> v{ld, r2, f};
   // a ring sweep covers the ball unless the +-rmax clip cuts into it
   ball_sweep<PER>(B, np, qx, qy, qz, !sweep_complete<PER>(B, qx, qy, qz, true, r2), v);
 }

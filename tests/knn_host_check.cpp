@@ -82,12 +82,12 @@ static int run(int n, int k, float L, int periodic, int lattice, unsigned seed, 
     float dk2;
     if (periodic) {
       dk2 = knn_kth<1>(B, np, qx, qy, qz, r0, k, h, scanned);
-      knn_each<1>(B, np, qx, qy, qz, k, dk2, scanned, h, take);
-      ball_each<1>(B, np, qx, qy, qz, r2, inball);
+      knn_each<1>(B, np, qx, qy, qz, k, dk2, scanned, h, PosLoad{B.pos}, take);
+      ball_each<1>(B, np, qx, qy, qz, r2, PosLoad{B.pos}, inball);
     } else {
       dk2 = knn_kth<0>(B, np, qx, qy, qz, r0, k, h, scanned);
-      knn_each<0>(B, np, qx, qy, qz, k, dk2, scanned, h, take);
-      ball_each<0>(B, np, qx, qy, qz, r2, inball);
+      knn_each<0>(B, np, qx, qy, qz, k, dk2, scanned, h, PosLoad{B.pos}, take);
+      ball_each<0>(B, np, qx, qy, qz, r2, PosLoad{B.pos}, inball);
     }
     std::vector<std::pair<float, uint32_t>> all;
     std::multiset<uint32_t> want_ball;
