@@ -93,6 +93,7 @@ struct avism_ctx {
   avism::DBuf marcap;                          // inner MARCAP (optional)
   bool has_marcap = false;
   float mass_max = 0;                          // max particle mass (code units)
+  bool mass_uniform = false;                   // every particle has the same mass
   avism::DBuf mass_fix;                        // DDENS cell sums, int64 fixed point
   float meandens = 0;                          // MEANDENS in Msun/Mpc^3
   float meandens_raw = 0;                      // MEANDENS before unit scaling

@@ -115,13 +115,16 @@ struct Kahan {
 
 // ----------------------------------------------------------------- kernels
 // unit conversion (reader.f90:457-461), the box check (avism.f90:573-575) and
-// the largest |mass| (for the DDENS fixed point); grid-stride with one
-// atomic per block, since same-address atomics serialise
+// the largest |mass| (for the DDENS fixed point), and whether every mass
+// equals the first (m0raw: the first mass as given, copied aside before the
+// conversion in place); grid-stride with one atomic per block, since
+// same-address atomics serialise
 __global__ void __launch_bounds__(256) k_convert_check(int64_t np, float* vx, float* vy, float* vz, float* m,
                                                       const float* x, const float* y, const float* z, float half,
-                                                      int* bad, unsigned* mmax) {
+                                                      int* bad, unsigned* mmax, const float* m0raw, int* mixed) {
   unsigned mb = 0;
-  bool out = false;
+  bool out = false, diff = false;
+  const float m0 = *m0raw / kUM;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
     const float mi = m[i] / kUM;
     m[i] = mi;
@@ -131,20 +134,23 @@ __global__ void __launch_bounds__(256) k_convert_check(int64_t np, float* vx, fl
     const float a = x[i], b = y[i], c = z[i];
     out = out || (a < -half || a > half || b < -half || b > half || c < -half || c > half);
     mb = max(mb, __float_as_uint(fabsf(mi)));  // non-negative floats order as their bits
+    diff = diff || __float_as_uint(mi) != __float_as_uint(m0);
   }
   for (int o = 32; o > 0; o >>= 1) mb = max(mb, (unsigned)__shfl_xor((int)mb, o));
   __shared__ unsigned wmax[4];
-  __shared__ int wbad[4];
-  const bool anyout = __ballot(out) != 0;
+  __shared__ int wbad[4], wdiff[4];
+  const bool anyout = __ballot(out) != 0, anydiff = __ballot(diff) != 0;
   if ((threadIdx.x & 63) == 0) {
     wmax[threadIdx.x >> 6] = mb;
     wbad[threadIdx.x >> 6] = anyout ? 1 : 0;
+    wdiff[threadIdx.x >> 6] = anydiff ? 1 : 0;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned bm = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
     if (bm) atomicMax(mmax, bm);
     if (wbad[0] | wbad[1] | wbad[2] | wbad[3]) atomicOr(bad, 1);
+    if (wdiff[0] | wdiff[1] | wdiff[2] | wdiff[3]) atomicOr(mixed, 1);
   }
 }
 
@@ -188,17 +194,23 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
   return x * q + (x < r ? x : r) + y;
 }
 
+// uniform: every particle has the same mass, so the FP64 sum of the k
+// nearest masses is k * m exactly (each partial sum j * m, j <= 64, needs at
+// most 30 significant bits) and the neighbours themselves are not needed
 template <int PER, int K>
-__device__ __forceinline__ void ppart_one(const Bins& B, int64_t np, int k, float r0, int64_t i, uint32_t* keys,
-                                          uint32_t* gmx, float* pdens, float* hk) {
+__device__ __forceinline__ void ppart_one(const Bins& B, int64_t np, int k, float r0, int uniform, int64_t i,
+                                          uint32_t* keys, uint32_t* gmx, float* pdens, float* hk) {
   const float4 pq = B.pos[i];
   Sel<KQ, K / 4> h;
   h.init(keys, gmx, k);
   bool scanned;
   const float dk2 = knn_kth<PER>(B, np, pq.x, pq.y, pq.z, r0, k, h, scanned);
   double bas8 = 0.0;  // particles.f90:779-788
-  knn_each<PER>(B, np, pq.x, pq.y, pq.z, k, dk2, scanned, h, PosLoad{B.pos},
-                [&](int, float, const float4& p) { bas8 += (double)p.w; });
+  if (uniform)
+    bas8 = (double)k * (double)pq.w;
+  else
+    knn_each<PER>(B, np, pq.x, pq.y, pq.z, k, dk2, scanned, h, PosLoad{B.pos},
+                  [&](int, float, const float4& p) { bas8 += (double)p.w; });
   const float dk = sqrtf(dk2);  // kdtree.f90:539
   hk[i] = dk;
   const double vol = (4.0 / 3.0 * kPi) * (double)((dk * dk) * dk);
@@ -206,14 +218,15 @@ __device__ __forceinline__ void ppart_one(const Bins& B, int64_t np, int k, floa
 }
 
 template <int K>
-__global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float r0, float* pdens, float* hk) {
+__global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float r0, int uniform, float* pdens,
+                                             float* hk) {
   __shared__ uint32_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
   const int64_t i = xcd_block(blockIdx.x, gridDim.x) * KQ + threadIdx.x;
   if (i >= np) return;
   if (B.periodic)
-    ppart_one<1, K>(B, np, k, r0, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
+    ppart_one<1, K>(B, np, k, r0, uniform, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
   else
-    ppart_one<0, K>(B, np, k, r0, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
+    ppart_one<0, K>(B, np, k, r0, uniform, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
 }
 
 // VVEL_INTERP_SPH_VW (particles.f90:806-930): per cell the k nearest (or,
@@ -605,17 +618,20 @@ int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, con
   }
   c->np = np;
   c->fields_ready = false;
-  int* bad = (int*)c->scalars.p;  // [0] out-of-box flag, [1] max |mass| bits
-  AV_CHECK_HIP(c, hipMemsetAsync(bad, 0, 8, c->stream));
+  // [0] out-of-box flag, [1] max |mass| bits, [2] the first mass as given, [3] masses differ
+  int* bad = (int*)c->scalars.p;
+  AV_CHECK_HIP(c, hipMemsetAsync(bad, 0, 16, c->stream));
+  AV_CHECK_HIP(c, hipMemcpyAsync(bad + 2, c->pm.p, 4, hipMemcpyDeviceToDevice, c->stream));
   k_convert_check<<<(unsigned)std::min<int64_t>(nblk(np), 2048), 256, 0, c->stream>>>(np, c->pvx.as<float>(), c->pvy.as<float>(),
                                                    c->pvz.as<float>(), c->pm.as<float>(),
                                                    c->px.as<float>(), c->py.as<float>(),
                                                    c->pz.as<float>(), c->g.L0 / 2.0f, bad,
-                                                   (unsigned*)bad + 1);
-  int hbad[2] = {0, 0};
-  AV_CHECK_HIP(c, hipMemcpyAsync(hbad, bad, 8, hipMemcpyDeviceToHost, c->stream));
+                                                   (unsigned*)bad + 1, (const float*)(bad + 2), bad + 3);
+  int hbad[4] = {0, 0, 0, 0};
+  AV_CHECK_HIP(c, hipMemcpyAsync(hbad, bad, 16, hipMemcpyDeviceToHost, c->stream));
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   memcpy(&c->mass_max, &hbad[1], 4);
+  c->mass_uniform = hbad[3] == 0;
   if (!std::isfinite(c->mass_max)) AV_FAIL(c, AVISM_ERR_INPUT, "non-finite particle mass");
   if (hbad[0]) AV_FAIL(c, AVISM_ERR_INPUT, "Particles outside the bounding box! Check L0! Particles must be inside [-L0/2, L0/2] in each direction.");
   return AVISM_OK;
@@ -702,9 +718,11 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   {
     StageTimer t(c, "ppart_dens");
     if (k <= 32)
-      k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->pdens.as<float>(), c->hk.as<float>());
+      k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->mass_uniform ? 1 : 0,
+                                                       c->pdens.as<float>(), c->hk.as<float>());
     else
-      k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->pdens.as<float>(), c->hk.as<float>());
+      k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->mass_uniform ? 1 : 0,
+                                                       c->pdens.as<float>(), c->hk.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   AV_CHECK_HIP(c, hipMemsetAsync(c->hpart.p, 0, (size_t)np * 4, c->stream));

@@ -81,6 +81,31 @@ def test_fields_vs_oracle(case):
     assert _rms_rel(f["div"], ref["div"]) <= 1e-4
 
 
+@pytest.mark.parametrize("per", [0, 1])
+def test_fields_mixed_masses(per):
+    """Particles of different masses: PPART sums the k nearest masses one by
+    one (with equal masses the library takes k * m, which is the same FP64
+    value); checked against the oracle."""
+    import pyoracle as O
+    from avism_amd import synth
+    n, L0 = 24, 48.0
+    p = synth.zeldovich(n, L0, seed=37)
+    arr = list(p.numpy())
+    rng = np.random.default_rng(38)
+    arr[6] = (arr[6] * rng.uniform(0.5, 1.5, arr[6].size)).astype(np.float32)
+    if not per:  # keep the particles inside the box without wrapping
+        for a in arr[:3]:
+            np.clip(a, -L0 / 2, L0 / 2 - 1e-4, out=a)
+    print(f"[fields mixed masses periodic={per}]", flush=True)
+    ref = O.fields_from_particles(*arr, n, L0, per, k=32)
+    _, f, pd, hp = _gpu_fields(arr, n, L0, per, 32)
+    assert np.array_equal(hp, ref["hpart"])
+    assert _rel(pd, ref["part_dens"]) <= REL
+    for c in ("u", "v", "w"):
+        assert _rms_rel(f[c], ref[c]) <= REL, c
+    assert _rel(f["u1co"], ref["u1co"]) <= REL
+
+
 def test_fields_deterministic():
     from avism_amd import synth
     p = synth.zeldovich(24, 48.0, seed=33)
