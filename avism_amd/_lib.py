@@ -43,7 +43,7 @@ class VoidParams(C.Structure):
 
 class LevelStats(C.Structure):
     _fields_ = [("ncand", C.c_int64), ("ncubes", C.c_int32), ("nvoid2", C.c_int32),
-                ("nvoidt", C.c_int32), ("pad", C.c_int32), ("volt_clean", C.c_double),
+                ("nvoidt", C.c_int32), ("nperiodic", C.c_int32), ("volt_clean", C.c_double),
                 ("meandens", C.c_float), ("ff", C.c_float)]
 
 

@@ -117,7 +117,7 @@ struct avism_ctx {
   avism::DBuf indice, rank_of, paintP, paintK, jstar, pi_ptr, group, uvoid, marca;
   avism::DBuf ev_key, ev_val, ev_key2, ev_val2, grp_start, vf, ncellv, removed;
   avism::DBuf ccl, cellid, cellid2, cellkey, cellkey2, vstart, inert, per_tmp;
-  int32_t nvoid2 = 0, nvoidt = 0;
+  int32_t nvoid2 = 0, nvoidt = 0, nperiodic = 0;
   double volt_clean = 0;
   int64_t growth_stats[8] = {0};
 };

@@ -619,14 +619,6 @@ __global__ void k_absent_removed(int nv, const int32_t* __restrict__ uvoid, cons
   if (v < nv) removed[v] = (uvoid[v] == -1 && !has[v]) ? 1 : 0;
 }
 
-__global__ void k_cell_pairs(int64_t ne3, const int32_t* __restrict__ marca, uint32_t* key, uint32_t* val) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne3) return;
-  int id = marca[e];
-  key[e] = id > 0 ? (uint32_t)id : 0xffffffffu;
-  val[e] = (uint32_t)e;
-}
-
 __global__ void k_label_ranges(int64_t ne3, const uint32_t* __restrict__ key, int32_t* lstart, int32_t* lend) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ne3) return;
@@ -648,7 +640,7 @@ __device__ __forceinline__ int resolve(const int32_t* lmap, int l) {
 __global__ void k_vp_flag(int nv, ExtG G, const int32_t* __restrict__ indice, const int32_t* __restrict__ uvoid,
                           const int32_t* __restrict__ marca_base, const float* __restrict__ gxc,
                           const float* __restrict__ gyc, const float* __restrict__ gzc, float half, float dx,
-                          float rx1, int32_t* flag, int4* info) {
+                          float rx1, int32_t* flag, int4* info, int32_t* vlab) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nv) return;
   const int id = indice[I];
@@ -666,6 +658,7 @@ __global__ void k_vp_flag(int nv, ExtG G, const int32_t* __restrict__ indice, co
         f = b2 > 0;
         // id, base label at the image of the centre, image shifts (+1 so they fit 2 bits)
         info[I] = make_int4(id, b2, (sx + 1) | ((sy + 1) << 2) | ((sz + 1) << 4), 0);
+        vlab[id - 1] = f;
       }
     }
   }
@@ -678,47 +671,199 @@ __global__ void k_vp_list(int nv, const int32_t* __restrict__ flag, const int32_
   if (I < nv && flag[I]) list[pos[I]] = info[I];
 }
 
-// one workgroup walks the flagged main voids in INDICE order
+// the cells of the flagged voids, as (label, ext cell) pairs, compacted with
+// one atomic per wave (their order is irrelevant: they are sorted by label next)
+__global__ void __launch_bounds__(256) k_vp_cells(int64_t ne3, const int32_t* __restrict__ marca,
+                                                 const int32_t* __restrict__ vlab, uint32_t* key, uint32_t* val,
+                                                 unsigned* count) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int id = 0;
+  bool sel = false;
+  if (e < ne3) {
+    id = marca[e];
+    sel = id > 0 && vlab[id - 1];
+  }
+  const unsigned long long m = __ballot(sel);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63, leader = __builtin_ctzll(m);
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(m));
+  base = __shfl(base, leader);
+  if (sel) {
+    const unsigned r = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    key[r] = (uint32_t)id;
+    val[r] = (uint32_t)e;
+  }
+}
+
+// per flagged void, its overlap test counted before any earlier void of the
+// list changes labels: cnt2 = cells whose image lies in the ext grid, and the
+// histogram of the base labels met at those images (up to VPT labels, else
+// -1).  The ordered pass then needs no recount for a void unless labels were
+// merged into it before its turn or its histogram overflowed: the cells
+// labelled ind are still its own, and a label's count goes to ind2 when it
+// resolves to ind2 (voidfinding.f90:1476-1480).
+constexpr int VPT = 16, VPW = 2 + 2 * VPT;  // cnt2, nlab (or -1), labels, counts
+__global__ void __launch_bounds__(256) k_vp_count(const int4* __restrict__ list, ExtG G,
+                                                 const int32_t* __restrict__ marca_base,
+                                                 const uint32_t* __restrict__ cells,
+                                                 const int32_t* __restrict__ lstart, const int32_t* __restrict__ lend,
+                                                 int32_t* pre) {
+  __shared__ int s_c2, s_ovf, s_lab[VPT], s_num[VPT];
+  const int L = blockIdx.x, tid = threadIdx.x;
+  const int4 in = list[L];
+  const int ind = in.x;
+  const int sx = (in.z & 3) - 1, sy = ((in.z >> 2) & 3) - 1, sz = ((in.z >> 4) & 3) - 1;
+  if (tid == 0) {
+    s_c2 = 0;
+    s_ovf = 0;
+  }
+  if (tid < VPT) {
+    s_lab[tid] = 0;
+    s_num[tid] = 0;
+  }
+  __syncthreads();
+  int c2 = 0;
+  const int qe = lend[ind - 1];
+  for (int q = lstart[ind - 1] + tid; q < qe; q += blockDim.x) {
+    const int cell = (int)cells[q];
+    const int i = cell % G.ne + G.low1, j = (cell / G.ne) % G.ne + G.low1, k = cell / (G.ne * G.ne) + G.low1;
+    const int a = i + sx * G.n, bb = j + sy * G.n, cc = k + sz * G.n;
+    if (a >= G.low1 && a <= G.low2 && bb >= G.low1 && bb <= G.low2 && cc >= G.low1 && cc <= G.low2) {
+      c2++;
+      const int l = marca_base[G.ext(a, bb, cc)];
+      if (l > 0) {  // histogram slot of l (a small open table in LDS)
+        int t = 0;
+        for (; t < VPT; ++t) {
+          const int old = atomicCAS(&s_lab[t], 0, l);
+          if (old == 0 || old == l) break;
+        }
+        if (t < VPT)
+          atomicAdd(&s_num[t], 1);
+        else
+          s_ovf = 1;
+      }
+    }
+  }
+  atomicAdd(&s_c2, c2);
+  __syncthreads();
+  if (tid == 0) {
+    int nt = 0;
+    while (nt < VPT && s_lab[nt] != 0) ++nt;
+    int32_t* o = pre + (size_t)L * VPW;
+    o[0] = s_c2;
+    o[1] = s_ovf ? -1 : nt;
+    for (int t = 0; t < VPT; ++t) {
+      o[2 + t] = s_lab[t];
+      o[2 + VPT + t] = s_num[t];
+    }
+  }
+}
+
+// the label map for the ordered pass: in LDS as 16-bit entries when the
+// labels fit (it is read in every resolve), else the global map itself
+struct VpMap {
+  uint16_t* s;  // LDS copy, or null
+  int32_t* g;
+  __device__ __forceinline__ int get(int l) const { return s ? (int)s[l - 1] : g[l - 1]; }
+  __device__ __forceinline__ void set(int l, int v) const {
+    if (s) s[l - 1] = (uint16_t)v;
+    g[l - 1] = v;
+  }
+  __device__ __forceinline__ int resolve(int l) const {
+    while (l > 0 && get(l) != l) l = get(l);
+    return l;
+  }
+};
+
+// CHANGE_BELONG(IND2, IND) when the overlap exceeds 0.4 (voidfinding.f90:
+// 1476-1480, 1520-1566): labels of IND resolve to IND2 from now on, IND's
+// member list is appended to IND2's, and IND2 is marked as having members.
+__device__ __forceinline__ void vp_decide(int ind, int ind2, int cnt, int cnt2, const VpMap& M, int32_t* mnext,
+                                          int32_t* mtail, int32_t* selfm, uint32_t* grown, int32_t* stats) {
+  const float inter = (cnt2 == 0) ? 0.0f : (float)cnt / (float)cnt2;
+  if (!(inter > 0.4f)) return;
+  stats[0]++;
+  if (ind2 == ind) {
+    selfm[ind - 1] = 1;
+    return;
+  }
+  M.set(ind, ind2);
+  mnext[mtail[ind2 - 1] - 1] = ind;
+  mtail[ind2 - 1] = mtail[ind - 1];
+  grown[(ind2 - 1) >> 5] |= 1u << ((ind2 - 1) & 31);
+}
+
+// one workgroup walks the flagged main voids in INDICE order (each id occurs
+// once).  Thread 0 decides every void whose counts follow from its histogram;
+// a void that gained members, or whose histogram overflowed, is counted again
+// by the whole workgroup with the labels of now.  Dynamic LDS: the grown
+// bitmap ((nv + 32) / 32 words; in grown_g instead when that is given, zeroed)
+// and the 16-bit map ((nv + 1) entries) when use_lds.
 __global__ void __launch_bounds__(1024) k_void_periodic(
-    const int4* __restrict__ list, const int32_t* __restrict__ nlist, ExtG G,
+    const int4* __restrict__ list, const int32_t* __restrict__ nlist, int nv, int use_lds, uint32_t* grown_g, ExtG G,
     const int32_t* __restrict__ marca_base, const uint32_t* __restrict__ cells,
-    const int32_t* __restrict__ lstart, const int32_t* __restrict__ lend, int32_t* lmap, int32_t* mnext,
-    int32_t* mtail, int32_t* selfm, int32_t* stats) {
-  __shared__ int s_cnt, s_cnt2, s_go, s_ind, s_ind2, s_sx, s_sy, s_sz;
+    const int32_t* __restrict__ lstart, const int32_t* __restrict__ lend, const int32_t* __restrict__ pre,
+    int32_t* lmap, int32_t* mnext, int32_t* mtail, int32_t* selfm, int32_t* stats) {
+  extern __shared__ uint32_t dyn[];
+  __shared__ int s_cnt, s_cnt2, s_go, s_ind, s_ind2, s_sx, s_sy, s_sz, s_next;
   const int tid = threadIdx.x;
   const int nl = *nlist;
-  for (int L = 0; L < nl; ++L) {
-    if (tid == 0) {  // the part of the test that earlier voids can change (voidfinding.f90:1476-1480)
-      const int4 in = list[L];
-      const int id = in.x;
+  const int nw = (nv + 32) / 32;
+  uint32_t* grown = grown_g ? grown_g : dyn;
+  uint16_t* lm16 = use_lds ? (uint16_t*)(grown_g ? dyn : dyn + nw) : nullptr;
+  if (!grown_g)
+    for (int t = tid; t < nw; t += blockDim.x) grown[t] = 0u;
+  if (use_lds)
+    for (int t = tid; t < nv; t += blockDim.x) lm16[t] = (uint16_t)lmap[t];
+  if (tid == 0) s_next = 0;
+  __syncthreads();
+  const VpMap M{lm16, lmap};
+  for (;;) {
+    if (tid == 0) {
+      int L = s_next;
       s_go = 0;
-      if (!selfm[id - 1]) {
-        s_go = 1;
-        s_ind = id;
-        s_ind2 = resolve(lmap, in.y);
-        s_sx = (in.z & 3) - 1;
-        s_sy = ((in.z >> 2) & 3) - 1;
-        s_sz = ((in.z >> 4) & 3) - 1;
+      for (; L < nl; ++L) {
+        const int4 in = list[L];
+        const int id = in.x;
+        const int32_t* pr = pre + (size_t)L * VPW;
+        const int ind2 = M.resolve(in.y);
+        const bool gr = (grown[(id - 1) >> 5] >> ((id - 1) & 31)) & 1u;
+        const int nlab = pr[1];
+        if (gr || nlab < 0) {  // count this one again (voidfinding.f90:1476-1480 with the labels of now)
+          s_go = 1;
+          s_ind = id;
+          s_ind2 = ind2;
+          s_sx = (in.z & 3) - 1;
+          s_sy = ((in.z >> 2) & 3) - 1;
+          s_sz = ((in.z >> 4) & 3) - 1;
+          break;
+        }
+        int cnt = 0;
+        for (int t = 0; t < nlab; ++t)
+          if (M.resolve(pr[2 + t]) == ind2) cnt += pr[2 + VPT + t];
+        vp_decide(id, ind2, cnt, pr[0], M, mnext, mtail, selfm, grown, stats);
       }
+      s_next = L;
       s_cnt = 0;
       s_cnt2 = 0;
     }
     __syncthreads();
-    if (s_go) {
+    if (!s_go) break;
+    {
       const int ind = s_ind, ind2 = s_ind2;
       int c1 = 0, c2 = 0;
       // cells labelled ind now: base labels of the member list of ind
       for (int b = ind; b > 0; b = mnext[b - 1]) {
         int s = lstart[b - 1], e = lend[b - 1];
         for (int q = s + tid; q < e; q += blockDim.x) {
-          int64_t cell = cells[q];
-          int i = (int)(cell % G.ne) + G.low1, j = (int)((cell / G.ne) % G.ne) + G.low1,
-              k = (int)(cell / ((int64_t)G.ne * G.ne)) + G.low1;
+          const int cell = (int)cells[q];
+          const int i = cell % G.ne + G.low1, j = (cell / G.ne) % G.ne + G.low1, k = cell / (G.ne * G.ne) + G.low1;
           int a = i + s_sx * G.n, bb = j + s_sy * G.n, cc = k + s_sz * G.n;
           if (a >= G.low1 && a <= G.low2 && bb >= G.low1 && bb <= G.low2 && cc >= G.low1 && cc <= G.low2) {
             c2++;
             int l = marca_base[G.ext(a, bb, cc)];
-            if (l > 0 && resolve(lmap, l) == ind2) c1++;
+            if (l > 0 && M.resolve(l) == ind2) c1++;
           }
         }
         if (b == mtail[ind - 1]) break;
@@ -727,18 +872,8 @@ __global__ void __launch_bounds__(1024) k_void_periodic(
       atomicAdd(&s_cnt2, c2);
       __syncthreads();
       if (tid == 0) {
-        float inter = (s_cnt2 == 0) ? 0.0f : (float)s_cnt / (float)s_cnt2;
-        if (inter > 0.4f) {  // CHANGE_BELONG(IND2, IND)
-          stats[0]++;
-          if (ind2 == ind) {
-            selfm[ind - 1] = 1;
-          } else {
-            lmap[ind - 1] = ind2;
-            // append ind's member list to ind2's
-            mnext[mtail[ind2 - 1] - 1] = ind;
-            mtail[ind2 - 1] = mtail[ind - 1];
-          }
-        }
+        vp_decide(ind, ind2, s_cnt, s_cnt2, M, mnext, mtail, selfm, grown, stats);
+        s_next = s_next + 1;
       }
     }
     __syncthreads();
@@ -823,6 +958,7 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
   const int64_t ne3 = (int64_t)g.next * g.next * g.next;
   c->nvoid2 = 0;
   c->nvoidt = 0;
+  c->nperiodic = 0;
   c->volt_clean = 0.0;
   if (!c->marca.ensure((size_t)ne3 * 4) || !c->vf.ensure((size_t)12 * (nv + 1) * 4) ||
       !c->indice.ensure((size_t)(nv + 1) * 4) || !c->uvoid.ensure((size_t)(nv + 1) * 4) ||
@@ -989,45 +1125,75 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     k_absent_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, has, removed);
     k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
     k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, marca);
-    // cell lists per label (sorted by label, then cell order)
-    if (!c->cellkey.ensure((size_t)ne3 * 4) || !c->cellkey2.ensure((size_t)ne3 * 4) ||
-        !c->cellid.ensure((size_t)ne3 * 4) || !c->cellid2.ensure((size_t)ne3 * 4))
-      AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (periodic)");
-    uint32_t* ka = c->cellkey.as<uint32_t>();
-    uint32_t* kb2 = c->cellkey2.as<uint32_t>();
-    uint32_t* va = c->cellid.as<uint32_t>();
-    uint32_t* vb = c->cellid2.as<uint32_t>();
-    k_cell_pairs<<<nbk(ne3), 256, 0, c->stream>>>(ne3, marca, ka, va);
-    size_t tmp = 0;
-    AV_CHECK_HIP(c, rocprim::radix_sort_pairs(nullptr, tmp, ka, kb2, va, vb, (size_t)ne3, 0, 32, c->stream));
-    if (!c->sort_tmp.ensure(tmp)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (periodic sort)");
-    AV_CHECK_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp.p, tmp, ka, kb2, va, vb, (size_t)ne3, 0, 32, c->stream));
-    int32_t* lstart = gend;
-    int32_t* lend = par2;
-    AV_CHECK_HIP(c, hipMemsetAsync(lstart, 0, (size_t)(nv + 1) * 4, c->stream));
-    AV_CHECK_HIP(c, hipMemsetAsync(lend, 0, (size_t)(nv + 1) * 4, c->stream));
-    k_label_ranges<<<nbk(ne3), 256, 0, c->stream>>>(ne3, kb2, lstart, lend);
     int32_t* selfm = removed;
     k_init_lmap<<<nbk(nv), 256, 0, c->stream>>>(nv, lmap, mnext, mtail, selfm);
-    AV_CHECK_HIP(c, hipMemsetAsync(small + 3, 0, 4, c->stream));
-    // statically possible images, compacted in INDICE order (the cell lists
-    // vb are no longer needed as sort input, ka/va are free)
-    int32_t* vflag = (int32_t*)ka;
-    int32_t* vpos = (int32_t*)va;
-    int4* vinfo = (int4*)(vflag + ((nv + 2 + 3) & ~3));
-    int4* vlist = vinfo + (nv + 1);
+    AV_CHECK_HIP(c, hipMemsetAsync(small + 3, 0, 8, c->stream));  // [3] merges, [4] cell pairs
+    // the statically possible images, compacted in INDICE order, and a flag per label
+    // per void (nv + 1 entries each): flag, scan position, label flag, counts
+    // (VPW ints), then 16-byte aligned the per-rank info and the compacted list
+    const size_t nv1 = (size_t)nv + 1;
+    const size_t ints = (nv1 * (3 + VPW) + 3) & ~(size_t)3;
+    if (!c->cellkey.ensure((size_t)ne3 * 4) || !c->cellkey2.ensure((size_t)ne3 * 4) ||
+        !c->cellid.ensure((size_t)ne3 * 4) || !c->cellid2.ensure((size_t)ne3 * 4) ||
+        !c->per_tmp.ensure(ints * 4 + nv1 * 32))
+      AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (periodic)");
+    int32_t* vflag = c->per_tmp.as<int32_t>();
+    int32_t* vpos = vflag + nv1;
+    int32_t* vlab = vpos + nv1;
+    int32_t* pre = vlab + nv1;
+    int4* vinfo = (int4*)(vflag + ints);
+    int4* vl = vinfo + nv1;
+    AV_CHECK_HIP(c, hipMemsetAsync(vlab, 0, (size_t)(nv + 1) * 4, c->stream));
     k_vp_flag<<<nbk(nv), 256, 0, c->stream>>>(nv, G, indice, uvoid, marca, GX, GY, GZ, g.L0 / 2.0f, g.dx, g.rx1,
-                                              vflag, vinfo);
+                                              vflag, vinfo, vlab);
     AV_CHECK_HIP(c, hipMemsetAsync(vflag + nv, 0, 4, c->stream));
     size_t tmp2 = 0;
     AV_CHECK_HIP(c, rocprim::exclusive_scan(nullptr, tmp2, vflag, vpos, 0, (size_t)nv + 1,
                                             rocprim::plus<int32_t>(), c->stream));
-    if (!c->sort_tmp.ensure(std::max(tmp, tmp2))) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (periodic)");
+    if (!c->sort_tmp.ensure(tmp2)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (periodic)");
     AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmp2, vflag, vpos, 0, (size_t)nv + 1,
                                             rocprim::plus<int32_t>(), c->stream));
-    k_vp_list<<<nbk(nv), 256, 0, c->stream>>>(nv, vflag, vpos, vinfo, vlist);
-    k_void_periodic<<<1, 1024, 0, c->stream>>>(vlist, vpos + nv, G, marca, vb, lstart, lend, lmap, mnext, mtail,
-                                                selfm, small + 3);
+    k_vp_list<<<nbk(nv), 256, 0, c->stream>>>(nv, vflag, vpos, vinfo, vl);
+    // cell lists of the flagged voids (at most ne3 pairs), sorted by label
+    uint32_t* ka = c->cellkey.as<uint32_t>();
+    uint32_t* kb2 = c->cellkey2.as<uint32_t>();
+    uint32_t* va = c->cellid.as<uint32_t>();
+    uint32_t* vb = c->cellid2.as<uint32_t>();
+    k_vp_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, marca, vlab, ka, va, (unsigned*)(small + 4));
+    int32_t hcnt[2] = {0, 0};  // flagged voids, their cells
+    AV_CHECK_HIP(c, hipMemcpyAsync(&hcnt[0], vpos + nv, 4, hipMemcpyDeviceToHost, c->stream));
+    AV_CHECK_HIP(c, hipMemcpyAsync(&hcnt[1], small + 4, 4, hipMemcpyDeviceToHost, c->stream));
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    const int nl = hcnt[0];
+    const int64_t mcell = hcnt[1];
+    if (nl > 0 && mcell > 0) {
+      unsigned bits = 1;
+      while ((1ll << bits) <= (int64_t)nv) bits++;
+      size_t tmp = 0;
+      AV_CHECK_HIP(c, rocprim::radix_sort_pairs(nullptr, tmp, ka, kb2, va, vb, (size_t)mcell, 0, bits, c->stream));
+      if (!c->sort_tmp.ensure(tmp)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (periodic sort)");
+      AV_CHECK_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp.p, tmp, ka, kb2, va, vb, (size_t)mcell, 0, bits,
+                                                c->stream));
+      int32_t* lstart = gend;
+      int32_t* lend = par2;
+      AV_CHECK_HIP(c, hipMemsetAsync(lstart, 0, (size_t)(nv + 1) * 4, c->stream));
+      AV_CHECK_HIP(c, hipMemsetAsync(lend, 0, (size_t)(nv + 1) * 4, c->stream));
+      k_label_ranges<<<nbk(mcell), 256, 0, c->stream>>>(mcell, kb2, lstart, lend);
+      k_vp_count<<<(unsigned)nl, 256, 0, c->stream>>>(vl, G, marca, vb, lstart, lend, pre);
+      // LDS (at most 56 KB): the grown bitmap, and the 16-bit label map below
+      // 65535 labels; what does not fit stays in HBM
+      const size_t words = (size_t)(nv + 32) / 32, map16 = (size_t)(nv + 1) * 2;
+      const bool bits_lds = words * 4 <= 56 * 1024;
+      const bool use_lds = nv < 65535 && (bits_lds ? words * 4 : 0) + map16 <= 56 * 1024;
+      const size_t dyn = (bits_lds ? words * 4 : 0) + (use_lds ? map16 : 0);
+      uint32_t* grown_g = nullptr;
+      if (!bits_lds) {
+        grown_g = (uint32_t*)c->cellkey.p;  // the pair keys are sorted into cellkey2 by now
+        AV_CHECK_HIP(c, hipMemsetAsync(grown_g, 0, words * 4, c->stream));
+      }
+      k_void_periodic<<<1, 1024, dyn, c->stream>>>(vl, vpos + nv, nv, use_lds ? 1 : 0, grown_g, G, marca, vb,
+                                                    lstart, lend, pre, lmap, mnext, mtail, selfm, small + 3);
+    }
     k_apply_lmap<<<nbk(nv), 256, 0, c->stream>>>(nv, lmap, selfm, uvoid, VOLNEW);
     k_relabel_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, lmap, marca);
     AV_CHECK_HIP(c, hipGetLastError());
@@ -1053,11 +1219,13 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     k_req<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, VOLNEW, REQ, small + 4);
     AV_CHECK_HIP(c, hipGetLastError());
     unsigned long long ht = 0;
-    int32_t hn = 0;
+    int32_t hn = 0, hp = 0;
     AV_CHECK_HIP(c, hipMemcpyAsync(&ht, tot, 8, hipMemcpyDeviceToHost, c->stream));
     AV_CHECK_HIP(c, hipMemcpyAsync(&hn, small + 4, 4, hipMemcpyDeviceToHost, c->stream));
+    if (g.periodic) AV_CHECK_HIP(c, hipMemcpyAsync(&hp, small + 3, 4, hipMemcpyDeviceToHost, c->stream));
     AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
     c->nvoidt = hn;
+    c->nperiodic = hp;
     c->volt_clean = (double)ht * (double)((g.dx * g.dx) * g.dx);
   }
   return AVISM_OK;
@@ -1077,7 +1245,7 @@ int get_catalogue(avism_ctx* c, int32_t* indice, int32_t* uvoid, int32_t* ncellv
     st->ncubes = c->nvoid;
     st->nvoid2 = c->nvoid2;
     st->nvoidt = c->nvoidt;
-    st->pad = 0;
+    st->nperiodic = c->nperiodic;
     st->volt_clean = c->volt_clean;
     st->meandens = c->meandens;
     float l3 = (c->g.L0 * c->g.L0) * c->g.L0;

@@ -25,7 +25,7 @@ module avism_c
 
   type, bind(C) :: avism_level_stats
     integer(c_int64_t) :: ncand
-    integer(c_int32_t) :: ncubes, nvoid2, nvoidt, pad
+    integer(c_int32_t) :: ncubes, nvoid2, nvoidt, nperiodic
     real(c_double) :: volt_clean
     real(c_float) :: meandens, ff
   end type

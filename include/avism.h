@@ -72,7 +72,7 @@ typedef struct {
     int32_t ncubes;     /* NVOID after VOIDFIND */
     int32_t nvoid2;     /* voids after MERGE_VOID */
     int32_t nvoidt;     /* voids in the catalogue (avism.f90:1973) */
-    int32_t pad;
+    int32_t nperiodic;  /* main voids VOID_PERIODIC relabelled (CHANGE_BELONG, voidfinding.f90:1476-1480) */
     double volt_clean;  /* total void volume (avism.f90:1928) */
     float meandens;     /* MEANDENS in Msun/Mpc^3 (avism.f90:1409) */
     float ff;           /* VOLT_CLEAN / L0^3 (catalogue header, avism.f90:2047) */
