@@ -420,17 +420,21 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
     int j0 = max(iy - reach, 1), j1 = min(iy + reach, n);
     const int i0 = max(ix - reach, 1), i1 = min(ix + reach, n);
     // ... and to the cells whose centre can lie within h (beyond it M4(2d/h)
-    // is exactly 0, and adding +0 leaves the FP32 NORM unchanged); the bounds
-    // keep a margin of one cell and 1e-5 h^2, far above the rounding involved
+    // is exactly 0, and adding +0 leaves the FP32 NORM unchanged).  In cell
+    // units u = (x - RADX(1)) / DX + 1 the centre of cell i is at u = i, so the
+    // centres within r of x are the integers in [u - r/DX, u + r/DX]; the
+    // bounds keep a margin of 1e-5 h^2 and 1e-3 cells, far above the rounding
+    // involved
     const float hm2 = hh * hh * 1.00001f;
     const float rdx = 1.0f / G.dx;
-    k0 = max(k0, (int)floorf((p.z - hh - G.rx1) * rdx) + 1);
-    k1 = min(k1, (int)floorf((p.z + hh - G.rx1) * rdx) + 2);
-    j0 = max(j0, (int)floorf((p.y - hh - G.rx1) * rdx) + 1);
-    j1 = min(j1, (int)floorf((p.y + hh - G.rx1) * rdx) + 2);
-    const float ra = sqrtf(hm2);  // bounds every row's half-width below
-    const int ia = max(i0, (int)floorf((p.x - ra - G.rx1) * rdx) + 1);
-    const int ib = min(i1, (int)floorf((p.x + ra - G.rx1) * rdx) + 2);
+    const float ux = (p.x - G.rx1) * rdx + 1.0f, uy = (p.y - G.rx1) * rdx + 1.0f, uz = (p.z - G.rx1) * rdx + 1.0f;
+    const float hc = hh * 1.00001f * rdx + 1e-3f;
+    k0 = max(k0, (int)ceilf(uz - hc));
+    k1 = min(k1, (int)floorf(uz + hc));
+    j0 = max(j0, (int)ceilf(uy - hc));
+    j1 = min(j1, (int)floorf(uy + hc));
+    const int ia = max(i0, (int)ceilf(ux - hc));  // bounds every row's run below
+    const int ib = min(i1, (int)floorf(ux + hc));
     const bool local = ia >= bx0 && ib < bx0 + DBX && j0 >= by0 && j1 < by0 + DBY && k0 >= bz0 && k1 < bz0 + DBZ;
     auto deposit = [&](int ci, int cj, int ck, float v) {
       const unsigned long long f = (unsigned long long)llrint((double)v * scale);
@@ -462,9 +466,9 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
           const float ty = p.y - G.rad[j2 - G.rlo];
           const float rem = hm2 - (ty * ty + tz * tz);
           if (rem < 0.0f) continue;  // no centre of this row within h
-          const float rr = sqrtf(rem);
-          const int a = max(i0, (int)floorf((p.x - rr - G.rx1) * rdx) + 1);
-          const int b = min(i1, (int)floorf((p.x + rr - G.rx1) * rdx) + 2);
+          const float rc = bound_sqrt(rem) * rdx + 1e-3f;
+          const int a = max(i0, (int)ceilf(ux - rc));
+          const int b = min(i1, (int)floorf(ux + rc));
           for (int i2 = a; i2 <= b; ++i2) {
             bool ok;
             const float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);

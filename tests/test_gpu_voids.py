@@ -38,7 +38,7 @@ def gpu_void_outputs(c, growth_mode=0):
     cat = c.catalogue(nv)
     out = dict(nvoid=nv, cubes_i=ci, cubes_f=cf, father=fa, indice=cat["indice"], uvoid=cat["uvoid"],
                ncellv=cat["ncellv"], vf=cat["vf"], nvoid2=cat["stats"].nvoid2, nvoidt=cat["stats"].nvoidt,
-               volt_clean=cat["stats"].volt_clean, marca=c.marca())
+               volt_clean=cat["stats"].volt_clean, marca=c.marca(), nperiodic=cat["stats"].nperiodic)
     return out
 
 
@@ -105,6 +105,9 @@ def test_grid_input_vs_oracle(n, per, seed):
     _setup(c, n, L0, per)
     c.load_grid_fields(d, vx, vy, vz)
     out = gpu_void_outputs(c)
+    if per:  # the periodic-image relabelling (VOID_PERIODIC) is exercised, and matches
+        print(f"[voids n={n}] VOID_PERIODIC relabels: {out['nperiodic']}", flush=True)
+        assert out["nperiodic"] > 0
     lin, r = c.candidates()
     assert np.array_equal(lin, ref.cand_lin)
     grown = ref.cand_r >= 0
