@@ -566,15 +566,46 @@ __global__ void __launch_bounds__(256) k_cc_local(ExtG G, const int32_t* __restr
   }
 }
 
-// joins across the low faces of the blocks
-__global__ void k_cc_faces(int64_t ne3, ExtG G, int32_t* lab) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne3 || lab[e] < 0) return;
-  const int i = (int)(e % G.ne), j = (int)((e / G.ne) % G.ne), k = (int)(e / ((int64_t)G.ne * G.ne));
-  if (i > 0 && i % CCB == 0 && lab[e - 1] >= 0) cc_union(lab, (int)e, (int)(e - 1));
-  if (j > 0 && j % CCB == 0 && lab[e - G.ne] >= 0) cc_union(lab, (int)e, (int)(e - G.ne));
-  if (k > 0 && k % CCB == 0 && lab[e - (int64_t)G.ne * G.ne] >= 0)
-    cc_union(lab, (int)e, (int)(e - (int64_t)G.ne * G.ne));
+// joins across the low faces of the blocks: one wave per 8 x 8 block face
+// (grid: blocks, 3 directions), lane (u, v) on the face.  A pair of local
+// labels equal to its left or lower neighbour's on the face is already joined
+// by that neighbour (the first lane of each distinct pair has neither), so a
+// face whose two sides are one component each makes one union, not 64.
+__global__ void __launch_bounds__(64) k_cc_faces(ExtG G, int nbc, int32_t* lab) {
+  const int dir = blockIdx.y, b = blockIdx.x;
+  const int bx = b % nbc, by = (b / nbc) % nbc, bz = b / (nbc * nbc);
+  const int l = threadIdx.x, u = l & (CCB - 1), v = l / CCB;
+  int i, j, k;
+  int64_t step;
+  if (dir == 0) {
+    if (bx == 0) return;
+    i = bx * CCB;
+    j = by * CCB + u;
+    k = bz * CCB + v;
+    step = 1;
+  } else if (dir == 1) {
+    if (by == 0) return;
+    i = bx * CCB + u;
+    j = by * CCB;
+    k = bz * CCB + v;
+    step = G.ne;
+  } else {
+    if (bz == 0) return;
+    i = bx * CCB + u;
+    j = by * CCB + v;
+    k = bz * CCB;
+    step = (int64_t)G.ne * G.ne;
+  }
+  int la = -1, lb = -1;
+  if (i < G.ne && j < G.ne && k < G.ne) {
+    const int64_t e = (int64_t)i + (int64_t)G.ne * ((int64_t)j + (int64_t)G.ne * k);
+    la = lab[e];
+    lb = lab[e - step];
+  }
+  const int pl = l > 0 ? l - 1 : 0, ql = l >= CCB ? l - CCB : 0;
+  const int pa = __shfl(la, pl), pb = __shfl(lb, pl), qa = __shfl(la, ql), qb = __shfl(lb, ql);
+  const bool dup = (u > 0 && pa == la && pb == lb) || (v > 0 && qa == la && qb == lb);
+  if (la >= 0 && lb >= 0 && !dup) cc_union(lab, la, lb);
 }
 
 __global__ void k_cc_flatten(int64_t ne3, int32_t* lab) {
@@ -585,15 +616,15 @@ __global__ void k_cc_flatten(int64_t ne3, int32_t* lab) {
 
 // holes (empty cells not connected to the corner, voidfinding.f90:1040-1135)
 // take the id of the first non-hole cell in +x (1140-1156)
-__global__ void k_fill_holes(int64_t ne3, ExtG G, const int32_t* __restrict__ lab, int32_t* marca,
-                             int32_t* nholes) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne3) return;
+// (grid: x chunks, y, z)
+__global__ void k_fill_holes(ExtG G, const int32_t* __restrict__ lab, int32_t* marca, int32_t* nholes) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.ne) return;
+  const int64_t e = (int64_t)i + (int64_t)G.ne * ((int64_t)blockIdx.y + (int64_t)G.ne * blockIdx.z);
   int outside = lab[0];
   int l = lab[e];
   if (l < 0 || l == outside) return;
   atomicAdd(nholes, 1);
-  int i = (int)(e % G.ne);
   int64_t f = e;
   while (i < G.ne && lab[f] >= 0 && lab[f] != outside) {
     ++i;
@@ -1106,10 +1137,11 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     int32_t* lab = P;  // reuse
     const int nbc = (G.ne + CCB - 1) / CCB;
     k_cc_local<<<(unsigned)((int64_t)nbc * nbc * nbc), 256, 0, c->stream>>>(G, marca, lab);
-    k_cc_faces<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, lab);
+    const dim3 g3((unsigned)((G.ne + 255) / 256), (unsigned)G.ne, (unsigned)G.ne);
+    k_cc_faces<<<dim3((unsigned)((int64_t)nbc * nbc * nbc), 3), 64, 0, c->stream>>>(G, nbc, lab);
     k_cc_flatten<<<nbk(ne3), 256, 0, c->stream>>>(ne3, lab);
     AV_CHECK_HIP(c, hipMemsetAsync(small + 2, 0, 4, c->stream));
-    k_fill_holes<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, lab, marca, small + 2);
+    k_fill_holes<<<g3, 256, 0, c->stream>>>(G, lab, marca, small + 2);
     AV_CHECK_HIP(c, hipGetLastError());
   }
   if (stop_after == 3) {
