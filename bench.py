@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed-mass", action="store_true", help="skip the mixed-mass field-stage timing")
     ap.add_argument("--cpu-n", type=int, default=64, help="side of the CPU-baseline sample")
     ap.add_argument("--mode", choices=["both", "plain", "learned"], default="both")
     return ap.parse_args()
@@ -146,6 +147,22 @@ def main():
         catalogues[mode] = (ctx.marca(), cat["uvoid"].copy(), cat["indice"].copy())
         results[mode] = dict(ms=dt * 1e3, stages={k: v / args.steps for k, v in tim.items()},
                              ncubes=int(st.ncubes), nvoids=int(st.nvoidt), ncand=int(st.ncand))
+    # The workload's particles all have the same mass (as N-body snapshots do),
+    # so PPART takes k*m as the exact mass sum.  For the record, the field stage
+    # once more with masses spread over [0.5, 1.5] x m (the neighbour-sum path).
+    mixed = None
+    if not args.no_mixed_mass:
+        from avism_amd._lib import AVISM_DEVICE
+        gen = torch.Generator(device=dev).manual_seed(7)
+        m2 = parts.m * (0.5 + torch.rand(parts.m.shape, device=dev, generator=gen))
+        torch.cuda.synchronize()
+        ctx.load_particles(parts.x, parts.y, parts.z, parts.vx, parts.vy, parts.vz, m2, where=AVISM_DEVICE)
+        ctx.set_timing(True)
+        ctx.particles_to_grid(k=args.k)
+        tim = stage_table(ctx.timings())
+        ctx.set_timing(False)
+        mixed = {s2: round(tim[s2], 3) for s2 in ("ppart_dens", "vvel_sph_vw", "ddens_scatter") if s2 in tim}
+        del m2
     identical = None
     if 0 in catalogues and GL in catalogues:
         import numpy as np
@@ -208,6 +225,7 @@ def main():
         "stages_ms": {m: {k2: round(v, 3) for k2, v in results[m]["stages"].items()} for m in results},
         "ncand": r["ncand"], "ncubes": r["ncubes"], "nvoids": r["nvoids"],
         "roofline": roofline,
+        "field_stage_ms_mixed_masses": mixed,
         "input_gen_s": round(t_gen, 2),
     }
     if not args.no_cpu_baseline:
