@@ -154,6 +154,11 @@ int cnn_backward(avism_ctx* c, const float* params, int64_t n, const uint16_t* p
 int cnn_adam(avism_ctx* c, float* params, const float* grads, float* m, float* v, float lr, int32_t step);
 int cnn_propose(avism_ctx* c);
 
+// the small device scalars: counters and flags in the first 4 KB, RADX at
+// 4096, then 8-byte slots for kRedSlices partial sums
+inline size_t scalars_slices_offset(const avism_ctx* c) { return (4096 + c->g.rad.size() * 4 + 127) & ~(size_t)63; }
+inline size_t scalars_bytes(const avism_ctx* c) { return scalars_slices_offset(c) + 512 * 8; }
+
 // scoped stage timer on the context stream
 struct StageTimer {
   avism_ctx* c;

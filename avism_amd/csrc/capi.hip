@@ -115,7 +115,7 @@ int avism_set_grid(avism_ctx* c, int32_t n, float L0, int32_t periodic) {
     AV_FAIL(c, AVISM_ERR_LIMIT, "extended grid above 2^31 cells is not supported by this build");
   // small device scalars: counters and sums in the first 4 KiB, the cell-centre
   // table RADX from byte 4096 on (read by the field and shape kernels)
-  if (!c->scalars.ensure(4096 + c->g.rad.size() * 4))
+  if (!c->scalars.ensure(avism::scalars_bytes(c)))
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (scalars)");
   c->grid_set = true;
   c->fields_ready = false;

@@ -524,11 +524,20 @@ __global__ void __launch_bounds__(256) k_ddens_finalize(int n, const unsigned lo
   }
 }
 
-__global__ void k_reduce_mean(const double* partial, int64_t nparts, int64_t ncell, double* sum_out,
-                              float* mean_out) {
+// FP64 sum of the block partials in a fixed order (deterministic): each block
+// of the first pass sums a contiguous slice, the single block of the second
+// pass sums the slices and forms the mean
+constexpr int kRedSlices = 512;
+static_assert(kRedSlices == 512, "scalars_bytes reserves 512 slots for the slices");
+__global__ void __launch_bounds__(256) k_reduce_mean(const double* partial, int64_t nparts, int64_t ncell,
+                                                    double* slices, int final_pass, double* sum_out,
+                                                    float* mean_out) {
   __shared__ double red[256];
+  const int64_t per = final_pass ? nparts : (nparts + kRedSlices - 1) / kRedSlices;
+  const int64_t b0 = final_pass ? 0 : (int64_t)blockIdx.x * per;
+  const int64_t b1 = b0 + per < nparts ? b0 + per : nparts;
   double s = 0.0;
-  for (int64_t i = threadIdx.x; i < nparts; i += 256) s += partial[i];
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) s += partial[i];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
@@ -536,8 +545,12 @@ __global__ void k_reduce_mean(const double* partial, int64_t nparts, int64_t nce
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    *sum_out = red[0];
-    *mean_out = (float)(red[0] / (double)ncell);  // avism.f90:1405
+    if (!final_pass) {
+      slices[blockIdx.x] = red[0];
+    } else {
+      *sum_out = red[0];
+      *mean_out = (float)(red[0] / (double)ncell);  // avism.f90:1405
+    }
   }
 }
 
@@ -711,7 +724,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
       !c->fw.ensure((size_t)ncell * 4) || !c->u_mass.ensure((size_t)ncell * 4) ||
       !c->u1co.ensure((size_t)ncell * 4) || !c->div.ensure((size_t)ncell * 4))
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (fields)");
-  float* d_rad = (float*)c->scalars.ensure(4096 + c->g.rad.size() * 4);
+  float* d_rad = (float*)c->scalars.ensure(scalars_bytes(c));
   float* rad = d_rad + 1024;
   AV_CHECK_HIP(c, hipMemcpyAsync(rad, c->g.rad.data(), c->g.rad.size() * 4, hipMemcpyHostToDevice,
                                  c->stream));
@@ -779,7 +792,10 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   float* dmean = (float*)((char*)c->scalars.p + 264);
   {
     StageTimer t(c, "mean_div");
-    k_reduce_mean<<<1, 256, 0, c->stream>>>(c->partial64.as<double>(), nparts, ncell, dsum, dmean);
+    double* slices = (double*)((char*)c->scalars.p + scalars_slices_offset(c));
+    k_reduce_mean<<<kRedSlices, 256, 0, c->stream>>>(c->partial64.as<double>(), nparts, ncell, slices, 0, dsum,
+                                                     dmean);
+    k_reduce_mean<<<1, 256, 0, c->stream>>>(slices, kRedSlices, ncell, nullptr, 1, dsum, dmean);
     dim3 g2((n + 255) / 256, n, n);
     k_div_finalize<<<g2, 256, 0, c->stream>>>(n, c->g.dx, c->g.periodic, c->fu.as<float>(),
                                               c->fv.as<float>(), c->fw.as<float>(),

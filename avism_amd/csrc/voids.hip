@@ -49,17 +49,26 @@ struct MarkParams {
   float two_dx;
 };
 
-// voidfinding.f90:256-432 for one cell; ref order +X,-X,+Y,-Y,+Z,-Z
+// voidfinding.f90:256-432 for one cell; ref order +X,-X,+Y,-Y,+Z,-Z.
+// Grid (x chunks, y, z) over the ext grid, so the coordinates need no
+// division; the periodic wrap of the (at most 3-cell) neighbour offsets is
+// taken once per axis.
 __global__ void k_mark(ExtGeom G, MarkParams P, const float* __restrict__ u1co,
                        const float* __restrict__ div, const int32_t* __restrict__ marcap,
                        uint8_t* flags) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t ne3 = (int64_t)G.ne * G.ne * G.ne;
-  if (e >= ne3) return;
-  int I = (int)(e % G.ne) + G.low1;
-  int J = (int)((e / G.ne) % G.ne) + G.low1;
-  int K = (int)(e / ((int64_t)G.ne * G.ne)) + G.low1;
-  size_t ci = G.inner(I, J, K);
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= G.ne) return;
+  const int I = x + G.low1, J = (int)blockIdx.y + G.low1, K = (int)blockIdx.z + G.low1;
+  const int n = G.n;
+  // inner offsets of the cell and of its neighbours at -3..+3 along each axis
+  size_t ox[7], oy[7], oz[7];
+#pragma unroll
+  for (int t = 0; t < 7; ++t) {
+    ox[t] = (size_t)(G.wrap(I + t - 3) - 1);
+    oy[t] = (size_t)n * (size_t)(G.wrap(J + t - 3) - 1);
+    oz[t] = (size_t)n * (size_t)n * (size_t)(G.wrap(K + t - 3) - 1);
+  }
+  const size_t ci = ox[3] + oy[3] + oz[3];
   int mp = marcap ? marcap[ci] : -1;
   uint8_t f = 0;
   // MARK_ALL (voidfinding.f90:30-52); SMASK = 1
@@ -68,32 +77,31 @@ __global__ void k_mark(ExtGeom G, MarkParams P, const float* __restrict__ u1co,
     if (u1co[ci] < P.dthr1 && div[ci] > 0.0f) f |= FL_V;
   }
   const int L1 = G.low1, L2 = G.low2;
+#pragma unroll
   for (int dir = 0; dir < 6; ++dir) {
-    int axis = dir >> 1;
-    bool plus = (dir & 1) == 0;
-    int pos = axis == 0 ? I : (axis == 1 ? J : K);
-    uint8_t bit = axis == 0 ? (plus ? FL_X2 : FL_X1) : axis == 1 ? (plus ? FL_Y2 : FL_Y1) : (plus ? FL_Z2 : FL_Z1);
+    const int axis = dir >> 1;
+    const bool plus = (dir & 1) == 0;
+    const int pos = axis == 0 ? I : (axis == 1 ? J : K);
+    const uint8_t bit = axis == 0 ? (plus ? FL_X2 : FL_X1) : axis == 1 ? (plus ? FL_Y2 : FL_Y1) : (plus ? FL_Z2 : FL_Z1);
+    // the cell at offset o (-3..3) along this axis
+    auto at = [&](int o) -> size_t {
+      return axis == 0 ? ox[3 + o] + oy[3] + oz[3] : axis == 1 ? ox[3] + oy[3 + o] + oz[3] : ox[3] + oy[3] + oz[3 + o];
+    };
     bool flag = plus ? (pos >= L2 - kNsec) : (pos <= L1 + kNsec);
     if (!flag) {
-      int s = plus ? 1 : -1;
-      int II = pos + s;
-      int i1 = I, j1 = J, k1 = K;
-      if (axis == 0) i1 = II; else if (axis == 1) j1 = II; else k1 = II;
-      size_t cn = G.inner(i1, j1, k1);
-      int edge = plus ? L2 : L1;
+      const int s = plus ? 1 : -1;
+      const int II = pos + s;
+      const size_t cn = at(s);
+      const int edge = plus ? L2 : L1;
       if (II == edge) {
         if (div[cn] < P.divthr) flag = true;
         if (u1co[cn] >= P.dthr2) flag = true;
       } else {
-        int ia = i1, ja = j1, ka = k1, ib = i1, jb = j1, kb = k1;
-        if (axis == 0) { ia += s; ib -= s; } else if (axis == 1) { ja += s; jb -= s; } else { ka += s; kb -= s; }
-        float dd = (u1co[G.inner(ia, ja, ka)] - u1co[G.inner(ib, jb, kb)]) / P.two_dx;
+        const float dd = (u1co[at(2 * s)] - u1co[ci]) / P.two_dx;
         if (dd >= P.grad) flag = true;
-        bool nextok = plus ? (II < L2 - 1) : (II > L1 + 1);
+        const bool nextok = plus ? (II < L2 - 1) : (II > L1 + 1);
         if (flag && nextok) {
-          int i2 = i1, j2 = j1, k2 = k1;
-          if (axis == 0) i2 += 2 * s; else if (axis == 1) j2 += 2 * s; else k2 += 2 * s;
-          float dd2 = (u1co[G.inner(i2, j2, k2)] - u1co[cn]) / P.two_dx;
+          const float dd2 = (u1co[at(3 * s)] - u1co[cn]) / P.two_dx;
           if (dd2 < P.grad) flag = false;
         }
         if (div[cn] < P.divthr) flag = true;
@@ -102,7 +110,7 @@ __global__ void k_mark(ExtGeom G, MarkParams P, const float* __restrict__ u1co,
     }
     if (flag) f |= bit;
   }
-  flags[e] = f;
+  flags[(size_t)x + (size_t)G.ne * ((size_t)blockIdx.y + (size_t)G.ne * blockIdx.z)] = f;
 }
 
 constexpr int kChunk = 16;  // cells per thread in the ordered compaction
@@ -472,8 +480,8 @@ int void_find(avism_ctx* c, int32_t growth_mode) {
   if (!c->flags.ensure((size_t)ne3)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (flags)");
   {
     StageTimer t(c, "mark_flags");
-    k_mark<<<nbk(ne3), 256, 0, c->stream>>>(G, P, c->u1co.as<float>(), c->div.as<float>(), marcap,
-                                            c->flags.as<uint8_t>());
+    k_mark<<<dim3((unsigned)((G.ne + 255) / 256), (unsigned)G.ne, (unsigned)G.ne), 256, 0, c->stream>>>(
+        G, P, c->u1co.as<float>(), c->div.as<float>(), marcap, c->flags.as<uint8_t>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   // ordered compaction
