@@ -94,6 +94,7 @@ struct avism_ctx {
   bool has_marcap = false;
   float mass_max = 0;                          // max particle mass (code units)
   bool mass_uniform = false;                   // every particle has the same mass
+  avism::DBuf tsc_acc;                         // TSC velocity sums, FP64 [4][N^3]
   avism::DBuf mass_fix;                        // DDENS cell sums, int64 fixed point
   float meandens = 0;                          // MEANDENS in Msun/Mpc^3
   float meandens_raw = 0;                      // MEANDENS before unit scaling

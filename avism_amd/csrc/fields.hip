@@ -323,6 +323,126 @@ __global__ void k_fill_h(int64_t np, float* hpart, const float* hk) {
   if (!(hpart[i] > 0.0f)) hpart[i] = hk[i];
 }
 
+// ---------------------------------------------------------------- TSC
+// DENS_INTERP_TSC (particles.f90:16-128) and VEL_INTERP_TSC (132-272): each
+// particle spreads over the 3 x 3 x 3 cells around its nearest centre with the
+// triangular-shaped-cloud weights, in the reference's FP32 operations.  The
+// sums are int64 fixed point (order-free, so reproducible); the reference adds
+// REAL*8 (mass) and REAL*4 under OMP ATOMIC (velocities), whose results depend
+// on its thread schedule.  Non-periodic: cells outside the grid are dropped
+// (the reference writes outside its arrays there).  The sums are FP64 atomic
+// adds: weights reach 1e-20 and below in the tails, which a fixed point would
+// drop; FP64 addition order changes only the last bits of the FP64 sums,
+// below the FP32 rounding of the results.
+struct TscGeom {
+  const float* __restrict__ rad;  // RADX, reference index i at rad[i - rlo]
+  int rlo, n, periodic;
+  float dx;
+};
+
+// the periodic wrap of a coordinate (particles.f90:165-181, 187-194)
+__device__ __forceinline__ float tsc_wrap(const TscGeom& T, float p) {
+  if (T.periodic) {
+    const float lo = T.rad[1 - T.rlo] - 0.5f * T.dx, hi = T.rad[T.n - T.rlo] + 0.5f * T.dx;
+    const float leng = hi - lo;
+    if (p < lo) p = p + leng;
+    if (p >= hi) p = p - leng;
+  }
+  return p;
+}
+
+// the nearest centre I and the weights of I-1, I, I+1 (particles.f90:196-217)
+__device__ __forceinline__ int tsc_w(const TscGeom& T, float p, float V[3]) {
+  const int I = (int)(((p - T.rad[1 - T.rlo]) / T.dx) + 0.49999f) + 1;
+  float bas = fabsf(T.rad[I - 1 - T.rlo] - p);
+  float t = 1.5f - bas / T.dx;
+  V[0] = 0.5f * (t * t);
+  bas = fabsf(T.rad[I - T.rlo] - p);
+  t = bas / T.dx;
+  V[1] = 0.75f - t * t;
+  bas = fabsf(T.rad[I + 1 - T.rlo] - p);
+  t = 1.5f - bas / T.dx;
+  V[2] = 0.5f * (t * t);
+  return I;
+}
+
+// cell index after the periodic wrap (particles.f90:223-230); -1 outside
+__device__ __forceinline__ int tsc_cell(const TscGeom& T, int i) {
+  if (T.periodic) {
+    if (i < 1) i += T.n;
+    if (i > T.n) i -= T.n;
+  }
+  return (i < 1 || i > T.n) ? -1 : i;
+}
+
+// one thread per particle (input order): the mass deposits (mass != null)
+// into mass[N^3], the velocity sums (acc != null) into acc[4][N^3]
+__global__ void __launch_bounds__(256) k_tsc(TscGeom T, int64_t np, const float* __restrict__ x,
+                                            const float* __restrict__ y, const float* __restrict__ z,
+                                            const float* __restrict__ m, const float* __restrict__ vx,
+                                            const float* __restrict__ vy, const float* __restrict__ vz,
+                                            double* mass, double* acc) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  float VX[3], VY[3], VZ[3];
+  const int I = tsc_w(T, tsc_wrap(T, x[p]), VX);
+  const int J = tsc_w(T, tsc_wrap(T, y[p]), VY);
+  const int K = tsc_w(T, tsc_wrap(T, z[p]), VZ);
+  const size_t ncell = (size_t)T.n * T.n * T.n;
+  const float mp = mass ? m[p] : 0.0f;
+  const float up = acc ? vx[p] : 0.0f, vp = acc ? vy[p] : 0.0f, wp = acc ? vz[p] : 0.0f;
+  for (int a = -1; a <= 1; ++a) {
+    const int k3 = tsc_cell(T, K + a);
+    if (k3 < 0) continue;
+    for (int b = -1; b <= 1; ++b) {
+      const int j3 = tsc_cell(T, J + b);
+      if (j3 < 0) continue;
+      for (int cc = -1; cc <= 1; ++cc) {
+        const int i3 = tsc_cell(T, I + cc);
+        if (i3 < 0) continue;
+        const size_t o = (size_t)(i3 - 1) + (size_t)T.n * ((size_t)(j3 - 1) + (size_t)T.n * (k3 - 1));
+        if (mass) {
+          const float inc = ((mp * VX[cc + 1]) * VY[b + 1]) * VZ[a + 1];  // particles.f90:100-118
+          atomicAdd(&mass[o], (double)inc);
+        }
+        if (acc) {
+          const float wt = (VX[cc + 1] * VY[b + 1]) * VZ[a + 1];  // particles.f90:232
+          atomicAdd(&acc[o], (double)(up * wt));
+          atomicAdd(&acc[ncell + o], (double)(vp * wt));
+          atomicAdd(&acc[2 * ncell + o], (double)(wp * wt));
+          atomicAdd(&acc[3 * ncell + o], (double)wt);
+        }
+      }
+    }
+  }
+}
+
+// U = sum(v * w) / WEI with WEI = sum(w), 1 where no particle reached the cell
+// (particles.f90:255-267)
+__global__ void k_tsc_vel_finalize(int64_t ncell, const double* __restrict__ acc, float* u, float* v, float* w) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  float wei = (float)acc[3 * ncell + c];
+  if (wei == 0.0f) wei = 1.0f;
+  u[c] = (float)acc[c] / wei;
+  v[c] = (float)acc[ncell + c] / wei;
+  w[c] = (float)acc[2 * ncell + c] / wei;
+}
+
+// FILL_H_ZEROS (particles.f90:398-431) when no PPART query ran (TSC
+// velocities): h = the k-th neighbour distance of each particle with h = 0
+template <int PER, int K>
+__global__ void __launch_bounds__(KQ) k_fill_h_knn(Bins B, int64_t np, int k, float r0, float* hpart) {
+  __shared__ uint32_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * KQ + threadIdx.x;
+  if (i >= np || hpart[i] > 0.0f) return;
+  const float4 pq = B.pos[i];
+  Sel<KQ, K / 4> h;
+  h.init(keys_s + threadIdx.x, gmx_s + threadIdx.x, k);
+  bool scanned;
+  hpart[i] = sqrtf(knn_kth<PER>(B, np, pq.x, pq.y, pq.z, r0, k, h, scanned));
+}
+
 // DDENS_INTERP_SPH pass 1 (particles.f90:479-547): per particle the NORM, in
 // the reference's loop order and FP32, plus the reach of its support in cells.
 struct DdensGeom {
@@ -495,16 +615,17 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
 
 // cell sums -> U1 = mass/(dx^3 rete^3)/rote*rote (avism.f90:1249-1250, 1386)
 // and the FP64 block sums of U1 (avism.f90:1395-1405), fixed reduction order
+// acc: int64 fixed-point cell masses (fp64 == 0) or FP64 ones (fp64 != 0)
 __global__ void __launch_bounds__(256) k_ddens_finalize(int n, const unsigned long long* __restrict__ acc,
-                                                       double inv_scale, float den, float rote, float* umass,
-                                                       double* partial) {
+                                                       double inv_scale, int fp64, float den, float rote,
+                                                       float* umass, double* partial) {
   int ix = blockIdx.x * 8 + threadIdx.x % 8;
   int iy = blockIdx.y * 8 + (threadIdx.x / 8) % 8;
   int iz = blockIdx.z * 4 + threadIdx.x / 64;
   double contrib = 0.0;
   if (ix < n && iy < n && iz < n) {
     size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
-    float mass = (float)((double)acc[c] * inv_scale);
+    float mass = fp64 ? (float)__longlong_as_double((long long)acc[c]) : (float)((double)acc[c] * inv_scale);
     float u = mass / den;
     u = u / rote;
     u = u * rote;
@@ -711,8 +832,9 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   if (k <= 0) AV_FAIL(c, AVISM_ERR_INPUT, "KNEIGHBOURS must be positive");
   if (k > 64) AV_FAIL(c, AVISM_ERR_UNSUPPORTED, "KNEIGHBOURS above 64 is not supported by this build");
   if (k > c->np) AV_FAIL(c, AVISM_ERR_INPUT, "KNEIGHBOURS exceeds the number of particles");
-  if (vel_interp != 1 || dens_interp != 1)
-    AV_FAIL(c, AVISM_ERR_UNSUPPORTED, "only SPH interpolation (FLAG_VEL_INTERP=FLAG_DENS_INTERP=1) is built");
+  // FLAG_VEL_INTERP / FLAG_DENS_INTERP: 0 = TSC, 1 = SPH (avism.f90:1178-1179, 1220-1221)
+  if ((vel_interp != 0 && vel_interp != 1) || (dens_interp != 0 && dens_interp != 1))
+    AV_FAIL(c, AVISM_ERR_INPUT, "FLAG_VEL_INTERP and FLAG_DENS_INTERP must be 0 (TSC) or 1 (SPH)");
   const int64_t np = c->np;
   const int n = c->g.n;
   const int64_t ncell = (int64_t)n * n * n;
@@ -732,34 +854,68 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   // first search radius: 1.3 x the k-th neighbour distance at the mean density
   const double rho = (double)np / ((double)c->g.L0 * c->g.L0 * c->g.L0);
   const float r0 = (float)(1.3 * std::cbrt(3.0 * k / (4.0 * kPi * rho)));
-  {
-    StageTimer t(c, "ppart_dens");
-    if (k <= 32)
-      k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->mass_uniform ? 1 : 0,
-                                                       c->pdens.as<float>(), c->hk.as<float>());
-    else
-      k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->mass_uniform ? 1 : 0,
-                                                       c->pdens.as<float>(), c->hk.as<float>());
-    AV_CHECK_HIP(c, hipGetLastError());
-  }
   AV_CHECK_HIP(c, hipMemsetAsync(c->hpart.p, 0, (size_t)np * 4, c->stream));
-  {
+  const TscGeom T{rad, c->g.rlo, n, c->g.periodic, c->g.dx};
+  if (vel_interp == 1) {
+    {
+      StageTimer t(c, "ppart_dens");
+      if (k <= 32)
+        k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->mass_uniform ? 1 : 0,
+                                                         c->pdens.as<float>(), c->hk.as<float>());
+      else
+        k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->mass_uniform ? 1 : 0,
+                                                         c->pdens.as<float>(), c->hk.as<float>());
+      AV_CHECK_HIP(c, hipGetLastError());
+    }
     k_velrec<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), c->spos4.as<float4>(),
                                               c->pdens.as<float>(), c->pvx.as<float>(), c->pvy.as<float>(),
                                               c->pvz.as<float>(), c->vel4.as<float4>());
-    StageTimer t(c, "vvel_sph_vw");
-    const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
-    const VvelOut O{c->vel4.as<float4>(), c->hpart.as<uint32_t>(), c->fu.as<float>(), c->fv.as<float>(),
-                    c->fw.as<float>(), 1};
-    if (k <= 32)
-      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
-    else
-      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
+    {
+      StageTimer t(c, "vvel_sph_vw");
+      const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
+      const VvelOut O{c->vel4.as<float4>(), c->hpart.as<uint32_t>(), c->fu.as<float>(), c->fv.as<float>(),
+                      c->fw.as<float>(), 1};
+      if (k <= 32)
+        k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
+      else
+        k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
+      AV_CHECK_HIP(c, hipGetLastError());
+    }
+    {
+      StageTimer t(c, "fill_h");
+      k_fill_h<<<nblk(np), 256, 0, c->stream>>>(np, c->hpart.as<float>(), c->hk.as<float>());
+      AV_CHECK_HIP(c, hipGetLastError());
+    }
+  } else {  // VEL_INTERP_TSC; PART_DENS stays 0 as in the reference
+    if (!c->tsc_acc.ensure((size_t)ncell * 32)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (TSC)");
+    AV_CHECK_HIP(c, hipMemsetAsync(c->pdens.p, 0, (size_t)np * 4, c->stream));
+    StageTimer t(c, "vel_tsc");
+    k_fill_u64<<<nblk(ncell * 4), 256, 0, c->stream>>>(ncell * 4, c->tsc_acc.as<unsigned long long>(), 0ull);
+    k_tsc<<<nblk(np), 256, 0, c->stream>>>(T, np, c->px.as<float>(), c->py.as<float>(), c->pz.as<float>(),
+                                           c->pm.as<float>(), c->pvx.as<float>(), c->pvy.as<float>(),
+                                           c->pvz.as<float>(), nullptr, c->tsc_acc.as<double>());
+    k_tsc_vel_finalize<<<nblk(ncell), 256, 0, c->stream>>>(ncell, c->tsc_acc.as<double>(), c->fu.as<float>(),
+                                                           c->fv.as<float>(), c->fw.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
-  {
-    StageTimer t(c, "fill_h");
-    k_fill_h<<<nblk(np), 256, 0, c->stream>>>(np, c->hpart.as<float>(), c->hk.as<float>());
+  if (dens_interp == 1 && vel_interp == 0) {  // H_DISTANCE (particles.f90:276-351), then FILL_H_ZEROS
+    StageTimer t(c, "h_distance");
+    const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
+    const VvelOut O{nullptr, c->hpart.as<uint32_t>(), nullptr, nullptr, nullptr, 0};
+    const unsigned nbq = nblk(np, KQ);
+    if (k <= 32) {
+      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
+      if (c->g.periodic)
+        k_fill_h_knn<1, 32><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
+      else
+        k_fill_h_knn<0, 32><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
+    } else {
+      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
+      if (c->g.periodic)
+        k_fill_h_knn<1, 64><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
+      else
+        k_fill_h_knn<0, 64><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
+    }
     AV_CHECK_HIP(c, hipGetLastError());
   }
   DdensGeom G{rad, c->g.rlo, n, c->g.dx, c->g.rx1};
@@ -774,17 +930,25 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   int e2 = 0;
   std::frexp(bound, &e2);  // bound < 2^e2
   const double scale = std::ldexp(1.0, 62 - e2);
-  {
+  if (dens_interp == 1) {
     StageTimer t(c, "ddens_scatter");
     k_fill_u64<<<nblk(ncell), 256, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 0ull);
     const unsigned ntiles = (unsigned)(((n + DTX - 1) / DTX) * ((n + DTY - 1) / DTY) * ((n + DTZ - 1) / DTZ));
     k_ddens_scatter<<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
                                                    c->mass_fix.as<unsigned long long>(), c->norm.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
+  } else {  // DENS_INTERP_TSC: FP64 cell masses in mass_fix (zero bits are +0.0)
+    StageTimer t(c, "dens_tsc");
+    k_fill_u64<<<nblk(ncell), 256, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 0ull);
+    k_tsc<<<nblk(np), 256, 0, c->stream>>>(T, np, c->px.as<float>(), c->py.as<float>(), c->pz.as<float>(),
+                                           c->pm.as<float>(), c->pvx.as<float>(), c->pvy.as<float>(),
+                                           c->pvz.as<float>(), c->mass_fix.as<double>(), nullptr);
+    AV_CHECK_HIP(c, hipGetLastError());
   }
   {
     StageTimer t(c, "ddens_finalize");
-    k_ddens_finalize<<<grid, 256, 0, c->stream>>>(n, c->mass_fix.as<unsigned long long>(), 1.0 / scale, den,
+    k_ddens_finalize<<<grid, 256, 0, c->stream>>>(n, c->mass_fix.as<unsigned long long>(), 1.0 / scale,
+                                                  dens_interp == 1 ? 0 : 1, den,
                                                   c->rote, c->u_mass.as<float>(), c->partial64.as<double>());
     AV_CHECK_HIP(c, hipGetLastError());
   }

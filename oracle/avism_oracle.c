@@ -563,10 +563,15 @@ void orc_dens_tsc(const orc_grid* g, int64_t np, const float* x, const float* y,
 
 void orc_vel_tsc(const orc_grid* g, int64_t np, const float* x, const float* y,
                  const float* z, const float* vx, const float* vy, const float* vz,
-                 float* u, float* v, float* w) {
+                 int32_t exact, float* u, float* v, float* w) {
+    /* exact == 0: the reference's REAL*4 sums in its one-thread order (it adds
+     * under OMP ATOMIC, particles.f90:226-249, so with more threads its last bits
+     * depend on the schedule); exact != 0: FP64 sums, the order-free value */
     int32_t n = g->n;
     size_t ncell = (size_t)n * n * n;
-    double* a8 = (double*)calloc(4 * ncell, sizeof(double));
+    float* wt = (float*)calloc(ncell, sizeof(float));
+    double* a8 = exact ? (double*)calloc(4 * ncell, sizeof(double)) : NULL;
+    for (size_t c = 0; c < ncell; c++) { u[c] = 0.0f; v[c] = 0.0f; w[c] = 0.0f; }
     for (int64_t p = 0; p < np; p++) {
         int32_t I, J, K; float VX[3], VY[3], VZ[3];
         tsc_w(g, tsc_wrap_pos(g, x[p]), &I, VX);
@@ -577,19 +582,31 @@ void orc_vel_tsc(const orc_grid* g, int64_t np, const float* x, const float* y,
                 for (int c = -1; c <= 1; c++) {
                     int32_t i3 = tsc_cell(g, I + c), j3 = tsc_cell(g, J + b), k3 = tsc_cell(g, K + a);
                     if (i3 < 0 || j3 < 0 || k3 < 0) continue;
-                    float wt = (VX[c + 1] * VY[b + 1]) * VZ[a + 1];   /* particles.f90:232 */
+                    float inc_wt = (VX[c + 1] * VY[b + 1]) * VZ[a + 1];   /* particles.f90:232 */
                     size_t o = (size_t)(i3 - 1) + (size_t)n * ((j3 - 1) + (size_t)n * (k3 - 1));
-                    a8[o] += (double)(vx[p] * wt);
-                    a8[ncell + o] += (double)(vy[p] * wt);
-                    a8[2 * ncell + o] += (double)(vz[p] * wt);
-                    a8[3 * ncell + o] += (double)wt;
+                    if (exact) {
+                        a8[o] += (double)(vx[p] * inc_wt);
+                        a8[ncell + o] += (double)(vy[p] * inc_wt);
+                        a8[2 * ncell + o] += (double)(vz[p] * inc_wt);
+                        a8[3 * ncell + o] += (double)inc_wt;
+                    } else {
+                        u[o] = u[o] + vx[p] * inc_wt;
+                        v[o] = v[o] + vy[p] * inc_wt;
+                        w[o] = w[o] + vz[p] * inc_wt;
+                        wt[o] = wt[o] + inc_wt;
+                    }
                 }
     }
     for (size_t c = 0; c < ncell; c++) {   /* particles.f90:255-267 */
-        float wei = (float)a8[3 * ncell + c];
+        if (exact) {
+            u[c] = (float)a8[c]; v[c] = (float)a8[ncell + c]; w[c] = (float)a8[2 * ncell + c];
+            wt[c] = (float)a8[3 * ncell + c];
+        }
+        float wei = wt[c];
         if (wei == 0.0f) wei = 1.0f;
-        u[c] = (float)a8[c] / wei; v[c] = (float)a8[ncell + c] / wei; w[c] = (float)a8[2 * ncell + c] / wei;
+        u[c] = u[c] / wei; v[c] = v[c] / wei; w[c] = w[c] / wei;
     }
+    free(wt);
     free(a8);
 }
 

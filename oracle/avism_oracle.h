@@ -78,7 +78,7 @@ void orc_dens_tsc(const orc_grid* g, int64_t np, const float* x, const float* y,
                   const float* z, const float* m, float* mass_out);
 void orc_vel_tsc(const orc_grid* g, int64_t np, const float* x, const float* y,
                  const float* z, const float* vx, const float* vy, const float* vz,
-                 float* u, float* v, float* w);
+                 int32_t exact, float* u, float* v, float* w);
 /* avism.f90:1249-1250, 1371-1414: mass -> U1CO (=1+delta); returns MEANDENS
  * (Msun/Mpc^3) and the raw FP64 sum through sum_out. */
 float orc_normalise(const orc_grid* g, float ache, float omega0, float zeta,

@@ -71,7 +71,8 @@ def lib():
     L.orc_fill_h_zeros.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, C.c_int32, _f32p]
     L.orc_ddens_sph.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f64p]
     L.orc_dens_tsc.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p]
-    L.orc_vel_tsc.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.orc_vel_tsc.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_int32, _f32p, _f32p,
+                              _f32p]
     L.orc_normalise.restype = C.c_float
     L.orc_normalise.argtypes = [vp, C.c_float, C.c_float, C.c_float, _f32p, _f32p, C.POINTER(C.c_double)]
     L.orc_diver_uniform.argtypes = [C.c_int32, C.c_float, _f32p, _f32p, _f32p, C.c_int32, _f32p]
@@ -196,9 +197,11 @@ class LevelResult:
 
 
 def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.678,
-                          omega0=0.31, zeta=0.0, vel_interp=1, dens_interp=1):
+                          omega0=0.31, zeta=0.0, vel_interp=1, dens_interp=1, tsc_exact=False):
     """Field stage (avism.f90:1177-1257, 1371-1414). Inputs in file units
-    (reader.f90:447-461 conversion applied here).  Returns a dict of arrays."""
+    (reader.f90:447-461 conversion applied here).  Returns a dict of arrays.
+    tsc_exact: TSC velocity sums in FP64 instead of the reference's REAL*4
+    one-thread order."""
     L = lib()
     g = Grid(n, L0, periodic)
     x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
@@ -216,7 +219,7 @@ def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.
         L.orc_vvel_vw(knn._p, g._p, k, m, pdens, vx, vy, vz, None, hpart, u, v, w)
         L.orc_fill_h_zeros(knn._p, npart, x, y, z, k, hpart)
     else:
-        L.orc_vel_tsc(g._p, npart, x, y, z, vx, vy, vz, u, v, w)
+        L.orc_vel_tsc(g._p, npart, x, y, z, vx, vy, vz, 1 if tsc_exact else 0, u, v, w)
     mass = np.zeros(ncell, np.float32)
     mass64 = np.zeros(ncell, np.float64)
     if dens_interp == 1:

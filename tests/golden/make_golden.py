@@ -10,9 +10,16 @@ Each fixture holds the inputs and the reference's outputs for one level:
   part_p{0,1}     : particle input (reader.f90:410-465) -> kNN, PPART_DENS,
                     VVEL_INTERP_SPH_VW, FILL_H_ZEROS, DDENS_INTERP_SPH, mean
                     density, divergence and the same void stage
+  tsc_p{0,1}      : particle input with FLAG_VEL_INTERP = FLAG_DENS_INTERP = 0
+                    -> VEL_INTERP_TSC, DENS_INTERP_TSC and the rest as above
+  tscv_p{0,1}     : FLAG_VEL_INTERP = 0, FLAG_DENS_INTERP = 1 -> VEL_INTERP_TSC,
+                    H_DISTANCE, FILL_H_ZEROS, DDENS_INTERP_SPH and the rest
+
+`python tests/golden/make_golden.py tsc` writes only the tsc fixtures.
 """
 from __future__ import annotations
 
+import os
 import sys
 import tempfile
 from pathlib import Path
@@ -42,12 +49,40 @@ def _save(name, inputs: dict, ref: dict, meta: dict):
                  if k in ("nvoid", "nvoid2", "nvoidt")})
 
 
+def tsc_fixtures(td: Path):
+    """The TSC interpolations (particles.f90:16-351) through the reference's own
+    routines: a Zel'dovich box (periodic) and mock voids (non-periodic)."""
+    mock = synth.mock_voids(n0=5000, L0=60.0, nvoids=4, rmin=6.0, rmax=12.0, seed=25)[0].numpy()
+    # non-periodic: particles kept two cells away from the faces -- the
+    # reference's TSC writes outside its arrays for particles in the outer cells
+    # (particles.f90:104-113 without the periodic wrap)
+    keep = np.all(np.abs(np.stack(mock[:3])) < 30.0 - 2 * 3.0, axis=0)
+    cases = (
+        (1, synth.zeldovich(16, 40.0, seed=24, rms_disp=2.0).numpy(), 20, 40.0, 24),
+        (0, [a[keep].copy() for a in mock], 20, 60.0, 25),
+    )
+    for per, arr, n, L0, seed in cases:
+        for name, vi, di in (("tsc", 0, 0), ("tscv", 0, 1)):
+            # one thread: VEL_INTERP_TSC adds REAL*4 with OMP ATOMIC, so only the
+            # one-thread order is reproducible (particles.f90:226-249)
+            os.environ["OMP_NUM_THREADS"] = "1"
+            ref = O.run_reference(td / f"{name}{per}", 1, n, L0, per, PARAMS, k=16, parts=arr,
+                                  vel_interp=vi, dens_interp=di)
+            del os.environ["OMP_NUM_THREADS"]
+            _save(f"{name}_p{per}", dict(zip(("x", "y", "z", "vx", "vy", "vz", "m"), arr)), ref,
+                  dict(n=n, L0=L0, periodic=per, mode=1, k=16, seed=seed, vel_interp=vi, dens_interp=di))
+
+
 def main():
     for per in (0, 1):
         if not O.ref_available(per):
             raise SystemExit("oracle/_ref not built: run `make -C oracle ref` first")
     with tempfile.TemporaryDirectory(dir=ROOT / "_work" if (ROOT / "_work").exists() else None) as td:
         td = Path(td)
+        if sys.argv[1:] == ["tsc"]:
+            tsc_fixtures(td)
+            return
+        tsc_fixtures(td)
         for per, seed in ((0, 11), (1, 12)):
             n, L0 = 24, 48.0
             d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, seed))

@@ -73,6 +73,65 @@ def test_particle_input_matches_reference(golden_dir, name):
     compare_void_outputs(res, lambda kk: g[f"ref_{kk}"])
 
 
+@pytest.mark.parametrize("name", ["tsc_p0", "tsc_p1", "tscv_p0", "tscv_p1"])
+def test_tsc_input_matches_reference(golden_dir, name):
+    """VEL_INTERP_TSC, DENS_INTERP_TSC and H_DISTANCE (particles.f90:16-351)."""
+    g = _load(golden_dir, name)
+    n, L0, per, k = (int(g["meta_n"]), float(g["meta_L0"]), int(g["meta_periodic"]), int(g["meta_k"]))
+    vi, di = int(g["meta_vel_interp"]), int(g["meta_dens_interp"])
+    arr = [g[f"in_{c}"] for c in ("x", "y", "z", "vx", "vy", "vz", "m")]
+    f = O.fields_from_particles(*arr, n, L0, per, k=k, vel_interp=vi, dens_interp=di)
+    keys = ["u", "v", "w", "mass", "u1co", "div"] + (["hpart"] if di == 1 else [])
+    for key in keys:
+        assert np.array_equal(f[key], g[f"ref_{key}"]), key
+    assert f["meandens"] == float(g["ref_meandens"])
+    rete = O.cosmo(0.678, 0.31, 0.0)[1]
+    res = O.void_stage(f["grid"], f["u1co"], f["div"], PARAMS, meandens=f["meandens"], rete=rete)
+    compare_void_outputs(res, lambda kk: g[f"ref_{kk}"])
+
+
+def test_tsc_conserves_mass_and_weights():
+    """Periodic TSC: each particle's 27 weights sum to 1 (up to FP32 rounding),
+    so the deposited mass equals the total mass; a particle at a cell centre
+    spreads 1/8, 3/4, 1/8 per axis."""
+    rng = np.random.default_rng(9)
+    n, L0 = 12, 24.0
+    npart = 3000
+    x, y, z = (rng.uniform(-L0 / 2, L0 / 2, npart).astype(np.float32) for _ in range(3))
+    m = rng.uniform(0.5, 1.5, npart).astype(np.float32) * np.float32(9.1717e18)
+    zero = np.zeros(npart, np.float32)
+    f = O.fields_from_particles(x, y, z, zero, zero, zero, m, n, L0, 1, k=8, vel_interp=0, dens_interp=0)
+    total = float(np.sum((m / np.float32(9.1717e18)).astype(np.float64)))
+    assert abs(float(np.sum(f["mass"].astype(np.float64))) - total) <= 1e-5 * total
+    g = O.Grid(n, L0, 1)
+    c = np.float32(g.rad(5))  # one particle on the centre of cell (5, 5, 5)
+    one = np.array([c], np.float32)
+    f1 = O.fields_from_particles(one, one, one, np.zeros(1, np.float32), np.zeros(1, np.float32),
+                                 np.zeros(1, np.float32), np.array([9.1717e18], np.float32), n, L0, 1, k=1,
+                                 vel_interp=0, dens_interp=0)
+    mass = f1["mass"].reshape(n, n, n)  # [z, y, x]
+    w = np.array([0.125, 0.75, 0.125])
+    expect = np.einsum("i,j,k->ijk", w, w, w)
+    assert np.allclose(mass[3:6, 3:6, 3:6], expect, rtol=1e-6, atol=0)
+    assert abs(float(mass.sum()) - 1.0) < 1e-6
+
+
+def test_tsc_velocity_exact_and_reference_order_agree(golden_dir):
+    """The exact (FP64) TSC velocity sums stay within the REAL*4 rounding of the
+    reference's own: |u_exact - u_ref| <= 64 float32 epsilons of the largest
+    velocity in the field, in every cell."""
+    g = _load(golden_dir, "tsc_p1")
+    n, L0, per = int(g["meta_n"]), float(g["meta_L0"]), int(g["meta_periodic"])
+    arr = [g[f"in_{c}"] for c in ("x", "y", "z", "vx", "vy", "vz", "m")]
+    fx = O.fields_from_particles(*arr, n, L0, per, k=16, vel_interp=0, dens_interp=0, tsc_exact=True)
+    speed = np.max(np.abs(np.stack([g["ref_u"], g["ref_v"], g["ref_w"]])))
+    for c in ("u", "v", "w"):
+        d = np.abs(fx[c].astype(np.float64) - g[f"ref_{c}"])
+        assert np.max(d) <= 64 * np.finfo(np.float32).eps * speed, c
+    # a 1% scale error is far outside that bound
+    assert not np.allclose(fx["u"], g["ref_u"] * 1.01, rtol=0, atol=64 * np.finfo(np.float32).eps * speed)
+
+
 def test_indexx_matches_reference(golden_dir):
     z = np.load(golden_dir / "indexx.npz")
     names = [k[5:] for k in z.files if k.startswith("keys_")]
