@@ -105,7 +105,9 @@ int avism_load_particles(avism_ctx* ctx, int64_t np, const float* x, const float
                          const float* z, const float* vx, const float* vy, const float* vz,
                          const float* m, int32_t where);
 /* PPART_DENS, VVEL_INTERP_SPH_VW / VEL_INTERP_TSC, FILL_H_ZEROS, H_DISTANCE,
- * DDENS_INTERP_SPH / DENS_INTERP_TSC, mean density, DIVER_UNIFORM */
+ * DDENS_INTERP_SPH / DENS_INTERP_TSC, mean density, DIVER_UNIFORM
+ * (avism.f90:1177-1257).  vel_interp, dens_interp: FLAG_VEL_INTERP and
+ * FLAG_DENS_INTERP of voids.dat, 1 = SPH, 0 = TSC. */
 int avism_particles_to_grid(avism_ctx* ctx, int32_t k, int32_t vel_interp, int32_t dens_interp);
 /* READ_BINARY_GRID (reader.f90:523-552): delta and velocities (km/s) on the
  * inner grid; then DIVER_UNIFORM.  Used for grid input (FLAG_DATA=2). */
