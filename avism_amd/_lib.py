@@ -50,7 +50,8 @@ class LevelStats(C.Structure):
 EXPORTS = [
     "avism_create", "avism_destroy", "avism_last_error", "avism_set_timing", "avism_get_timings",
     "avism_set_grid", "avism_get_grid", "avism_set_cosmology", "avism_set_void_params",
-    "avism_set_parent_map", "avism_load_particles", "avism_particles_to_grid",
+    "avism_set_parent_map", "avism_set_parent_radii", "avism_next_level", "avism_load_particles",
+    "avism_particles_to_grid",
     "avism_load_grid_fields", "avism_load_void_fields", "avism_get_fields", "avism_get_particle_fields", "avism_find_voids",
     "avism_set_proposals", "avism_get_candidates", "avism_get_cubes", "avism_indexx",
     "avism_merge_and_post", "avism_get_catalogue", "avism_get_marca", "avism_run_level",
@@ -88,6 +89,8 @@ def lib():
         "avism_set_cosmology": ([vp, f32, f32, f32], C.c_int),
         "avism_set_void_params": ([vp, C.POINTER(VoidParams)], C.c_int),
         "avism_set_parent_map": ([vp, vp, i32], C.c_int),
+        "avism_set_parent_radii": ([vp, vp, i32, i32], C.c_int),
+        "avism_next_level": ([vp], C.c_int),
         "avism_load_particles": ([vp, i64, vp, vp, vp, vp, vp, vp, vp, i32], C.c_int),
         "avism_particles_to_grid": ([vp, i32, i32, i32], C.c_int),
         "avism_load_grid_fields": ([vp, vp, vp, vp, vp, i32], C.c_int),
@@ -173,6 +176,31 @@ class Context:
                         rmin=1.0, rmin_sub=1.0, nvoid_max=30000000):
         p = VoidParams(dens_thre, dens_thre2, grad_thre, div_thre, rmin, rmin_sub, int(nvoid_max))
         self._chk(lib().avism_set_void_params(self._h, C.byref(p)))
+
+    def set_parent_map(self, marcap_inner):
+        """MARCAP of the previous level on the inner grid (None: -1 everywhere)."""
+        if marcap_inner is None:
+            self._chk(lib().avism_set_parent_map(self._h, None, AVISM_HOST))
+            return
+        a = np.ascontiguousarray(marcap_inner, np.int32).ravel()
+        gi = self.grid()
+        if a.size != gi.n ** 3:
+            raise ValueError(f"parent map has {a.size} cells, the grid {gi.n ** 3}")
+        self._chk(lib().avism_set_parent_map(self._h, _ptr(a), AVISM_HOST))
+
+    def set_parent_radii(self, reqp):
+        """REQP of the previous level, reqp[id-1] for parent ids 1..len (None: no test)."""
+        if reqp is None:
+            self._chk(lib().avism_set_parent_radii(self._h, None, 0, AVISM_HOST))
+            return
+        a = np.ascontiguousarray(reqp, np.float32).ravel()
+        self._chk(lib().avism_set_parent_radii(self._h, _ptr(a) if a.size else _ptr(np.zeros(1, np.float32)),
+                                               int(a.size), AVISM_HOST))
+
+    def next_level(self) -> GridInfo:
+        """Void-in-void hand-off: MARCAP, REQP and the grid of 2N (avism.f90:2123-2161)."""
+        self._chk(lib().avism_next_level(self._h))
+        return self.grid()
 
     def set_timing(self, on: bool):
         self._chk(lib().avism_set_timing(self._h, int(bool(on))))

@@ -92,6 +92,9 @@ struct avism_ctx {
   avism::DBuf partial64;
   avism::DBuf marcap;                          // inner MARCAP (optional)
   bool has_marcap = false;
+  avism::DBuf reqp;                            // REQP of the parent level, ids 1..nreqp
+  int32_t nreqp = 0;
+  bool has_reqp = false;
   float mass_max = 0;                          // max particle mass (code units)
   bool mass_uniform = false;                   // every particle has the same mass
   avism::DBuf tsc_acc;                         // TSC velocity sums, FP64 [4][N^3]
@@ -117,8 +120,9 @@ struct avism_ctx {
   // merge and post
   avism::DBuf indice, rank_of, paintP, paintK, jstar, pi_ptr, group, uvoid, marca;
   avism::DBuf ev_key, ev_val, ev_key2, ev_val2, grp_start, vf, ncellv, removed;
-  avism::DBuf ccl, cellid, cellid2, cellkey, cellkey2, vstart, inert, per_tmp;
+  avism::DBuf ccl, cellid, cellid2, cellkey, cellkey2, vstart, inert, per_tmp, bkt;
   int32_t nvoid2 = 0, nvoidt = 0, nperiodic = 0;
+  bool post_ready = false;                     // MARCA and the catalogue are final
   double volt_clean = 0;
   int64_t growth_stats[8] = {0};
 };
@@ -145,6 +149,7 @@ int fields_from_grid(avism_ctx* c, const float* delta, const float* vx, const fl
                      const float* vz, int where);
 int void_find(avism_ctx* c, int32_t growth_mode);
 int merge_and_post(avism_ctx* c, int32_t stop_after);
+int next_level(avism_ctx* c);
 int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx);
 void add_timing(avism_ctx* c, const char* name, hipEvent_t a, hipEvent_t b);
 int cnn_patches(avism_ctx* c, const int64_t* ranks, int64_t n, uint16_t* out);

@@ -119,8 +119,11 @@ int avism_set_grid(avism_ctx* c, int32_t n, float L0, int32_t periodic) {
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (scalars)");
   c->grid_set = true;
   c->fields_ready = false;
+  c->post_ready = false;
   c->np = 0;
   c->has_marcap = false;
+  c->has_reqp = false;
+  c->nreqp = 0;
   return AVISM_OK;
 }
 
@@ -176,6 +179,30 @@ int avism_set_parent_map(avism_ctx* c, const int32_t* marcap, int32_t where) {
                             where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   c->has_marcap = true;
   return AVISM_OK;
+}
+
+int avism_set_parent_radii(avism_ctx* c, const float* reqp, int32_t n, int32_t where) {
+  CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
+  if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "no grid set");
+  if (!reqp) {
+    c->has_reqp = false;
+    c->nreqp = 0;
+    return AVISM_OK;
+  }
+  if (n < 0) AV_FAIL(c, AVISM_ERR_ARG, "negative parent radii count");
+  if (!c->reqp.ensure((size_t)(n + 1) * 4)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+  if (n > 0)
+    AV_CHECK_HIP(c, hipMemcpy(c->reqp.p, reqp, (size_t)n * 4,
+                              where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  c->has_reqp = true;
+  c->nreqp = n;
+  return AVISM_OK;
+}
+
+int avism_next_level(avism_ctx* c) {
+  CTX_GUARD(c);
+  return next_level(c);
 }
 
 int avism_load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, const float* z,

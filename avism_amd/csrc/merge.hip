@@ -21,6 +21,15 @@
 //  * GXC/VOLNEW are the reference's FP32 running volume-weighted means, folded
 //    per group in the reference's event order (step, then ball order by
 //    distance, ties by id).
+//  * void-in-void levels: only cubes of the same parent void (FATHER) absorb
+//    each other, by either rule (voidfinding.f90:848-850, 939), so groups
+//    never mix fathers.  j* then takes the lowest rank among same-father
+//    cubes.  A cell whose +-2 boxes all come from one father gives that
+//    father's minimum rank directly; a cube whose box holds a cell where
+//    several fathers' boxes meet finds j* by enumerating the cubes near it
+//    (bucket grid of cube centres).  The close-brother test skips cells
+//    whose painter belongs to another father.
+#include <climits>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -74,6 +83,30 @@ __global__ void k_check_father(int nv, Cubes C, int32_t* flag) {
   if (v < nv && C.father[v] != C.father[0]) atomicOr(flag, 1);
 }
 
+constexpr int32_t kFatherEmpty = INT_MIN, kFatherMixed = INT_MIN + 1;
+
+// the father of the +-2 boxes over each cell: one father, or kFatherMixed
+// where boxes of several fathers meet (a mixed cell never changes back)
+__global__ void k_paint_father(int nv, ExtG G, Cubes C, int32_t* F) {
+  int v = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  int lane = threadIdx.x % 64;
+  if (v >= nv) return;
+  const int f = C.father[v];
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = max(C.ini(a, v) - 2, G.low1);
+    hi[a] = min(C.fin(a, v) + 2, G.low2);
+  }
+  int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
+  int vol = sx * sy * sz;
+  for (int q = lane; q < vol; q += 64) {
+    int a = q % sx, b = (q / sx) % sy, c = q / (sx * sy);
+    int32_t* cell = &F[G.ext(lo[0] + a, lo[1] + b, lo[2] + c)];
+    int old = atomicCAS(cell, kFatherEmpty, f);
+    if (old != kFatherEmpty && old != f && old != kFatherMixed) atomicExch(cell, kFatherMixed);
+  }
+}
+
 // box of cube v grown by g, clipped to the grid, painted with atomicMin(val)
 __global__ void k_paint_box(int nv, ExtG G, Cubes C, const int32_t* __restrict__ indice,
                             const int32_t* __restrict__ keyval, int grow, int32_t* grid) {
@@ -95,9 +128,13 @@ __global__ void k_paint_box(int nv, ExtG G, Cubes C, const int32_t* __restrict__
   }
 }
 
-// j*(c): min of the +-2 painting over c's own box (rank order)
+// j*(c): min of the +-2 painting over c's own box (rank order).  With the
+// father grid F: c's own +-2 box covers each of these cells, so a cell of one
+// father is c's father's and P holds that father's minimum; a mixed cell
+// leaves j* to k_jstar_enumerate (jstar = -2, counted in namb).
 __global__ void k_jstar(int nv, ExtG G, Cubes C, const int32_t* __restrict__ indice,
-                        const int32_t* __restrict__ P, int32_t* jstar, int32_t* key) {
+                        const int32_t* __restrict__ P, const int32_t* __restrict__ F, int32_t* jstar,
+                        int32_t* key, int32_t* namb) {
   int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   int lane = threadIdx.x % 64;
   if (r >= nv) return;
@@ -109,9 +146,75 @@ __global__ void k_jstar(int nv, ExtG G, Cubes C, const int32_t* __restrict__ ind
   }
   int vol = sz[0] * sz[1] * sz[2];
   int m = 0x7fffffff;
+  bool mixed = false;
   for (int q = lane; q < vol; q += 64) {
     int a = q % sz[0], b = (q / sz[0]) % sz[1], c = q / (sz[0] * sz[1]);
-    m = min(m, P[G.ext(lo[0] + a, lo[1] + b, lo[2] + c)]);
+    size_t e = G.ext(lo[0] + a, lo[1] + b, lo[2] + c);
+    if (F && F[e] == kFatherMixed) mixed = true;
+    else m = min(m, P[e]);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
+  mixed = __any(mixed);
+  if (lane == 0) {
+    if (mixed) {
+      jstar[r] = -2;
+      atomicAdd(namb, 1);
+      return;
+    }
+    int js = m < r ? m : -1;
+    jstar[r] = js;
+    key[r] = js >= 0 ? 2 * js + 1 : 2 * r;
+  }
+}
+
+// cube centres bucketed on a grid of kBkt^3 ext cells (key = bucket + 1)
+constexpr int kBkt = 8;
+__global__ void k_bucket_keys(int nv, ExtG G, Cubes C, const int32_t* __restrict__ indice, int nbx,
+                              uint32_t* key, int32_t* val, int32_t* rmax) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nv) return;
+  int v = indice[r] - 1;
+  int bx = (C.ic(0, v) - G.low1) / kBkt, by = (C.ic(1, v) - G.low1) / kBkt, bz = (C.ic(2, v) - G.low1) / kBkt;
+  key[r] = (uint32_t)(bx + nbx * (by + nbx * bz)) + 1u;
+  val[r] = r;
+  atomicMax(rmax, (C.fin(0, v) - C.ini(0, v)) / 2);
+}
+
+// j* of a cube left undecided by k_jstar: the lowest rank j < r of its father
+// whose box grown by 1 overlaps its own grown by 1, over the cubes whose
+// centres lie within r + rmax + 2 cells (lanes over buckets)
+__global__ void k_jstar_enumerate(int nv, ExtG G, Cubes C, const int32_t* __restrict__ indice, int nbx,
+                                  const int32_t* __restrict__ bstart, const int32_t* __restrict__ bend,
+                                  const int32_t* __restrict__ branks, const int32_t* __restrict__ rmaxp,
+                                  int32_t* jstar, int32_t* key) {
+  int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  int lane = threadIdx.x % 64;
+  if (r >= nv || jstar[r] != -2) return;
+  const int v = indice[r] - 1;
+  const int f = C.father[v];
+  int lo[3], hi[3], b0[3], nb[3];
+  const int reach = (C.fin(0, v) - C.ini(0, v)) / 2 + *rmaxp + 2;
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = C.ini(a, v) - 1;
+    hi[a] = C.fin(a, v) + 1;
+    int c0 = max(C.ic(a, v) - reach, G.low1), c1 = min(C.ic(a, v) + reach, G.low2);
+    b0[a] = (c0 - G.low1) / kBkt;
+    nb[a] = (c1 - G.low1) / kBkt - b0[a] + 1;
+  }
+  const int ntot = nb[0] * nb[1] * nb[2];
+  int m = 0x7fffffff;
+  for (int q = lane; q < ntot; q += 64) {
+    int bx = b0[0] + q % nb[0], by = b0[1] + (q / nb[0]) % nb[1], bz = b0[2] + q / (nb[0] * nb[1]);
+    int b = bx + nbx * (by + nbx * bz);
+    for (int i = bstart[b]; i < bend[b]; ++i) {
+      int j = branks[i];
+      if (j >= r || j >= m) continue;
+      int w = indice[j] - 1;
+      if (C.father[w] != f) continue;
+      bool ov = true;
+      for (int a = 0; a < 3; ++a) ov = ov && C.ini(a, w) - 1 <= hi[a] && lo[a] <= C.fin(a, w) + 1;
+      if (ov) m = j;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
   if (lane == 0) {
@@ -124,7 +227,7 @@ __global__ void k_jstar(int nv, ExtG G, Cubes C, const int32_t* __restrict__ ind
 // parent pointer (rank space): absorber, close brother, or self (major)
 __global__ void k_parent(int nv, ExtG G, Cubes C, const int32_t* __restrict__ indice,
                          const int32_t* __restrict__ jstar, const int32_t* __restrict__ K,
-                         int32_t* parent) {
+                         int check_father, int32_t* parent) {
   int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   int lane = threadIdx.x % 64;
   if (r >= nv) return;
@@ -141,13 +244,15 @@ __global__ void k_parent(int nv, ExtG G, Cubes C, const int32_t* __restrict__ in
     cc[a] = C.ic(a, v);
   }
   int vol = sz[0] * sz[1] * sz[2];
+  const int f = C.father[v];
   unsigned long long best = ~0ull;
   int bestkey = -1;
   for (int q = lane; q < vol; q += 64) {  // q follows the reference's loop order
     int a = q % sz[0], b = (q / sz[0]) % sz[1], c = q / (sz[0] * sz[1]);
     int x = lo[0] + a, y = lo[1] + b, z = lo[2] + c;
     int kk = K[G.ext(x, y, z)];
-    if (kk < 2 * r) {
+    // the painter's group has the painter's father (voidfinding.f90:848-850)
+    if (kk < 2 * r && (!check_father || C.father[indice[kk >> 1] - 1] == f)) {
       long long d2 = (long long)(cc[0] - x) * (cc[0] - x) + (long long)(cc[1] - y) * (cc[1] - y) +
                      (long long)(cc[2] - z) * (cc[2] - z);
       unsigned long long kq = ((unsigned long long)d2 << 32) | (unsigned)q;
@@ -991,6 +1096,7 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
   c->nvoidt = 0;
   c->nperiodic = 0;
   c->volt_clean = 0.0;
+  c->post_ready = false;
   if (!c->marca.ensure((size_t)ne3 * 4) || !c->vf.ensure((size_t)12 * (nv + 1) * 4) ||
       !c->indice.ensure((size_t)(nv + 1) * 4) || !c->uvoid.ensure((size_t)(nv + 1) * 4) ||
       !c->ncellv.ensure((size_t)(nv + 1) * 4))
@@ -998,6 +1104,7 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
   if (nv == 0) {
     AV_CHECK_HIP(c, hipMemsetAsync(c->marca.p, 0, (size_t)ne3 * 4, c->stream));
     AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    c->post_ready = stop_after == 0;
     return AVISM_OK;
   }
   Cubes C{c->cube_i.as<int32_t>(), c->cube_f.as<float>(), c->cube_father.as<int32_t>(), nv};
@@ -1032,21 +1139,62 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     k_indice<<<nbk(nv), 256, 0, c->stream>>>(nv, asc, indice, rank);
     AV_CHECK_HIP(c, hipMemsetAsync(small, 0, 64, c->stream));
     k_check_father<<<nbk(nv), 256, 0, c->stream>>>(nv, C, small);
-    int32_t hf = 0;
-    AV_CHECK_HIP(c, hipMemcpyAsync(&hf, small, 4, hipMemcpyDeviceToHost, c->stream));
-    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
-    if (hf) AV_FAIL(c, AVISM_ERR_UNSUPPORTED, "cubes with different parent voids (void-in-void levels) are not supported by the GPU merge");
   }
+  int32_t hf = 0;  // cubes of several parent voids
+  AV_CHECK_HIP(c, hipMemcpyAsync(&hf, small, 4, hipMemcpyDeviceToHost, c->stream));
+  AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   int32_t* P = c->paintP.as<int32_t>();
   int32_t* K = c->paintK.as<int32_t>();
+  int32_t* F = nullptr;
+  if (hf) {
+    if (!c->cover_acc.ensure((size_t)ne3 * 4)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (merge)");
+    F = c->cover_acc.as<int32_t>();
+  }
   {
     StageTimer t(c, "merge");
     k_fill<<<nbk(ne3), 256, 0, c->stream>>>(ne3, P, 0x7fffffff);
     k_fill<<<nbk(ne3), 256, 0, c->stream>>>(ne3, K, 0x7fffffff);
     k_paint_box<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, nullptr, 2, P);
-    k_jstar<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, P, jstar, keyv);
+    if (F) {
+      k_fill<<<nbk(ne3), 256, 0, c->stream>>>(ne3, F, kFatherEmpty);
+      k_paint_father<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, F);
+    }
+    k_jstar<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, P, F, jstar, keyv, small + 2);
+    AV_CHECK_HIP(c, hipGetLastError());
+    if (F) {
+      int32_t namb = 0;
+      AV_CHECK_HIP(c, hipMemcpyAsync(&namb, small + 2, 4, hipMemcpyDeviceToHost, c->stream));
+      AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+      if (namb > 0) {
+        const int nbx = (g.next + kBkt - 1) / kBkt;
+        const int64_t nbt = (int64_t)nbx * nbx * nbx;
+        if (!c->bkt.ensure((size_t)(nbt * 2 + 4 * (int64_t)(nv + 1)) * 4))
+          AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (merge buckets)");
+        int32_t* bstart = c->bkt.as<int32_t>();
+        int32_t* bend = bstart + nbt;
+        uint32_t* bkey = (uint32_t*)(bend + nbt);
+        uint32_t* bkey2 = bkey + (nv + 1);
+        int32_t* bval = (int32_t*)(bkey2 + (nv + 1));
+        int32_t* bval2 = bval + (nv + 1);
+        AV_CHECK_HIP(c, hipMemsetAsync(small + 5, 0, 4, c->stream));
+        k_bucket_keys<<<nbk(nv), 256, 0, c->stream>>>(nv, G, C, indice, nbx, bkey, bval, small + 5);
+        int bits = 1;
+        while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)nbt + 1) ++bits;
+        size_t tmp = 0;
+        AV_CHECK_HIP(c, rocprim::radix_sort_pairs(nullptr, tmp, bkey, bkey2, bval, bval2, (size_t)nv, 0, bits,
+                                                  c->stream));
+        if (!c->sort_tmp.ensure(tmp + 256)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+        AV_CHECK_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp.p, tmp, bkey, bkey2, bval, bval2, (size_t)nv, 0,
+                                                  bits, c->stream));
+        AV_CHECK_HIP(c, hipMemsetAsync(bstart, 0, (size_t)nbt * 8, c->stream));
+        k_label_ranges<<<nbk(nv), 256, 0, c->stream>>>(nv, bkey2, bstart, bend);
+        k_jstar_enumerate<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, nbx, bstart, bend, bval2,
+                                                                small + 5, jstar, keyv);
+        AV_CHECK_HIP(c, hipGetLastError());
+      }
+    }
     k_paint_box<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, keyv, 0, K);
-    k_parent<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, jstar, K, par);
+    k_parent<<<nbk(nv, W), 64 * W, 0, c->stream>>>(nv, G, C, indice, jstar, K, hf ? 1 : 0, par);
     int32_t* a = par;
     int32_t* b = par2;
     for (int it = 0; it < 64; ++it) {
@@ -1260,6 +1408,59 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     c->nperiodic = hp;
     c->volt_clean = (double)ht * (double)((g.dx * g.dx) * g.dx);
   }
+  c->post_ready = true;
+  return AVISM_OK;
+}
+
+namespace {
+// MARCAP(X,Y,Z) = MARCA(INT((X-1)/2)+1, ...) on the grid of 2N (avism.f90:2148-2156);
+// grid (x chunks, y, z) over 2N
+__global__ void k_upsample_marca(ExtG G, const int32_t* __restrict__ marca, int32_t* marcap) {
+  const int n2 = 2 * G.n;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n2) return;
+  const int y = blockIdx.y, z = blockIdx.z;
+  marcap[(size_t)x + (size_t)n2 * ((size_t)y + (size_t)n2 * z)] = marca[G.ext(x / 2 + 1, y / 2 + 1, z / 2 + 1)];
+}
+
+// REQP(IND) = REQ(IND) for the main voids (avism.f90:2140-2144); 0 for the rest
+__global__ void k_reqp(int nv, const int32_t* __restrict__ uvoid, const float* __restrict__ req, float* reqp) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nv) reqp[v] = uvoid[v] == -1 ? req[v] : 0.0f;
+}
+}  // namespace
+
+// hand-off to the next void-in-void level (avism.f90:2123-2161, 742-759):
+// MARCAP and REQP from this level's catalogue, then the grid of 2N cells per
+// axis over the same box.  The particles stay loaded, so the next level can
+// interpolate them at its own resolution.
+int next_level(avism_ctx* c) {
+  if (!c->post_ready) AV_FAIL(c, AVISM_ERR_STATE, "avism_merge_and_post must complete before the next level");
+  const GridGeom g = c->g;
+  const int n2 = 2 * g.n;
+  GridGeom g2 = make_grid(n2, g.L0, g.periodic);
+  if ((int64_t)g2.next * g2.next * g2.next >= (int64_t)0x7fffffff)
+    AV_FAIL(c, AVISM_ERR_LIMIT, "extended grid of the next level above 2^31 cells is not supported by this build");
+  const int nv = c->nvoid;
+  if (!c->marcap.ensure((size_t)n2 * n2 * n2 * 4) || !c->reqp.ensure((size_t)(nv + 1) * 4))
+    AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (next level)");
+  ExtG G{g.n, g.low1, g.low2, g.next, g.periodic};
+  k_upsample_marca<<<dim3((unsigned)((n2 + 255) / 256), (unsigned)n2, (unsigned)n2), 256, 0, c->stream>>>(
+      G, c->marca.as<int32_t>(), c->marcap.as<int32_t>());
+  if (nv > 0)
+    k_reqp<<<nbk(nv), 256, 0, c->stream>>>(nv, c->uvoid.as<int32_t>(), c->vf.as<float>() + 11 * (size_t)nv,
+                                           c->reqp.as<float>());
+  AV_CHECK_HIP(c, hipGetLastError());
+  AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  c->g = g2;
+  if (!c->scalars.ensure(scalars_bytes(c))) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (scalars)");
+  c->has_marcap = true;
+  c->has_reqp = true;
+  c->nreqp = nv;
+  c->fields_ready = false;
+  c->post_ready = false;
+  c->nvoid = 0;
+  c->ncand = 0;
   return AVISM_OK;
 }
 
@@ -1289,7 +1490,7 @@ int get_catalogue(avism_ctx* c, int32_t* indice, int32_t* uvoid, int32_t* ncellv
 int get_marca(avism_ctx* c, int32_t* marca, int32_t inner_only, int32_t where) {
   const GridGeom& g = c->g;
   const int64_t ne3 = (int64_t)g.next * g.next * g.next;
-  if (!c->marca.p) AV_FAIL(c, AVISM_ERR_STATE, "no MARCA computed");
+  if (!c->marca.p || !c->post_ready) AV_FAIL(c, AVISM_ERR_STATE, "no MARCA computed (run avism_merge_and_post)");
   hipMemcpyKind kind = where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   if (!inner_only) {
     AV_CHECK_HIP(c, hipMemcpy(marca, c->marca.p, (size_t)ne3 * 4, kind));

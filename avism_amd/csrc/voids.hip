@@ -19,6 +19,7 @@
 //     predecessor over it and is accepted, one whose centre is covered by an
 //     accepted lower rank is rejected.  The lowest undecided rank is always
 //     decided, so the rounds terminate; ids are the ranks' prefix count.
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -243,10 +244,17 @@ __device__ __forceinline__ void finish_growth(int sblock, int szone, bool zflag2
   }
 }
 
+// ParentRadii: REQP of the parent level, ids 1..n (null: no parent-size test)
+struct ParentRadii {
+  const float* reqp;
+  int n;
+  float rmin_sub;
+};
+
 __global__ void k_grow_lane(int64_t ncand, ExtGeom G, const int32_t* __restrict__ order,
                             const int32_t* __restrict__ lin, const uint8_t* __restrict__ flags,
-                            const int32_t* __restrict__ marcap, const int8_t* __restrict__ prop,
-                            int smax, int8_t* status, int16_t* rr) {
+                            const int32_t* __restrict__ marcap, ParentRadii PR,
+                            const int8_t* __restrict__ prop, int smax, int8_t* status, int16_t* rr) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncand) return;
   int e = lin[order[t]];
@@ -260,6 +268,13 @@ __global__ void k_grow_lane(int64_t ncand, ExtGeom G, const int32_t* __restrict_
     return;
   }
   int indv = marcap ? marcap[G.inner(I, J, K)] : -1;
+  // FLAG1_GRID (voidfinding.f90:267-274, 491): a parent void smaller than
+  // RMIN_SUB stops the cube at its centre cell, which FLAG2 then rejects (657-659)
+  if (indv > 0 && PR.reqp && indv <= PR.n && PR.reqp[indv - 1] < PR.rmin_sub) {
+    status[t] = ST_INVALID;
+    rr[t] = 0;
+    return;
+  }
   int szone;
   bool zf2;
   zone_info(G, I, J, K, szone, zf2);
@@ -456,6 +471,15 @@ __global__ void k_write_cubes(int64_t ncand, ExtGeom G, const int32_t* __restric
   father[v] = marcap ? marcap[G.inner(I, J, K)] : -1;
 }
 
+// largest id in the parent map (grid-stride; one atomic per wave)
+__global__ void k_max_id(int64_t n, const int32_t* __restrict__ a, int32_t* out) {
+  int m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, a[i]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+  if (threadIdx.x % 64 == 0) atomicMax(out, m);
+}
+
 // processing order t -> compacted candidate (largest divergence first)
 __global__ void k_reverse_order(int64_t n, const int32_t* __restrict__ indx, int32_t* ord) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -467,6 +491,7 @@ inline unsigned nbk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b)
 
 int void_find(avism_ctx* c, int32_t growth_mode) {
   if (!c->fields_ready) AV_FAIL(c, AVISM_ERR_STATE, "fields not computed");
+  c->post_ready = false;
   const GridGeom& g = c->g;
   ExtGeom G{g.n, g.low1, g.low2, g.next, g.periodic};
   const int64_t ne3 = (int64_t)g.next * g.next * g.next;
@@ -477,6 +502,17 @@ int void_find(avism_ctx* c, int32_t growth_mode) {
   P.divthr = c->vp.div_thre;
   P.two_dx = 2.0f * g.dx;  // exact; see voidfinding.f90:290
   const int32_t* marcap = c->has_marcap ? c->marcap.as<int32_t>() : nullptr;
+  if (marcap && c->has_reqp) {  // every parent id must have its REQP
+    const int64_t n3 = (int64_t)g.n * g.n * g.n;
+    int32_t* mx = (int32_t*)((char*)c->scalars.ensure(4096) + 768);
+    AV_CHECK_HIP(c, hipMemsetAsync(mx, 0, 4, c->stream));
+    k_max_id<<<(unsigned)std::min<int64_t>(nbk(n3), 4096), 256, 0, c->stream>>>(n3, marcap, mx);
+    AV_CHECK_HIP(c, hipGetLastError());
+    int32_t h = 0;
+    AV_CHECK_HIP(c, hipMemcpyAsync(&h, mx, 4, hipMemcpyDeviceToHost, c->stream));
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    if (h > c->nreqp) AV_FAIL(c, AVISM_ERR_INPUT, "the parent map holds void ids above the number of parent radii");
+  }
   if (!c->flags.ensure((size_t)ne3)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (flags)");
   {
     StageTimer t(c, "mark_flags");
@@ -538,8 +574,9 @@ int void_find(avism_ctx* c, int32_t growth_mode) {
   }
   {
     StageTimer t(c, prop ? "grow_proposal" : "grow");
+    ParentRadii PR{c->has_reqp && marcap ? c->reqp.as<float>() : nullptr, c->nreqp, c->vp.rmin_sub};
     k_grow_lane<<<nbk(nc), 256, 0, c->stream>>>(nc, G, order, c->cand_lin.as<int32_t>(),
-                                                c->flags.as<uint8_t>(), marcap, prop, 4, status, rr);
+                                                c->flags.as<uint8_t>(), marcap, PR, prop, 4, status, rr);
     k_grow_wave<<<nbk(nc, 4), 256, 0, c->stream>>>(nc, G, order, c->cand_lin.as<int32_t>(),
                                                    c->flags.as<uint8_t>(), marcap, status, rr);
     AV_CHECK_HIP(c, hipGetLastError());

@@ -93,8 +93,24 @@ int avism_get_grid(avism_ctx* ctx, avism_grid_info* out, float* rad_ext /* next 
 /* cosmology of avism.f90:469-479, 599-606 (h, Omega_m, redshift) */
 int avism_set_cosmology(avism_ctx* ctx, float ache, float omega0, float zeta);
 int avism_set_void_params(avism_ctx* ctx, const avism_void_params* p);
-/* MARCAP of the previous level (avism.f90:2123-2161); NULL = -1 everywhere */
+/* MARCAP of the previous level (avism.f90:2123-2161); NULL = -1 everywhere.
+ * Cells with MARCAP = 0 can hold no cube; a cube grows only over cells of its
+ * centre's parent id, and only cubes of one parent merge (voidfinding.f90:34,
+ * 487-601, 848-850, 939). */
 int avism_set_parent_map(avism_ctx* ctx, const int32_t* marcap_inner, int32_t where);
+/* REQP of the previous level, reqp[id-1] for parent ids 1..n (avism.f90:2138-2144):
+ * a candidate whose parent has REQP < RMIN_SUB grows no cube
+ * (voidfinding.f90:267-274, 491).  NULL: no parent-size test.  Fails with
+ * AVISM_ERR_INPUT at avism_find_voids if the parent map holds ids above n. */
+int avism_set_parent_radii(avism_ctx* ctx, const float* reqp, int32_t n, int32_t where);
+/* the void-in-void hand-off (avism.f90:2123-2161, 742-759), after
+ * avism_merge_and_post: MARCAP = this level's MARCA upsampled by 2 per axis,
+ * REQP = REQ of its main voids, and the grid becomes 2N cells per axis over
+ * the same box.  Loaded particles are kept, so avism_particles_to_grid then
+ * interpolates them at the new resolution (the reference frees them inside
+ * its level loop, avism.f90:1613-1629); grid fields for 2N can be loaded
+ * instead. */
+int avism_next_level(avism_ctx* ctx);
 
 /* ---- field stage (avism.f90:1177-1257, 1371-1414) --------------------- */
 /* READ_BINARY_PART_1 data (reader.f90:410-465): positions Mpc/h, velocities

@@ -318,6 +318,23 @@ def void_stage(g: Grid, u1co, div, params: VoidParams, marcap=None, reqp=None, s
     return res
 
 
+def inner_of_ext(g: Grid, a_ext: np.ndarray) -> np.ndarray:
+    """The inner cells 1..N of an ext-grid array (x fastest)."""
+    ne, n, o = g.next, g.n, 1 - g.low1
+    return np.ascontiguousarray(np.asarray(a_ext).reshape(ne, ne, ne)[o:o + n, o:o + n, o:o + n]).ravel()
+
+
+def next_level_inputs(g: Grid, marca_ext, uvoid, req):
+    """Hand-off to the next void-in-void level (avism.f90:2123-2161): MARCAP on
+    the grid of 2N, MARCAP(IX,JY,KZ) = MARCA(INT((IX-1)/2.)+1, ...), and
+    REQP(IND) = REQ(IND) for the main voids (UVOID = -1), 0 for the others."""
+    n = g.n
+    mi = inner_of_ext(g, marca_ext).reshape(n, n, n)
+    up = mi.repeat(2, axis=0).repeat(2, axis=1).repeat(2, axis=2)
+    reqp = np.where(np.asarray(uvoid) == -1, np.asarray(req, np.float32), np.float32(0.0)).astype(np.float32)
+    return np.ascontiguousarray(up, np.int32).ravel(), reqp
+
+
 # ----------------------------------------------------------------- reference
 REF_DIR = HERE / "_ref"
 
@@ -340,12 +357,24 @@ def run_reference_indexx(jobdir: Path, keys: np.ndarray) -> np.ndarray:
 
 def run_reference(jobdir: Path, mode: int, n: int, L0: float, periodic: int, params: VoidParams,
                   ache=0.678, omega0=0.31, zeta=0.0, k=32, vel_interp=1, dens_interp=1,
-                  parts=None, fields=None, timeout=600) -> dict:
-    """Run the reference routines (oracle/_ref) on one level and read the outputs."""
+                  parts=None, fields=None, marcap=None, reqp=None, timeout=600) -> dict:
+    """Run the reference routines (oracle/_ref) on one level and read the outputs.
+    marcap (inner n^3 int32) and reqp (float32, ids 1..len) are a parent level's
+    MARCAP and REQP (avism.f90:2123-2161); None runs the first level."""
     import resource
 
     jobdir = Path(jobdir)
     jobdir.mkdir(parents=True, exist_ok=True)
+    for f in ("marcap.bin", "reqp.bin"):
+        (jobdir / f).unlink(missing_ok=True)
+    if marcap is not None:
+        mp = np.ascontiguousarray(marcap, np.int32)
+        assert mp.size == n ** 3
+        rq = np.zeros(params.nvoid_max, np.float32)
+        if reqp is not None:
+            rq[: len(reqp)] = np.asarray(reqp, np.float32)
+        (jobdir / "marcap.bin").write_bytes(mp.tobytes())
+        (jobdir / "reqp.bin").write_bytes(rq.tobytes())
     npart = 0
     if mode == 1:
         arrs = [np.ascontiguousarray(a, np.float32) for a in parts]

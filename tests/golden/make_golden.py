@@ -15,7 +15,16 @@ Each fixture holds the inputs and the reference's outputs for one level:
   tscv_p{0,1}     : FLAG_VEL_INTERP = 0, FLAG_DENS_INTERP = 1 -> VEL_INTERP_TSC,
                     H_DISTANCE, FILL_H_ZEROS, DDENS_INTERP_SPH and the rest
 
-`python tests/golden/make_golden.py tsc` writes only the tsc fixtures.
+  lev_p{0,1}      : two void-in-void levels on one particle set: the first
+                    level at N, then the same particles at 2N with the first
+                    level's MARCAP and REQP (avism.f90:2123-2161) -> the
+                    second level's fields, cubes with FATHER, merge and
+                    catalogue.  The reference's driver frees the particles
+                    inside its level loop (avism.f90:1613-1629), so each level
+                    is one run of the reference routines; RMIN_SUB is set so
+                    that some parents are below it (voidfinding.f90:267-274).
+
+`python tests/golden/make_golden.py tsc` (or `lev`) writes only those fixtures.
 """
 from __future__ import annotations
 
@@ -73,6 +82,28 @@ def tsc_fixtures(td: Path):
                   dict(n=n, L0=L0, periodic=per, mode=1, k=16, seed=seed, vel_interp=vi, dens_interp=di))
 
 
+def lev_fixtures(td: Path):
+    cases = (
+        (1, synth.zeldovich(16, 40.0, seed=22, rms_disp=2.0).numpy(), 16, 40.0, 22, 5.0),
+        (0, synth.mock_voids(n0=5000, L0=60.0, nvoids=4, rmin=6.0, rmax=12.0, seed=23)[0].numpy(), 20, 60.0,
+         23, 6.0),
+    )
+    for per, arr, n, L0, seed, rmin_sub in cases:
+        P = O.VoidParams(**{**PARAMS.__dict__, "rmin_sub": rmin_sub})
+        r1 = O.run_reference(td / f"lev{per}_1", 1, n, L0, per, P, k=16, parts=arr)
+        marcap, reqp = O.next_level_inputs(O.Grid(n, L0, per), r1["marca"], r1["uvoid"], r1["vf"][11])
+        r2 = O.run_reference(td / f"lev{per}_2", 1, 2 * n, L0, per, P, k=16, parts=arr, marcap=marcap, reqp=reqp)
+        inputs = dict(zip(("x", "y", "z", "vx", "vy", "vz", "m"), arr))
+        inputs.update(marcap=marcap, reqp=reqp)
+        # the velocity and mass grids are checked at one level by part_p*; here
+        # u1co, div and hpart (whose VVEL ownership depends on the grid) are kept
+        ref = {k: v for k, v in r2.items() if k not in ("u", "v", "w", "mass", "part_dens")}
+        ref.update(l1_marca=r1["marca"], l1_uvoid=r1["uvoid"], l1_req=r1["vf"][11], l1_nvoid=r1["nvoid"],
+                   l1_nvoidt=r1["nvoidt"])
+        _save(f"lev_p{per}", inputs, ref, dict(n=n, n2=2 * n, L0=L0, periodic=per, mode=1, k=16, seed=seed,
+                                               rmin_sub=rmin_sub))
+
+
 def main():
     for per in (0, 1):
         if not O.ref_available(per):
@@ -82,7 +113,11 @@ def main():
         if sys.argv[1:] == ["tsc"]:
             tsc_fixtures(td)
             return
+        if sys.argv[1:] == ["lev"]:
+            lev_fixtures(td)
+            return
         tsc_fixtures(td)
+        lev_fixtures(td)
         for per, seed in ((0, 11), (1, 12)):
             n, L0 = 24, 48.0
             d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, seed))

@@ -206,3 +206,103 @@ def test_indexx_is_a_sort_and_planted_mistake_is_caught():
     # a stable sort has the same keys but (here) a different tie order
     stable = np.argsort(keys, kind="stable") + 1
     assert not np.array_equal(stable, idx)
+
+
+# ------------------------------------------------------------ void-in-void levels
+def mixed_father_cubes(ci, father, low1, ne):
+    """Cubes whose own box holds a cell covered by the +-2 boxes of cubes of
+    two or more fathers (where the merge must tell fathers apart), and the
+    pairs of cubes of different fathers whose boxes grown by 1 overlap."""
+    nv = ci.shape[1]
+    seen = np.full(ne ** 3, np.iinfo(np.int32).min, np.int64)
+    mixed = np.zeros(ne ** 3, bool)
+    for v in range(nv):
+        lo = np.maximum(ci[0:3, v] - 2, low1) - low1
+        hi = np.minimum(ci[3:6, v] + 2, low1 + ne - 1) - low1
+        zz, yy, xx = np.meshgrid(*(np.arange(lo[a], hi[a] + 1) for a in (2, 1, 0)), indexing="ij")
+        e = (xx + ne * (yy + ne * zz)).ravel()
+        prev = seen[e]
+        mixed[e[(prev != np.iinfo(np.int32).min) & (prev != father[v])]] = True
+        seen[e] = np.where(prev == np.iinfo(np.int32).min, father[v], prev)
+    amb = 0
+    for v in range(nv):
+        lo, hi = ci[0:3, v] - low1, ci[3:6, v] - low1
+        zz, yy, xx = np.meshgrid(*(np.arange(lo[a], hi[a] + 1) for a in (2, 1, 0)), indexing="ij")
+        amb += bool(mixed[(xx + ne * (yy + ne * zz)).ravel()].any())
+    lo, hi = ci[0:3] - 1, ci[3:6] + 1
+    pairs = 0
+    for v in range(nv):
+        ov = np.all((lo[:, v:v + 1] <= hi) & (lo <= hi[:, v:v + 1]), axis=0)
+        pairs += int(np.sum(ov & (father != father[v])))
+    return amb, pairs // 2
+
+
+def flag1_candidates(g, marcap, reqp, rmin_sub, cand_lin):
+    """Candidates whose parent void is smaller than RMIN_SUB."""
+    ne, n = g.next, g.n
+    lin = np.asarray(cand_lin, np.int64)
+    idx = [lin % ne + g.low1, (lin // ne) % ne + g.low1, lin // (ne * ne) + g.low1]
+    idx = [np.where(a < 1, a + n, np.where(a > n, a - n, a)) - 1 for a in idx]
+    par = np.asarray(marcap).reshape(n, n, n)[idx[2], idx[1], idx[0]]
+    small = np.array([i + 1 for i in range(len(reqp)) if reqp[i] < rmin_sub], np.int64)
+    return int(np.isin(par, small).sum())
+
+
+@pytest.mark.parametrize("name", ["lev_p0", "lev_p1"])
+def test_sublevel_matches_reference(golden_dir, name):
+    """Two void-in-void levels run by the reference's own routines: the oracle
+    reproduces the first level, the hand-off, the second level's fields and
+    its cubes (FATHER, FLAG1 of small parents), merge and catalogue."""
+    g = _load(golden_dir, name)
+    n, n2, L0, per, k = (int(g["meta_n"]), int(g["meta_n2"]), float(g["meta_L0"]), int(g["meta_periodic"]),
+                         int(g["meta_k"]))
+    P = O.VoidParams(**{**PARAMS.__dict__, "rmin_sub": float(g["meta_rmin_sub"])})
+    arr = [g[f"in_{c}"] for c in ("x", "y", "z", "vx", "vy", "vz", "m")]
+    rete = O.cosmo(0.678, 0.31, 0.0)[1]
+    f1 = O.fields_from_particles(*arr, n, L0, per, k=k)
+    r1 = O.void_stage(f1["grid"], f1["u1co"], f1["div"], P, meandens=f1["meandens"], rete=rete)
+    assert np.array_equal(r1.marca, g["ref_l1_marca"])
+    assert np.array_equal(r1.uvoid, g["ref_l1_uvoid"])
+    assert np.array_equal(r1.vf[11], g["ref_l1_req"])
+    marcap, reqp = O.next_level_inputs(f1["grid"], r1.marca, r1.uvoid, r1.vf[11])
+    assert np.array_equal(marcap, g["in_marcap"]) and np.array_equal(reqp, g["in_reqp"])
+    f2 = O.fields_from_particles(*arr, n2, L0, per, k=k)
+    for key in ("hpart", "u1co", "div"):
+        assert np.array_equal(f2[key], g[f"ref_{key}"]), key
+    assert f2["meandens"] == float(g["ref_meandens"])
+    res = O.void_stage(f2["grid"], f2["u1co"], f2["div"], P, marcap=marcap, reqp=reqp, meandens=f2["meandens"],
+                       rete=rete)
+    compare_void_outputs(res, lambda kk: g[f"ref_{kk}"])
+    assert np.array_equal(res.father, g["ref_father"])
+    # what the fixture exercises: cubes of several parents, parents below
+    # RMIN_SUB with candidates in them, and (periodic case) cubes of different
+    # parents close enough that only the FATHER test keeps them apart
+    assert res.nvoid > 0 and len(np.unique(res.father)) >= 2
+    cand = O.void_stage(f2["grid"], f2["u1co"], f2["div"], P, marcap=marcap, reqp=reqp, stop_after=-1,
+                        want_candidates=True)
+    assert flag1_candidates(f2["grid"], marcap, reqp, P.rmin_sub, cand.cand_lin) > 0
+    if per:
+        amb, pairs = mixed_father_cubes(res.cubes_i, res.father, f2["grid"].low1, f2["grid"].next)
+        assert amb > 0 and pairs > 0
+
+
+@pytest.mark.parametrize("per", [0, 1])
+def test_next_level_inputs_formula(per):
+    """MARCAP(IX,JY,KZ) = MARCA(INT((IX-1)/2.)+1, ...) and REQP(IND) = REQ(IND)
+    for main voids (avism.f90:2140-2156), written out cell by cell."""
+    rng = np.random.default_rng(5 + per)
+    n = 5
+    g = O.Grid(n, 10.0, per)
+    ne = g.next
+    marca = rng.integers(0, 7, size=ne ** 3).astype(np.int32)
+    uvoid = np.array([-1, 3, -1, 0, -1, 1], np.int32)
+    req = rng.uniform(1, 5, size=6).astype(np.float32)
+    mp, rq = O.next_level_inputs(g, marca, uvoid, req)
+    m3 = marca.reshape(ne, ne, ne)
+    for kz in range(1, 2 * n + 1):
+        for jy in range(1, 2 * n + 1):
+            for ix in range(1, 2 * n + 1):
+                ixc, jyc, kzc = (int((a - 1) / 2.0) + 1 for a in (ix, jy, kz))
+                want = m3[kzc - g.low1, jyc - g.low1, ixc - g.low1]
+                assert mp[(ix - 1) + 2 * n * ((jy - 1) + 2 * n * (kz - 1))] == want
+    assert rq.tolist() == [req[0], 0.0, req[2], 0.0, req[4], 0.0]
