@@ -118,6 +118,10 @@ module avism_c
       type(c_ptr), value :: ctx, marca
       integer(c_int32_t), value :: inner_only, where
     end function
+    integer(c_int) function avism_next_level(ctx) bind(C, name="avism_next_level")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
     integer(c_int) function avism_cnn_set_params(ctx, params, where) bind(C, name="avism_cnn_set_params")
       import :: c_ptr, c_int, c_int32_t, c_float
       type(c_ptr), value :: ctx

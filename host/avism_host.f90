@@ -1,13 +1,23 @@
 !-----------------------------------------------------------------------
 ! avism_host -- Fortran host of the MI355X grid-and-void path.
 !
-! Drop-in for the single-level path of PROGRAM AVISM (avism.f90): reads
+! Drop-in for the grid-and-void path of PROGRAM AVISM (avism.f90): reads
 ! config/voids.dat with the same positional READs (avism.f90:176-253), reads
 ! input_data/bin_file_partNNNNN (reader.f90:410-450, FLAG_DATA=1) or
 ! input_data/bin_file_gridNNNNN (reader.f90:521-540, FLAG_DATA=2), runs the
-! path through the C ABI (host/avism_c.f90) and writes output_files/voidsNNNNN,
-! mapNNNNN and, with FLAG_WRITE_CUBES=1, cubesNNNNN as avism.f90:707-722 and
-! 2028-2085 do.
+! path through the C ABI (host/avism_c.f90) for the levels LEVMIN..LEVMAX
+! (avism.f90:742-2181) and writes output_files/voidsNNNNN, mapNNNNN and, with
+! FLAG_WRITE_CUBES=1, cubesNNNNN as avism.f90:707-722 and 2028-2085 do, one
+! block per level.
+!
+! Void-in-void levels (particle input): level IR has NCOX*2**(IR-LEVMIN)
+! cells per axis (avism.f90:755-758); avism_next_level hands MARCAP and REQP
+! over (avism.f90:2123-2161) and the same particles are interpolated again
+! (the reference frees them inside its level loop, avism.f90:1613-1629).
+! The catalogue's parent radius column is REQP of the parent void (the
+! reference prints REQ(INDP), the current level's array at the parent's id,
+! avism.f90:2076-2078); NVOIDP is 0 at the first level (uninitialised in the
+! reference, avism.f90:70).
 !
 ! Usage:  avism_host.x [--learned weights.f32]
 !   --learned: growth with the half-width proposal network whose fp32
@@ -15,8 +25,8 @@
 !   catalogue is the same as without it.
 !
 ! Not covered (the library returns or this program stops with a message):
-! LEVMAX > LEVMIN (void-in-void levels), survey mode, MASCLET/GADGET input,
-! TSC interpolation, an input grid size different from NCOX.
+! LEVMAX > LEVMIN with grid input, survey mode, MASCLET/GADGET input, an input
+! grid size different from NCOX.
 !-----------------------------------------------------------------------
 program avism_host
   use iso_c_binding
@@ -39,7 +49,8 @@ program avism_host
   type(avism_level_stats) :: st
   integer(c_int) :: rc
   integer(c_int32_t) :: ncubes, nvoids, growth
-  integer :: niter, ifi, iter, n, nlev, ir
+  integer :: niter, ifi, iter, n, nlev, ir, nvoidp
+  real(c_float), allocatable :: reqp(:)
   real :: zeta
   real(c_float), allocatable, target :: params(:)
   character(len=512) :: arg, weights
@@ -77,20 +88,38 @@ program avism_host
   end if
 
   niter = int((last - first) / every) + 1
+  nlev = levmax - levmin + 1
   do ifi = 1, niter
     iter = first + every * (ifi - 1)
-    if (flag_data == 1) then
-      call run_particles(iter, zeta)
-    else
-      call run_grid(iter, zeta)
-    end if
-    call avism_check(ctx, avism_find_voids(ctx, growth, ncubes), 'avism_find_voids')
-    call avism_check(ctx, avism_merge_and_post(ctx, nvoids), 'avism_merge_and_post')
-    n = ncox
-    nlev = levmax - levmin + 1
-    ir = levmin
-    call write_outputs(iter, ir, nlev, n, ncubes)
-    write(*,*) 'ITER', iter, ': cubes', ncubes, ' voids', nvoids
+    if (ifi > 1) call avism_check(ctx, avism_set_grid(ctx, int(ncox, c_int32_t), real(lado0, c_float), &
+                                                   int(flag_period, c_int32_t)), 'avism_set_grid')
+    call open_outputs(iter)
+    nvoidp = 0
+    allocate(reqp(0))
+    do ir = levmin, levmax
+      n = ncox * 2**(ir - levmin)
+      if (ir == levmin) then
+        if (flag_data == 1) then
+          call run_particles(iter, zeta)
+        else
+          call run_grid(iter, zeta)
+        end if
+      else
+        call avism_check(ctx, avism_next_level(ctx), 'avism_next_level')
+        call avism_check(ctx, avism_particles_to_grid(ctx, int(kneighbours, c_int32_t), &
+                                                   int(flag_vel_interp, c_int32_t), &
+                                                   int(flag_dens_interp, c_int32_t)), 'avism_particles_to_grid')
+      end if
+      call avism_check(ctx, avism_find_voids(ctx, growth, ncubes), 'avism_find_voids')
+      call avism_check(ctx, avism_merge_and_post(ctx, nvoids), 'avism_merge_and_post')
+      call write_level(ir, n, ncubes, nvoidp, reqp)
+      write(*,*) 'ITER', iter, ' LEVEL', ir, ': cubes', ncubes, ' voids', nvoids
+      nvoidp = ncubes
+    end do
+    deallocate(reqp)
+    close(10)
+    close(11)
+    if (flag_write_cubes == 1) close(12)
   end do
   rc = avism_destroy(ctx)
 
@@ -143,8 +172,12 @@ contains
       write(*,*) '     ERROR: NCOX, NCOY and NCOZ must be equal'
       stop 1
     end if
-    if (levmax /= levmin) then
-      write(*,*) '     ERROR: void-in-void levels (LEVMAX > LEVMIN) are not supported by this build'
+    if (levmax < levmin) then
+      write(*,*) '     ERROR: LEVMAX must not be below LEVMIN'
+      stop 1
+    end if
+    if (levmax /= levmin .and. flag_data /= 1) then
+      write(*,*) '     ERROR: void-in-void levels (LEVMAX > LEVMIN) need particle input (FLAG_DATA = 1) in this build'
       stop 1
     end if
     if (flag_data /= 1 .and. flag_data /= 2) then
@@ -218,13 +251,28 @@ contains
     call avism_check(ctx, avism_load_grid_fields(ctx, d, u, v, w, AVISM_HOST), 'avism_load_grid_fields')
   end subroutine
 
-  subroutine write_outputs(it, lev, nl, nn, nv)
-    integer, intent(in) :: it, lev, nl, nn
+  ! file headers (avism.f90:707-722); the map file takes one block per level
+  subroutine open_outputs(it)
+    integer, intent(in) :: it
+    open(unit=10, file=dir//'/voids'//iter_tag(it), status='replace')
+    write(10, *) nlev, levmin, levmax, ncox, ncoy, ncoz, lado0
+    open(unit=11, file=dir//'/map'//iter_tag(it), form='unformatted', access='stream', status='replace')
+    if (flag_write_cubes == 1) then
+      open(unit=12, file=dir//'/cubes'//iter_tag(it), status='replace')
+      write(12, *) nlev, levmin, levmax
+    end if
+  end subroutine
+
+  ! one level's block of the catalogue, cubes and map files (avism.f90:2028-2085);
+  ! on return reqp holds this level's REQP (REQ of the main voids, by id) for the next
+  subroutine write_level(lev, nn, nv, nvp, reqp)
+    integer, intent(in) :: lev, nn, nvp
     integer(c_int32_t), intent(in) :: nv
+    real(c_float), allocatable, intent(inout) :: reqp(:)
     integer(c_int32_t), allocatable, target :: indice(:), uvoid(:), ncellv(:), marca(:), ci(:,:)
     real(c_float), allocatable, target :: vf(:,:), u1co(:), div(:), cf(:,:)
     integer(c_int32_t), allocatable, target :: father(:)
-    integer :: iv, ind, nvp, nvs
+    integer :: iv, ind, nvs
     integer(8) :: ncell
     real :: volm, req, reqpp
     integer :: indp
@@ -233,24 +281,17 @@ contains
     allocate(indice(nvs), uvoid(nvs), ncellv(nvs), vf(nvs, 12))
     call avism_check(ctx, avism_get_catalogue(ctx, c_loc(indice), c_loc(uvoid), c_loc(ncellv), c_loc(vf), st), &
                   'avism_get_catalogue')
+    allocate(ci(nvs, 9), cf(nvs, 7), father(nvs))
+    call avism_check(ctx, avism_get_cubes(ctx, c_loc(ci), c_loc(cf), c_loc(father)), 'avism_get_cubes')
     ncell = int(nn, 8)**3
     allocate(marca(ncell), u1co(ncell), div(ncell))
     call avism_check(ctx, avism_get_marca(ctx, c_loc(marca), 1_c_int32_t, AVISM_HOST), 'avism_get_marca')
     call avism_check(ctx, avism_get_fields(ctx, c_loc(u1co), c_loc(div), c_null_ptr, c_null_ptr, c_null_ptr, &
                                         AVISM_HOST), 'avism_get_fields')
 
-    ! voids catalogue: file header (avism.f90:713-714), level header (2047), one line per main void
-    open(unit=10, file=dir//'/voids'//iter_tag(it), status='replace')
-    write(10, *) nl, levmin, levmax, ncox, ncoy, ncoz, lado0
-    nvp = 0
+    ! level header (avism.f90:2047), one line per main void
     write(10, *) lev, int(nv), int(st%nvoidt), nvp, real(st%volt_clean / (real(lado0, 8)**3)), st%meandens
-    if (flag_write_cubes == 1) then
-      allocate(ci(nvs, 9), cf(nvs, 7), father(nvs))
-      call avism_check(ctx, avism_get_cubes(ctx, c_loc(ci), c_loc(cf), c_loc(father)), 'avism_get_cubes')
-      open(unit=12, file=dir//'/cubes'//iter_tag(it), status='replace')
-      write(12, *) nl, levmin, levmax
-      write(12, *) lev, int(nv), int(st%nvoidt)
-    end if
+    if (flag_write_cubes == 1) write(12, *) lev, int(nv), int(st%nvoidt)
     do iv = 1, int(nv)
       ind = indice(iv)
       if (flag_write_cubes == 1) then
@@ -262,18 +303,26 @@ contains
       req = vf(ind, 12)
       indp = 0
       reqpp = 0.0
+      if (lev > levmin) then
+        indp = father(ind)
+        if (indp >= 1 .and. indp <= size(reqp)) reqpp = reqp(indp)
+      end if
       write(10, *) ind, vf(ind, 4), vf(ind, 5), vf(ind, 6), vf(ind, 1), vf(ind, 2), vf(ind, 3), volm, req, &
                    vf(ind, 8), vf(ind, 10), vf(ind, 11), indp, reqpp, vf(ind, 9)
     end do
-    close(10)
-    if (flag_write_cubes == 1) close(12)
 
     ! map: MARCA, delta and divergence on the inner grid (avism.f90:2027-2037)
-    open(unit=11, file=dir//'/map'//iter_tag(it), form='unformatted', access='stream', status='replace')
     write(11) marca
     write(11) u1co - 1.0
     write(11) div
-    close(11)
+
+    ! REQP(IND) = REQ(IND) for the main voids (avism.f90:2138-2144)
+    deallocate(reqp)
+    allocate(reqp(int(nv)))
+    do ind = 1, int(nv)
+      reqp(ind) = 0.0
+      if (uvoid(ind) == -1) reqp(ind) = vf(ind, 12)
+    end do
   end subroutine
 
 end program avism_host

@@ -144,3 +144,67 @@ def test_host_particle_input_matches_binding(tmp_path):
     main_ids = cat["indice"][cat["uvoid"][cat["indice"] - 1] == -1]
     assert np.array_equal(rows[:, 0].astype(int), main_ids)
     assert np.allclose(rows[:, 7], cat["vf"][6][main_ids - 1], rtol=1e-6)
+
+
+def test_host_two_levels_match_binding(tmp_path):
+    """LEVMIN=0, LEVMAX=1 with particle input: one catalogue block and one map
+    block per level; the second level's INDP is the cube's FATHER and its
+    parent radius the parent's REQ (avism.f90:2073-2083, 2123-2161)."""
+    from avism_amd import synth
+    from avism_amd.config import VoidsConfig
+    from avism_amd._lib import Context
+    n, L0 = 16, 40.0
+    arr = synth.zeldovich(16, L0, seed=22, rms_disp=2.0).numpy()
+    cfg = VoidsConfig(first=4, last=4, every=1, ncox=n, ncoy=n, ncoz=n, levmin=0, levmax=1, lado0=L0,
+                      flag_period=1, flag_data=1, kneighbours=16, partired=10 ** 8, rmin_sub=5.0)
+    (tmp_path / "input_data").mkdir()
+    with open(tmp_path / "input_data" / "bin_file_part00004", "wb") as f:
+        _record(f, np.int64(arr[0].size), np.float32(0.0))
+        for a in arr:
+            _record(f, a.astype(np.float32))
+    _run_host(tmp_path, cfg)
+    c = Context(0)
+    c.set_grid(n, L0, 1)
+    c.set_cosmology(cfg.ache, cfg.omega0, 0.0)
+    c.set_void_params(cfg.dens_thre, cfg.dens_thre2, cfg.grad_thre, cfg.div_thre, cfg.rmin, cfg.rmin_sub,
+                      cfg.nvoid_max)
+    c.load_particles(*arr)
+    tok = (tmp_path / "output_files" / "voids00004").read_text().split()
+    assert tok[:3] == ["2", "0", "1"]
+    pos = 7
+    raw = np.fromfile(tmp_path / "output_files" / "map00004", dtype=np.uint8)
+    mpos = 0
+    req_prev = None
+    for lev in (0, 1):
+        nn = n * 2 ** lev
+        if lev:
+            assert c.next_level().n == nn
+        c.particles_to_grid(k=16)
+        nv = c.find_voids(0)
+        c.merge_and_post()
+        cat = c.catalogue(nv)
+        _, _, father = c.cubes(nv)
+        f = c.fields()
+        head = tok[pos:pos + 6]
+        pos += 6
+        nvt = int(cat["stats"].nvoidt)
+        assert int(head[0]) == lev and int(head[1]) == nv and int(head[2]) == nvt
+        rows = np.array([float(t) for t in tok[pos:pos + 15 * nvt]]).reshape(-1, 15)
+        pos += 15 * nvt
+        main_ids = cat["indice"][cat["uvoid"][cat["indice"] - 1] == -1]
+        assert np.array_equal(rows[:, 0].astype(int), main_ids)
+        assert np.allclose(rows[:, 8], cat["vf"][11][main_ids - 1], rtol=1e-6)
+        if lev == 0:
+            assert np.all(rows[:, 12] == 0) and np.all(rows[:, 13] == 0)
+        else:
+            assert nv > 0 and len(np.unique(father)) >= 2
+            assert np.array_equal(rows[:, 12].astype(int), father[main_ids - 1])
+            assert np.allclose(rows[:, 13], req_prev[father[main_ids - 1] - 1], rtol=1e-6)
+        req_prev = np.where(cat["uvoid"] == -1, cat["vf"][11], 0.0)
+        m = nn ** 3 * 4
+        blk = raw[mpos:mpos + 3 * m]
+        mpos += 3 * m
+        assert np.array_equal(blk[:m].view(np.int32), c.marca(inner_only=True))
+        assert np.array_equal(blk[m:2 * m].view(np.float32), f["u1co"] - np.float32(1.0))
+        assert np.array_equal(blk[2 * m:].view(np.float32), f["div"])
+    assert pos == len(tok) and mpos == raw.size
