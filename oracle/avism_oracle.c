@@ -1204,17 +1204,25 @@ int32_t orc_merge_post(const orc_grid* g, const orc_void_params* p, int32_t nvoi
         int32_t id = marca[c];
         if (id > 0 && UV(id) == -1) ncellv[id - 1]++;
     }
-    /* remove a main void everywhere (the WHERE blocks of 1817-1826) */
-#define KILL(id) do { int32_t _id = (id);                                   \
-        for (size_t c_ = 0; c_ < ne3; c_++) if (marca[c_] == _id) marca[c_] = 0; \
-        UV(_id) = 0;                                                        \
-        for (int32_t v_ = 0; v_ < nvoid; v_++) if (uvoid[v_] == _id) uvoid[v_] = 0; \
+    /* remove a main void everywhere (the WHERE blocks of 1817-1826).  A
+     * removal changes only the removed void's own cells and members, so the
+     * removals decided in one pass over INDICE are applied together after it
+     * (KILL_APPLY), with the same result as one at a time. */
+    uint8_t* dead = (uint8_t*)calloc(nvoid > 0 ? nvoid : 1, 1);
+#define KILL(id) do { int32_t _id = (id); dead[_id - 1] = 1; UV(_id) = 0;     \
         VOLNEW[_id - 1] = 0.0f; ncellv[_id - 1] = 0; } while (0)
+#define KILL_APPLY() do {                                                     \
+        for (size_t c_ = 0; c_ < ne3; c_++) {                                 \
+            int32_t m_ = marca[c_]; if (m_ > 0 && dead[m_ - 1]) marca[c_] = 0; } \
+        for (int32_t v_ = 0; v_ < nvoid; v_++) {                              \
+            int32_t u_ = uvoid[v_]; if (u_ > 0 && dead[u_ - 1]) uvoid[v_] = 0; } \
+    } while (0)
     for (int32_t I = 0; I < nvoid; I++) {
         int32_t id = indice[I];
         if (UV(id) >= 0) continue;
         if (ncellv[id - 1] < NCELL_MIN) KILL(id);
     }
+    KILL_APPLY();
     /* VOID_SHAPE (shape.f90:140-225) */
     float* inert = (float*)calloc((size_t)9 * (nvoid > 0 ? nvoid : 1), sizeof(float));
     for (int32_t kz = L1; kz <= L2; kz++)
@@ -1269,6 +1277,7 @@ int32_t orc_merge_post(const orc_grid* g, const orc_void_params* p, int32_t nvoi
         if (UV(id) >= 0) continue;
         if ((double)VOLNEW[id - 1] < rmin3) KILL(id);
     }
+    KILL_APPLY();
     for (int32_t I = 0; I < nvoid; I++) {
         int32_t id = indice[I];
         if (UV(id) >= 0) continue;
@@ -1318,24 +1327,47 @@ int32_t orc_merge_post(const orc_grid* g, const orc_void_params* p, int32_t nvoi
     if (g->periodic) {
         int32_t* lind = (int32_t*)malloc(sizeof(int32_t) * (2 * (size_t)nvoid + 1));
         int32_t nloc = 0;
+        /* main voids with no cell in the inner box are removed; a removal
+         * touches only the removed void's cells (all outside the inner box),
+         * so presence is taken in one pass beforehand */
+        const int32_t ne = g->next;
+        uint8_t* present = (uint8_t*)calloc(nvoid > 0 ? nvoid : 1, 1);
+        for (int32_t kz = 1; kz <= n; kz++)
+            for (int32_t jy = 1; jy <= n; jy++)
+                for (int32_t ix = 1; ix <= n; ix++) {
+                    int32_t m = marca[EXT(g, ix, jy, kz)];
+                    if (m > 0) present[m - 1] = 1;
+                }
         for (int32_t I = 0; I < nvoid; I++) {
             int32_t id = indice[I];
             if (UV(id) >= 0) continue;
             lind[nloc++] = id;
-            int flag = 0;
-            for (int32_t kz = 1; kz <= n && !flag; kz++)
-                for (int32_t jy = 1; jy <= n && !flag; jy++)
-                    for (int32_t ix = 1; ix <= n; ix++)
-                        if (marca[EXT(g, ix, jy, kz)] == id) { flag = 1; break; }
-            if (!flag) {
+            if (!present[id - 1]) {
                 VOLNEW[id - 1] = 0.0f;
                 UV(id) = 0;
-                for (int32_t v = 0; v < nvoid; v++) if (uvoid[v] == id) uvoid[v] = 0;
-                for (size_t c = 0; c < ne3; c++) if (marca[c] == id) marca[c] = 0;
+                dead[id - 1] = 1;
             } else {
                 lind[nloc++] = id;
             }
         }
+        KILL_APPLY();
+        free(present);
+        /* the cells of each label (one counting sort of the ext grid); when
+         * CHANGE_BELONG moves label IND to IND2, IND's member labels are
+         * appended to IND2's, so a count visits exactly the cells carrying
+         * the label at that moment (the counts do not depend on the order) */
+        int32_t* cst = (int32_t*)calloc((size_t)nvoid + 2, sizeof(int32_t));
+        for (size_t c = 0; c < ne3; c++) if (marca[c] > 0) cst[marca[c] + 1]++;
+        for (int32_t i = 1; i <= nvoid + 1; i++) cst[i] += cst[i - 1];
+        int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nvoid + 2));
+        memcpy(cur, cst, sizeof(int32_t) * ((size_t)nvoid + 2));
+        uint32_t* clist = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)cst[nvoid + 1] + 1));
+        for (size_t c = 0; c < ne3; c++) if (marca[c] > 0) clist[cur[marca[c]]++] = (uint32_t)c;
+        free(cur);
+        int32_t* mhead = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nvoid + 1));
+        int32_t* mnext = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nvoid + 1));
+        int32_t* mtail = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nvoid + 1));
+        for (int32_t i = 0; i <= nvoid; i++) { mhead[i] = i; mnext[i] = 0; mtail[i] = i; }
         float half = g->L0 / 2.0f;
         float* lc = (float*)malloc(sizeof(float) * 3 * (size_t)(nloc > 0 ? nloc : 1));
         for (int32_t j = 0; j < nloc; j++) {
@@ -1357,25 +1389,35 @@ int32_t orc_merge_post(const orc_grid* g, const orc_void_params* p, int32_t nvoi
             int32_t id2 = marca[EXT(g, ax, ay, az)];
             if (id2 == 0) continue;
             int64_t cnt = 0, cnt2 = 0;
-            for (int32_t kz = L1; kz <= L2; kz++)
-                for (int32_t jy = L1; jy <= L2; jy++)
-                    for (int32_t ix = L1; ix <= L2; ix++) {
-                        if (marca[EXT(g, ix, jy, kz)] != id) continue;
-                        int32_t a2 = ix + sx * n, b2 = jy + sy * n, c2 = kz + sz * n;
-                        if (a2 >= L1 && a2 <= L2 && b2 >= L1 && b2 <= L2 && c2 >= L1 && c2 <= L2) {
-                            cnt2++;
-                            if (marca[EXT(g, a2, b2, c2)] == id2) cnt++;
-                        }
+            for (int32_t m = mhead[id]; m > 0; m = mnext[m])
+                for (int32_t q = cst[m]; q < cst[m + 1]; q++) {
+                    size_t c = clist[q];
+                    if (marca[c] != id) continue;
+                    int32_t ix = (int32_t)(c % (size_t)ne) + L1;
+                    int32_t jy = (int32_t)((c / (size_t)ne) % (size_t)ne) + L1;
+                    int32_t kz = (int32_t)(c / ((size_t)ne * ne)) + L1;
+                    int32_t a2 = ix + sx * n, b2 = jy + sy * n, c2 = kz + sz * n;
+                    if (a2 >= L1 && a2 <= L2 && b2 >= L1 && b2 <= L2 && c2 >= L1 && c2 <= L2) {
+                        cnt2++;
+                        if (marca[EXT(g, a2, b2, c2)] == id2) cnt++;
                     }
+                }
             float inter = (cnt2 == 0) ? 0.0f : (float)cnt / (float)cnt2;
             if (inter > 0.4f) {  /* CHANGE_BELONG(IND2, IND) */
                 UV(id) = id2;
                 for (int32_t v = 0; v < nvoid; v++) if (uvoid[v] == id) uvoid[v] = id2;
-                for (size_t c = 0; c < ne3; c++) if (marca[c] == id) marca[c] = id2;
+                for (int32_t m = mhead[id]; m > 0; m = mnext[m])
+                    for (int32_t q = cst[m]; q < cst[m + 1]; q++)
+                        if (marca[clist[q]] == id) marca[clist[q]] = id2;
+                if (mhead[id] > 0) {
+                    if (mhead[id2] > 0) { mnext[mtail[id2]] = mhead[id]; mtail[id2] = mtail[id]; }
+                    else { mhead[id2] = mhead[id]; mtail[id2] = mtail[id]; }
+                    mhead[id] = 0;
+                }
                 VOLNEW[id - 1] = 0.0f;
             }
         }
-        free(lind); free(lc);
+        free(lind); free(lc); free(cst); free(clist); free(mhead); free(mnext); free(mtail);
     }
 
     /* final properties over the inner box (1918-1973) */
@@ -1408,6 +1450,8 @@ int32_t orc_merge_post(const orc_grid* g, const orc_void_params* p, int32_t nvoi
         if (UV(id) >= 0) continue;
         if ((double)VOLNEW[id - 1] < rmin3) KILL(id);
     }
+    KILL_APPLY();
+    free(dead);
     int32_t nvoidt = 0;
     for (int32_t i = 0; i < nvoid; i++) if (uvoid[i] == -1) nvoidt++;
     for (int32_t I = 0; I < nvoid; I++) {   /* catalogue radius (2074) */

@@ -63,7 +63,8 @@ def compare(gpu, ref_get, check_vf=True):
             a = gpu["vf"][i][main].astype(np.float64)
             b = rvf[i][main].astype(np.float64)
             if nm in VF_TOL:
-                scale = np.maximum(np.abs(b), 1.0 if nm in ("gxc", "gyc", "gzc") else 1e-30)
+                # centres in Mpc, EPS and IP dimensionless O(1): absolute below 1
+                scale = np.maximum(np.abs(b), 1.0 if nm in ("gxc", "gyc", "gzc", "eps", "ip") else 1e-30)
                 assert np.max(np.abs(a - b) / scale, initial=0.0) <= VF_TOL[nm], nm
             else:
                 assert np.array_equal(gpu["vf"][i][main], rvf[i][main]), nm
