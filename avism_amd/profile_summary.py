@@ -2,9 +2,11 @@
 
     python -m avism_amd.profile_summary stats  <kernel_stats.csv> <out.md> <title> <command>
     python -m avism_amd.profile_summary traffic <fetch_dir> <write_dir> <n> <k> <out.json> <command...>
+    python -m avism_amd.profile_summary counters <pmc_dir> <out.md> <title> <command...>
 
 `stats` condenses a `rocprofv3 --kernel-trace --stats` kernel_stats CSV into a
-markdown table.  `traffic` reads the counter_collection CSVs of two `--pmc`
+markdown table.  `counters` tabulates any `--pmc` run's counters per field kernel (mean per
+dispatch), with VALU instructions per wave.  `traffic` reads the counter_collection CSVs of two `--pmc`
 runs (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch) and scales each
 counter by what it reads for a copy of known size: the n^3 x 4-byte
 device-to-device copies that `load_particles` makes (`__amd_rocclr_copyBuffer`
@@ -96,9 +98,32 @@ def traffic(fetch_dir: str, write_dir: str, n: int, k: int, dst: str, cmd: str) 
     return out
 
 
+def counters(pmc_dir: str, dst: str, title: str, cmd: str) -> None:
+    f = next(Path(pmc_dir).glob("**/*counter_collection.csv"))
+    acc: dict[str, dict[str, list[float]]] = {}
+    for r in csv.DictReader(open(f)):
+        base = short(r["Kernel_Name"]).split("<")[0]
+        if base not in STAGE_OF:
+            continue
+        acc.setdefault(base, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    names = sorted({c for d in acc.values() for c in d})
+    out = [f"# {title}", "", "Command (run from /tmp, TMPDIR=/tmp, repository at $R):", "", "```", cmd, "```", "",
+           "Mean per dispatch.", "", "| kernel | " + " | ".join(names) + " | VALU instr / wave |",
+           "|---|" + "---:|" * (len(names) + 1)]
+    for base, d in acc.items():
+        mean = {c: sum(v) / len(v) for c, v in d.items()}
+        per_wave = (mean["SQ_INSTS_VALU"] / mean["SQ_WAVES"]) if mean.get("SQ_WAVES") and "SQ_INSTS_VALU" in mean \
+            else float("nan")
+        out.append(f"| {base} | " + " | ".join(f"{mean.get(c, float('nan')):.4g}" for c in names) +
+                   f" | {per_wave:.0f} |")
+    Path(dst).write_text("\n".join(out) + "\n")
+
+
 def main(argv: list[str]) -> None:
     if argv[0] == "stats":
         stats(argv[1], argv[2], argv[3], argv[4])
+    elif argv[0] == "counters":
+        counters(argv[1], argv[2], argv[3], " ".join(argv[4:]))
     elif argv[0] == "traffic":
         print(json.dumps(traffic(argv[1], argv[2], int(argv[3]), int(argv[4]), argv[5], " ".join(argv[6:]))))
     else:

@@ -13,6 +13,11 @@ their catalogues must be identical.  `value` is the learned-proposal path's
 cells/s (N^3 inner cells per second of wall time); the plain path's figure and
 the ratio are reported beside it.
 
+With --levels L (default 1) a step runs L void-in-void levels on the same
+particles (N^3, then (2N)^3, ... cells; avism_next_level between them) and
+the line carries a per-level record; `value` is then N^3 base cells per
+second of the whole multi-level step.
+
 Prints one JSON line (see the keys below).
 """
 from __future__ import annotations
@@ -43,6 +48,7 @@ def parse():
     ap.add_argument("--no-mixed-mass", action="store_true", help="skip the mixed-mass field-stage timing")
     ap.add_argument("--cpu-n", type=int, default=64, help="side of the CPU-baseline sample")
     ap.add_argument("--mode", choices=["both", "plain", "learned"], default="both")
+    ap.add_argument("--levels", type=int, default=1, help="void-in-void levels per step (N, 2N, ...)")
     return ap.parse_args()
 
 
@@ -53,12 +59,25 @@ def stage_table(tim):
     return out
 
 
-def run_step(ctx, parts, k, growth_mode, proposer=None):
+def run_step(ctx, parts, k, growth_mode, proposer=None, levels=1, grid=None, per_level=None):
     from avism_amd._lib import AVISM_DEVICE
+    if levels > 1:
+        ctx.set_grid(*grid)  # back to the base level
     ctx.load_particles(parts.x, parts.y, parts.z, parts.vx, parts.vy, parts.vz, parts.m, where=AVISM_DEVICE)
-    ctx.particles_to_grid(k=k)
-    nv = ctx.find_voids(growth_mode)
-    ctx.merge_and_post()
+    for lev in range(levels):
+        t0 = time.perf_counter()
+        if lev:
+            ctx.next_level()
+        ctx.particles_to_grid(k=k)
+        nv = ctx.find_voids(growth_mode)
+        ctx.merge_and_post()
+        if per_level is not None:  # the library calls return with their work complete
+            st = ctx.stats()
+            rec = per_level.setdefault(lev, dict(n=ctx.grid().n, ms=0.0, stages={}))
+            rec["ms"] += (time.perf_counter() - t0) * 1e3
+            rec.update(ncand=int(st.ncand), ncubes=int(nv), nvoids=int(st.nvoidt))
+            for name, ms in ctx.timings():
+                rec["stages"][name] = rec["stages"].get(name, 0.0) + ms
     return nv
 
 
@@ -126,21 +145,30 @@ def main():
 
     results = {}
     catalogues = {}
+    levels = {}
     GL = 2  # GROW_LEARNED
     modes = {"both": [0, GL], "plain": [0], "learned": [GL]}[args.mode]
     for mode in modes:
         if mode == GL and proposer is None:
             continue
+        grid = (n, args.box, 1)
         for _ in range(args.warmup):
-            run_step(ctx, parts, args.k, mode, proposer)
+            run_step(ctx, parts, args.k, mode, proposer, args.levels, grid)
         torch.cuda.synchronize()
         ctx.set_timing(True)
+        per_level = {} if args.levels > 1 else None
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            nv = run_step(ctx, parts, args.k, mode, proposer)
+            nv = run_step(ctx, parts, args.k, mode, proposer, args.levels, grid, per_level)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
         tim = stage_table(ctx.timings())
+        if per_level is not None:
+            levels[mode] = [dict(n=v["n"], ms=round(v["ms"] / args.steps, 3), ncand=v["ncand"], ncubes=v["ncubes"],
+                                 nvoids=v["nvoids"],
+                                 stages_ms={k2: round(x / args.steps, 3) for k2, x in v["stages"].items()})
+                            for _, v in sorted(per_level.items())]
+            tim = stage_table([(k2, x * args.steps) for k2, x in levels[mode][0]["stages_ms"].items()])
         ctx.set_timing(False)
         st = ctx.stats()
         cat = ctx.catalogue(nv)
@@ -151,7 +179,7 @@ def main():
     # so PPART takes k*m as the exact mass sum.  For the record, the field stage
     # once more with masses spread over [0.5, 1.5] x m (the neighbour-sum path).
     mixed = None
-    if not args.no_mixed_mass:
+    if not args.no_mixed_mass and args.levels == 1:
         from avism_amd._lib import AVISM_DEVICE
         gen = torch.Generator(device=dev).manual_seed(7)
         m2 = parts.m * (0.5 + torch.rand(parts.m.shape, device=dev, generator=gen))
@@ -217,7 +245,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Zel'dovich particles generated on the GPU)",
         "config": {"workload": f"zeldovich {n}^3 particles, {n}^3 grid (ext {gi.next}^3), periodic "
-                               f"{args.box:g} Mpc/h, SPH k={k}", "growth": "learned" if main_mode else "plain"},
+                               f"{args.box:g} Mpc/h, SPH k={k}"
+                               + (f", {args.levels} void-in-void levels ({n}^3 .. {n * 2 ** (args.levels - 1)}^3 cells)"
+                                  if args.levels > 1 else ""),
+                   "growth": "learned" if main_mode else "plain"},
         "plain_ms_per_step": results.get(0, {}).get("ms"),
         "learned_ms_per_step": results.get(GL, {}).get("ms"),
         "learned_over_plain": (results[0]["ms"] / results[GL]["ms"]) if (0 in results and GL in results) else None,
@@ -226,6 +257,7 @@ def main():
         "ncand": r["ncand"], "ncubes": r["ncubes"], "nvoids": r["nvoids"],
         "roofline": roofline,
         "field_stage_ms_mixed_masses": mixed,
+        "levels": {("learned" if m else "plain"): v for m, v in levels.items()} or None,
         "input_gen_s": round(t_gen, 2),
     }
     if not args.no_cpu_baseline:
