@@ -50,68 +50,149 @@ struct MarkParams {
   float two_dx;
 };
 
-// voidfinding.f90:256-432 for one cell; ref order +X,-X,+Y,-Y,+Z,-Z.
-// Grid (x chunks, y, z) over the ext grid, so the coordinates need no
-// division; the periodic wrap of the (at most 3-cell) neighbour offsets is
-// taken once per axis.
-__global__ void k_mark(ExtGeom G, MarkParams P, const float* __restrict__ u1co,
-                       const float* __restrict__ div, const int32_t* __restrict__ marcap,
-                       uint8_t* flags) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= G.ne) return;
-  const int I = x + G.low1, J = (int)blockIdx.y + G.low1, K = (int)blockIdx.z + G.low1;
-  const int n = G.n;
-  // inner offsets of the cell and of its neighbours at -3..+3 along each axis
-  size_t ox[7], oy[7], oz[7];
-#pragma unroll
-  for (int t = 0; t < 7; ++t) {
-    ox[t] = (size_t)(G.wrap(I + t - 3) - 1);
-    oy[t] = (size_t)n * (size_t)(G.wrap(J + t - 3) - 1);
-    oz[t] = (size_t)n * (size_t)n * (size_t)(G.wrap(K + t - 3) - 1);
+// voidfinding.f90:256-432 (and MARK_ALL, 30-52) for every ext cell; ref
+// order +X,-X,+Y,-Y,+Z,-Z.  A block takes a 64 x 8 (x, y) column of cells and
+// walks kMarkZ planes of it in z, kMarkP planes per step: the planes' U1CO
+// (halo 3) and DIVERCO (halo 1) are staged in LDS for the x and y stencils,
+// and each cell's z line (U1CO at z-3..z+3, DIVERCO at z-1..z+1) slides
+// through registers.  Ext cells read the inner fields through the periodic
+// wrap; halo cells outside LOW1..LOW2 are clamped into it (the stencil never
+// uses them).
+constexpr int kMarkX = 64, kMarkY = 8, kMarkZ = 32, kMarkP = 4;
+constexpr int kMarkT = kMarkX * kMarkY;                          // threads per block
+constexpr int kMarkU = (kMarkY + 6) * (kMarkX + 6);              // U1CO tile entries per plane
+constexpr int kMarkD = (kMarkY + 2) * (kMarkX + 2);              // DIVERCO tile entries per plane
+constexpr int kMarkUP = (kMarkU + kMarkT - 1) / kMarkT;          // per thread
+constexpr int kMarkDP = (kMarkD + kMarkT - 1) / kMarkT;
+static_assert(kMarkZ % kMarkP == 0, "whole steps per block");
+
+struct Line {  // one axis' stencil values at offsets -3..3 (div at -1..1)
+  float u[7];
+  float d[3];
+};
+
+__device__ __forceinline__ bool mark_dir(const MarkParams& P, const Line& A, int pos, bool plus, int L1,
+                                         int L2) {
+  bool flag = plus ? (pos >= L2 - kNsec) : (pos <= L1 + kNsec);
+  if (flag) return true;
+  const int s = plus ? 1 : -1;
+  const int II = pos + s;
+  const float un = A.u[3 + s], dn = A.d[1 + s];
+  const int edge = plus ? L2 : L1;
+  if (II == edge) {
+    if (dn < P.divthr) flag = true;
+    if (un >= P.dthr2) flag = true;
+  } else {
+    const float dd = (A.u[3 + 2 * s] - A.u[3]) / P.two_dx;
+    if (dd >= P.grad) flag = true;
+    const bool nextok = plus ? (II < L2 - 1) : (II > L1 + 1);
+    if (flag && nextok) {
+      const float dd2 = (A.u[3 + 3 * s] - un) / P.two_dx;
+      if (dd2 < P.grad) flag = false;
+    }
+    if (dn < P.divthr) flag = true;
+    if (un >= P.dthr2) flag = true;
   }
-  const size_t ci = ox[3] + oy[3] + oz[3];
-  int mp = marcap ? marcap[ci] : -1;
-  uint8_t f = 0;
-  // MARK_ALL (voidfinding.f90:30-52); SMASK = 1
-  if (mp != 0) {
-    f |= FL_SUB;
-    if (u1co[ci] < P.dthr1 && div[ci] > 0.0f) f |= FL_V;
-  }
-  const int L1 = G.low1, L2 = G.low2;
+  return flag;
+}
+
+__global__ void __launch_bounds__(kMarkT) k_mark(ExtGeom G, MarkParams P, const float* __restrict__ u1co,
+                                                 const float* __restrict__ div,
+                                                 const int32_t* __restrict__ marcap, uint8_t* flags) {
+  __shared__ float su[kMarkP][kMarkY + 6][kMarkX + 6];
+  __shared__ float sd[kMarkP][kMarkY + 2][kMarkX + 2];
+  const int L1 = G.low1, L2 = G.low2, n = G.n, ne = G.ne;
+  const int tx = threadIdx.x % kMarkX, ty = threadIdx.x / kMarkX;
+  const int x0 = blockIdx.x * kMarkX, y0 = blockIdx.y * kMarkY, z0 = blockIdx.z * kMarkZ;
+  const int I = x0 + tx + L1, J = y0 + ty + L1;
+  const bool live = x0 + tx < ne && y0 + ty < ne;
+  auto clampc = [&](int a) { return a < L1 ? L1 : (a > L2 ? L2 : a); };
+  auto inner_xy = [&](int xe, int ye) {
+    return (size_t)(G.wrap(clampc(xe)) - 1) + (size_t)n * (size_t)(G.wrap(clampc(ye)) - 1);
+  };
+  auto plane = [&](int k) { return (size_t)n * (size_t)n * (size_t)(G.wrap(clampc(k)) - 1); };
+  const size_t cxy = inner_xy(I, J);
+  // the (x, y) inner offsets of the tile entries this thread stages (fixed over z)
+  size_t uo[kMarkUP], dof[kMarkDP];
 #pragma unroll
-  for (int dir = 0; dir < 6; ++dir) {
-    const int axis = dir >> 1;
-    const bool plus = (dir & 1) == 0;
-    const int pos = axis == 0 ? I : (axis == 1 ? J : K);
-    const uint8_t bit = axis == 0 ? (plus ? FL_X2 : FL_X1) : axis == 1 ? (plus ? FL_Y2 : FL_Y1) : (plus ? FL_Z2 : FL_Z1);
-    // the cell at offset o (-3..3) along this axis
-    auto at = [&](int o) -> size_t {
-      return axis == 0 ? ox[3 + o] + oy[3] + oz[3] : axis == 1 ? ox[3] + oy[3 + o] + oz[3] : ox[3] + oy[3] + oz[3 + o];
-    };
-    bool flag = plus ? (pos >= L2 - kNsec) : (pos <= L1 + kNsec);
-    if (!flag) {
-      const int s = plus ? 1 : -1;
-      const int II = pos + s;
-      const size_t cn = at(s);
-      const int edge = plus ? L2 : L1;
-      if (II == edge) {
-        if (div[cn] < P.divthr) flag = true;
-        if (u1co[cn] >= P.dthr2) flag = true;
-      } else {
-        const float dd = (u1co[at(2 * s)] - u1co[ci]) / P.two_dx;
-        if (dd >= P.grad) flag = true;
-        const bool nextok = plus ? (II < L2 - 1) : (II > L1 + 1);
-        if (flag && nextok) {
-          const float dd2 = (u1co[at(3 * s)] - u1co[cn]) / P.two_dx;
-          if (dd2 < P.grad) flag = false;
-        }
-        if (div[cn] < P.divthr) flag = true;
-        if (u1co[cn] >= P.dthr2) flag = true;
+  for (int m = 0; m < kMarkUP; ++m) {
+    const int q = threadIdx.x + m * kMarkT;
+    uo[m] = inner_xy(x0 + q % (kMarkX + 6) - 3 + L1, y0 + q / (kMarkX + 6) - 3 + L1);
+  }
+#pragma unroll
+  for (int m = 0; m < kMarkDP; ++m) {
+    const int q = threadIdx.x + m * kMarkT;
+    dof[m] = inner_xy(x0 + q % (kMarkX + 2) - 1 + L1, y0 + q / (kMarkX + 2) - 1 + L1);
+  }
+  const int z1 = min(z0 + kMarkZ, ne);
+  // the z line of the step's planes: U1CO at z-3..z+kMarkP+2, DIVERCO at z-1..z+kMarkP
+  float zu[kMarkP + 6], zd[kMarkP + 2];
+#pragma unroll
+  for (int o = 0; o < kMarkP + 6; ++o) zu[o] = u1co[cxy + plane(z0 + L1 + o - 3)];
+#pragma unroll
+  for (int o = 0; o < kMarkP + 2; ++o) zd[o] = div[cxy + plane(z0 + L1 + o - 1)];
+  for (int z = z0; z < z1; z += kMarkP) {
+#pragma unroll
+    for (int pl = 0; pl < kMarkP; ++pl) {
+      const size_t pk = plane(z + pl + L1);
+#pragma unroll
+      for (int m = 0; m < kMarkUP; ++m) {
+        const int q = threadIdx.x + m * kMarkT;
+        if (q < kMarkU) (&su[pl][0][0])[q] = u1co[uo[m] + pk];
+      }
+#pragma unroll
+      for (int m = 0; m < kMarkDP; ++m) {
+        const int q = threadIdx.x + m * kMarkT;
+        if (q < kMarkD) (&sd[pl][0][0])[q] = div[dof[m] + pk];
       }
     }
-    if (flag) f |= bit;
+    __syncthreads();
+#pragma unroll
+    for (int pl = 0; pl < kMarkP; ++pl) {
+      const int K = z + pl + L1;
+      if (live && z + pl < z1) {
+        Line X, Y, Z;
+#pragma unroll
+        for (int o = 0; o < 7; ++o) {
+          X.u[o] = su[pl][ty + 3][tx + o];
+          Y.u[o] = su[pl][ty + o][tx + 3];
+          Z.u[o] = zu[pl + o];
+        }
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+          X.d[o] = sd[pl][ty + 1][tx + o];
+          Y.d[o] = sd[pl][ty + o][tx + 1];
+          Z.d[o] = zd[pl + o];
+        }
+        const int mp = marcap ? marcap[cxy + plane(K)] : -1;
+        uint8_t f = 0;
+        // MARK_ALL (voidfinding.f90:30-52); SMASK = 1
+        if (mp != 0) {
+          f |= FL_SUB;
+          if (Z.u[3] < P.dthr1 && Z.d[1] > 0.0f) f |= FL_V;
+        }
+        if (mark_dir(P, X, I, true, L1, L2)) f |= FL_X2;
+        if (mark_dir(P, X, I, false, L1, L2)) f |= FL_X1;
+        if (mark_dir(P, Y, J, true, L1, L2)) f |= FL_Y2;
+        if (mark_dir(P, Y, J, false, L1, L2)) f |= FL_Y1;
+        if (mark_dir(P, Z, K, true, L1, L2)) f |= FL_Z2;
+        if (mark_dir(P, Z, K, false, L1, L2)) f |= FL_Z1;
+        flags[(size_t)(x0 + tx) + (size_t)ne * ((size_t)(y0 + ty) + (size_t)ne * (z + pl))] = f;
+      }
+    }
+    // slide the z line kMarkP planes on
+    if (z + kMarkP < z1) {
+#pragma unroll
+      for (int o = 0; o < 6; ++o) zu[o] = zu[o + kMarkP];
+#pragma unroll
+      for (int o = 6; o < kMarkP + 6; ++o) zu[o] = u1co[cxy + plane(z + kMarkP + L1 + o - 3)];
+#pragma unroll
+      for (int o = 0; o < 2; ++o) zd[o] = zd[o + kMarkP];
+#pragma unroll
+      for (int o = 2; o < kMarkP + 2; ++o) zd[o] = div[cxy + plane(z + kMarkP + L1 + o - 1)];
+    }
+    __syncthreads();
   }
-  flags[(size_t)x + (size_t)G.ne * ((size_t)blockIdx.y + (size_t)G.ne * blockIdx.z)] = f;
 }
 
 constexpr int kChunk = 16;  // cells per thread in the ordered compaction
@@ -516,7 +597,9 @@ int void_find(avism_ctx* c, int32_t growth_mode) {
   if (!c->flags.ensure((size_t)ne3)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (flags)");
   {
     StageTimer t(c, "mark_flags");
-    k_mark<<<dim3((unsigned)((G.ne + 255) / 256), (unsigned)G.ne, (unsigned)G.ne), 256, 0, c->stream>>>(
+    k_mark<<<dim3((unsigned)((G.ne + kMarkX - 1) / kMarkX), (unsigned)((G.ne + kMarkY - 1) / kMarkY),
+                  (unsigned)((G.ne + kMarkZ - 1) / kMarkZ)),
+             kMarkT, 0, c->stream>>>(
         G, P, c->u1co.as<float>(), c->div.as<float>(), marcap, c->flags.as<uint8_t>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
