@@ -613,51 +613,66 @@ __device__ void cc_union(int32_t* lab, int a, int b) {
 
 // block-local components first: an 8^3 block labels its empty cells in LDS
 // (label = the component's smallest cell, which is also its smallest global
-// index), so the global union-find only joins components across block faces
+// index), so the global union-find only joins components across block faces.
+// A block whose cells are all empty (most of the grid) is one component with
+// label 0; otherwise labels propagate to neighbours until none changes.
+// isrep marks each component's smallest cell (its representative).
 constexpr int CCB = 8;
 
-__global__ void __launch_bounds__(256) k_cc_local(ExtG G, const int32_t* __restrict__ marca, int32_t* lab) {
+__global__ void __launch_bounds__(256) k_cc_local(ExtG G, const int32_t* __restrict__ marca, int32_t* lab,
+                                                 uint8_t* isrep) {
   __shared__ int16_t L[CCB * CCB * CCB];
   __shared__ int changed;
   const int nbx = (G.ne + CCB - 1) / CCB;
   const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nbx, bz = blockIdx.x / (nbx * nbx);
   const int tid = threadIdx.x;
   int64_t eg[2];
+  bool full = true;  // every in-grid cell of this thread is empty
   for (int r = 0; r < 2; ++r) {
     const int c = tid + 256 * r;
     const int gx = bx * CCB + c % CCB, gy = by * CCB + (c / CCB) % CCB, gz = bz * CCB + c / (CCB * CCB);
     const bool in = gx < G.ne && gy < G.ne && gz < G.ne;
     eg[r] = in ? (int64_t)gx + (int64_t)G.ne * ((int64_t)gy + (int64_t)G.ne * gz) : -1;
-    L[c] = (in && marca[eg[r]] == 0) ? (int16_t)c : (int16_t)-1;
+    const bool empty = in && marca[eg[r]] == 0;
+    L[c] = empty ? (int16_t)c : (int16_t)-1;
+    full = full && (empty || !in);
   }
-  for (;;) {
-    if (tid == 0) changed = 0;
-    __syncthreads();
-    bool ch = false;
+  if (__syncthreads_and(full)) {
     for (int r = 0; r < 2; ++r) {
       const int c = tid + 256 * r;
-      int m = L[c];
-      if (m < 0) continue;
-      const int x = c % CCB, y = (c / CCB) % CCB, z = c / (CCB * CCB);
-      int t;
-      if (x > 0 && (t = L[c - 1]) >= 0) m = min(m, t);
-      if (x < CCB - 1 && (t = L[c + 1]) >= 0) m = min(m, t);
-      if (y > 0 && (t = L[c - CCB]) >= 0) m = min(m, t);
-      if (y < CCB - 1 && (t = L[c + CCB]) >= 0) m = min(m, t);
-      if (z > 0 && (t = L[c - CCB * CCB]) >= 0) m = min(m, t);
-      if (z < CCB - 1 && (t = L[c + CCB * CCB]) >= 0) m = min(m, t);
-      m = min(m, (int)L[m]);  // jump along the label chain
-      if (m < L[c]) {
-        L[c] = (int16_t)m;
-        ch = true;
-      }
+      if (eg[r] >= 0) L[c] = 0;
     }
-    if (ch) changed = 1;
-    __syncthreads();
-    const int any = changed;
-    __syncthreads();
-    if (!any) break;
+  } else {
+    for (;;) {
+      if (tid == 0) changed = 0;
+      __syncthreads();
+      bool ch = false;
+      for (int r = 0; r < 2; ++r) {
+        const int c = tid + 256 * r;
+        int m = L[c];
+        if (m < 0) continue;
+        const int x = c % CCB, y = (c / CCB) % CCB, z = c / (CCB * CCB);
+        int t;
+        if (x > 0 && (t = L[c - 1]) >= 0) m = min(m, t);
+        if (x < CCB - 1 && (t = L[c + 1]) >= 0) m = min(m, t);
+        if (y > 0 && (t = L[c - CCB]) >= 0) m = min(m, t);
+        if (y < CCB - 1 && (t = L[c + CCB]) >= 0) m = min(m, t);
+        if (z > 0 && (t = L[c - CCB * CCB]) >= 0) m = min(m, t);
+        if (z < CCB - 1 && (t = L[c + CCB * CCB]) >= 0) m = min(m, t);
+        m = min(m, (int)L[m]);  // jump along the label chain
+        if (m < L[c]) {
+          L[c] = (int16_t)m;
+          ch = true;
+        }
+      }
+      if (ch) changed = 1;
+      __syncthreads();
+      const int any = changed;
+      __syncthreads();
+      if (!any) break;
+    }
   }
+  __syncthreads();
   for (int r = 0; r < 2; ++r) {
     const int c = tid + 256 * r;
     if (eg[r] < 0) continue;
@@ -668,6 +683,7 @@ __global__ void __launch_bounds__(256) k_cc_local(ExtG G, const int32_t* __restr
       const int gx = bx * CCB + m % CCB, gy = by * CCB + (m / CCB) % CCB, gz = bz * CCB + m / (CCB * CCB);
       lab[eg[r]] = (int32_t)((int64_t)gx + (int64_t)G.ne * ((int64_t)gy + (int64_t)G.ne * gz));
     }
+    isrep[eg[r]] = m == c ? 1 : 0;
   }
 }
 
@@ -713,10 +729,11 @@ __global__ void __launch_bounds__(64) k_cc_faces(ExtG G, int nbc, int32_t* lab) 
   if (la >= 0 && lb >= 0 && !dup) cc_union(lab, la, lb);
 }
 
-__global__ void k_cc_flatten(int64_t ne3, int32_t* lab) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne3 || lab[e] < 0) return;
-  lab[e] = cc_find(lab, (int)e);
+// every representative points at its root; any empty cell's root is then
+// lab[lab[cell]] (a cell's entry is its representative, a root its own)
+__global__ void k_cc_flatten(int64_t ne3, const uint8_t* __restrict__ isrep, int32_t* lab) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne3 && isrep[e]) lab[e] = cc_find(lab, (int)e);
 }
 
 // holes (empty cells not connected to the corner, voidfinding.f90:1040-1135)
@@ -726,12 +743,13 @@ __global__ void k_fill_holes(ExtG G, const int32_t* __restrict__ lab, int32_t* m
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= G.ne) return;
   const int64_t e = (int64_t)i + (int64_t)G.ne * ((int64_t)blockIdx.y + (int64_t)G.ne * blockIdx.z);
-  int outside = lab[0];
-  int l = lab[e];
-  if (l < 0 || l == outside) return;
+  const int l0 = lab[0];
+  const int outside = l0 >= 0 ? lab[l0] : -1;
+  const int l = lab[e];
+  if (l < 0 || lab[l] == outside) return;
   atomicAdd(nholes, 1);
   int64_t f = e;
-  while (i < G.ne && lab[f] >= 0 && lab[f] != outside) {
+  for (int lf; i < G.ne && (lf = lab[f]) >= 0 && lab[lf] != outside;) {
     ++i;
     ++f;
   }
@@ -1284,10 +1302,11 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     StageTimer t(c, "hole_fill");
     int32_t* lab = P;  // reuse
     const int nbc = (G.ne + CCB - 1) / CCB;
-    k_cc_local<<<(unsigned)((int64_t)nbc * nbc * nbc), 256, 0, c->stream>>>(G, marca, lab);
+    uint8_t* isrep = c->paintK.as<uint8_t>();  // free after the merge
+    k_cc_local<<<(unsigned)((int64_t)nbc * nbc * nbc), 256, 0, c->stream>>>(G, marca, lab, isrep);
     const dim3 g3((unsigned)((G.ne + 255) / 256), (unsigned)G.ne, (unsigned)G.ne);
     k_cc_faces<<<dim3((unsigned)((int64_t)nbc * nbc * nbc), 3), 64, 0, c->stream>>>(G, nbc, lab);
-    k_cc_flatten<<<nbk(ne3), 256, 0, c->stream>>>(ne3, lab);
+    k_cc_flatten<<<nbk(ne3), 256, 0, c->stream>>>(ne3, isrep, lab);
     AV_CHECK_HIP(c, hipMemsetAsync(small + 2, 0, 4, c->stream));
     k_fill_holes<<<g3, 256, 0, c->stream>>>(G, lab, marca, small + 2);
     AV_CHECK_HIP(c, hipGetLastError());
