@@ -411,23 +411,39 @@ __global__ void k_copy_nonleaders(int nv, const int32_t* __restrict__ indice, co
 }
 
 // ---------------------------------------------------------- post-processing
-__global__ void k_count_cells(int64_t ne3, ExtG G, const int32_t* __restrict__ marca, int inner_only,
-                              int32_t* ncell) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne3) return;
-  int id = marca[e];
-  if (id <= 0) return;
-  if (inner_only) {
-    int I = (int)(e % G.ne) + G.low1, J = (int)((e / G.ne) % G.ne) + G.low1,
-        K = (int)(e / ((int64_t)G.ne * G.ne)) + G.low1;
-    if (I < 1 || I > G.n || J < 1 || J > G.n || K < 1 || K > G.n) return;
-  }
-  atomicAdd(&ncell[id - 1], 1);
+// the id shared by a wave's nonzero ids (0 if it has none, -1 if they differ)
+__device__ __forceinline__ int wave_uniform_id(int id) {
+  int lead = id;
+  for (int off = 32; off > 0; off >>= 1) lead = max(lead, __shfl_xor(lead, off));
+  return __all(id == 0 || id == lead) ? lead : -1;
 }
 
-// NCELL_MIN / RMIN removals (avism.f90:1812-1827, 1845-1859): decided per main void
+__device__ __forceinline__ double wave_sum(double s) {
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  return s;
+}
+
+// cells per void over the ext grid (grid: x chunks, y, z); a wave whose void
+// cells share one id adds once
+__global__ void k_count_cells(ExtG G, const int32_t* __restrict__ marca, int32_t* ncell) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = (int64_t)x + (int64_t)G.ne * ((int64_t)blockIdx.y + (int64_t)G.ne * blockIdx.z);
+  const int id = x < G.ne ? marca[e] : 0;
+  const int u = wave_uniform_id(id);
+  if (u == 0) return;
+  if (u > 0) {
+    const int cnt = __popcll(__ballot(id > 0));
+    if (threadIdx.x % 64 == 0) atomicAdd(&ncell[u - 1], cnt);
+  } else if (id > 0) {
+    atomicAdd(&ncell[id - 1], 1);
+  }
+}
+
+// NCELL_MIN / RMIN removals (avism.f90:1812-1827, 1845-1859): decided per main void;
+// nremoved counts them (zeroed by the caller)
 __global__ void k_mark_removed(int nv, const int32_t* __restrict__ uvoid, const int32_t* __restrict__ ncell,
-                               int mode, float dx, double rmin3, int32_t* removed, float* volnew) {
+                               int mode, float dx, double rmin3, int32_t* removed, float* volnew,
+                               int32_t* nremoved) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
   removed[v] = 0;
@@ -439,6 +455,7 @@ __global__ void k_mark_removed(int nv, const int32_t* __restrict__ uvoid, const 
     volnew[v] = vol;
     if ((double)vol < rmin3) removed[v] = 1;
   }
+  if (removed[v]) atomicAdd(nremoved, 1);
 }
 
 __global__ void k_apply_removed_cubes(int nv, const int32_t* __restrict__ removed, int32_t* uvoid,
@@ -455,7 +472,9 @@ __global__ void k_apply_removed_cubes(int nv, const int32_t* __restrict__ remove
   }
 }
 
-__global__ void k_apply_removed_cells(int64_t ne3, const int32_t* __restrict__ removed, int32_t* marca) {
+__global__ void k_apply_removed_cells(int64_t ne3, const int32_t* __restrict__ removed,
+                                      const int32_t* __restrict__ nremoved, int32_t* marca) {
+  if (*nremoved == 0) return;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne3) return;
   int id = marca[e];
@@ -463,15 +482,17 @@ __global__ void k_apply_removed_cells(int64_t ne3, const int32_t* __restrict__ r
 }
 
 // VOID_SHAPE (shape.f90:155-172): per-cell FP32 terms, FP64 sums
-__global__ void k_inertia(int64_t ne3, ExtG G, const float* __restrict__ rad, int rlo,
+// (grid: x chunks, y, z over the ext grid)
+__global__ void k_inertia(ExtG G, const float* __restrict__ rad, int rlo,
                           const int32_t* __restrict__ marca, const float* __restrict__ gxc,
                           const float* __restrict__ gyc, const float* __restrict__ gzc, double* inert) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int id = e < ne3 ? marca[e] : 0;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = (int64_t)x + (int64_t)G.ne * ((int64_t)blockIdx.y + (int64_t)G.ne * blockIdx.z);
+  int id = x < G.ne ? marca[e] : 0;
+  if (__all(id == 0)) return;
   double t[6] = {0, 0, 0, 0, 0, 0};
   if (id > 0) {
-    int I = (int)(e % G.ne) + G.low1, J = (int)((e / G.ne) % G.ne) + G.low1,
-        K = (int)(e / ((int64_t)G.ne * G.ne)) + G.low1;
+    int I = x + G.low1, J = (int)blockIdx.y + G.low1, K = (int)blockIdx.z + G.low1;
     float rk[3] = {rad[I - rlo] - gxc[id - 1], rad[J - rlo] - gyc[id - 1], rad[K - rlo] - gzc[id - 1]};
     float rr = sqrtf((rk[0] * rk[0] + rk[1] * rk[1]) + rk[2] * rk[2]);
     float d = (1.0f * rr) * rr;
@@ -768,9 +789,11 @@ __global__ void k_inner_presence(int64_t ne3, ExtG G, const int32_t* __restrict_
 }
 
 __global__ void k_absent_removed(int nv, const int32_t* __restrict__ uvoid, const int32_t* __restrict__ has,
-                                 int32_t* removed) {
+                                 int32_t* removed, int32_t* nremoved) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < nv) removed[v] = (uvoid[v] == -1 && !has[v]) ? 1 : 0;
+  if (v >= nv) return;
+  removed[v] = (uvoid[v] == -1 && !has[v]) ? 1 : 0;
+  if (removed[v]) atomicAdd(nremoved, 1);
 }
 
 __global__ void k_label_ranges(int64_t ne3, const uint32_t* __restrict__ key, int32_t* lstart, int32_t* lend) {
@@ -1065,23 +1088,40 @@ __global__ void k_init_lmap(int nv, int32_t* lmap, int32_t* mnext, int32_t* mtai
 }
 
 // final properties over the inner box (avism.f90:1918-1945)
-__global__ void k_final_props(int64_t ne3, ExtG G, const int32_t* __restrict__ marca,
-                              const int32_t* __restrict__ uvoid, const float* __restrict__ u1co,
-                              float mtot_scale_md, float dx, float rete3, int32_t* ncell, double* umean,
-                              double* mtot, unsigned long long* ncells_total) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne3) return;
-  int id = marca[e];
-  if (id <= 0 || uvoid[id - 1] != -1) return;
-  int I = (int)(e % G.ne) + G.low1, J = (int)((e / G.ne) % G.ne) + G.low1,
-      K = (int)(e / ((int64_t)G.ne * G.ne)) + G.low1;
-  if (I < 1 || I > G.n || J < 1 || J > G.n || K < 1 || K > G.n) return;
-  float u = u1co[(size_t)(I - 1) + (size_t)G.n * ((size_t)(J - 1) + (size_t)G.n * (K - 1))];
-  atomicAdd(&ncell[id - 1], 1);
-  atomicAdd(&umean[id - 1], (double)u);
-  float m = ((((u * mtot_scale_md) * dx) * dx) * dx) * rete3;
-  atomicAdd(&mtot[id - 1], (double)m);
-  atomicAdd(ncells_total, 1ull);
+// (grid: x chunks, y, z over the inner grid); a wave whose void cells share
+// one id adds once, and the total cell count takes one atomic per wave
+__global__ void k_final_props(ExtG G, const int32_t* __restrict__ marca, const int32_t* __restrict__ uvoid,
+                              const float* __restrict__ u1co, float mtot_scale_md, float dx, float rete3,
+                              int32_t* ncell, double* umean, double* mtot, unsigned long long* ncells_total) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int I = x + 1, J = (int)blockIdx.y + 1, K = (int)blockIdx.z + 1;
+  int id = 0;
+  float u = 0.0f, m = 0.0f;
+  if (x < G.n) {
+    id = marca[G.ext(I, J, K)];
+    if (id > 0 && uvoid[id - 1] != -1) id = 0;
+    if (id > 0) {
+      u = u1co[(size_t)(I - 1) + (size_t)G.n * ((size_t)(J - 1) + (size_t)G.n * (K - 1))];
+      m = ((((u * mtot_scale_md) * dx) * dx) * dx) * rete3;
+    }
+  }
+  const int w = wave_uniform_id(id);
+  if (w == 0) return;
+  const unsigned long long inv = __ballot(id > 0);
+  const bool lane0 = threadIdx.x % 64 == 0;
+  if (lane0) atomicAdd(ncells_total, (unsigned long long)__popcll(inv));
+  if (w > 0) {
+    const double su = wave_sum((double)u), sm = wave_sum((double)m);
+    if (lane0) {
+      atomicAdd(&ncell[w - 1], __popcll(inv));
+      atomicAdd(&umean[w - 1], su);
+      atomicAdd(&mtot[w - 1], sm);
+    }
+  } else if (id > 0) {
+    atomicAdd(&ncell[id - 1], 1);
+    atomicAdd(&umean[id - 1], (double)u);
+    atomicAdd(&mtot[id - 1], (double)m);
+  }
 }
 
 __global__ void k_finish_catalogue(int nv, const int32_t* __restrict__ ncell, const double* __restrict__ umean,
@@ -1268,10 +1308,12 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
   {
     StageTimer t(c, "post_filters");
     AV_CHECK_HIP(c, hipMemsetAsync(ncell, 0, (size_t)(nv + 1) * 4, c->stream));
-    k_count_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, marca, 0, ncell);
-    k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 0, g.dx, 0.0, removed, VOLNEW);
+    k_count_cells<<<dim3((unsigned)((G.ne + 255) / 256), (unsigned)G.ne, (unsigned)G.ne), 256, 0, c->stream>>>(
+        G, marca, ncell);
+    AV_CHECK_HIP(c, hipMemsetAsync(small + 7, 0, 4, c->stream));
+    k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 0, g.dx, 0.0, removed, VOLNEW, small + 7);
     k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
-    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, marca);
+    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
   }
   // ---- VOID_SHAPE (avism.f90:1831)
   float* d_rad = (float*)((char*)c->scalars.p + 4096);
@@ -1280,7 +1322,8 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     AV_CHECK_HIP(c, hipMemcpyAsync(d_rad, g.rad.data(), g.rad.size() * 4, hipMemcpyHostToDevice, c->stream));
     double* inert = (double*)c->inert.p;
     AV_CHECK_HIP(c, hipMemsetAsync(inert, 0, (size_t)(nv + 1) * 6 * 8, c->stream));
-    k_inertia<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, d_rad, g.rlo, marca, GX, GY, GZ, inert);
+    k_inertia<<<dim3((unsigned)((G.ne + 255) / 256), (unsigned)G.ne, (unsigned)G.ne), 256, 0, c->stream>>>(
+        G, d_rad, g.rlo, marca, GX, GY, GZ, inert);
     k_shape<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, inert, g.dx, EPS, IPR);
   }
   // ---- VOLNEW from NCELLV + RMIN filter (1834-1859), centres XC (1863-1870)
@@ -1288,9 +1331,10 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
   {
     StageTimer t(c, "post_filters");
     AV_CHECK_HIP(c, hipMemsetAsync(VOLNEW, 0, (size_t)nv * 4, c->stream));
-    k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 1, g.dx, rmin3, removed, VOLNEW);
+    AV_CHECK_HIP(c, hipMemsetAsync(small + 7, 0, 4, c->stream));
+    k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 1, g.dx, rmin3, removed, VOLNEW, small + 7);
     k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
-    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, marca);
+    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
     k_xc<<<nbk(nv), 256, 0, c->stream>>>(nv, C, uvoid, g.dx, g.rx1, XC, YC, ZC);
   }
   if (stop_after == 2) {
@@ -1321,9 +1365,10 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     int32_t* has = gstart;
     AV_CHECK_HIP(c, hipMemsetAsync(has, 0, (size_t)(nv + 1) * 4, c->stream));
     k_inner_presence<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, marca, has);
-    k_absent_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, has, removed);
+    AV_CHECK_HIP(c, hipMemsetAsync(small + 7, 0, 4, c->stream));
+    k_absent_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, has, removed, small + 7);
     k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
-    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, marca);
+    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
     int32_t* selfm = removed;
     k_init_lmap<<<nbk(nv), 256, 0, c->stream>>>(nv, lmap, mnext, mtail, selfm);
     AV_CHECK_HIP(c, hipMemsetAsync(small + 3, 0, 8, c->stream));  // [3] merges, [4] cell pairs
@@ -1407,13 +1452,14 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     AV_CHECK_HIP(c, hipMemsetAsync(um, 0, (size_t)(nv + 1) * 16, c->stream));
     AV_CHECK_HIP(c, hipMemsetAsync(tot, 0, 8, c->stream));
     float rete3 = (c->rete * c->rete) * c->rete;
-    k_final_props<<<nbk(ne3), 256, 0, c->stream>>>(ne3, G, marca, uvoid, c->u1co.as<float>(), c->meandens,
-                                                   g.dx, rete3, ncell, um, mt, tot);
+    k_final_props<<<dim3((unsigned)((g.n + 255) / 256), (unsigned)g.n, (unsigned)g.n), 256, 0, c->stream>>>(
+        G, marca, uvoid, c->u1co.as<float>(), c->meandens, g.dx, rete3, ncell, um, mt, tot);
     k_finish_catalogue<<<nbk(nv), 256, 0, c->stream>>>(nv, ncell, um, mt, UMEAN, MTOT);
     AV_CHECK_HIP(c, hipMemsetAsync(VOLNEW, 0, (size_t)nv * 4, c->stream));
-    k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 1, g.dx, rmin3, removed, VOLNEW);
+    AV_CHECK_HIP(c, hipMemsetAsync(small + 7, 0, 4, c->stream));
+    k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 1, g.dx, rmin3, removed, VOLNEW, small + 7);
     k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
-    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, marca);
+    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
     AV_CHECK_HIP(c, hipMemsetAsync(small + 4, 0, 4, c->stream));
     k_req<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, VOLNEW, REQ, small + 4);
     AV_CHECK_HIP(c, hipGetLastError());
