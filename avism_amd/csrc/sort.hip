@@ -51,9 +51,7 @@ __global__ void k_init_indx(int64_t n, int32_t* indx, int32_t* segid) {
 }
 
 // per segment: insertion sort (small) or median-of-three set-up (num.f90:33-48)
-__global__ void k_seg_setup(int nseg, SegTab T, const float* __restrict__ arr, int32_t* indx) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nseg) return;
+__device__ void seg_setup(int s, SegTab T, const float* __restrict__ arr, int32_t* indx) {
   int l = T.l[s], r = T.r[s];
   auto IX = [&](int p) -> int32_t& { return indx[p - 1]; };
   auto A = [&](int32_t id) -> float { return arr[id - 1]; };
@@ -79,6 +77,12 @@ __global__ void k_seg_setup(int nseg, SegTab T, const float* __restrict__ arr, i
   if (A(IX(l + 1)) > A(IX(l))) { t = IX(l + 1); IX(l + 1) = IX(l); IX(l) = t; }
   T.pid[s] = IX(l);
   T.piv[s] = A(IX(l));
+}
+
+__global__ void k_seg_setup(const int32_t* __restrict__ nsegp, SegTab T, const float* __restrict__ arr,
+                            int32_t* indx) {
+  const int nseg = *nsegp;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) seg_setup(s, T, arr, indx);
 }
 
 // g/s flags per position
@@ -122,11 +126,8 @@ __global__ void k_scatter_gs(int64_t n, SegTab T, const int32_t* __restrict__ se
   }
 }
 
-__global__ void k_seg_partition(int nseg, SegTab T, const int32_t* __restrict__ gx,
-                                const int32_t* __restrict__ sx, const int32_t* __restrict__ gpos,
-                                const int32_t* __restrict__ spos) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nseg || T.small[s]) return;
+__device__ void seg_partition(int s, SegTab T, const int32_t* __restrict__ gx, const int32_t* __restrict__ sx,
+                              const int32_t* __restrict__ gpos, const int32_t* __restrict__ spos) {
   int l = T.l[s], r = T.r[s], base = l - 1;
   int ng = gx[r] - gx[l - 1];
   int ns = sx[r] - sx[l - 1];
@@ -156,6 +157,15 @@ __global__ void k_seg_partition(int nseg, SegTab T, const int32_t* __restrict__ 
   T.jff[s] = j_f;
 }
 
+__global__ void k_seg_partition(const int32_t* __restrict__ nsegp, SegTab T, const int32_t* __restrict__ gx,
+                                const int32_t* __restrict__ sx, const int32_t* __restrict__ gpos,
+                                const int32_t* __restrict__ spos) {
+  const int nseg = *nsegp;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x) {
+    if (!T.small[s]) seg_partition(s, T, gx, sx, gpos, spos);
+  }
+}
+
 __global__ void k_exchange(int64_t n, SegTab T, const int32_t* __restrict__ segid,
                            const int32_t* __restrict__ gf, const int32_t* __restrict__ gx,
                            const int32_t* __restrict__ spos, int32_t* indx) {
@@ -171,9 +181,7 @@ __global__ void k_exchange(int64_t n, SegTab T, const int32_t* __restrict__ segi
   indx[partner - 1] = a;
 }
 
-__global__ void k_seg_finish(int nseg, SegTab T, SegTab N, int32_t* indx, int32_t* counter) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nseg) return;
+__device__ void seg_finish(int s, SegTab T, SegTab N, int32_t* indx, int32_t* counter) {
   T.cl[s] = -1;
   T.cr[s] = -1;
   if (T.small[s]) return;
@@ -192,6 +200,13 @@ __global__ void k_seg_finish(int nseg, SegTab T, SegTab N, int32_t* indx, int32_
     N.r[id] = r;
     T.cr[s] = id;
   }
+}
+
+__global__ void k_seg_finish(const int32_t* __restrict__ nsegp, SegTab T, SegTab N, int32_t* indx,
+                             int32_t* counter) {
+  const int nseg = *nsegp;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += gridDim.x * blockDim.x)
+    seg_finish(s, T, N, indx, counter);
 }
 
 __global__ void k_relabel(int64_t n, SegTab T, int32_t* segid) {
@@ -219,7 +234,7 @@ int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx)
   const size_t N1 = (size_t)n + 1;
   // scratch: segid, gf, sf, gx, sx, gpos, spos (n+1 each) and two segment tables
   const size_t tab_entries = (size_t)n / 2 + 2;
-  size_t need = 7 * N1 * 4 + 2 * 10 * tab_entries * 4 + 64;
+  size_t need = 7 * N1 * 4 + 2 * 10 * tab_entries * 4 + 64;  // + the two level counters
   char* base = (char*)c->sort_misc.ensure(need);
   if (!base) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (indexx)");
   int32_t* segid = (int32_t*)base;
@@ -244,31 +259,39 @@ int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx)
                                           c->stream));
   if (!c->sort_tmp.ensure(tmpb)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (scan)");
   k_init_indx<<<nb(n), 256, 0, c->stream>>>(n, d_indx, segid);
-  int32_t one[2] = {1, (int32_t)n};
+  int32_t one[3] = {1, (int32_t)n, 1};
   AV_CHECK_HIP(c, hipMemcpyAsync(A.l, &one[0], 4, hipMemcpyHostToDevice, c->stream));
   AV_CHECK_HIP(c, hipMemcpyAsync(A.r, &one[1], 4, hipMemcpyHostToDevice, c->stream));
-  int nseg = 1;
+  AV_CHECK_HIP(c, hipMemcpyAsync(counter, &one[2], 4, hipMemcpyHostToDevice, c->stream));
+  // levels run in batches with the segment count kept on the device; the
+  // host checks it between batches (a level with no segment does nothing)
+  constexpr int kBatch = 4;
+  const unsigned gseg = nb((int64_t)tab_entries) < 1024 ? nb((int64_t)tab_entries) : 1024;
   int levels = 0;
-  while (nseg > 0) {
-    if (++levels > 4096) AV_FAIL(c, AVISM_ERR_LIMIT, "indexx did not converge");
-    k_seg_setup<<<nb(nseg), 256, 0, c->stream>>>(nseg, A, d_keys, d_indx);
-    k_flags<<<nb((int64_t)N1), 256, 0, c->stream>>>(n, A, segid, d_keys, d_indx, gf, sf);
-    AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmpb, gf, gx, 0, N1,
-                                            rocprim::plus<int32_t>(), c->stream));
-    AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmpb, sf, sx, 0, N1,
-                                            rocprim::plus<int32_t>(), c->stream));
-    k_scatter_gs<<<nb(n), 256, 0, c->stream>>>(n, A, segid, gf, sf, gx, sx, gpos, spos);
-    k_seg_partition<<<nb(nseg), 256, 0, c->stream>>>(nseg, A, gx, sx, gpos, spos);
-    k_exchange<<<nb(n), 256, 0, c->stream>>>(n, A, segid, gf, gx, spos, d_indx);
-    AV_CHECK_HIP(c, hipMemsetAsync(counter, 0, 4, c->stream));
-    k_seg_finish<<<nb(nseg), 256, 0, c->stream>>>(nseg, A, B, d_indx, counter);
-    k_relabel<<<nb(n), 256, 0, c->stream>>>(n, A, segid);
-    AV_CHECK_HIP(c, hipGetLastError());
-    int32_t next = 0;
-    AV_CHECK_HIP(c, hipMemcpyAsync(&next, counter, 4, hipMemcpyDeviceToHost, c->stream));
+  for (;;) {
+    for (int b = 0; b < kBatch; ++b, ++levels) {
+      int32_t* cur = counter + (levels & 1);
+      int32_t* nxt = counter + ((levels + 1) & 1);
+      k_seg_setup<<<gseg, 256, 0, c->stream>>>(cur, A, d_keys, d_indx);
+      k_flags<<<nb((int64_t)N1), 256, 0, c->stream>>>(n, A, segid, d_keys, d_indx, gf, sf);
+      AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmpb, gf, gx, 0, N1,
+                                              rocprim::plus<int32_t>(), c->stream));
+      AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmpb, sf, sx, 0, N1,
+                                              rocprim::plus<int32_t>(), c->stream));
+      k_scatter_gs<<<nb(n), 256, 0, c->stream>>>(n, A, segid, gf, sf, gx, sx, gpos, spos);
+      k_seg_partition<<<gseg, 256, 0, c->stream>>>(cur, A, gx, sx, gpos, spos);
+      k_exchange<<<nb(n), 256, 0, c->stream>>>(n, A, segid, gf, gx, spos, d_indx);
+      AV_CHECK_HIP(c, hipMemsetAsync(nxt, 0, 4, c->stream));
+      k_seg_finish<<<gseg, 256, 0, c->stream>>>(cur, A, B, d_indx, nxt);
+      k_relabel<<<nb(n), 256, 0, c->stream>>>(n, A, segid);
+      AV_CHECK_HIP(c, hipGetLastError());
+      std::swap(A, B);
+    }
+    int32_t left = 0;
+    AV_CHECK_HIP(c, hipMemcpyAsync(&left, counter + (levels & 1), 4, hipMemcpyDeviceToHost, c->stream));
     AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
-    nseg = next;
-    std::swap(A, B);
+    if (left == 0) break;
+    if (levels > 4096) AV_FAIL(c, AVISM_ERR_LIMIT, "indexx did not converge");
   }
   return AVISM_OK;
 }
