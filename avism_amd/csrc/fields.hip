@@ -614,34 +614,50 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
 }
 
 // cell sums -> U1 = mass/(dx^3 rete^3)/rote*rote (avism.f90:1249-1250, 1386)
-// and the FP64 block sums of U1 (avism.f90:1395-1405), fixed reduction order
-// acc: int64 fixed-point cell masses (fp64 == 0) or FP64 ones (fp64 != 0)
-__global__ void __launch_bounds__(256) k_ddens_finalize(int n, const unsigned long long* __restrict__ acc,
-                                                       double inv_scale, int fp64, float den, float rote,
-                                                       float* umass, double* partial) {
-  int ix = blockIdx.x * 8 + threadIdx.x % 8;
-  int iy = blockIdx.y * 8 + (threadIdx.x / 8) % 8;
-  int iz = blockIdx.z * 4 + threadIdx.x / 64;
+// and the FP64 block sums of U1 (avism.f90:1395-1405), fixed reduction order.
+// acc: int64 fixed-point cell masses (fp64 == 0) or FP64 ones (fp64 != 0).
+// A thread takes kFinV consecutive cells (16-byte loads and stores), a block
+// kFinB * kFinV cells.
+constexpr int kFinB = 256, kFinV = 4;
+
+__device__ __forceinline__ float fin_cell(unsigned long long a, double inv_scale, int fp64, float den, float rote) {
+  const float mass = fp64 ? (float)__longlong_as_double((long long)a) : (float)((double)a * inv_scale);
+  float u = mass / den;
+  u = u / rote;
+  return u * rote;
+}
+
+__global__ void __launch_bounds__(kFinB) k_ddens_finalize(int64_t ncell, const unsigned long long* __restrict__ acc,
+                                                         double inv_scale, int fp64, float den, float rote,
+                                                         float* umass, double* partial) {
+  const int64_t c0 = ((int64_t)blockIdx.x * kFinB + threadIdx.x) * kFinV;
   double contrib = 0.0;
-  if (ix < n && iy < n && iz < n) {
-    size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
-    float mass = fp64 ? (float)__longlong_as_double((long long)acc[c]) : (float)((double)acc[c] * inv_scale);
-    float u = mass / den;
-    u = u / rote;
-    u = u * rote;
-    umass[c] = u;
-    contrib = (double)u;
+  if (c0 + kFinV <= ncell) {
+    const ulonglong2 a01 = *reinterpret_cast<const ulonglong2*>(acc + c0);
+    const ulonglong2 a23 = *reinterpret_cast<const ulonglong2*>(acc + c0 + 2);
+    float4 u;
+    u.x = fin_cell(a01.x, inv_scale, fp64, den, rote);
+    u.y = fin_cell(a01.y, inv_scale, fp64, den, rote);
+    u.z = fin_cell(a23.x, inv_scale, fp64, den, rote);
+    u.w = fin_cell(a23.y, inv_scale, fp64, den, rote);
+    *reinterpret_cast<float4*>(umass + c0) = u;
+    contrib = (((double)u.x + (double)u.y) + (double)u.z) + (double)u.w;
+  } else {
+    for (int64_t c = c0; c < ncell; ++c) {
+      const float u = fin_cell(acc[c], inv_scale, fp64, den, rote);
+      umass[c] = u;
+      contrib += (double)u;
+    }
   }
-  __shared__ double red[256];
-  red[threadIdx.x] = contrib;
+  // block sum in a fixed order: waves by shuffles, then the four wave sums
+  for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off);
+  __shared__ double wsum[kFinB / 64];
+  if (threadIdx.x % 64 == 0) wsum[threadIdx.x / 64] = contrib;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    size_t b = blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
-    partial[b] = red[0];
+    double t = 0.0;
+    for (int w = 0; w < kFinB / 64; ++w) t += wsum[w];
+    partial[blockIdx.x] = t;
   }
 }
 
@@ -921,8 +937,8 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   DdensGeom G{rad, c->g.rlo, n, c->g.dx, c->g.rx1};
   // cosmology constants (avism.f90:469-479, 599-606, 1249-1250)
   float den = ((c->g.dx * c->g.dx) * c->g.dx) * ((c->rete * c->rete) * c->rete);
-  dim3 grid((n + 7) / 8, (n + 7) / 8, (n + 3) / 4);
-  int64_t nparts = (int64_t)grid.x * grid.y * grid.z;
+  const unsigned grid = (unsigned)((ncell + (int64_t)kFinB * kFinV - 1) / ((int64_t)kFinB * kFinV));
+  int64_t nparts = grid;
   if (!c->partial64.ensure((size_t)nparts * 8) || !c->mass_fix.ensure((size_t)ncell * 8))
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
   // fixed point: every cell sum stays below np * max mass * scale <= 2^62
@@ -947,7 +963,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   }
   {
     StageTimer t(c, "ddens_finalize");
-    k_ddens_finalize<<<grid, 256, 0, c->stream>>>(n, c->mass_fix.as<unsigned long long>(), 1.0 / scale,
+    k_ddens_finalize<<<grid, kFinB, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 1.0 / scale,
                                                   dens_interp == 1 ? 0 : 1, den,
                                                   c->rote, c->u_mass.as<float>(), c->partial64.as<double>());
     AV_CHECK_HIP(c, hipGetLastError());
