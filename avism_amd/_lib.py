@@ -50,7 +50,7 @@ class LevelStats(C.Structure):
 EXPORTS = [
     "avism_create", "avism_destroy", "avism_last_error", "avism_set_timing", "avism_get_timings",
     "avism_set_grid", "avism_get_grid", "avism_set_cosmology", "avism_set_void_params",
-    "avism_set_parent_map", "avism_set_parent_radii", "avism_next_level", "avism_load_particles",
+    "avism_set_parent_map", "avism_set_parent_radii", "avism_set_mask", "avism_next_level", "avism_load_particles",
     "avism_particles_to_grid",
     "avism_load_grid_fields", "avism_load_void_fields", "avism_get_fields", "avism_get_particle_fields", "avism_find_voids",
     "avism_set_proposals", "avism_get_candidates", "avism_get_cubes", "avism_indexx",
@@ -90,6 +90,7 @@ def lib():
         "avism_set_void_params": ([vp, C.POINTER(VoidParams)], C.c_int),
         "avism_set_parent_map": ([vp, vp, i32], C.c_int),
         "avism_set_parent_radii": ([vp, vp, i32, i32], C.c_int),
+        "avism_set_mask": ([vp, vp, i32], C.c_int),
         "avism_next_level": ([vp], C.c_int),
         "avism_load_particles": ([vp, i64, vp, vp, vp, vp, vp, vp, vp, i32], C.c_int),
         "avism_particles_to_grid": ([vp, i32, i32, i32], C.c_int),
@@ -196,6 +197,18 @@ class Context:
         a = np.ascontiguousarray(reqp, np.float32).ravel()
         self._chk(lib().avism_set_parent_radii(self._h, _ptr(a) if a.size else _ptr(np.zeros(1, np.float32)),
                                                int(a.size), AVISM_HOST))
+
+    def set_mask(self, smask_inner):
+        """SMASK on the inner grid (int8, 1 = observed; None: no mask)."""
+        if smask_inner is None:
+            self._chk(lib().avism_set_mask(self._h, None, AVISM_HOST))
+            return
+        a = np.ascontiguousarray(smask_inner, np.int8).ravel()
+        gi = self.grid()
+        if a.size != gi.n ** 3:
+            raise ValueError(f"mask has {a.size} cells, the grid {gi.n ** 3}")
+        self._mask = a
+        self._chk(lib().avism_set_mask(self._h, _ptr(a), AVISM_HOST))
 
     def next_level(self) -> GridInfo:
         """Void-in-void hand-off: MARCAP, REQP and the grid of 2N (avism.f90:2123-2161)."""

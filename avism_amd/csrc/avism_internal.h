@@ -92,6 +92,8 @@ struct avism_ctx {
   avism::DBuf partial64;
   avism::DBuf marcap;                          // inner MARCAP (optional)
   bool has_marcap = false;
+  avism::DBuf smask;                           // inner SMASK (int8), when has_mask
+  bool has_mask = false;
   avism::DBuf reqp;                            // REQP of the parent level, ids 1..nreqp
   int32_t nreqp = 0;
   bool has_reqp = false;

@@ -124,6 +124,7 @@ int avism_set_grid(avism_ctx* c, int32_t n, float L0, int32_t periodic) {
   c->has_marcap = false;
   c->has_reqp = false;
   c->nreqp = 0;
+  c->has_mask = false;
   return AVISM_OK;
 }
 
@@ -197,6 +198,22 @@ int avism_set_parent_radii(avism_ctx* c, const float* reqp, int32_t n, int32_t w
                               where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   c->has_reqp = true;
   c->nreqp = n;
+  return AVISM_OK;
+}
+
+int avism_set_mask(avism_ctx* c, const int8_t* smask, int32_t where) {
+  CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
+  if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "no grid set");
+  if (!smask) {
+    c->has_mask = false;
+    return AVISM_OK;
+  }
+  size_t b = (size_t)c->g.n * c->g.n * c->g.n;
+  if (!c->smask.ensure(b)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+  AV_CHECK_HIP(c, hipMemcpy(c->smask.p, smask, b,
+                            where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  c->has_mask = true;
   return AVISM_OK;
 }
 

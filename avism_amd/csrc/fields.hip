@@ -867,6 +867,8 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   AV_CHECK_HIP(c, hipMemcpyAsync(rad, c->g.rad.data(), c->g.rad.size() * 4, hipMemcpyHostToDevice,
                                  c->stream));
   Bins B = make_bins(c);
+  // SMASK: VVEL_INTERP_SPH_VW and H_DISTANCE skip masked cells (particles.f90:312, 856)
+  const int8_t* smask = c->has_mask ? c->smask.as<int8_t>() : nullptr;
   // first search radius: 1.3 x the k-th neighbour distance at the mean density
   const double rho = (double)np / ((double)c->g.L0 * c->g.L0 * c->g.L0);
   const float r0 = (float)(1.3 * std::cbrt(3.0 * k / (4.0 * kPi * rho)));
@@ -891,10 +893,15 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
       const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
       const VvelOut O{c->vel4.as<float4>(), c->hpart.as<uint32_t>(), c->fu.as<float>(), c->fv.as<float>(),
                       c->fw.as<float>(), 1};
+      if (smask) {  // masked cells keep zero velocity (particles.f90:856)
+        AV_CHECK_HIP(c, hipMemsetAsync(c->fu.p, 0, (size_t)ncell * 4, c->stream));
+        AV_CHECK_HIP(c, hipMemsetAsync(c->fv.p, 0, (size_t)ncell * 4, c->stream));
+        AV_CHECK_HIP(c, hipMemsetAsync(c->fw.p, 0, (size_t)ncell * 4, c->stream));
+      }
       if (k <= 32)
-        k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
+        k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
       else
-        k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
+        k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
       AV_CHECK_HIP(c, hipGetLastError());
     }
     {
@@ -920,13 +927,13 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     const VvelOut O{nullptr, c->hpart.as<uint32_t>(), nullptr, nullptr, nullptr, 0};
     const unsigned nbq = nblk(np, KQ);
     if (k <= 32) {
-      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
+      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
       if (c->g.periodic)
         k_fill_h_knn<1, 32><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
       else
         k_fill_h_knn<0, 32><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
     } else {
-      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, nullptr, O);
+      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
       if (c->g.periodic)
         k_fill_h_knn<1, 64><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
       else

@@ -1488,6 +1488,16 @@ __global__ void k_upsample_marca(ExtG G, const int32_t* __restrict__ marca, int3
   marcap[(size_t)x + (size_t)n2 * ((size_t)y + (size_t)n2 * z)] = marca[G.ext(x / 2 + 1, y / 2 + 1, z / 2 + 1)];
 }
 
+// SMASK of the finer grid from the coarser one (avism.f90:875-889, NDOWN = 2)
+__global__ void k_upsample_mask(int n, const int8_t* __restrict__ m, int8_t* m2) {
+  const int n2 = 2 * n;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n2) return;
+  const int y = blockIdx.y, z = blockIdx.z;
+  m2[(size_t)x + (size_t)n2 * ((size_t)y + (size_t)n2 * z)] =
+      m[(size_t)(x / 2) + (size_t)n * ((size_t)(y / 2) + (size_t)n * (z / 2))];
+}
+
 // REQP(IND) = REQ(IND) for the main voids (avism.f90:2140-2144); 0 for the rest
 __global__ void k_reqp(int nv, const int32_t* __restrict__ uvoid, const float* __restrict__ req, float* reqp) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1515,6 +1525,16 @@ int next_level(avism_ctx* c) {
   if (nv > 0)
     k_reqp<<<nbk(nv), 256, 0, c->stream>>>(nv, c->uvoid.as<int32_t>(), c->vf.as<float>() + 11 * (size_t)nv,
                                            c->reqp.as<float>());
+  if (c->has_mask) {  // the survey mask at the finer resolution (the old one is no longer needed)
+    DBuf m2;
+    if (!m2.ensure((size_t)n2 * n2 * n2)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (next level)");
+    k_upsample_mask<<<dim3((unsigned)((n2 + 255) / 256), (unsigned)n2, (unsigned)n2), 256, 0, c->stream>>>(
+        g.n, c->smask.as<int8_t>(), m2.as<int8_t>());
+    AV_CHECK_HIP(c, hipGetLastError());
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    std::swap(c->smask.p, m2.p);
+    std::swap(c->smask.bytes, m2.bytes);
+  }
   AV_CHECK_HIP(c, hipGetLastError());
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   c->g = g2;

@@ -96,11 +96,15 @@ __device__ __forceinline__ bool mark_dir(const MarkParams& P, const Line& A, int
   return flag;
 }
 
+// MASK: SMASK given (inner int8; the periodic shell is masked, avism.f90:866-904)
+template <int MASK>
 __global__ void __launch_bounds__(kMarkT) k_mark(ExtGeom G, MarkParams P, const float* __restrict__ u1co,
                                                  const float* __restrict__ div,
-                                                 const int32_t* __restrict__ marcap, uint8_t* flags) {
+                                                 const int32_t* __restrict__ marcap,
+                                                 const int8_t* __restrict__ smask, uint8_t* flags) {
   __shared__ float su[kMarkP][kMarkY + 6][kMarkX + 6];
   __shared__ float sd[kMarkP][kMarkY + 2][kMarkX + 2];
+  __shared__ int8_t sm[MASK ? kMarkP : 1][kMarkY + 2][kMarkX + 2];
   const int L1 = G.low1, L2 = G.low2, n = G.n, ne = G.ne;
   const int tx = threadIdx.x % kMarkX, ty = threadIdx.x / kMarkX;
   const int x0 = blockIdx.x * kMarkX, y0 = blockIdx.y * kMarkY, z0 = blockIdx.z * kMarkZ;
@@ -124,9 +128,31 @@ __global__ void __launch_bounds__(kMarkT) k_mark(ExtGeom G, MarkParams P, const 
     const int q = threadIdx.x + m * kMarkT;
     dof[m] = inner_xy(x0 + q % (kMarkX + 2) - 1 + L1, y0 + q / (kMarkX + 2) - 1 + L1);
   }
+  // SMASK of an ext cell: outside the inner box it is 0
+  auto inner_or = [&](int xe, int ye) -> int64_t {
+    return (xe < 1 || xe > n || ye < 1 || ye > n) ? -1 : (int64_t)(xe - 1) + (int64_t)n * (ye - 1);
+  };
+  auto mask_at = [&](int64_t xy, int ze) -> int8_t {
+    return (xy < 0 || ze < 1 || ze > n) ? (int8_t)0 : smask[xy + (int64_t)n * n * (ze - 1)];
+  };
+  int64_t mo[MASK ? kMarkDP : 1];
+  const int64_t mxy = MASK ? inner_or(I, J) : 0;
+  if (MASK) {
+#pragma unroll
+    for (int m = 0; m < kMarkDP; ++m) {
+      const int q = threadIdx.x + m * kMarkT;
+      mo[m] = inner_or(x0 + q % (kMarkX + 2) - 1 + L1, y0 + q / (kMarkX + 2) - 1 + L1);
+    }
+  }
   const int z1 = min(z0 + kMarkZ, ne);
-  // the z line of the step's planes: U1CO at z-3..z+kMarkP+2, DIVERCO at z-1..z+kMarkP
+  // the z line of the step's planes: U1CO at z-3..z+kMarkP+2, DIVERCO (and
+  // SMASK) at z-1..z+kMarkP
   float zu[kMarkP + 6], zd[kMarkP + 2];
+  int8_t zm[MASK ? kMarkP + 2 : 1];
+  if (MASK) {
+#pragma unroll
+    for (int o = 0; o < kMarkP + 2; ++o) zm[o] = mask_at(mxy, z0 + L1 + o - 1);
+  }
 #pragma unroll
   for (int o = 0; o < kMarkP + 6; ++o) zu[o] = u1co[cxy + plane(z0 + L1 + o - 3)];
 #pragma unroll
@@ -144,6 +170,7 @@ __global__ void __launch_bounds__(kMarkT) k_mark(ExtGeom G, MarkParams P, const 
       for (int m = 0; m < kMarkDP; ++m) {
         const int q = threadIdx.x + m * kMarkT;
         if (q < kMarkD) (&sd[pl][0][0])[q] = div[dof[m] + pk];
+        if (MASK && q < kMarkD) (&sm[pl][0][0])[q] = mask_at(mo[m], z + pl + L1);
       }
     }
     __syncthreads();
@@ -166,17 +193,24 @@ __global__ void __launch_bounds__(kMarkT) k_mark(ExtGeom G, MarkParams P, const 
         }
         const int mp = marcap ? marcap[cxy + plane(K)] : -1;
         uint8_t f = 0;
-        // MARK_ALL (voidfinding.f90:30-52); SMASK = 1
-        if (mp != 0) {
+        // MARK_ALL (voidfinding.f90:30-52): nothing in a masked cell
+        if ((!MASK || zm[pl + 1] != 0) && mp != 0) {
           f |= FL_SUB;
           if (Z.u[3] < P.dthr1 && Z.d[1] > 0.0f) f |= FL_V;
         }
-        if (mark_dir(P, X, I, true, L1, L2)) f |= FL_X2;
-        if (mark_dir(P, X, I, false, L1, L2)) f |= FL_X1;
-        if (mark_dir(P, Y, J, true, L1, L2)) f |= FL_Y2;
-        if (mark_dir(P, Y, J, false, L1, L2)) f |= FL_Y1;
-        if (mark_dir(P, Z, K, true, L1, L2)) f |= FL_Z2;
-        if (mark_dir(P, Z, K, false, L1, L2)) f |= FL_Z1;
+        // directions in the reference's order; a masked neighbour ends the
+        // cell (CYCLE, voidfinding.f90:283, 308, ...): later directions stay 0
+        auto step = [&](const Line& A, int pos, bool plus, int8_t nmask, uint8_t bit) -> bool {
+          const bool edge = plus ? (pos >= L2 - kNsec) : (pos <= L1 + kNsec);
+          if (!edge && MASK && nmask == 0) return false;
+          if (mark_dir(P, A, pos, plus, L1, L2)) f |= bit;
+          return true;
+        };
+        const int8_t mxp = MASK ? sm[pl][ty + 1][tx + 2] : 1, mxm = MASK ? sm[pl][ty + 1][tx] : 1;
+        const int8_t myp = MASK ? sm[pl][ty + 2][tx + 1] : 1, mym = MASK ? sm[pl][ty][tx + 1] : 1;
+        const int8_t mzp = MASK ? zm[pl + 2] : 1, mzm = MASK ? zm[pl] : 1;
+        (void)(step(X, I, true, mxp, FL_X2) && step(X, I, false, mxm, FL_X1) && step(Y, J, true, myp, FL_Y2) &&
+               step(Y, J, false, mym, FL_Y1) && step(Z, K, true, mzp, FL_Z2) && step(Z, K, false, mzm, FL_Z1));
         flags[(size_t)(x0 + tx) + (size_t)ne * ((size_t)(y0 + ty) + (size_t)ne * (z + pl))] = f;
       }
     }
@@ -190,6 +224,12 @@ __global__ void __launch_bounds__(kMarkT) k_mark(ExtGeom G, MarkParams P, const 
       for (int o = 0; o < 2; ++o) zd[o] = zd[o + kMarkP];
 #pragma unroll
       for (int o = 2; o < kMarkP + 2; ++o) zd[o] = div[cxy + plane(z + kMarkP + L1 + o - 1)];
+      if (MASK) {
+#pragma unroll
+        for (int o = 0; o < 2; ++o) zm[o] = zm[o + kMarkP];
+#pragma unroll
+        for (int o = 2; o < kMarkP + 2; ++o) zm[o] = mask_at(mxy, z + kMarkP + L1 + o - 1);
+      }
     }
     __syncthreads();
   }
@@ -597,10 +637,14 @@ int void_find(avism_ctx* c, int32_t growth_mode) {
   if (!c->flags.ensure((size_t)ne3)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (flags)");
   {
     StageTimer t(c, "mark_flags");
-    k_mark<<<dim3((unsigned)((G.ne + kMarkX - 1) / kMarkX), (unsigned)((G.ne + kMarkY - 1) / kMarkY),
-                  (unsigned)((G.ne + kMarkZ - 1) / kMarkZ)),
-             kMarkT, 0, c->stream>>>(
-        G, P, c->u1co.as<float>(), c->div.as<float>(), marcap, c->flags.as<uint8_t>());
+    const dim3 gm((unsigned)((G.ne + kMarkX - 1) / kMarkX), (unsigned)((G.ne + kMarkY - 1) / kMarkY),
+                  (unsigned)((G.ne + kMarkZ - 1) / kMarkZ));
+    if (c->has_mask)
+      k_mark<1><<<gm, kMarkT, 0, c->stream>>>(G, P, c->u1co.as<float>(), c->div.as<float>(), marcap,
+                                              c->smask.as<int8_t>(), c->flags.as<uint8_t>());
+    else
+      k_mark<0><<<gm, kMarkT, 0, c->stream>>>(G, P, c->u1co.as<float>(), c->div.as<float>(), marcap, nullptr,
+                                              c->flags.as<uint8_t>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   // ordered compaction

@@ -23,7 +23,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-__all__ = ["Particles", "zeldovich", "mock_voids", "grid_fields", "particle_mass"]
+__all__ = ["Particles", "zeldovich", "mock_voids", "grid_fields", "particle_mass", "survey_mask"]
 
 RHO_CRIT = 2.775e11  # h^2 Msun / Mpc^3
 
@@ -145,6 +145,20 @@ def mock_voids(n0: int = 1_000_000, L0: float = 200.0, nvoids: int = 40, rmin: f
     parts = Particles(t(pos[:, 0]), t(pos[:, 1]), t(pos[:, 2]), t(vel[:, 0]), t(vel[:, 1]),
                       t(vel[:, 2]), t(m), float(L0), 0)
     return parts, centres, radii
+
+
+def survey_mask(n: int, empty_frac: float = 0.2, seed: int = 0, rmin: float = 1.5, rmax: float = 5.0) -> np.ndarray:
+    """A seeded survey-like mask on an n^3 grid (int8, x fastest, 1 = observed):
+    random balls of radius rmin..rmax cells are cut out until empty_frac of the
+    cells are empty (the shape the reference reads as SMASK, avism.f90:403-418)."""
+    rng = np.random.default_rng(seed)
+    m = np.ones((n, n, n), np.int8)  # [z, y, x]
+    zz, yy, xx = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    while (m == 0).mean() < empty_frac:
+        c = rng.uniform(0, n, size=3)
+        r = rng.uniform(rmin, rmax)
+        m[(xx - c[0]) ** 2 + (yy - c[1]) ** 2 + (zz - c[2]) ** 2 < r * r] = 0
+    return m.ravel()
 
 
 def grid_fields(n: int, L0: float, seed: int, sigma: float = 0.8, kc: float = 0.5,

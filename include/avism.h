@@ -103,10 +103,17 @@ int avism_set_parent_map(avism_ctx* ctx, const int32_t* marcap_inner, int32_t wh
  * (voidfinding.f90:267-274, 491).  NULL: no parent-size test.  Fails with
  * AVISM_ERR_INPUT at avism_find_voids if the parent map holds ids above n. */
 int avism_set_parent_radii(avism_ctx* ctx, const float* reqp, int32_t n, int32_t where);
+/* SMASK on the inner grid (int8, 1 = observed; avism.f90:403-418, 866-904);
+ * NULL = 1 everywhere.  The periodic shell counts as masked, as in survey
+ * mode.  VVEL_INTERP_SPH_VW and H_DISTANCE skip masked cells
+ * (particles.f90:312, 856), MARK_ALL marks nothing there and the flag stencil
+ * ends a cell at a masked neighbour (voidfinding.f90:33, 283-300); DDENS and
+ * the mean density do not read it (the reference's non-survey mean). */
+int avism_set_mask(avism_ctx* ctx, const int8_t* smask_inner, int32_t where);
 /* the void-in-void hand-off (avism.f90:2123-2161, 742-759), after
  * avism_merge_and_post: MARCAP = this level's MARCA upsampled by 2 per axis,
- * REQP = REQ of its main voids, and the grid becomes 2N cells per axis over
- * the same box.  Loaded particles are kept, so avism_particles_to_grid then
+ * REQP = REQ of its main voids, the survey mask (if set) upsampled likewise,
+ * and the grid becomes 2N cells per axis over the same box.  Loaded particles are kept, so avism_particles_to_grid then
  * interpolates them at the new resolution (the reference frees them inside
  * its level loop, avism.f90:1613-1629); grid fields for 2N can be loaded
  * instead. */

@@ -197,11 +197,12 @@ class LevelResult:
 
 
 def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.678,
-                          omega0=0.31, zeta=0.0, vel_interp=1, dens_interp=1, tsc_exact=False):
+                          omega0=0.31, zeta=0.0, vel_interp=1, dens_interp=1, tsc_exact=False, smask=None):
     """Field stage (avism.f90:1177-1257, 1371-1414). Inputs in file units
     (reader.f90:447-461 conversion applied here).  Returns a dict of arrays.
     tsc_exact: TSC velocity sums in FP64 instead of the reference's REAL*4
-    one-thread order."""
+    one-thread order.  smask: inner n^3 int8 SMASK (None: all ones); VVEL and
+    H_DISTANCE skip masked cells (particles.f90:312, 856), DDENS ignores it."""
     L = lib()
     g = Grid(n, L0, periodic)
     x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
@@ -214,9 +215,10 @@ def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.
     pdens = np.zeros(npart, np.float32)
     hk = np.zeros(npart, np.float32)
     knn = KNN(x, y, z, L0, periodic)
+    sm = None if smask is None else np.ascontiguousarray(smask, np.int8).ctypes.data_as(C.c_void_p)
     if vel_interp == 1:
         L.orc_ppart_dens(knn._p, npart, x, y, z, m, k, pdens, hk)
-        L.orc_vvel_vw(knn._p, g._p, k, m, pdens, vx, vy, vz, None, hpart, u, v, w)
+        L.orc_vvel_vw(knn._p, g._p, k, m, pdens, vx, vy, vz, sm, hpart, u, v, w)
         L.orc_fill_h_zeros(knn._p, npart, x, y, z, k, hpart)
     else:
         L.orc_vel_tsc(g._p, npart, x, y, z, vx, vy, vz, 1 if tsc_exact else 0, u, v, w)
@@ -224,7 +226,7 @@ def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.
     mass64 = np.zeros(ncell, np.float64)
     if dens_interp == 1:
         if vel_interp == 0:
-            L.orc_h_distance(knn._p, g._p, k, None, hpart)
+            L.orc_h_distance(knn._p, g._p, k, sm, hpart)
             L.orc_fill_h_zeros(knn._p, npart, x, y, z, k, hpart)
         L.orc_ddens_sph(g._p, npart, x, y, z, hpart, m, mass, mass64)
     else:
@@ -265,7 +267,15 @@ def void_stage(g: Grid, u1co, div, params: VoidParams, marcap=None, reqp=None, s
     mp = np.full(g.n ** 3, -1, np.int32) if marcap is None else np.ascontiguousarray(marcap, np.int32)
     mp_ext = np.zeros(ne3, np.int32)
     L.orc_extend_i32(g._p, mp, mp_ext)
-    sm = np.ones(ne3, np.int8) if smask is None else np.ascontiguousarray(smask, np.int8)
+    if smask is None:
+        sm = np.ones(ne3, np.int8)
+    elif np.asarray(smask).size == g.n ** 3:  # inner SMASK: the periodic shell stays 0 (avism.f90:866-904)
+        sm = np.zeros((g.next,) * 3, np.int8)
+        o = 1 - g.low1
+        sm[o:o + g.n, o:o + g.n, o:o + g.n] = np.asarray(smask, np.int8).reshape((g.n,) * 3)
+        sm = sm.ravel()
+    else:
+        sm = np.ascontiguousarray(smask, np.int8)
     NV = params.nvoid_max
     ci = np.zeros(9 * NV, np.int32)
     cf = np.zeros(7 * NV, np.float32)
@@ -357,7 +367,7 @@ def run_reference_indexx(jobdir: Path, keys: np.ndarray) -> np.ndarray:
 
 def run_reference(jobdir: Path, mode: int, n: int, L0: float, periodic: int, params: VoidParams,
                   ache=0.678, omega0=0.31, zeta=0.0, k=32, vel_interp=1, dens_interp=1,
-                  parts=None, fields=None, marcap=None, reqp=None, timeout=600) -> dict:
+                  parts=None, fields=None, marcap=None, reqp=None, smask=None, timeout=600) -> dict:
     """Run the reference routines (oracle/_ref) on one level and read the outputs.
     marcap (inner n^3 int32) and reqp (float32, ids 1..len) are a parent level's
     MARCAP and REQP (avism.f90:2123-2161); None runs the first level."""
@@ -365,8 +375,12 @@ def run_reference(jobdir: Path, mode: int, n: int, L0: float, periodic: int, par
 
     jobdir = Path(jobdir)
     jobdir.mkdir(parents=True, exist_ok=True)
-    for f in ("marcap.bin", "reqp.bin"):
+    for f in ("marcap.bin", "reqp.bin", "smask.bin"):
         (jobdir / f).unlink(missing_ok=True)
+    if smask is not None:
+        sm = np.ascontiguousarray(smask, np.int8)
+        assert sm.size == n ** 3
+        (jobdir / "smask.bin").write_bytes(sm.tobytes())
     if marcap is not None:
         mp = np.ascontiguousarray(marcap, np.int32)
         assert mp.size == n ** 3

@@ -20,6 +20,9 @@
 !                        reference driver hands them to the next level
 !                        (avism.f90:2123-2161); without them MARCAP = -1 and
 !                        REQP = 0 (avism.f90:957-964)
+!   <jobdir>/smask.bin   (optional) SMASK int8 [n^3] on the inner grid; the
+!                        periodic shell stays 0 as in survey mode
+!                        (avism.f90:866-904); without it SMASK = 1
 ! Outputs <jobdir>/out_*.bin as raw stream files.
 !-----------------------------------------------------------------------------
 PROGRAM REF_HARNESS
@@ -31,7 +34,7 @@ PROGRAM REF_HARNESS
    CHARACTER(LEN=512) :: JOB
    INTEGER :: MODE, N, NP4, KNEIGH, NPLUS, LOW1, LOW2, IX, JY, KZ, I, IV, IND, NVOID, NVOID2
    INTEGER :: NVOIDT
-   LOGICAL :: HAVE_MARCAP
+   LOGICAL :: HAVE_MARCAP, HAVE_MASK
    INTEGER(KIND=8) :: I8, CNT8
    REAL*4 :: ACHE, OMEGA0, ZETA, ACHE0, RODO, RE0, ROTE, RETE, LADO, SIZING, MEANDENS
    REAL*4 :: DXX, RX1
@@ -98,6 +101,13 @@ PROGRAM REF_HARNESS
    ENDIF
    ALLOCATE(SMASK(LOW1:LOW2,LOW1:LOW2,LOW1:LOW2))
    SMASK = 1
+   INQUIRE(FILE=TRIM(JOB)//'/smask.bin', EXIST=HAVE_MASK)
+   IF (HAVE_MASK) THEN
+      SMASK = 0
+      OPEN(2, FILE=TRIM(JOB)//'/smask.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+      READ(2) SMASK(1:N,1:N,1:N)
+      CLOSE(2)
+   ENDIF
    ALLOCATE(UBAS(N,N,N), U1C(N,N,N), U2C(N,N,N), U3C(N,N,N), U4C(N,N,N))
    U1C = 0.; U2C = 0.; U3C = 0.; U4C = 0.
 

@@ -24,7 +24,13 @@ Each fixture holds the inputs and the reference's outputs for one level:
                     is one run of the reference routines; RMIN_SUB is set so
                     that some parents are below it (voidfinding.f90:267-274).
 
-`python tests/golden/make_golden.py tsc` (or `lev`) writes only those fixtures.
+  mask_p{0,1}, mask_g{0,1}
+                  : particle (grid) input with a survey mask (SMASK, 20% of the cells
+                    empty, the periodic shell masked as in survey mode): VVEL and
+                    H_DISTANCE skip masked cells, MARK_ALL and the flag stencil
+                    read it (voidfinding.f90:33, 283-300), DDENS ignores it
+
+`python tests/golden/make_golden.py tsc` (or `lev`, `mask`) writes only those fixtures.
 """
 from __future__ import annotations
 
@@ -104,6 +110,27 @@ def lev_fixtures(td: Path):
                                                rmin_sub=rmin_sub))
 
 
+def mask_fixtures(td: Path):
+    cases = (
+        (1, synth.zeldovich(16, 40.0, seed=26, rms_disp=2.0).numpy(), 20, 40.0, 26),
+        (0, synth.mock_voids(n0=5000, L0=60.0, nvoids=4, rmin=6.0, rmax=12.0, seed=27)[0].numpy(), 20, 60.0, 27),
+    )
+    for per, arr, n, L0, seed in cases:
+        sm = synth.survey_mask(n, 0.2, seed=seed)
+        ref = O.run_reference(td / f"mask{per}", 1, n, L0, per, PARAMS, k=16, parts=arr, smask=sm)
+        inputs = dict(zip(("x", "y", "z", "vx", "vy", "vz", "m"), arr))
+        inputs.update(smask=sm)
+        _save(f"mask_p{per}", inputs, ref, dict(n=n, L0=L0, periodic=per, mode=1, k=16, seed=seed))
+    # grid input with a mask: more voids for the masked marking and growth
+    for per, seed in ((1, 28), (0, 29)):
+        n, L0 = 24, 48.0
+        d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, seed))
+        sm = synth.survey_mask(n, 0.2, seed=seed)
+        ref = O.run_reference(td / f"maskg{per}", 2, n, L0, per, PARAMS, fields=(d, vx, vy, vz), smask=sm)
+        _save(f"mask_g{per}", dict(delta=d, vx=vx, vy=vy, vz=vz, smask=sm), ref,
+              dict(n=n, L0=L0, periodic=per, mode=2, seed=seed))
+
+
 def main():
     for per in (0, 1):
         if not O.ref_available(per):
@@ -116,8 +143,12 @@ def main():
         if sys.argv[1:] == ["lev"]:
             lev_fixtures(td)
             return
+        if sys.argv[1:] == ["mask"]:
+            mask_fixtures(td)
+            return
         tsc_fixtures(td)
         lev_fixtures(td)
+        mask_fixtures(td)
         for per, seed in ((0, 11), (1, 12)):
             n, L0 = 24, 48.0
             d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, seed))
