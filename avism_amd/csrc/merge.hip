@@ -472,13 +472,14 @@ __global__ void k_apply_removed_cubes(int nv, const int32_t* __restrict__ remove
   }
 }
 
+// grid-stride over a fixed grid, so that a pass with nothing removed costs one small launch
 __global__ void k_apply_removed_cells(int64_t ne3, const int32_t* __restrict__ removed,
                                       const int32_t* __restrict__ nremoved, int32_t* marca) {
   if (*nremoved == 0) return;
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne3) return;
-  int id = marca[e];
-  if (id > 0 && removed[id - 1]) marca[e] = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne3; e += (int64_t)gridDim.x * blockDim.x) {
+    const int id = marca[e];
+    if (id > 0 && removed[id - 1]) marca[e] = 0;
+  }
 }
 
 // VOID_SHAPE (shape.f90:155-172): per-cell FP32 terms, FP64 sums
@@ -1313,7 +1314,7 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     AV_CHECK_HIP(c, hipMemsetAsync(small + 7, 0, 4, c->stream));
     k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 0, g.dx, 0.0, removed, VOLNEW, small + 7);
     k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
-    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
+    k_apply_removed_cells<<<4096, 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
   }
   // ---- VOID_SHAPE (avism.f90:1831)
   float* d_rad = (float*)((char*)c->scalars.p + 4096);
@@ -1334,7 +1335,7 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     AV_CHECK_HIP(c, hipMemsetAsync(small + 7, 0, 4, c->stream));
     k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 1, g.dx, rmin3, removed, VOLNEW, small + 7);
     k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
-    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
+    k_apply_removed_cells<<<4096, 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
     k_xc<<<nbk(nv), 256, 0, c->stream>>>(nv, C, uvoid, g.dx, g.rx1, XC, YC, ZC);
   }
   if (stop_after == 2) {
@@ -1368,7 +1369,7 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     AV_CHECK_HIP(c, hipMemsetAsync(small + 7, 0, 4, c->stream));
     k_absent_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, has, removed, small + 7);
     k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
-    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
+    k_apply_removed_cells<<<4096, 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
     int32_t* selfm = removed;
     k_init_lmap<<<nbk(nv), 256, 0, c->stream>>>(nv, lmap, mnext, mtail, selfm);
     AV_CHECK_HIP(c, hipMemsetAsync(small + 3, 0, 8, c->stream));  // [3] merges, [4] cell pairs
@@ -1459,7 +1460,7 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     AV_CHECK_HIP(c, hipMemsetAsync(small + 7, 0, 4, c->stream));
     k_mark_removed<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, ncell, 1, g.dx, rmin3, removed, VOLNEW, small + 7);
     k_apply_removed_cubes<<<nbk(nv), 256, 0, c->stream>>>(nv, removed, uvoid, ncell, VOLNEW);
-    k_apply_removed_cells<<<nbk(ne3), 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
+    k_apply_removed_cells<<<4096, 256, 0, c->stream>>>(ne3, removed, small + 7, marca);
     AV_CHECK_HIP(c, hipMemsetAsync(small + 4, 0, 4, c->stream));
     k_req<<<nbk(nv), 256, 0, c->stream>>>(nv, uvoid, VOLNEW, REQ, small + 4);
     AV_CHECK_HIP(c, hipGetLastError());

@@ -104,3 +104,35 @@ def test_masked_grid_vs_oracle_and_learned(n, per, seed):
                         meandens=f2["meandens"])
     out2 = gpu_void_outputs(c2)
     compare(out2, lambda k: _as_ref(ref2)[k])
+
+
+def test_uneven_masked_patches():
+    """Survey-like patches of uneven size through one context (grids 32^3 to
+    64^3, Zel'dovich particles at 0.5 to 2 per cell, 20% masked,
+    non-periodic): each patch's void stage on its GPU fields equals the
+    oracle's, and masked cells carry no velocity."""
+    from avism_amd import synth
+    from avism_amd._lib import Context
+    rng = np.random.default_rng(3000)
+    ctx = Context(0)  # one context, reused across patches of different sizes
+    for i in range(6):
+        n = int(rng.choice([32, 40, 48, 64]))
+        side = int(round(n * rng.uniform(0.8, 1.26)))
+        L0 = 2.0 * n
+        p = synth.zeldovich(side, L0, seed=3000 + i)
+        arr = p.numpy()
+        sm = synth.survey_mask(n, 0.2, seed=3000 + i)
+        ctx.set_grid(n, L0, 0)
+        ctx.set_cosmology(0.678, 0.31, 0.0)
+        ctx.set_void_params(PARAMS.dens_thre, PARAMS.dens_thre2, PARAMS.grad_thre, PARAMS.div_thre, PARAMS.rmin,
+                            PARAMS.rmin_sub, PARAMS.nvoid_max)
+        ctx.set_mask(sm)
+        ctx.load_particles(*arr)
+        ctx.particles_to_grid(k=32)
+        f = ctx.fields()
+        out = gpu_void_outputs(ctx)
+        ref = O.void_stage(O.Grid(n, L0, 0), f["u1co"], f["div"], PARAMS, smask=sm, meandens=ctx.stats().meandens,
+                           rete=O.cosmo(0.678, 0.31, 0.0)[1])
+        print(f"[patch {i}] n={n} np={side ** 3} cubes={out['nvoid']} voids={out['nvoidt']}", flush=True)
+        compare(out, lambda k: _as_ref(ref)[k])
+        assert np.all(f["u"][sm == 0] == 0.0)
