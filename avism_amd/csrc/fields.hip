@@ -506,21 +506,23 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
   const int tx0 = (int)(bt % ntx) * DTX, ty0 = (int)((bt / ntx) % nty) * DTY, tz0 = (int)(bt / ((int64_t)ntx * nty)) * DTZ;
   const int tid = threadIdx.x;
   for (int c = tid; c < DBX * DBY * DBZ; c += 256) box[c] = 0ull;
-  if (tid == 0) {  // the tile's particles: one contiguous range per tile row (bins = cells)
+  if (tid == 0) {  // the tile's particles: one contiguous range per row of bins (a bin is s^3 cells)
+    const int nb = B.nb, s = n / nb;
+    const int bty = DTY / s, btz = DTZ / s, bx0 = tx0 / s, by0 = ty0 / s, bz0 = tz0 / s;
     int accn = 0;
-    for (int r = 0; r < DTY * DTZ; ++r) {
-      const int y = ty0 + r % DTY, z = tz0 + r / DTY;
+    for (int r = 0; r < bty * btz; ++r) {
+      const int y = by0 + r % bty, z = bz0 + r / bty;
       int cnt = 0, b0 = 0;
-      if (y < n && z < n) {
-        const int rowb = n * (y + n * z);
-        b0 = B.start[rowb + tx0];
-        cnt = B.start[rowb + min(tx0 + DTX, n)] - b0;
+      if (y < nb && z < nb) {
+        const int rowb = nb * (y + nb * z);
+        b0 = B.start[rowb + bx0];
+        cnt = B.start[rowb + min(bx0 + DTX / s, nb)] - b0;
       }
       qrow[r] = accn;
       qbeg[r] = b0;
       accn += cnt;
     }
-    qrow[DTY * DTZ] = accn;
+    for (int r = bty * btz; r <= DTY * DTZ; ++r) qrow[r] = accn;
   }
   __syncthreads();
   // widened tile in 1-based cells
@@ -794,7 +796,12 @@ int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, con
 static int build_bins(avism_ctx* c) {
   StageTimer t(c, "bin");
   const int64_t np = c->np;
-  int nb = c->g.n;  // one bin per grid cell
+  // bins of s^3 cells (s = 1, 2 or 4, dividing n and the DDENS tile): about one
+  // particle per bin, also on the finer grids of void-in-void levels
+  const double ppc = (double)np / ((double)c->g.n * c->g.n * c->g.n);
+  int s = 1;
+  while (s < 4 && c->g.n % (2 * s) == 0 && (double)(8 * s * s * s) * ppc <= 1.5) s *= 2;
+  int nb = c->g.n / s;
   if (nb < 1) nb = 1;
   c->nb = nb;
   c->bsz = c->g.L0 / (float)nb;
