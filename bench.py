@@ -120,8 +120,8 @@ def cpu_baseline(n_side: int, box_per_cell: float, k: int):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU layout is "
-              "described in DESIGN.md and is not part of this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this library drives a single "
+              "device, and a sharded multi-GPU run is not implemented.", file=sys.stderr)
         sys.exit(2)
     import torch
     from avism_amd import synth
