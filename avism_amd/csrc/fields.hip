@@ -21,6 +21,7 @@
 #include <cstring>
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include <rocprim/rocprim.hpp>
 
@@ -182,7 +183,9 @@ __global__ void k_gather_sorted(int64_t np, const uint32_t* perm, const float* x
 // masses, self included) / (4/3 pi d_k^3); d_k is kept for FILL_H_ZEROS.
 // One particle per lane, in bin-sorted order (a wave's queries are neighbours).
 // The search returns input indices, so the masses are read in input order.
-constexpr int KQ = 128;  // threads per block of the kNN kernels
+// threads per block of the kNN kernels: a lane keeps its k selection slots
+// in LDS (5K/4 words), so K = 128 and 256 run 64 lanes a block (40 and 80 KB)
+constexpr int kq_of(int K) { return K <= 64 ? 128 : 64; }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
 // and each XCD has its own L2.  Renumbering the blocks so that every XCD gets
@@ -201,7 +204,7 @@ template <int PER, int K>
 __device__ __forceinline__ void ppart_one(const Bins& B, int64_t np, int k, float r0, int uniform, int64_t i,
                                           uint32_t* keys, uint32_t* gmx, float* pdens, float* hk) {
   const float4 pq = B.pos[i];
-  Sel<KQ, K / 4> h;
+  Sel<kq_of(K), K / 4> h;
   h.init(keys, gmx, k);
   bool scanned;
   const float dk2 = knn_kth<PER>(B, np, pq.x, pq.y, pq.z, r0, k, h, scanned);
@@ -218,10 +221,11 @@ __device__ __forceinline__ void ppart_one(const Bins& B, int64_t np, int k, floa
 }
 
 template <int K>
-__global__ void __launch_bounds__(KQ) k_ppart(Bins B, int64_t np, int k, float r0, int uniform, float* pdens,
-                                             float* hk) {
-  __shared__ uint32_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
-  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * KQ + threadIdx.x;
+__global__ void __launch_bounds__(kq_of(K)) k_ppart(Bins B, int64_t np, int k, float r0, int uniform,
+                                                   float* pdens, float* hk) {
+  constexpr int BQ = kq_of(K);
+  __shared__ uint32_t keys_s[K * BQ], gmx_s[K / 4 * BQ];
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * BQ + threadIdx.x;
   if (i >= np) return;
   if (B.periodic)
     ppart_one<1, K>(B, np, k, r0, uniform, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
@@ -243,7 +247,7 @@ struct VvelOut {
 template <int PER, int K>
 __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float r0, float dx, float qx, float qy,
                                          float qz, size_t c, uint32_t* keys, uint32_t* gmx, const VvelOut& O) {
-  Sel<KQ, K / 4> h;
+  Sel<kq_of(K), K / 4> h;
   h.init(keys, gmx, k);
   bool scanned;
   const float dk2 = knn_kth<PER>(B, np, qx, qy, qz, r0, k, h, scanned);
@@ -285,14 +289,16 @@ __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float
 }
 
 template <int K>
-__global__ void __launch_bounds__(KQ) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad, int rlo,
-                                            float dx, int k, float r0, const int8_t* __restrict__ smask, VvelOut O) {
-  __shared__ uint32_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
+__global__ void __launch_bounds__(kq_of(K)) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad,
+                                                  int rlo, float dx, int k, float r0,
+                                                  const int8_t* __restrict__ smask, VvelOut O) {
+  constexpr int BQ = kq_of(K), TZ = BQ / 32;  // tile of 8 x 4 x TZ cells
+  __shared__ uint32_t keys_s[K * BQ], gmx_s[K / 4 * BQ];
   const int ntx = (n + 7) / 8, nty = (n + 3) / 4;
   const int tid = threadIdx.x;
   const int64_t bt = xcd_block(blockIdx.x, gridDim.x);
   const int ix = (int)(bt % ntx) * 8 + tid % 8, iy = (int)((bt / ntx) % nty) * 4 + (tid / 8) % 4,
-            iz = (int)(bt / ((int64_t)ntx * nty)) * 4 + tid / 32;
+            iz = (int)(bt / ((int64_t)ntx * nty)) * TZ + tid / 32;
   if (ix >= n || iy >= n || iz >= n) return;
   const size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
   if (smask && smask[c] == 0) return;
@@ -432,12 +438,13 @@ __global__ void k_tsc_vel_finalize(int64_t ncell, const double* __restrict__ acc
 // FILL_H_ZEROS (particles.f90:398-431) when no PPART query ran (TSC
 // velocities): h = the k-th neighbour distance of each particle with h = 0
 template <int PER, int K>
-__global__ void __launch_bounds__(KQ) k_fill_h_knn(Bins B, int64_t np, int k, float r0, float* hpart) {
-  __shared__ uint32_t keys_s[K * KQ], gmx_s[K / 4 * KQ];
-  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * KQ + threadIdx.x;
+__global__ void __launch_bounds__(kq_of(K)) k_fill_h_knn(Bins B, int64_t np, int k, float r0, float* hpart) {
+  constexpr int BQ = kq_of(K);
+  __shared__ uint32_t keys_s[K * BQ], gmx_s[K / 4 * BQ];
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * BQ + threadIdx.x;
   if (i >= np || hpart[i] > 0.0f) return;
   const float4 pq = B.pos[i];
-  Sel<KQ, K / 4> h;
+  Sel<BQ, K / 4> h;
   h.init(keys_s + threadIdx.x, gmx_s + threadIdx.x, k);
   bool scanned;
   hpart[i] = sqrtf(knn_kth<PER>(B, np, pq.x, pq.y, pq.z, r0, k, h, scanned));
@@ -850,10 +857,30 @@ static int build_bins(avism_ctx* c) {
   return AVISM_OK;
 }
 
+// f(std::integral_constant<int, K>) for the smallest K in {32, 64, 128, 256}
+// with k <= K (k <= 256 checked by the caller): the selection slots per lane
+template <class F>
+static void with_k(int k, F&& f) {
+  if (k <= 32)
+    f(std::integral_constant<int, 32>{});
+  else if (k <= 64)
+    f(std::integral_constant<int, 64>{});
+  else if (k <= 128)
+    f(std::integral_constant<int, 128>{});
+  else
+    f(std::integral_constant<int, 256>{});
+}
+
+// blocks of k_vvel: tiles of 8 x 4 x (lanes / 32) cells
+static unsigned vvel_tiles(int n, int K) {
+  const int tz = kq_of(K) / 32;
+  return (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + tz - 1) / tz));
+}
+
 int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp) {
   if (c->np <= 0) AV_FAIL(c, AVISM_ERR_STATE, "no particles loaded");
   if (k <= 0) AV_FAIL(c, AVISM_ERR_INPUT, "KNEIGHBOURS must be positive");
-  if (k > 64) AV_FAIL(c, AVISM_ERR_UNSUPPORTED, "KNEIGHBOURS above 64 is not supported by this build");
+  if (k > 256) AV_FAIL(c, AVISM_ERR_UNSUPPORTED, "KNEIGHBOURS above 256 is not supported by this build");
   if (k > c->np) AV_FAIL(c, AVISM_ERR_INPUT, "KNEIGHBOURS exceeds the number of particles");
   // FLAG_VEL_INTERP / FLAG_DENS_INTERP: 0 = TSC, 1 = SPH (avism.f90:1178-1179, 1220-1221)
   if ((vel_interp != 0 && vel_interp != 1) || (dens_interp != 0 && dens_interp != 1))
@@ -884,12 +911,11 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   if (vel_interp == 1) {
     {
       StageTimer t(c, "ppart_dens");
-      if (k <= 32)
-        k_ppart<32><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->mass_uniform ? 1 : 0,
-                                                         c->pdens.as<float>(), c->hk.as<float>());
-      else
-        k_ppart<64><<<nblk(np, KQ), KQ, 0, c->stream>>>(B, np, k, r0, c->mass_uniform ? 1 : 0,
-                                                         c->pdens.as<float>(), c->hk.as<float>());
+      with_k(k, [&](auto KC) {
+        constexpr int K = decltype(KC)::value;
+        k_ppart<K><<<nblk(np, kq_of(K)), kq_of(K), 0, c->stream>>>(B, np, k, r0, c->mass_uniform ? 1 : 0,
+                                                                   c->pdens.as<float>(), c->hk.as<float>());
+      });
       AV_CHECK_HIP(c, hipGetLastError());
     }
     k_velrec<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), c->spos4.as<float4>(),
@@ -897,7 +923,6 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
                                               c->pvz.as<float>(), c->vel4.as<float4>());
     {
       StageTimer t(c, "vvel_sph_vw");
-      const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
       const VvelOut O{c->vel4.as<float4>(), c->hpart.as<uint32_t>(), c->fu.as<float>(), c->fv.as<float>(),
                       c->fw.as<float>(), 1};
       if (smask) {  // masked cells keep zero velocity (particles.f90:856)
@@ -905,10 +930,10 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
         AV_CHECK_HIP(c, hipMemsetAsync(c->fv.p, 0, (size_t)ncell * 4, c->stream));
         AV_CHECK_HIP(c, hipMemsetAsync(c->fw.p, 0, (size_t)ncell * 4, c->stream));
       }
-      if (k <= 32)
-        k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
-      else
-        k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
+      with_k(k, [&](auto KC) {
+        constexpr int K = decltype(KC)::value;
+        k_vvel<K><<<vvel_tiles(n, K), kq_of(K), 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
+      });
       AV_CHECK_HIP(c, hipGetLastError());
     }
     {
@@ -930,22 +955,15 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   }
   if (dens_interp == 1 && vel_interp == 0) {  // H_DISTANCE (particles.f90:276-351), then FILL_H_ZEROS
     StageTimer t(c, "h_distance");
-    const unsigned ntiles = (unsigned)(((n + 7) / 8) * ((n + 3) / 4) * ((n + 3) / 4));
     const VvelOut O{nullptr, c->hpart.as<uint32_t>(), nullptr, nullptr, nullptr, 0};
-    const unsigned nbq = nblk(np, KQ);
-    if (k <= 32) {
-      k_vvel<32><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
+    with_k(k, [&](auto KC) {
+      constexpr int K = decltype(KC)::value, BQ = kq_of(K);
+      k_vvel<K><<<vvel_tiles(n, K), BQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
       if (c->g.periodic)
-        k_fill_h_knn<1, 32><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
+        k_fill_h_knn<1, K><<<nblk(np, BQ), BQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
       else
-        k_fill_h_knn<0, 32><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
-    } else {
-      k_vvel<64><<<ntiles, KQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
-      if (c->g.periodic)
-        k_fill_h_knn<1, 64><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
-      else
-        k_fill_h_knn<0, 64><<<nbq, KQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
-    }
+        k_fill_h_knn<0, K><<<nblk(np, BQ), BQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
+    });
     AV_CHECK_HIP(c, hipGetLastError());
   }
   DdensGeom G{rad, c->g.rlo, n, c->g.dx, c->g.rx1};

@@ -131,6 +131,21 @@ def mask_fixtures(td: Path):
               dict(n=n, L0=L0, periodic=per, mode=2, seed=seed))
 
 
+def kbig_fixtures(td: Path):
+    """Neighbour counts above 64: the reference's default KNEIGHBOURS = 256
+    (periodic Zel'dovich, 4096 particles) and k = 100 (mock voids)."""
+    p = synth.zeldovich(16, 40.0, seed=24, rms_disp=2.0)
+    arr = p.numpy()
+    ref = O.run_reference(td / "k256", 1, 16, 40.0, 1, PARAMS, k=256, parts=arr)
+    _save("part_k256_p1", dict(zip(("x", "y", "z", "vx", "vy", "vz", "m"), arr)), ref,
+          dict(n=16, L0=40.0, periodic=1, mode=1, k=256, seed=24))
+    p, _, _ = synth.mock_voids(n0=5000, L0=60.0, nvoids=4, rmin=6.0, rmax=12.0, seed=25)
+    arr = p.numpy()
+    ref = O.run_reference(td / "k100", 1, 20, 60.0, 0, PARAMS, k=100, parts=arr)
+    _save("part_k100_p0", dict(zip(("x", "y", "z", "vx", "vy", "vz", "m"), arr)), ref,
+          dict(n=20, L0=60.0, periodic=0, mode=1, k=100, seed=25))
+
+
 def main():
     for per in (0, 1):
         if not O.ref_available(per):
@@ -146,9 +161,13 @@ def main():
         if sys.argv[1:] == ["mask"]:
             mask_fixtures(td)
             return
+        if sys.argv[1:] == ["kbig"]:
+            kbig_fixtures(td)
+            return
         tsc_fixtures(td)
         lev_fixtures(td)
         mask_fixtures(td)
+        kbig_fixtures(td)
         for per, seed in ((0, 11), (1, 12)):
             n, L0 = 24, 48.0
             d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, seed))

@@ -44,7 +44,7 @@ def _gpu_fields(arr, n, L0, per, k, vel_interp=1, dens_interp=1):
     return c, f, pd, hp
 
 
-@pytest.mark.parametrize("name", ["part_p0", "part_p1"])
+@pytest.mark.parametrize("name", ["part_p0", "part_p1", "part_k100_p0", "part_k256_p1"])
 def test_fields_vs_reference_fixture(golden_dir, name):
     g = dict(np.load(golden_dir / f"{name}.npz"))
     n, L0, per, k = (int(g["meta_n"]), float(g["meta_L0"]), int(g["meta_periodic"]), int(g["meta_k"]))
@@ -127,6 +127,44 @@ def test_tsc_vs_reference_fixture(golden_dir, name):
         assert _rms_rel(f[c], g[f"ref_{c}"]) <= REL, c
     assert _rel(f["u1co"], g["ref_u1co"]) <= REL
     assert _rms_rel(f["div"], g["ref_div"]) <= 1e-4
+
+
+@pytest.mark.parametrize("k,per,vi,di", [(33, 0, 1, 1), (65, 1, 1, 1), (128, 0, 1, 1), (256, 1, 1, 1),
+                                         (256, 1, 0, 1), (200, 0, 0, 1)])
+def test_fields_large_k_vs_oracle(k, per, vi, di):
+    """Each neighbour-count class (k up to 64, 128, 256 selection slots; the
+    reference's default KNEIGHBOURS is 256) on SPH velocities and on
+    H_DISTANCE with TSC velocities, at 24^3 with 13824 particles."""
+    import pyoracle as O
+    from avism_amd import synth
+    n, L0 = 24, 48.0
+    p = synth.zeldovich(24, L0, seed=70 + k)
+    arr = list(p.numpy())
+    if not per:
+        keep = np.all(np.abs(np.stack(arr[:3])) < L0 / 2 - 2 * L0 / n, axis=0)
+        arr = [a[keep].copy() for a in arr]
+    print(f"[fields k={k} per={per} vel={vi} dens={di}] np={arr[0].size}", flush=True)
+    ref = O.fields_from_particles(*arr, n, L0, per, k=k, vel_interp=vi, dens_interp=di, tsc_exact=True)
+    _, f, pd, hp = _gpu_fields(arr, n, L0, per, k, vi, di)
+    assert np.array_equal(hp, ref["hpart"])
+    if vi == 1:
+        assert _rel(pd, ref["part_dens"]) <= REL
+    for c in ("u", "v", "w"):
+        assert _rms_rel(f[c], ref[c]) <= REL, c
+    assert _rel(f["u1co"], ref["u1co"]) <= REL
+    assert _rms_rel(f["div"], ref["div"]) <= 1e-4
+
+
+def test_k_above_256_rejected():
+    from avism_amd._lib import AvismError
+    from avism_amd import synth
+    arr = synth.zeldovich(8, 16.0, seed=5).numpy()
+    c = _ctx()
+    c.set_grid(8, 16.0, 1)
+    c.set_cosmology(0.678, 0.31, 0.0)
+    c.load_particles(*arr)
+    with pytest.raises(AvismError):
+        c.particles_to_grid(k=257)
 
 
 @pytest.mark.parametrize("per,vi,di", [(1, 0, 0), (0, 0, 0), (1, 0, 1), (1, 1, 0)])

@@ -59,7 +59,7 @@ def test_grid_input_matches_reference(golden_dir, name):
     compare_void_outputs(res, lambda k: g[f"ref_{k}"])
 
 
-@pytest.mark.parametrize("name", ["part_p0", "part_p1"])
+@pytest.mark.parametrize("name", ["part_p0", "part_p1", "part_k100_p0", "part_k256_p1"])
 def test_particle_input_matches_reference(golden_dir, name):
     g = _load(golden_dir, name)
     n, L0, per, k = (int(g["meta_n"]), float(g["meta_L0"]), int(g["meta_periodic"]), int(g["meta_k"]))
