@@ -52,6 +52,7 @@ EXPORTS = [
     "avism_set_grid", "avism_get_grid", "avism_set_cosmology", "avism_set_void_params",
     "avism_set_parent_map", "avism_set_parent_radii", "avism_set_mask", "avism_next_level", "avism_load_particles",
     "avism_particles_to_grid",
+    "avism_particles_to_density",
     "avism_load_grid_fields", "avism_load_void_fields", "avism_get_fields", "avism_get_particle_fields", "avism_find_voids",
     "avism_set_proposals", "avism_get_candidates", "avism_get_cubes", "avism_indexx",
     "avism_merge_and_post", "avism_get_catalogue", "avism_get_marca", "avism_run_level",
@@ -94,6 +95,7 @@ def lib():
         "avism_next_level": ([vp], C.c_int),
         "avism_load_particles": ([vp, i64, vp, vp, vp, vp, vp, vp, vp, i32], C.c_int),
         "avism_particles_to_grid": ([vp, i32, i32, i32], C.c_int),
+        "avism_particles_to_density": ([vp, i32, i32], C.c_int),
         "avism_load_grid_fields": ([vp, vp, vp, vp, vp, i32], C.c_int),
         "avism_load_void_fields": ([vp, vp, vp, f32, i32], C.c_int),
         "avism_get_fields": ([vp, vp, vp, vp, vp, vp, i32], C.c_int),
@@ -228,19 +230,25 @@ class Context:
 
     # ---- field stage
     def load_particles(self, x, y, z, vx, vy, vz, m, where=None):
+        """vx, vy, vz may all be None: particles without velocities."""
         arrs = [x, y, z, vx, vy, vz, m]
         if where is None:
             where = AVISM_HOST if isinstance(x, np.ndarray) else (
                 AVISM_DEVICE if getattr(x, "is_cuda", False) else AVISM_HOST)
         if where == AVISM_HOST:
-            arrs = [np.ascontiguousarray(a if isinstance(a, np.ndarray) else a.numpy(), np.float32)
-                    for a in arrs]
+            arrs = [None if a is None else np.ascontiguousarray(a if isinstance(a, np.ndarray) else a.numpy(),
+                                                                np.float32) for a in arrs]
         n = int(arrs[0].shape[0]) if isinstance(arrs[0], np.ndarray) else int(arrs[0].numel())
         self._keep = arrs
-        self._chk(lib().avism_load_particles(self._h, n, *[_ptr(a) for a in arrs], int(where)))
+        self._chk(lib().avism_load_particles(self._h, n, *[None if a is None else _ptr(a) for a in arrs],
+                                             int(where)))
 
     def particles_to_grid(self, k=32, vel_interp=1, dens_interp=1):
         self._chk(lib().avism_particles_to_grid(self._h, int(k), int(vel_interp), int(dens_interp)))
+
+    def particles_to_density(self, k=32, dens_interp=1):
+        """FLAG_VEL_AVAILABLE = 0: density only, DIVERCO from linear theory."""
+        self._chk(lib().avism_particles_to_density(self._h, int(k), int(dens_interp)))
 
     def load_grid_fields(self, delta, vx, vy, vz):
         arrs = [np.ascontiguousarray(a, np.float32) for a in (delta, vx, vy, vz)]

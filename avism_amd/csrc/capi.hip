@@ -227,13 +227,20 @@ int avism_load_particles(avism_ctx* c, int64_t np, const float* x, const float* 
                          int32_t where) {
   CTX_GUARD(c);
   DEVICE_ARGS(c, where);
-  if (!x || !y || !z || !vx || !vy || !vz || !m) AV_FAIL(c, AVISM_ERR_ARG, "null particle array");
+  if (!x || !y || !z || !m) AV_FAIL(c, AVISM_ERR_ARG, "null particle array");
+  if ((vx == nullptr) != (vy == nullptr) || (vx == nullptr) != (vz == nullptr))
+    AV_FAIL(c, AVISM_ERR_ARG, "velocities: all three arrays or none");
   return load_particles(c, np, x, y, z, vx, vy, vz, m, where);
 }
 
 int avism_particles_to_grid(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp) {
   CTX_GUARD(c);
   return fields_from_particles(c, k, vel_interp, dens_interp);
+}
+
+int avism_particles_to_density(avism_ctx* c, int32_t k, int32_t dens_interp) {
+  CTX_GUARD(c);
+  return fields_from_particles(c, k, -1, dens_interp);
 }
 
 int avism_load_grid_fields(avism_ctx* c, const float* delta, const float* vx, const float* vy,

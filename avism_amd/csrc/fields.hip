@@ -733,6 +733,18 @@ __global__ void k_div_finalize(int n, float dx, int periodic, const float* __res
   if (umass) u1co[c] = umass[c] / (*mean);
 }
 
+// velocities not available (FLAG_DIV = -1, avism.f90:1259-1263, 1646-1650):
+// U1CO = U1 / MEANDENS and DIVERCO = -(OMEGA0^0.6 RETE HTE) (U1CO - 1), the
+// constant formed on the host in the same FP32 order
+__global__ void k_div_linear(int64_t ncell, float lin, const float* __restrict__ umass,
+                             const float* __restrict__ mean, float* u1co, float* div) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const float u = umass[c] / (*mean);
+  u1co[c] = u;
+  div[c] = -(lin * (u - 1.0f));
+}
+
 __global__ void k_grid_input(int64_t ncell, const float* delta, const float* vx, const float* vy,
                              const float* vz, float* u1co, float* u, float* v, float* w) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -777,7 +789,10 @@ int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, con
   hipMemcpyKind kind = where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   for (int i = 0; i < 7; i++) {
     if (!bufs[i]->ensure(b)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (particles)");
-    AV_CHECK_HIP(c, hipMemcpyAsync(bufs[i]->p, src[i], b, kind, c->stream));
+    if (src[i])
+      AV_CHECK_HIP(c, hipMemcpyAsync(bufs[i]->p, src[i], b, kind, c->stream));
+    else  // no velocities (READ_BINARY_PART_2 data): zero, as the reader leaves them
+      AV_CHECK_HIP(c, hipMemsetAsync(bufs[i]->p, 0, b, c->stream));
   }
   c->np = np;
   c->fields_ready = false;
@@ -883,7 +898,8 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   if (k > 256) AV_FAIL(c, AVISM_ERR_UNSUPPORTED, "KNEIGHBOURS above 256 is not supported by this build");
   if (k > c->np) AV_FAIL(c, AVISM_ERR_INPUT, "KNEIGHBOURS exceeds the number of particles");
   // FLAG_VEL_INTERP / FLAG_DENS_INTERP: 0 = TSC, 1 = SPH (avism.f90:1178-1179, 1220-1221)
-  if ((vel_interp != 0 && vel_interp != 1) || (dens_interp != 0 && dens_interp != 1))
+  // vel_interp -1: FLAG_VEL_AVAILABLE = 0 (avism_particles_to_density)
+  if ((vel_interp != -1 && vel_interp != 0 && vel_interp != 1) || (dens_interp != 0 && dens_interp != 1))
     AV_FAIL(c, AVISM_ERR_INPUT, "FLAG_VEL_INTERP and FLAG_DENS_INTERP must be 0 (TSC) or 1 (SPH)");
   const int64_t np = c->np;
   const int n = c->g.n;
@@ -941,6 +957,11 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
       k_fill_h<<<nblk(np), 256, 0, c->stream>>>(np, c->hpart.as<float>(), c->hk.as<float>());
       AV_CHECK_HIP(c, hipGetLastError());
     }
+  } else if (vel_interp == -1) {  // no velocities: U2..U4 stay 0, as PART_DENS
+    AV_CHECK_HIP(c, hipMemsetAsync(c->pdens.p, 0, (size_t)np * 4, c->stream));
+    AV_CHECK_HIP(c, hipMemsetAsync(c->fu.p, 0, (size_t)ncell * 4, c->stream));
+    AV_CHECK_HIP(c, hipMemsetAsync(c->fv.p, 0, (size_t)ncell * 4, c->stream));
+    AV_CHECK_HIP(c, hipMemsetAsync(c->fw.p, 0, (size_t)ncell * 4, c->stream));
   } else {  // VEL_INTERP_TSC; PART_DENS stays 0 as in the reference
     if (!c->tsc_acc.ensure((size_t)ncell * 32)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (TSC)");
     AV_CHECK_HIP(c, hipMemsetAsync(c->pdens.p, 0, (size_t)np * 4, c->stream));
@@ -953,7 +974,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
                                                            c->fv.as<float>(), c->fw.as<float>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
-  if (dens_interp == 1 && vel_interp == 0) {  // H_DISTANCE (particles.f90:276-351), then FILL_H_ZEROS
+  if (dens_interp == 1 && vel_interp != 1) {  // H_DISTANCE (particles.f90:276-351), then FILL_H_ZEROS
     StageTimer t(c, "h_distance");
     const VvelOut O{nullptr, c->hpart.as<uint32_t>(), nullptr, nullptr, nullptr, 0};
     with_k(k, [&](auto KC) {
@@ -1008,11 +1029,17 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     k_reduce_mean<<<kRedSlices, 256, 0, c->stream>>>(c->partial64.as<double>(), nparts, ncell, slices, 0, dsum,
                                                      dmean);
     k_reduce_mean<<<1, 256, 0, c->stream>>>(slices, kRedSlices, ncell, nullptr, 1, dsum, dmean);
-    dim3 g2((n + 255) / 256, n, n);
-    k_div_finalize<<<g2, 256, 0, c->stream>>>(n, c->g.dx, c->g.periodic, c->fu.as<float>(),
-                                              c->fv.as<float>(), c->fw.as<float>(),
-                                              c->u_mass.as<float>(), dmean, c->u1co.as<float>(),
-                                              c->div.as<float>());
+    if (vel_interp == -1) {
+      const float lin = (powf(c->omega0, 0.6f) * c->rete) * c->hte;  // avism.f90:1648
+      k_div_linear<<<nblk(ncell), 256, 0, c->stream>>>(ncell, lin, c->u_mass.as<float>(), dmean,
+                                                      c->u1co.as<float>(), c->div.as<float>());
+    } else {
+      dim3 g2((n + 255) / 256, n, n);
+      k_div_finalize<<<g2, 256, 0, c->stream>>>(n, c->g.dx, c->g.periodic, c->fu.as<float>(),
+                                                c->fv.as<float>(), c->fw.as<float>(),
+                                                c->u_mass.as<float>(), dmean, c->u1co.as<float>(),
+                                                c->div.as<float>());
+    }
     AV_CHECK_HIP(c, hipGetLastError());
   }
   float hmean = 0;

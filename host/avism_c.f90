@@ -80,6 +80,12 @@ module avism_c
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: k, vel_interp, dens_interp
     end function
+    integer(c_int) function avism_particles_to_density(ctx, k, dens_interp) &
+        bind(C, name="avism_particles_to_density")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: k, dens_interp
+    end function
     integer(c_int) function avism_load_grid_fields(ctx, delta, vx, vy, vz, where) &
         bind(C, name="avism_load_grid_fields")
       import :: c_ptr, c_int, c_int32_t, c_float

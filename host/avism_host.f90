@@ -106,9 +106,7 @@ program avism_host
         end if
       else
         call avism_check(ctx, avism_next_level(ctx), 'avism_next_level')
-        call avism_check(ctx, avism_particles_to_grid(ctx, int(kneighbours, c_int32_t), &
-                                                   int(flag_vel_interp, c_int32_t), &
-                                                   int(flag_dens_interp, c_int32_t)), 'avism_particles_to_grid')
+        call particle_fields()
       end if
       call avism_check(ctx, avism_find_voids(ctx, growth, ncubes), 'avism_find_voids')
       call avism_check(ctx, avism_merge_and_post(ctx, nvoids), 'avism_merge_and_post')
@@ -184,8 +182,12 @@ contains
       write(*,*) '     ERROR: only FLAG_DATA = 1 (BinPart) and 2 (BinGrid) are supported by this build'
       stop 1
     end if
-    if (flag_survey /= 0 .or. flag_vel_available /= 1) then
-      write(*,*) '     ERROR: survey mode and velocity-free input are not supported by this build'
+    if (flag_survey /= 0) then
+      write(*,*) '     ERROR: survey mode is not supported by this build'
+      stop 1
+    end if
+    if (flag_vel_available == 0 .and. flag_data /= 1) then
+      write(*,*) '     ERROR: velocity-free input (FLAG_VEL_AVAILABLE = 0) needs particle input in this build'
       stop 1
     end if
     if (flag_data == 2 .and. (nhyx /= ncox .or. nhyy /= ncoy .or. nhyz /= ncoz)) then
@@ -220,18 +222,35 @@ contains
     read(41) x
     read(41) y
     read(41) zz
-    read(41) vx
-    read(41) vy
-    read(41) vz
+    if (flag_vel_available == 1) then  ! READ_BINARY_PART_1 (reader.f90:447-456)
+      read(41) vx
+      read(41) vy
+      read(41) vz
+    else  ! READ_BINARY_PART_2 (reader.f90:506-509): no velocity records
+      vx = 0.0
+      vy = 0.0
+      vz = 0.0
+    end if
     read(41) m
     close(41)
     write(*,*) '     TOTAL PARTICLES IN ITER=', np
     call avism_check(ctx, avism_set_cosmology(ctx, ache, omega0, z), 'avism_set_cosmology')
     call avism_check(ctx, avism_load_particles(ctx, int(np, c_int64_t), x, y, zz, vx, vy, vz, m, AVISM_HOST), &
                   'avism_load_particles')
-    call avism_check(ctx, avism_particles_to_grid(ctx, int(kneighbours, c_int32_t), &
-                                               int(flag_vel_interp, c_int32_t), &
-                                               int(flag_dens_interp, c_int32_t)), 'avism_particles_to_grid')
+    call particle_fields()
+  end subroutine
+
+  ! the field stage on the loaded particles: with velocities (avism.f90:1159-1257)
+  ! or without (FLAG_VEL_AVAILABLE = 0, avism.f90:1259-1300, 1646-1650)
+  subroutine particle_fields()
+    if (flag_vel_available == 1) then
+      call avism_check(ctx, avism_particles_to_grid(ctx, int(kneighbours, c_int32_t), &
+                                                 int(flag_vel_interp, c_int32_t), &
+                                                 int(flag_dens_interp, c_int32_t)), 'avism_particles_to_grid')
+    else
+      call avism_check(ctx, avism_particles_to_density(ctx, int(kneighbours, c_int32_t), &
+                                                    int(flag_dens_interp, c_int32_t)), 'avism_particles_to_density')
+    end if
   end subroutine
 
   subroutine run_grid(it, z)

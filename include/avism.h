@@ -123,7 +123,8 @@ int avism_next_level(avism_ctx* ctx);
 /* READ_BINARY_PART_1 data (reader.f90:410-465): positions Mpc/h, velocities
  * km/s, masses Msun.  Units are converted as the reader does (m/UM, v/UV).
  * Fails with AVISM_ERR_INPUT for particles outside [-L0/2, L0/2]
- * (avism.f90:560-589). */
+ * (avism.f90:560-589).  vx, vy, vz may all be NULL (READ_BINARY_PART_2 data,
+ * reader.f90:468-519: no velocities; they are taken as 0). */
 int avism_load_particles(avism_ctx* ctx, int64_t np, const float* x, const float* y,
                          const float* z, const float* vx, const float* vy, const float* vz,
                          const float* m, int32_t where);
@@ -132,6 +133,12 @@ int avism_load_particles(avism_ctx* ctx, int64_t np, const float* x, const float
  * (avism.f90:1177-1257).  vel_interp, dens_interp: FLAG_VEL_INTERP and
  * FLAG_DENS_INTERP of voids.dat, 1 = SPH, 0 = TSC. */
 int avism_particles_to_grid(avism_ctx* ctx, int32_t k, int32_t vel_interp, int32_t dens_interp);
+/* FLAG_VEL_AVAILABLE = 0 (avism.f90:1259-1300, 1646-1650): the density only,
+ * by DENS_INTERP_TSC (dens_interp 0) or H_DISTANCE, FILL_H_ZEROS and
+ * DDENS_INTERP_SPH (dens_interp 1); then the mean density, and DIVERCO from
+ * linear theory, -OMEGA0^0.6 RETE HTE (U1CO - 1).  The velocity fields read
+ * back as 0. */
+int avism_particles_to_density(avism_ctx* ctx, int32_t k, int32_t dens_interp);
 /* READ_BINARY_GRID (reader.f90:523-552): delta and velocities (km/s) on the
  * inner grid; then DIVER_UNIFORM.  Used for grid input (FLAG_DATA=2). */
 int avism_load_grid_fields(avism_ctx* ctx, const float* delta, const float* vx, const float* vy,

@@ -610,6 +610,14 @@ void orc_vel_tsc(const orc_grid* g, int64_t np, const float* x, const float* y,
     free(a8);
 }
 
+/* Velocities not available (FLAG_DIV = -1, avism.f90:1646-1650):
+ * DIVERCO = -OMEGA0**0.6 * RETE * HTE * (U1CO - 1), Fortran's left-to-right
+ * products in REAL*4, the unary minus over the whole product. */
+void orc_div_linear(int64_t ncell, const float* u1co, float omega0, float rete, float hte, float* div) {
+    const float lin = (powf(omega0, 0.6f) * rete) * hte;
+    for (int64_t c = 0; c < ncell; c++) div[c] = -(lin * (u1co[c] - 1.0f));
+}
+
 float orc_normalise(const orc_grid* g, float ache, float omega0, float zeta,
                     const float* mass, float* u1co, double* sum_out) {
     float rote, rete, hte;

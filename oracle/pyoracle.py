@@ -86,6 +86,8 @@ def lib():
     L.orc_merge_post.restype = C.c_int32
     L.orc_merge_post.argtypes = [vp, C.POINTER(_VP), C.c_int32, _i32p, _f32p, _i32p, _f32p, C.c_float,
                                  C.c_float, _i32p, _i32p, _i32p, _f32p, _i32p, _i32p, _f64p, C.c_int32]
+    L.orc_div_linear.restype = None
+    L.orc_div_linear.argtypes = [C.c_int64, _f32p, C.c_float, C.c_float, C.c_float, _f32p]
     L.orc_cosmo.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
                             C.POINTER(C.c_float)]
     _lib = L
@@ -202,10 +204,15 @@ def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.
     (reader.f90:447-461 conversion applied here).  Returns a dict of arrays.
     tsc_exact: TSC velocity sums in FP64 instead of the reference's REAL*4
     one-thread order.  smask: inner n^3 int8 SMASK (None: all ones); VVEL and
-    H_DISTANCE skip masked cells (particles.f90:312, 856), DDENS ignores it."""
+    H_DISTANCE skip masked cells (particles.f90:312, 856), DDENS ignores it.
+    vel_interp -1: velocities not available (FLAG_VEL_AVAILABLE = 0,
+    avism.f90:1259-1300): density only, DIVERCO from linear theory
+    (avism.f90:1646-1650); vx, vy, vz may be None."""
     L = lib()
     g = Grid(n, L0, periodic)
     x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    if vx is None:
+        vx = vy = vz = np.zeros(np.size(x), np.float32)
     vx, vy, vz, m = (np.array(a, np.float32, copy=True) for a in (vx, vy, vz, m))
     L.orc_convert_units(x.size, vx, vy, vz, m)
     npart = x.size
@@ -220,12 +227,12 @@ def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.
         L.orc_ppart_dens(knn._p, npart, x, y, z, m, k, pdens, hk)
         L.orc_vvel_vw(knn._p, g._p, k, m, pdens, vx, vy, vz, sm, hpart, u, v, w)
         L.orc_fill_h_zeros(knn._p, npart, x, y, z, k, hpart)
-    else:
+    elif vel_interp == 0:
         L.orc_vel_tsc(g._p, npart, x, y, z, vx, vy, vz, 1 if tsc_exact else 0, u, v, w)
     mass = np.zeros(ncell, np.float32)
     mass64 = np.zeros(ncell, np.float64)
     if dens_interp == 1:
-        if vel_interp == 0:
+        if vel_interp != 1:
             L.orc_h_distance(knn._p, g._p, k, sm, hpart)
             L.orc_fill_h_zeros(knn._p, npart, x, y, z, k, hpart)
         L.orc_ddens_sph(g._p, npart, x, y, z, hpart, m, mass, mass64)
@@ -235,7 +242,11 @@ def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.
     s = C.c_double()
     meandens = L.orc_normalise(g._p, ache, omega0, zeta, mass, u1co, C.byref(s))
     div = np.zeros(ncell, np.float32)
-    L.orc_diver_uniform(n, g.dx, u, v, w, periodic, div)
+    if vel_interp == -1:
+        _, rete, hte = cosmo(ache, omega0, zeta)
+        L.orc_div_linear(ncell, u1co, omega0, rete, hte, div)
+    else:
+        L.orc_diver_uniform(n, g.dx, u, v, w, periodic, div)
     return dict(grid=g, u=u, v=v, w=w, hpart=hpart, part_dens=pdens, hk=hk, mass=mass,
                 mass64=mass64, u1co=u1co, div=div, meandens=float(meandens), sum64=s.value)
 

@@ -36,7 +36,7 @@ PROGRAM REF_HARNESS
    INTEGER :: NVOIDT
    LOGICAL :: HAVE_MARCAP, HAVE_MASK
    INTEGER(KIND=8) :: I8, CNT8
-   REAL*4 :: ACHE, OMEGA0, ZETA, ACHE0, RODO, RE0, ROTE, RETE, LADO, SIZING, MEANDENS
+   REAL*4 :: ACHE, OMEGA0, ZETA, ACHE0, RODO, RE0, ROTE, RETE, HTE, LADO, SIZING, MEANDENS
    REAL*4 :: DXX, RX1
    REAL*8 :: SUM8, VOLM, VOLT_CLEAN
    REAL*4, ALLOCATABLE :: UBAS(:,:,:), U2C(:,:,:), U3C(:,:,:), U4C(:,:,:), U1C(:,:,:)
@@ -70,6 +70,8 @@ PROGRAM REF_HARNESS
    RE0 = 1.0/10.98
    ROTE = RODO*(1.0+ZETA)**3
    RETE = RE0/(1.0+ZETA)
+   HTE = OMEGA0*(((RE0/RETE)**3) - 1.0) + 1.0    ! avism.f90:601-602
+   HTE = ACHE*SQRT(HTE)
 
    ! ---- grid set-up (avism.f90:780-865)
    IF (FLAG_PERIOD .EQ. 0) THEN
@@ -162,7 +164,7 @@ PROGRAM REF_HARNESS
       ENDIF
       IF (FLAG_DENS_INTERP .EQ. 0) CALL DENS_INTERP_TSC(N,N,N,NPARTT,RXPA,RYPA,RZPA,MASAP,U1C)
       IF (FLAG_DENS_INTERP .EQ. 1) THEN
-         IF (FLAG_VEL_INTERP .EQ. 0) THEN
+         IF (FLAG_VEL_INTERP .LE. 0) THEN
             CALL H_DISTANCE(KNEIGH,N,N,N,NPARTT,TREE,SMASK(1:N,1:N,1:N),HPART)
             CALL FILL_H_ZEROS(KNEIGH,NPARTT,RXPA,RYPA,RZPA,TREE,HPART)
          ENDIF
@@ -210,6 +212,10 @@ PROGRAM REF_HARNESS
    CLOSE(3)
 
    DIVERCO(1:N,1:N,1:N) = UBAS
+   ! FLAG_VEL_INTERP = -1 here: velocities not available, FLAG_DIV = -1
+   ! (avism.f90:1259-1263, 1646-1650), on the inner grid
+   IF (MODE .EQ. 1 .AND. FLAG_VEL_INTERP .EQ. -1) &
+      DIVERCO(1:N,1:N,1:N) = - OMEGA0**(0.6)*RETE*HTE*(U1CO(1:N,1:N,1:N) - 1.0)
    IF (FLAG_PERIOD .EQ. 1) THEN
       CALL REAL_MAKE_PERIODIC(N,N,N,NPLUS,U1CO)
       CALL REAL_MAKE_PERIODIC(N,N,N,NPLUS,DIVERCO)

@@ -146,6 +146,22 @@ def kbig_fixtures(td: Path):
           dict(n=20, L0=60.0, periodic=0, mode=1, k=100, seed=25))
 
 
+def novel_fixtures(td: Path):
+    """Particles without velocities (FLAG_VEL_AVAILABLE = 0): density by SPH
+    (H_DISTANCE, FILL_H_ZEROS, DDENS) or TSC, DIVERCO from linear theory."""
+    for name, per, seed, di in (("nov_p0", 0, 41, 1), ("nov_p1", 1, 42, 1), ("novt_p1", 1, 43, 0)):
+        if per:
+            arr = synth.zeldovich(16, 40.0, seed=seed, rms_disp=2.0).numpy()
+            n, L0 = 20, 40.0
+        else:
+            p, _, _ = synth.mock_voids(n0=5000, L0=60.0, nvoids=4, rmin=6.0, rmax=12.0, seed=seed)
+            arr = p.numpy()
+            n, L0 = 20, 60.0
+        ref = O.run_reference(td / name, 1, n, L0, per, PARAMS, k=16, vel_interp=-1, dens_interp=di, parts=arr)
+        _save(name, dict(zip(("x", "y", "z", "vx", "vy", "vz", "m"), arr)), ref,
+              dict(n=n, L0=L0, periodic=per, mode=1, k=16, seed=seed, vel_interp=-1, dens_interp=di))
+
+
 def main():
     for per in (0, 1):
         if not O.ref_available(per):
@@ -164,10 +180,14 @@ def main():
         if sys.argv[1:] == ["kbig"]:
             kbig_fixtures(td)
             return
+        if sys.argv[1:] == ["novel"]:
+            novel_fixtures(td)
+            return
         tsc_fixtures(td)
         lev_fixtures(td)
         mask_fixtures(td)
         kbig_fixtures(td)
+        novel_fixtures(td)
         for per, seed in ((0, 11), (1, 12)):
             n, L0 = 24, 48.0
             d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, seed))

@@ -155,6 +155,66 @@ def test_fields_large_k_vs_oracle(k, per, vi, di):
     assert _rms_rel(f["div"], ref["div"]) <= 1e-4
 
 
+@pytest.mark.parametrize("name", ["nov_p0", "nov_p1", "novt_p1"])
+def test_velocity_free_vs_reference_fixture(golden_dir, name):
+    """avism_particles_to_density on particles without velocities (the
+    reference's FLAG_VEL_AVAILABLE = 0 run): HPART bit-exact, velocities 0,
+    U1CO within 1e-5, DIVERCO = -OMEGA0^0.6 RETE HTE (U1CO - 1) within 1e-5 of
+    its RMS."""
+    g = dict(np.load(golden_dir / f"{name}.npz"))
+    n, L0, per, k = (int(g["meta_n"]), float(g["meta_L0"]), int(g["meta_periodic"]), int(g["meta_k"]))
+    di = int(g["meta_dens_interp"])
+    print(f"[velocity-free fields {name}] n={n} dens_interp={di}", flush=True)
+    c = _ctx()
+    c.set_grid(n, L0, per)
+    c.set_cosmology(0.678, 0.31, 0.0)
+    c.load_particles(g["in_x"], g["in_y"], g["in_z"], None, None, None, g["in_m"])
+    c.particles_to_density(k=k, dens_interp=di)
+    f = c.fields()
+    _, hp = c.particle_fields(g["in_x"].size)
+    assert np.array_equal(hp, g["ref_hpart"])
+    for comp in ("u", "v", "w"):
+        assert not np.any(f[comp]), comp
+    assert _rel(f["u1co"], g["ref_u1co"]) <= REL
+    assert _rms_rel(f["div"], g["ref_div"]) <= REL
+
+
+@pytest.mark.parametrize("per,di", [(1, 1), (0, 1), (1, 0)])
+def test_velocity_free_vs_oracle(per, di):
+    """The velocity-free field stage at 32^3 against the oracle, then the void
+    stage on the GPU's fields against the oracle's on the same fields."""
+    import pyoracle as O
+    from avism_amd import synth
+    from test_gpu_voids import PARAMS, compare, gpu_void_outputs
+    n, L0 = 32, 62.5
+    arr = list(synth.zeldovich(32, L0, seed=90 + per).numpy())
+    if not per:
+        keep = np.all(np.abs(np.stack(arr[:3])) < L0 / 2 - 2 * L0 / n, axis=0)
+        arr = [a[keep].copy() for a in arr]
+    x, y, z, m = arr[0], arr[1], arr[2], arr[6]
+    print(f"[velocity-free oracle per={per} dens={di}] np={x.size}", flush=True)
+    ref = O.fields_from_particles(x, y, z, None, None, None, m, n, L0, per, k=32, vel_interp=-1, dens_interp=di)
+    c = _ctx()
+    c.set_grid(n, L0, per)
+    c.set_cosmology(0.678, 0.31, 0.0)
+    c.set_void_params(PARAMS.dens_thre, PARAMS.dens_thre2, PARAMS.grad_thre, PARAMS.div_thre, PARAMS.rmin,
+                      PARAMS.rmin_sub, PARAMS.nvoid_max)
+    c.load_particles(x, y, z, None, None, None, m)
+    c.particles_to_density(k=32, dens_interp=di)
+    f = c.fields()
+    _, hp = c.particle_fields(x.size)
+    assert np.array_equal(hp, ref["hpart"])
+    assert _rel(f["u1co"], ref["u1co"]) <= REL
+    assert _rms_rel(f["div"], ref["div"]) <= REL
+    out = gpu_void_outputs(c)
+    res = O.void_stage(O.Grid(n, L0, per), f["u1co"], f["div"], PARAMS, meandens=c.stats().meandens,
+                       rete=O.cosmo(0.678, 0.31, 0.0)[1])
+    assert res.nvoidt > 0
+    compare(out, lambda k: dict(nvoid=res.nvoid, cubes_i=res.cubes_i, cubes_f=res.cubes_f, indice=res.indice,
+                                nvoid2=res.nvoid2, uvoid=res.uvoid, marca=res.marca, nvoidt=res.nvoidt,
+                                ncellv=res.ncellv, volt_clean=res.volt_clean, vf=res.vf)[k])
+
+
 def test_k_above_256_rejected():
     from avism_amd._lib import AvismError
     from avism_amd import synth

@@ -109,7 +109,11 @@ def test_host_grid_input_matches_reference(tmp_path, golden_dir):
     assert (len(cubes) - 3 - 3) == 14 * int(g["ref_nvoid"])  # two headers, 14 values per cube
 
 
-def test_host_particle_input_matches_binding(tmp_path):
+@pytest.mark.parametrize("vel", [1, 0])
+def test_host_particle_input_matches_binding(tmp_path, vel):
+    """BinPart input with velocities (READ_BINARY_PART_1) and without
+    (READ_BINARY_PART_2, FLAG_VEL_AVAILABLE = 0: density only, linear-theory
+    divergence); the host's map and catalogue equal the binding's."""
     from avism_amd import synth
     from avism_amd.config import VoidsConfig
     from avism_amd._lib import Context
@@ -117,11 +121,11 @@ def test_host_particle_input_matches_binding(tmp_path):
     p = synth.zeldovich(n, L0, seed=17)
     arr = p.numpy()
     cfg = VoidsConfig(first=3, last=3, every=1, ncox=n, ncoy=n, ncoz=n, lado0=L0, flag_period=1, flag_data=1,
-                      kneighbours=32, partired=10 ** 8)
+                      kneighbours=32, partired=10 ** 8, flag_vel_available=vel)
     (tmp_path / "input_data").mkdir()
     with open(tmp_path / "input_data" / "bin_file_part00003", "wb") as f:
         _record(f, np.int64(arr[0].size), np.float32(0.0))
-        for a in arr:
+        for a in (arr if vel else [arr[0], arr[1], arr[2], arr[6]]):
             _record(f, a.astype(np.float32))
     _run_host(tmp_path, cfg)
     c = Context(0)
@@ -129,8 +133,12 @@ def test_host_particle_input_matches_binding(tmp_path):
     c.set_cosmology(cfg.ache, cfg.omega0, 0.0)
     c.set_void_params(cfg.dens_thre, cfg.dens_thre2, cfg.grad_thre, cfg.div_thre, cfg.rmin, cfg.rmin_sub,
                       cfg.nvoid_max)
-    c.load_particles(*arr)
-    c.particles_to_grid(k=32)
+    if vel:
+        c.load_particles(*arr)
+        c.particles_to_grid(k=32)
+    else:
+        c.load_particles(arr[0], arr[1], arr[2], None, None, None, arr[6])
+        c.particles_to_density(k=32)
     nv = c.find_voids(0)
     c.merge_and_post()
     cat = c.catalogue(nv)

@@ -81,7 +81,8 @@ def test_grid_input_vs_reference(golden_dir, name):
     compare(gpu_void_outputs(c), lambda k: g[f"ref_{k}"])
 
 
-@pytest.mark.parametrize("name", ["part_p0", "part_p1", "part_k100_p0", "part_k256_p1"])
+@pytest.mark.parametrize("name", ["part_p0", "part_p1", "part_k100_p0", "part_k256_p1", "nov_p0", "nov_p1",
+                                  "novt_p1"])
 def test_reference_particle_fields_vs_reference(golden_dir, name):
     # the reference's own U1CO/DIVERCO fed to the GPU void stage
     g = dict(np.load(golden_dir / f"{name}.npz"))

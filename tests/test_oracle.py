@@ -73,6 +73,24 @@ def test_particle_input_matches_reference(golden_dir, name):
     compare_void_outputs(res, lambda kk: g[f"ref_{kk}"])
 
 
+@pytest.mark.parametrize("name", ["nov_p0", "nov_p1", "novt_p1"])
+def test_velocity_free_input_matches_reference(golden_dir, name):
+    """Particles without velocities (FLAG_VEL_AVAILABLE = 0, avism.f90:1259-1300):
+    density only, DIVERCO = -OMEGA0^0.6 RETE HTE (U1CO - 1) (avism.f90:1646-1650)."""
+    g = _load(golden_dir, name)
+    n, L0, per, k = (int(g["meta_n"]), float(g["meta_L0"]), int(g["meta_periodic"]), int(g["meta_k"]))
+    di = int(g["meta_dens_interp"])
+    f = O.fields_from_particles(g["in_x"], g["in_y"], g["in_z"], None, None, None, g["in_m"], n, L0, per, k=k,
+                                vel_interp=-1, dens_interp=di)
+    for key in ("hpart", "u", "v", "w", "mass", "u1co", "div"):
+        assert np.array_equal(f[key], g[f"ref_{key}"]), key
+    assert not np.any(f["u"]) and np.any(f["div"])
+    assert f["meandens"] == float(g["ref_meandens"])
+    rete = O.cosmo(0.678, 0.31, 0.0)[1]
+    res = O.void_stage(f["grid"], f["u1co"], f["div"], PARAMS, meandens=f["meandens"], rete=rete)
+    compare_void_outputs(res, lambda kk: g[f"ref_{kk}"])
+
+
 @pytest.mark.parametrize("name", ["tsc_p0", "tsc_p1", "tscv_p0", "tscv_p1"])
 def test_tsc_input_matches_reference(golden_dir, name):
     """VEL_INTERP_TSC, DENS_INTERP_TSC and H_DISTANCE (particles.f90:16-351)."""
