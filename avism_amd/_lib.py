@@ -53,6 +53,8 @@ EXPORTS = [
     "avism_set_parent_map", "avism_set_parent_radii", "avism_set_mask", "avism_next_level", "avism_load_particles",
     "avism_particles_to_grid",
     "avism_particles_to_density",
+    "avism_load_randoms",
+    "avism_survey_density",
     "avism_load_grid_fields", "avism_load_void_fields", "avism_get_fields", "avism_get_particle_fields", "avism_find_voids",
     "avism_set_proposals", "avism_get_candidates", "avism_get_cubes", "avism_indexx",
     "avism_merge_and_post", "avism_get_catalogue", "avism_get_marca", "avism_run_level",
@@ -96,6 +98,8 @@ def lib():
         "avism_load_particles": ([vp, i64, vp, vp, vp, vp, vp, vp, vp, i32], C.c_int),
         "avism_particles_to_grid": ([vp, i32, i32, i32], C.c_int),
         "avism_particles_to_density": ([vp, i32, i32], C.c_int),
+        "avism_load_randoms": ([vp, C.c_int64, vp, vp, vp, vp, i32], C.c_int),
+        "avism_survey_density": ([vp, i32, i32], C.c_int),
         "avism_load_grid_fields": ([vp, vp, vp, vp, vp, i32], C.c_int),
         "avism_load_void_fields": ([vp, vp, vp, f32, i32], C.c_int),
         "avism_get_fields": ([vp, vp, vp, vp, vp, vp, i32], C.c_int),
@@ -249,6 +253,16 @@ class Context:
     def particles_to_density(self, k=32, dens_interp=1):
         """FLAG_VEL_AVAILABLE = 0: density only, DIVERCO from linear theory."""
         self._chk(lib().avism_particles_to_density(self._h, int(k), int(dens_interp)))
+
+    def load_randoms(self, x, y, z, m):
+        """Survey randoms (host arrays)."""
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (x, y, z, m)]
+        self._keep_rand = arrs
+        self._chk(lib().avism_load_randoms(self._h, int(arrs[0].size), *[_ptr(a) for a in arrs], AVISM_HOST))
+
+    def survey_density(self, k=32, dens_interp=1):
+        """FLAG_SURVEY = 1: D/(alpha R) density, linear-theory DIVERCO."""
+        self._chk(lib().avism_survey_density(self._h, int(k), int(dens_interp)))
 
     def load_grid_fields(self, delta, vx, vy, vz):
         arrs = [np.ascontiguousarray(a, np.float32) for a in (delta, vx, vy, vz)]

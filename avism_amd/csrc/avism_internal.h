@@ -105,6 +105,14 @@ struct avism_ctx {
   float meandens_raw = 0;                      // MEANDENS before unit scaling
   double meansum = 0;
   bool fields_ready = false;
+  // survey randoms (FLAG_SURVEY = 1): a second particle set, swapped with the
+  // first by swap_particle_sets; raw_mass: DDENS leaves cell masses unscaled
+  int64_t nrand = 0;
+  avism::DBuf rpx, rpy, rpz, rpvx, rpvy, rpvz, rpm, rspos4, rsorig, rvel4, rbin_start, rbin_reach;
+  avism::DBuf rpdens, rhk, rhpart, rnorm, u_rand;
+  int32_t rnb = 0;
+  float rbsz = 0, rmass_max = 0;
+  bool rmass_uniform = false, raw_mass = false;
 
   // void stage
   avism::DBuf flags;                           // ext flag bytes
@@ -149,6 +157,10 @@ namespace avism {
 int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp);
 int fields_from_grid(avism_ctx* c, const float* delta, const float* vx, const float* vy,
                      const float* vz, int where);
+int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, const float* z,
+                   const float* vx, const float* vy, const float* vz, const float* m, int where);
+void swap_particle_sets(avism_ctx* c);
+int survey_density(avism_ctx* c, int32_t k, int32_t dens_interp);
 int void_find(avism_ctx* c, int32_t growth_mode);
 int merge_and_post(avism_ctx* c, int32_t stop_after);
 int next_level(avism_ctx* c);

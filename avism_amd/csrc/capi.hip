@@ -243,6 +243,23 @@ int avism_particles_to_density(avism_ctx* c, int32_t k, int32_t dens_interp) {
   return fields_from_particles(c, k, -1, dens_interp);
 }
 
+int avism_load_randoms(avism_ctx* c, int64_t nrand, const float* x, const float* y, const float* z,
+                       const float* m, int32_t where) {
+  CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
+  if (!x || !y || !z || !m) AV_FAIL(c, AVISM_ERR_ARG, "null random array");
+  swap_particle_sets(c);
+  const int rc = load_particles(c, nrand, x, y, z, nullptr, nullptr, nullptr, m, where);
+  swap_particle_sets(c);
+  c->fields_ready = false;
+  return rc;
+}
+
+int avism_survey_density(avism_ctx* c, int32_t k, int32_t dens_interp) {
+  CTX_GUARD(c);
+  return survey_density(c, k, dens_interp);
+}
+
 int avism_load_grid_fields(avism_ctx* c, const float* delta, const float* vx, const float* vy,
                            const float* vz, int32_t where) {
   CTX_GUARD(c);

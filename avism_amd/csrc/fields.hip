@@ -1017,8 +1017,9 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   {
     StageTimer t(c, "ddens_finalize");
     k_ddens_finalize<<<grid, kFinB, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 1.0 / scale,
-                                                  dens_interp == 1 ? 0 : 1, den,
-                                                  c->rote, c->u_mass.as<float>(), c->partial64.as<double>());
+                                                  dens_interp == 1 ? 0 : 1, c->raw_mass ? 1.0f : den,
+                                                  c->raw_mass ? 1.0f : c->rote, c->u_mass.as<float>(),
+                                                  c->partial64.as<double>());
     AV_CHECK_HIP(c, hipGetLastError());
   }
   double* dsum = (double*)((char*)c->scalars.p + 256);
@@ -1049,6 +1050,153 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   c->meandens_raw = hmean;
   float ul3 = (kUL * kUL) * kUL;
   c->meandens = hmean * (kUM / ul3);  // avism.f90:1409
+  c->fields_ready = true;
+  return AVISM_OK;
+}
+
+// ---- survey mode (FLAG_SURVEY = 1, avism.f90:1415-1597)
+
+static void swap_buf(DBuf& a, DBuf& b) {
+  std::swap(a.p, b.p);
+  std::swap(a.bytes, b.bytes);
+}
+
+void swap_particle_sets(avism_ctx* c) {
+  swap_buf(c->px, c->rpx); swap_buf(c->py, c->rpy); swap_buf(c->pz, c->rpz);
+  swap_buf(c->pvx, c->rpvx); swap_buf(c->pvy, c->rpvy); swap_buf(c->pvz, c->rpvz); swap_buf(c->pm, c->rpm);
+  swap_buf(c->spos4, c->rspos4); swap_buf(c->sorig, c->rsorig); swap_buf(c->vel4, c->rvel4);
+  swap_buf(c->bin_start, c->rbin_start); swap_buf(c->bin_reach, c->rbin_reach);
+  swap_buf(c->pdens, c->rpdens); swap_buf(c->hk, c->rhk); swap_buf(c->hpart, c->rhpart); swap_buf(c->norm, c->rnorm);
+  std::swap(c->np, c->nrand); std::swap(c->nb, c->rnb); std::swap(c->bsz, c->rbsz);
+  std::swap(c->mass_max, c->rmass_max); std::swap(c->mass_uniform, c->rmass_uniform);
+}
+
+// per-block FP64 sums over the observed cells (SMASK > 0) of a and b, in a
+// fixed order (grid-stride per thread, then a tree in LDS): partial[2 * block]
+__global__ void __launch_bounds__(256) k_mask_sums(int64_t ncell, const float* __restrict__ a,
+                                                   const float* __restrict__ b,
+                                                   const int8_t* __restrict__ smask, double* partial) {
+  __shared__ double sa[256], sb[256];
+  double xa = 0.0, xb = 0.0;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * 256) {
+    if (smask[c] == 0) continue;
+    xa += (double)a[c];
+    if (b) xb += (double)b[c];
+  }
+  sa[threadIdx.x] = xa;
+  sb[threadIdx.x] = xb;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+      sb[threadIdx.x] += sb[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = sa[0];
+    partial[2 * blockIdx.x + 1] = sb[0];
+  }
+}
+
+// out[0..1] = the sums of the nparts partial pairs, in block order
+__global__ void k_sum_pairs(int nparts, const double* __restrict__ partial, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = 0.0, b = 0.0;
+  for (int i = 0; i < nparts; ++i) {
+    a += partial[2 * i];
+    b += partial[2 * i + 1];
+  }
+  out[0] = a;
+  out[1] = b;
+}
+
+// D/(alpha R) per observed cell (avism.f90:1562-1581): alpha = DSUM/RSUM,
+// the ratio clipped to [RATIOFLOOR, RATIOMAX] in FP64, stored in FP32;
+// unobserved cells keep U1CO
+__global__ void k_survey_ratio(int64_t ncell, const double* __restrict__ sums, const float* __restrict__ ugal,
+                               const float* __restrict__ urand, const int8_t* __restrict__ smask, float* u1co) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const float u = ugal[c];
+  if (smask[c] == 0) {
+    u1co[c] = u;
+    return;
+  }
+  const double alpha = sums[0] / sums[1];
+  const float r = urand[c];
+  double b = r > 0.0f ? (double)u / (alpha * (double)r) : (double)1e-4f;
+  b = fmax((double)1e-4f, b);
+  b = fmin(1000.0, b);
+  u1co[c] = (float)b;
+}
+
+// U1CO / MEANRATIO on observed cells (MEANRATIO = REAL*4 of the FP64 mean,
+// avism.f90:1584-1588), then the linear-theory DIVERCO over every cell
+__global__ void k_survey_final(int64_t ncell, const double* __restrict__ msum, double count, float lin,
+                               const int8_t* __restrict__ smask, float* u1co, float* div) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const float meanratio = (float)(msum[0] / count);
+  float u = u1co[c];
+  if (smask[c] != 0) u = u / meanratio;
+  u1co[c] = u;
+  div[c] = -(lin * (u - 1.0f));
+}
+
+int survey_density(avism_ctx* c, int32_t k, int32_t dens_interp) {
+  if (!c->has_mask) AV_FAIL(c, AVISM_ERR_STATE, "survey mode needs the survey mask (avism_set_mask)");
+  if (c->nrand <= 0) AV_FAIL(c, AVISM_ERR_STATE, "survey mode needs the randoms (avism_load_randoms)");
+  if (c->np <= 0) AV_FAIL(c, AVISM_ERR_STATE, "no particles loaded");
+  if (k <= 0) AV_FAIL(c, AVISM_ERR_INPUT, "KNEIGHBOURS must be positive");
+  const int n = c->g.n;
+  const int64_t ncell = (int64_t)n * n * n;
+  // randoms: DENS_INTERP_TSC, or H_DISTANCE and FILL_H_ZEROS with
+  // KNEIGHBOURS2 = MIN(8 KNEIGHBOURS, 256), then DDENS (avism.f90:1459-1500);
+  // U1RAND is the cell mass itself
+  const int k2 = std::min(8 * k, 256);
+  swap_particle_sets(c);
+  c->raw_mass = true;
+  int rc = fields_from_particles(c, k2, -1, dens_interp);
+  c->raw_mass = false;
+  swap_particle_sets(c);
+  if (rc) return rc;
+  if (!c->u_rand.ensure((size_t)ncell * 4)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (survey)");
+  AV_CHECK_HIP(c, hipMemcpyAsync(c->u_rand.p, c->u_mass.p, (size_t)ncell * 4, hipMemcpyDeviceToDevice,
+                                 c->stream));
+  // galaxies: the density without velocities; u_mass is then U1CO * ROTE's
+  // value before the mean (avism.f90:1386)
+  rc = fields_from_particles(c, k, -1, dens_interp);
+  if (rc) return rc;
+  StageTimer t(c, "survey_ratio");
+  const int nparts = (int)std::min<int64_t>(nblk(ncell), 1024);
+  if (!c->partial64.ensure((size_t)nparts * 16 + 64)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
+  double* part = c->partial64.as<double>();
+  double* sums = part + 2 * nparts;  // DSUM, RSUM; then the ratio sum
+  const int8_t* sm = c->smask.as<int8_t>();
+  k_mask_sums<<<nparts, 256, 0, c->stream>>>(ncell, c->u_mass.as<float>(), c->u_rand.as<float>(), sm, part);
+  k_sum_pairs<<<1, 64, 0, c->stream>>>(nparts, part, sums);
+  k_survey_ratio<<<nblk(ncell), 256, 0, c->stream>>>(ncell, sums, c->u_mass.as<float>(), c->u_rand.as<float>(),
+                                                      sm, c->u1co.as<float>());
+  k_mask_sums<<<nparts, 256, 0, c->stream>>>(ncell, c->u1co.as<float>(), nullptr, sm, part);
+  k_sum_pairs<<<1, 64, 0, c->stream>>>(nparts, part, sums + 2);
+  AV_CHECK_HIP(c, hipGetLastError());
+  // the observed-cell count, on the host from the mask
+  int64_t nobs = 0;
+  {
+    std::vector<int8_t> hm((size_t)ncell);
+    AV_CHECK_HIP(c, hipMemcpyAsync(hm.data(), sm, (size_t)ncell, hipMemcpyDeviceToHost, c->stream));
+    AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    for (int8_t v : hm) nobs += v > 0;
+  }
+  if (nobs == 0) AV_FAIL(c, AVISM_ERR_INPUT, "the survey mask has no observed cell");
+  const float lin = (powf(c->omega0, 0.6f) * c->rete) * c->hte;  // avism.f90:1648
+  k_survey_final<<<nblk(ncell), 256, 0, c->stream>>>(ncell, sums + 2, (double)nobs, lin, sm, c->u1co.as<float>(),
+                                                      c->div.as<float>());
+  AV_CHECK_HIP(c, hipGetLastError());
+  AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  const float ul3 = (kUL * kUL) * kUL;
+  c->meandens = c->rote * (kUM / ul3);  // avism.f90:1590
   c->fields_ready = true;
   return AVISM_OK;
 }

@@ -139,6 +139,18 @@ int avism_particles_to_grid(avism_ctx* ctx, int32_t k, int32_t vel_interp, int32
  * linear theory, -OMEGA0^0.6 RETE HTE (U1CO - 1).  The velocity fields read
  * back as 0. */
 int avism_particles_to_density(avism_ctx* ctx, int32_t k, int32_t dens_interp);
+/* Survey randoms (input_data/rands.dat, avism.f90:1431-1441): positions
+ * Mpc/h and masses Msun (m/UM applied), kept beside the particles. */
+int avism_load_randoms(avism_ctx* ctx, int64_t nrand, const float* x, const float* y, const float* z,
+                       const float* m, int32_t where);
+/* FLAG_SURVEY = 1 (avism.f90:1415-1597, 1646-1650): the galaxies' density
+ * as avism_particles_to_density makes it; the randoms' cell masses U1RAND by
+ * DENS_INTERP_TSC, or H_DISTANCE and FILL_H_ZEROS with MIN(8k, 256)
+ * neighbours and DDENS; on observed cells U1CO = D/(alpha R) clipped to
+ * [1e-4, 1000] with alpha = sum D / sum R, then divided by its mean; MEANDENS
+ * = the background density; DIVERCO from linear theory.  Needs
+ * avism_set_mask and avism_load_randoms. */
+int avism_survey_density(avism_ctx* ctx, int32_t k, int32_t dens_interp);
 /* READ_BINARY_GRID (reader.f90:523-552): delta and velocities (km/s) on the
  * inner grid; then DIVER_UNIFORM.  Used for grid input (FLAG_DATA=2). */
 int avism_load_grid_fields(avism_ctx* ctx, const float* delta, const float* vx, const float* vy,

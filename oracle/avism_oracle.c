@@ -640,6 +640,53 @@ float orc_normalise(const orc_grid* g, float ache, float omega0, float zeta,
     return meandens * (UM_CONST / ul3);                       /* avism.f90:1409 */
 }
 
+/* Survey mode (FLAG_SURVEY = 1, avism.f90:1386, 1543-1590, 1646-1650):
+ * U1CO = the galaxies' density (mass / (dx^3 RETE^3) / ROTE * ROTE); on
+ * observed cells U1CO = D/(alpha R) with alpha = DSUM/RSUM (FP64 sums in
+ * Fortran order), clipped to [RATIOFLOOR, RATIOMAX] = [1e-4f, 1000] in FP64;
+ * MEANRATIO = REAL*4 of the FP64 mean over observed cells; U1CO /= MEANRATIO
+ * there; DIVERCO = -OMEGA0**0.6 RETE HTE (U1CO - 1).  urand: the randoms'
+ * cell masses as DDENS (or TSC) leaves them.  Returns MEANDENS. */
+float orc_survey(const orc_grid* g, float ache, float omega0, float zeta, const float* mass,
+                 const float* urand, const int8_t* smask, float* u1co, float* div) {
+    float rote, rete, hte;
+    orc_cosmo(ache, omega0, zeta, &rote, &rete, &hte);
+    float dx = g->dx;
+    float den = ((dx * dx) * dx) * ((rete * rete) * rete);
+    size_t ncell = (size_t)g->n * g->n * g->n;
+    double dsum = 0.0, rsum = 0.0;
+    for (size_t c = 0; c < ncell; c++) {
+        float u = mass[c] / den;
+        u = u / rote;
+        u1co[c] = u * rote;
+        if (smask[c] == 0) continue;
+        dsum += (double)u1co[c];
+        rsum += (double)urand[c];
+    }
+    const double alpha = dsum / rsum;
+    const float floor4 = 1e-4f, max4 = 1000.0f;
+    for (size_t c = 0; c < ncell; c++) {
+        if (smask[c] == 0) continue;
+        double b = urand[c] > 0.0f ? (double)u1co[c] / (alpha * (double)urand[c]) : (double)floor4;
+        if (b < (double)floor4) b = (double)floor4;
+        if (b > (double)max4) b = (double)max4;
+        u1co[c] = (float)b;
+    }
+    double msum = 0.0;
+    int64_t cnt = 0;
+    for (size_t c = 0; c < ncell; c++)
+        if (smask[c] != 0) {
+            msum += (double)u1co[c];
+            cnt++;
+        }
+    const float meanratio = (float)(msum / (double)cnt);
+    for (size_t c = 0; c < ncell; c++)
+        if (smask[c] != 0) u1co[c] = u1co[c] / meanratio;
+    orc_div_linear((int64_t)ncell, u1co, omega0, rete, hte, div);
+    float ul3 = (UL_CONST * UL_CONST) * UL_CONST;
+    return rote * (UM_CONST / ul3);
+}
+
 void orc_diver_uniform(int32_t n, float dx, const float* u, const float* v,
                        const float* w, int32_t periodic, float* div) {
 #define F(a, i, j, k) a[(size_t)((i) - 1) + (size_t)n * (((j) - 1) + (size_t)n * ((k) - 1))]

@@ -86,6 +86,8 @@ def lib():
     L.orc_merge_post.restype = C.c_int32
     L.orc_merge_post.argtypes = [vp, C.POINTER(_VP), C.c_int32, _i32p, _f32p, _i32p, _f32p, C.c_float,
                                  C.c_float, _i32p, _i32p, _i32p, _f32p, _i32p, _i32p, _f64p, C.c_int32]
+    L.orc_survey.restype = C.c_float
+    L.orc_survey.argtypes = [vp, C.c_float, C.c_float, C.c_float, _f32p, _f32p, C.c_void_p, _f32p, _f32p]
     L.orc_div_linear.restype = None
     L.orc_div_linear.argtypes = [C.c_int64, _f32p, C.c_float, C.c_float, C.c_float, _f32p]
     L.orc_cosmo.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float),
@@ -251,6 +253,25 @@ def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.
                 mass64=mass64, u1co=u1co, div=div, meandens=float(meandens), sum64=s.value)
 
 
+def survey_fields(gal, rand, n, L0, periodic, smask, k=32, dens_interp=1, ache=0.678, omega0=0.31, zeta=0.0):
+    """Survey mode (FLAG_SURVEY = 1, avism.f90:1415-1597): galaxies (x, y, z, m)
+    without velocities, randoms (x, y, z, m) with KNEIGHBOURS2 = min(8k, 256)
+    (avism.f90:1482); D/(alpha R) density and linear-theory DIVERCO."""
+    sm = np.ascontiguousarray(smask, np.int8)
+    f = fields_from_particles(gal[0], gal[1], gal[2], None, None, None, gal[3], n, L0, periodic, k=k,
+                              vel_interp=-1, dens_interp=dens_interp, smask=sm, ache=ache, omega0=omega0,
+                              zeta=zeta)
+    r = fields_from_particles(rand[0], rand[1], rand[2], None, None, None, rand[3], n, L0, periodic,
+                              k=min(8 * k, 256), vel_interp=-1, dens_interp=dens_interp, smask=sm, ache=ache,
+                              omega0=omega0, zeta=zeta)
+    u1co = np.zeros(n ** 3, np.float32)
+    div = np.zeros(n ** 3, np.float32)
+    meandens = lib().orc_survey(f["grid"]._p, ache, omega0, zeta, f["mass"], r["mass"],
+                                sm.ctypes.data_as(C.c_void_p), u1co, div)
+    return dict(grid=f["grid"], u1co=u1co, div=div, meandens=float(meandens), hpart=f["hpart"],
+                hrand=r["hpart"], mass=f["mass"], urand=r["mass"])
+
+
 def fields_from_grid(delta, vx, vy, vz, n, L0, periodic, ache=0.678, omega0=0.31, zeta=0.0):
     """Grid input (reader.f90:541-549; avism.f90:1342 and 1599-1600)."""
     g = Grid(n, L0, periodic)
@@ -378,7 +399,7 @@ def run_reference_indexx(jobdir: Path, keys: np.ndarray) -> np.ndarray:
 
 def run_reference(jobdir: Path, mode: int, n: int, L0: float, periodic: int, params: VoidParams,
                   ache=0.678, omega0=0.31, zeta=0.0, k=32, vel_interp=1, dens_interp=1,
-                  parts=None, fields=None, marcap=None, reqp=None, smask=None, timeout=600) -> dict:
+                  parts=None, fields=None, marcap=None, reqp=None, smask=None, rands=None, timeout=600) -> dict:
     """Run the reference routines (oracle/_ref) on one level and read the outputs.
     marcap (inner n^3 int32) and reqp (float32, ids 1..len) are a parent level's
     MARCAP and REQP (avism.f90:2123-2161); None runs the first level."""
@@ -386,8 +407,13 @@ def run_reference(jobdir: Path, mode: int, n: int, L0: float, periodic: int, par
 
     jobdir = Path(jobdir)
     jobdir.mkdir(parents=True, exist_ok=True)
-    for f in ("marcap.bin", "reqp.bin", "smask.bin"):
+    for f in ("marcap.bin", "reqp.bin", "smask.bin", "rands.bin"):
         (jobdir / f).unlink(missing_ok=True)
+    if rands is not None:  # survey mode: NRAND (int32), then x, y, z, m
+        with open(jobdir / "rands.bin", "wb") as fh:
+            fh.write(np.int32(rands[0].size).tobytes())
+            for a in rands:
+                fh.write(np.ascontiguousarray(a, np.float32).tobytes())
     if smask is not None:
         sm = np.ascontiguousarray(smask, np.int8)
         assert sm.size == n ** 3
@@ -441,6 +467,8 @@ def run_reference(jobdir: Path, mode: int, n: int, L0: float, periodic: int, par
     out["u"], out["v"], out["w"] = vel[: n ** 3], vel[n ** 3: 2 * n ** 3], vel[2 * n ** 3:]
     if mode == 1:
         out["mass"] = np.fromfile(jobdir / "out_mass.bin", np.float32)
+        if (jobdir / "out_rands.bin").exists():
+            out["urand"] = np.fromfile(jobdir / "out_rands.bin", np.float32)
         pp = np.fromfile(jobdir / "out_parts.bin", np.float32)
         out["part_dens"], out["hpart"] = pp[:npart], pp[npart:]
     raw = open(jobdir / "out_cubes.bin", "rb").read()

@@ -34,7 +34,12 @@ PROGRAM REF_HARNESS
    CHARACTER(LEN=512) :: JOB
    INTEGER :: MODE, N, NP4, KNEIGH, NPLUS, LOW1, LOW2, IX, JY, KZ, I, IV, IND, NVOID, NVOID2
    INTEGER :: NVOIDT
-   LOGICAL :: HAVE_MARCAP, HAVE_MASK
+   LOGICAL :: HAVE_MARCAP, HAVE_MASK, HAVE_RANDS
+   INTEGER*4 :: NRAND, KNEIGH2
+   REAL*4, ALLOCATABLE :: RANDX(:), RANDY(:), RANDZ(:), MASSRAND(:), HRAND(:), U1RAND(:,:,:), RPTS(:,:)
+   TYPE(KDTreeNode), POINTER :: RTREE
+   REAL*8 :: DSUM, RSUM, ALPHA, BASS8
+   REAL*4 :: RATIOFLOOR, RATIOMAX, MEANRATIO
    INTEGER(KIND=8) :: I8, CNT8
    REAL*4 :: ACHE, OMEGA0, ZETA, ACHE0, RODO, RE0, ROTE, RETE, HTE, LADO, SIZING, MEANDENS
    REAL*4 :: DXX, RX1
@@ -181,6 +186,72 @@ PROGRAM REF_HARNESS
       CALL DIVER_UNIFORM(N,N,N,DXX,DXX,DXX,U2C,U3C,U4C,UBAS,FLAG_PERIOD)
       U1CO(1:N,1:N,1:N) = U1C
       U1CO = U1CO*ROTE
+      INQUIRE(FILE=TRIM(JOB)//'/rands.bin', EXIST=HAVE_RANDS)
+      IF (HAVE_RANDS) THEN
+         ! survey mode (avism.f90:1420-1590); rands.bin: NRAND, then x, y, z, m
+         OPEN(2, FILE=TRIM(JOB)//'/rands.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+         READ(2) NRAND
+         ALLOCATE(RANDX(NRAND), RANDY(NRAND), RANDZ(NRAND), MASSRAND(NRAND), U1RAND(N,N,N))
+         READ(2) RANDX, RANDY, RANDZ, MASSRAND
+         CLOSE(2)
+         MASSRAND = MASSRAND/UM
+         U1RAND = 0.
+         IF (FLAG_DENS_INTERP .EQ. 0) THEN
+            CALL DENS_INTERP_TSC(N,N,N,INT(NRAND,KIND=8),RANDX,RANDY,RANDZ,MASSRAND,U1RAND)
+         ELSE
+            ALLOCATE(HRAND(NRAND), RPTS(NRAND,3))
+            HRAND = 0.
+            RPTS(:,1) = RANDX(:)
+            RPTS(:,2) = RANDY(:)
+            RPTS(:,3) = RANDZ(:)
+#if periodic == 1
+            RTREE => build_kdtree(RPTS, LPER)
+#else
+            RTREE => build_kdtree(RPTS)
+#endif
+            KNEIGH2 = MIN(8*KNEIGH, 256)
+            CALL H_DISTANCE(KNEIGH2,N,N,N,INT(NRAND,KIND=8),RTREE,SMASK(1:N,1:N,1:N),HRAND)
+            CALL FILL_H_ZEROS(KNEIGH2,INT(NRAND,KIND=8),RANDX,RANDY,RANDZ,RTREE,HRAND)
+            CALL DDENS_INTERP_SPH(N,N,N,INT(NRAND,KIND=8),RANDX,RANDY,RANDZ,HRAND,MASSRAND, &
+                                  SMASK(1:N,1:N,1:N),U1RAND)
+         ENDIF
+         DSUM = 0.0D0
+         RSUM = 0.0D0
+         DO KZ=1,N
+            DO JY=1,N
+               DO IX=1,N
+                  IF (SMASK(IX,JY,KZ) .EQ. 0) CYCLE
+                  DSUM = DSUM + REAL(U1CO(IX,JY,KZ), KIND=8)
+                  RSUM = RSUM + REAL(U1RAND(IX,JY,KZ), KIND=8)
+               ENDDO
+            ENDDO
+         ENDDO
+         ALPHA = DSUM/RSUM
+         RATIOFLOOR = 1E-4
+         RATIOMAX = 1000.0D0
+         DO KZ=1,N
+            DO JY=1,N
+               DO IX=1,N
+                  IF (SMASK(IX,JY,KZ) .EQ. 0) CYCLE
+                  IF (U1RAND(IX,JY,KZ) > 0.0D0) THEN
+                     BASS8 = U1CO(IX,JY,KZ)/(ALPHA*U1RAND(IX,JY,KZ))
+                  ELSE
+                     BASS8 = RATIOFLOOR
+                  ENDIF
+                  BASS8 = MAX(REAL(RATIOFLOOR, KIND=8), BASS8)
+                  BASS8 = MIN(REAL(RATIOMAX, KIND=8), BASS8)
+                  U1CO(IX,JY,KZ) = BASS8
+               ENDDO
+            ENDDO
+         ENDDO
+         MEANRATIO = SUM(REAL(U1CO(1:N,1:N,1:N),8), MASK=SMASK(1:N,1:N,1:N)>0) / &
+                     REAL(COUNT(SMASK(1:N,1:N,1:N)>0),8)
+         WHERE (SMASK(1:N,1:N,1:N)>0) U1CO(1:N,1:N,1:N) = U1CO(1:N,1:N,1:N) / MEANRATIO
+         MEANDENS = ROTE * (UM / UL**3)
+         OPEN(3, FILE=TRIM(JOB)//'/out_rands.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+         WRITE(3) U1RAND
+         CLOSE(3)
+      ELSE
       SUM8 = 0.D0
       CNT8 = 0
       DO KZ=1,N
@@ -194,6 +265,7 @@ PROGRAM REF_HARNESS
       MEANDENS = SUM8 / REAL(CNT8, KIND=8)
       U1CO = U1CO / MEANDENS
       MEANDENS = MEANDENS * (UM / UL**3)
+      ENDIF
    ELSE
       ! ---- grid input (reader.f90:523-552, FLAG_DATA=2)
       OPEN(2, FILE=TRIM(JOB)//'/fields.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')

@@ -162,6 +162,27 @@ def novel_fixtures(td: Path):
               dict(n=n, L0=L0, periodic=per, mode=1, k=16, seed=seed, vel_interp=-1, dens_interp=di))
 
 
+def survey_fields_fixtures(td: Path):
+    """Survey mode (FLAG_SURVEY = 1): galaxies without velocities, randoms
+    uniform in the box at 4 per cell, a survey mask with 20% of cells empty;
+    density by SPH (KNEIGHBOURS2 = 128 for the randoms), periodic and not."""
+    for name, per, seed, di in (("svy_p0", 0, 51, 1), ("svy_p1", 1, 52, 1)):
+        p, _, _ = synth.mock_voids(n0=6000, L0=60.0, nvoids=4, rmin=6.0, rmax=12.0, seed=seed)
+        arr = p.numpy()
+        n, L0 = 20, 60.0
+        rng = np.random.default_rng(seed)
+        nr = 4 * n ** 3
+        rands = [rng.uniform(-L0 / 2 + 1e-3, L0 / 2 - 1e-3, nr).astype(np.float32) for _ in range(3)]
+        rands.append(np.full(nr, 1e10, np.float32))
+        sm = synth.survey_mask(n, 0.2, seed=seed)
+        ref = O.run_reference(td / name, 1, n, L0, per, PARAMS, k=16, vel_interp=-1, dens_interp=di, parts=arr,
+                              smask=sm, rands=rands)
+        inputs = dict(zip(("x", "y", "z", "vx", "vy", "vz", "m"), arr))
+        inputs.update(smask=sm, rx=rands[0], ry=rands[1], rz=rands[2], rm=rands[3])
+        _save(name, inputs, ref, dict(n=n, L0=L0, periodic=per, mode=1, k=16, seed=seed, vel_interp=-1,
+                                      dens_interp=di))
+
+
 def main():
     for per in (0, 1):
         if not O.ref_available(per):
@@ -183,11 +204,15 @@ def main():
         if sys.argv[1:] == ["novel"]:
             novel_fixtures(td)
             return
+        if sys.argv[1:] == ["survey"]:
+            survey_fields_fixtures(td)
+            return
         tsc_fixtures(td)
         lev_fixtures(td)
         mask_fixtures(td)
         kbig_fixtures(td)
         novel_fixtures(td)
+        survey_fields_fixtures(td)
         for per, seed in ((0, 11), (1, 12)):
             n, L0 = 24, 48.0
             d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, seed))

@@ -136,3 +136,34 @@ def test_uneven_masked_patches():
         print(f"[patch {i}] n={n} np={side ** 3} cubes={out['nvoid']} voids={out['nvoidt']}", flush=True)
         compare(out, lambda k: _as_ref(ref)[k])
         assert np.all(f["u"][sm == 0] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["svy_p0", "svy_p1"])
+def test_survey_vs_reference_fixture(golden_dir, name):
+    """avism_survey_density against the reference's survey-mode run: galaxy
+    HPART bit-exact, U1CO within 1e-5, DIVERCO within 1e-5 of its RMS, MEANDENS
+    the background; the void stage on the GPU's fields equals the oracle's, and
+    on the reference's fields the reference's catalogue."""
+    g = dict(np.load(golden_dir / f"{name}.npz"))
+    n, L0, per, k = int(g["meta_n"]), float(g["meta_L0"]), int(g["meta_periodic"]), int(g["meta_k"])
+    sm = g["in_smask"]
+    print(f"[survey {name}] n={n} randoms={g['in_rx'].size}", flush=True)
+    c = _ctx(n, L0, per)
+    c.set_mask(sm)
+    c.load_particles(g["in_x"], g["in_y"], g["in_z"], None, None, None, g["in_m"])
+    c.load_randoms(g["in_rx"], g["in_ry"], g["in_rz"], g["in_rm"])
+    c.survey_density(k=k, dens_interp=int(g["meta_dens_interp"]))
+    f = c.fields()
+    _, hp = c.particle_fields(g["in_x"].size)
+    assert np.array_equal(hp, g["ref_hpart"])
+    assert _rel(f["u1co"], g["ref_u1co"]) <= REL
+    assert _rms_rel(f["div"], g["ref_div"]) <= REL
+    assert np.float32(c.stats().meandens) == np.float32(g["ref_meandens"])
+    out = gpu_void_outputs(c)
+    ref = O.void_stage(O.Grid(n, L0, per), f["u1co"], f["div"], PARAMS, smask=sm, meandens=c.stats().meandens,
+                       rete=O.cosmo(0.678, 0.31, 0.0)[1])
+    compare(out, lambda kk: _as_ref(ref)[kk])
+    c2 = _ctx(n, L0, per)
+    c2.set_mask(sm)
+    c2.load_void_fields(g["ref_u1co"], g["ref_div"], float(g["ref_meandens"]))
+    compare(gpu_void_outputs(c2), lambda kk: g[f"ref_{kk}"])

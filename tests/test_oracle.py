@@ -91,6 +91,26 @@ def test_velocity_free_input_matches_reference(golden_dir, name):
     compare_void_outputs(res, lambda kk: g[f"ref_{kk}"])
 
 
+@pytest.mark.parametrize("name", ["svy_p0", "svy_p1"])
+def test_survey_input_matches_reference(golden_dir, name):
+    """Survey mode (FLAG_SURVEY = 1, avism.f90:1415-1597): the randoms'
+    density with min(8k, 256) neighbours, D/(alpha R) clipped and renormalised
+    on observed cells, linear-theory DIVERCO; then the masked void stage."""
+    g = _load(golden_dir, name)
+    n, L0, per, k = (int(g["meta_n"]), float(g["meta_L0"]), int(g["meta_periodic"]), int(g["meta_k"]))
+    f = O.survey_fields((g["in_x"], g["in_y"], g["in_z"], g["in_m"]), (g["in_rx"], g["in_ry"], g["in_rz"],
+                        g["in_rm"]), n, L0, per, g["in_smask"], k=k, dens_interp=int(g["meta_dens_interp"]))
+    for key in ("hpart", "mass", "urand", "u1co", "div"):
+        assert np.array_equal(f[key], g[f"ref_{key}"]), key
+    assert f["meandens"] == float(g["ref_meandens"])
+    obs = g["in_smask"] > 0
+    assert abs(np.mean(f["u1co"][obs].astype(np.float64)) - 1.0) < 1e-6  # renormalised on observed cells
+    rete = O.cosmo(0.678, 0.31, 0.0)[1]
+    res = O.void_stage(f["grid"], f["u1co"], f["div"], PARAMS, smask=g["in_smask"], meandens=f["meandens"],
+                       rete=rete)
+    compare_void_outputs(res, lambda kk: g[f"ref_{kk}"])
+
+
 @pytest.mark.parametrize("name", ["tsc_p0", "tsc_p1", "tscv_p0", "tscv_p1"])
 def test_tsc_input_matches_reference(golden_dir, name):
     """VEL_INTERP_TSC, DENS_INTERP_TSC and H_DISTANCE (particles.f90:16-351)."""
