@@ -86,6 +86,26 @@ module avism_c
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: k, dens_interp
     end function
+    integer(c_int) function avism_set_mask(ctx, smask, where) bind(C, name="avism_set_mask")
+      import :: c_ptr, c_int, c_int8_t, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int8_t) :: smask(*)
+      integer(c_int32_t), value :: where
+    end function
+    integer(c_int) function avism_load_randoms(ctx, nrand, x, y, z, m, where) &
+        bind(C, name="avism_load_randoms")
+      import :: c_ptr, c_int, c_int64_t, c_int32_t, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nrand
+      real(c_float) :: x(*), y(*), z(*), m(*)
+      integer(c_int32_t), value :: where
+    end function
+    integer(c_int) function avism_survey_density(ctx, k, dens_interp) &
+        bind(C, name="avism_survey_density")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: k, dens_interp
+    end function
     integer(c_int) function avism_load_grid_fields(ctx, delta, vx, vy, vz, where) &
         bind(C, name="avism_load_grid_fields")
       import :: c_ptr, c_int, c_int32_t, c_float

@@ -182,8 +182,12 @@ contains
       write(*,*) '     ERROR: only FLAG_DATA = 1 (BinPart) and 2 (BinGrid) are supported by this build'
       stop 1
     end if
-    if (flag_survey /= 0) then
-      write(*,*) '     ERROR: survey mode is not supported by this build'
+    if (flag_survey == 1 .and. flag_period == 1) then  ! avism.f90:278-281
+      write(*,*) 'PBCs considered in voids.dat but SURVEY/MASKED conditions activated'
+      stop 1
+    end if
+    if (flag_survey == 1 .and. (flag_data /= 1 .or. flag_vel_available /= 0)) then
+      write(*,*) '     ERROR: survey mode needs particle input without velocities in this build'
       stop 1
     end if
     if (flag_vel_available == 0 .and. flag_data /= 1) then
@@ -237,13 +241,49 @@ contains
     call avism_check(ctx, avism_set_cosmology(ctx, ache, omega0, z), 'avism_set_cosmology')
     call avism_check(ctx, avism_load_particles(ctx, int(np, c_int64_t), x, y, zz, vx, vy, vz, m, AVISM_HOST), &
                   'avism_load_particles')
+    if (flag_survey == 1) call load_survey()
     call particle_fields()
   end subroutine
 
   ! the field stage on the loaded particles: with velocities (avism.f90:1159-1257)
   ! or without (FLAG_VEL_AVAILABLE = 0, avism.f90:1259-1300, 1646-1650)
+  ! survey mode inputs (avism.f90:403-418, 1431-1441): input_data/mask.dat
+  ! (stream: NMASK, three REAL*4, NMASK^3 INTEGER*1; NMASK = NCOX) and
+  ! input_data/rands.dat (sequential: NRAND, then x, y, z, masses)
+  subroutine load_survey()
+    integer :: nmask
+    integer(c_int32_t) :: nrand
+    real :: mlow, mup, mside
+    integer(c_int8_t), allocatable :: sm(:)
+    real(c_float), allocatable :: rx(:), ry(:), rz(:), rm(:)
+    open(unit=16, file='input_data/mask.dat', access='stream', form='unformatted', status='old', action='read')
+    read(16) nmask
+    if (nmask /= ncox) then
+      write(*,*) 'MASK grid does not coincide with AVISM coarse grid', nmask, ncox
+      stop 1
+    end if
+    read(16) mlow, mup, mside
+    allocate(sm(int(nmask, 8)**3))
+    read(16) sm
+    close(16)
+    call avism_check(ctx, avism_set_mask(ctx, sm, AVISM_HOST), 'avism_set_mask')
+    open(unit=17, file='input_data/rands.dat', form='unformatted', status='old', action='read')
+    read(17) nrand
+    allocate(rx(nrand), ry(nrand), rz(nrand), rm(nrand))
+    read(17) rx
+    read(17) ry
+    read(17) rz
+    read(17) rm
+    close(17)
+    call avism_check(ctx, avism_load_randoms(ctx, int(nrand, c_int64_t), rx, ry, rz, rm, AVISM_HOST), &
+                     'avism_load_randoms')
+  end subroutine
+
   subroutine particle_fields()
-    if (flag_vel_available == 1) then
+    if (flag_survey == 1) then
+      call avism_check(ctx, avism_survey_density(ctx, int(kneighbours, c_int32_t), &
+                                              int(flag_dens_interp, c_int32_t)), 'avism_survey_density')
+    else if (flag_vel_available == 1) then
       call avism_check(ctx, avism_particles_to_grid(ctx, int(kneighbours, c_int32_t), &
                                                  int(flag_vel_interp, c_int32_t), &
                                                  int(flag_dens_interp, c_int32_t)), 'avism_particles_to_grid')

@@ -216,3 +216,56 @@ def test_host_two_levels_match_binding(tmp_path):
         assert np.array_equal(blk[m:2 * m].view(np.float32), f["u1co"] - np.float32(1.0))
         assert np.array_equal(blk[2 * m:].view(np.float32), f["div"])
     assert pos == len(tok) and mpos == raw.size
+
+
+def test_host_survey_matches_binding(tmp_path):
+    """Survey mode through the host: input_data/mask.dat (stream), rands.dat
+    (sequential records) and a bin_file_part without velocities; the map and
+    catalogue equal the binding's avism_survey_density run."""
+    from avism_amd import synth
+    from avism_amd.config import VoidsConfig
+    from avism_amd._lib import Context
+    n, L0 = 32, 64.0
+    arr = synth.zeldovich(n, L0, seed=19).numpy()
+    gal = [arr[0], arr[1], arr[2], arr[6]]
+    rng = np.random.default_rng(19)
+    nr = 2 * n ** 3
+    rands = [rng.uniform(-L0 / 2, L0 / 2, nr).astype(np.float32) for _ in range(3)]
+    rands.append(np.full(nr, 2e10, np.float32))
+    sm = synth.survey_mask(n, 0.2, seed=19)
+    # survey mode is non-periodic (avism.f90:278-281)
+    cfg = VoidsConfig(first=3, last=3, every=1, ncox=n, ncoy=n, ncoz=n, lado0=L0, flag_period=0, flag_data=1,
+                      kneighbours=16, partired=10 ** 8, flag_vel_available=0, flag_survey=1)
+    (tmp_path / "input_data").mkdir()
+    with open(tmp_path / "input_data" / "bin_file_part00003", "wb") as f:
+        _record(f, np.int64(gal[0].size), np.float32(0.0))
+        for a in gal:
+            _record(f, a.astype(np.float32))
+    with open(tmp_path / "input_data" / "mask.dat", "wb") as f:
+        f.write(np.int32(n).tobytes() + np.array([-L0 / 2, L0 / 2, L0], np.float32).tobytes())
+        f.write(np.ascontiguousarray(sm, np.int8).tobytes())
+    with open(tmp_path / "input_data" / "rands.dat", "wb") as f:
+        _record(f, np.int32(nr))
+        for a in rands:
+            _record(f, a)
+    _run_host(tmp_path, cfg)
+    c = Context(0)
+    c.set_grid(n, L0, 0)
+    c.set_cosmology(cfg.ache, cfg.omega0, 0.0)
+    c.set_void_params(cfg.dens_thre, cfg.dens_thre2, cfg.grad_thre, cfg.div_thre, cfg.rmin, cfg.rmin_sub,
+                      cfg.nvoid_max)
+    c.set_mask(sm)
+    c.load_particles(gal[0], gal[1], gal[2], None, None, None, gal[3])
+    c.load_randoms(*rands)
+    c.survey_density(k=16)
+    nv = c.find_voids(0)
+    c.merge_and_post()
+    cat = c.catalogue(nv)
+    f = c.fields()
+    marca, delta, div = _read_map(tmp_path / "output_files" / "map00003", n)
+    assert np.array_equal(marca, c.marca(inner_only=True))
+    assert np.array_equal(delta, f["u1co"] - np.float32(1.0))
+    assert np.array_equal(div, f["div"])
+    _, level, rows = _read_voids(tmp_path / "output_files" / "voids00003")
+    assert int(level[1]) == nv and int(level[2]) == int(cat["stats"].nvoidt)
+    assert int(cat["stats"].nvoidt) > 0
