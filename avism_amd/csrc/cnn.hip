@@ -500,10 +500,14 @@ __global__ void k_adam(int n, float* p, const float* __restrict__ g, float* m, f
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// proposal of every candidate: features, forward, argmax (first maximum wins)
+// proposal of every candidate: features, forward, argmax (first maximum wins).
+// Two waves per SIMD: the register cap spills 160 bytes per lane of weight
+// fragments, which costs less than the latency one wave per SIMD leaves
+// exposed (5.6 -> 4.0 ms at 123,455 candidates; three waves, with 540 bytes
+// spilled, took 5.0 ms)
 constexpr int FWD_LDS = 5840 + 352 * 16 * 2;
 
-__global__ void __launch_bounds__(64) k_propose(const float* __restrict__ P, Feat F, int64_t ncand,
+__global__ void __launch_bounds__(64, 2) k_propose(const float* __restrict__ P, Feat F, int64_t ncand,
                                                 int8_t* prop) {
   __shared__ __attribute__((aligned(16))) unsigned char sm[FWD_LDS];
   uint16_t* X0 = (uint16_t*)sm;
