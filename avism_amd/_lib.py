@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libavism_hip.so"
 
 AVISM_OK = 0
-AVISM_HOST, AVISM_DEVICE = 0, 1
+AVISM_HOST, AVISM_DEVICE, AVISM_HOST_ASYNC = 0, 1, 2
 GROW_PLAIN, GROW_PROPOSAL, GROW_LEARNED = 0, 1, 2
 CNN_NPARAM, CNN_NCLASS, CNN_CACHE, CNN_PATCH = 19064, 8, 7920, 2187
 
@@ -59,6 +59,7 @@ EXPORTS = [
     "avism_set_proposals", "avism_get_candidates", "avism_get_cubes", "avism_indexx",
     "avism_merge_and_post", "avism_get_catalogue", "avism_get_marca", "avism_run_level",
     "avism_cnn_set_params", "avism_cnn_patches", "avism_cnn_forward", "avism_cnn_backward", "avism_cnn_adam",
+    "avism_sync", "avism_get_counters",
 ]
 
 _lib = None
@@ -118,6 +119,8 @@ def lib():
         "avism_cnn_forward": ([vp, vp, i64, vp, vp, vp], C.c_int),
         "avism_cnn_backward": ([vp, vp, i64, vp, vp, vp, vp, vp, vp], C.c_int),
         "avism_cnn_adam": ([vp, vp, vp, vp, vp, f32, i32], C.c_int),
+        "avism_sync": ([vp], C.c_int),
+        "avism_get_counters": ([vp, vp, i32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -225,12 +228,20 @@ class Context:
         self._chk(lib().avism_set_timing(self._h, int(bool(on))))
 
     def timings(self):
-        n = C.c_int32(256)
-        ms = np.zeros(256, np.float32)
-        names = C.create_string_buffer(1 << 14)
-        self._chk(lib().avism_get_timings(self._h, names, len(names), _ptr(ms), C.byref(n)))
-        keys = names.value.decode().split(";")[: n.value]
-        return list(zip(keys, ms[: n.value].tolist()))
+        """(stage, ms) of every stage timed since the last call, in order."""
+        cap, nbytes = 256, 1 << 14
+        while True:
+            n = C.c_int32(cap)
+            ms = np.zeros(cap, np.float32)
+            names = C.create_string_buffer(nbytes)
+            rc = lib().avism_get_timings(self._h, names, len(names), _ptr(ms), C.byref(n))
+            if rc == -4:  # AVISM_ERR_LIMIT: buffers too small, nothing consumed
+                cap, nbytes = max(cap, n.value), max(2 * nbytes, 48 * max(cap, n.value))
+                continue
+            self._chk(rc)
+            keys = names.value.decode().split(";")[: n.value]
+            assert len(keys) == n.value
+            return list(zip(keys, ms[: n.value].tolist()))
 
     # ---- field stage
     def load_particles(self, x, y, z, vx, vy, vz, m, where=None):
@@ -279,6 +290,27 @@ class Context:
         self._chk(lib().avism_get_fields(self._h, _ptr(out["u1co"]), _ptr(out["div"]), _ptr(out["u"]),
                                          _ptr(out["v"]), _ptr(out["w"]), AVISM_HOST))
         return out
+
+    def fields_async(self, u1co=None, div=None):
+        """Queue the download of U1CO and DIVERCO into caller-owned pinned host
+        tensors (torch, pin_memory=True) behind the work queued so far; they are
+        complete after sync()."""
+        n3 = self.grid().n ** 3
+        for t in (u1co, div):
+            if t is not None and (not t.is_pinned() or t.numel() != n3 or t.element_size() != 4
+                                  or not t.is_contiguous()):
+                raise ValueError("fields_async needs contiguous pinned 4-byte host tensors of n^3 cells")
+        self._chk(lib().avism_get_fields(self._h, _ptr(u1co), _ptr(div), None, None, None, AVISM_HOST_ASYNC))
+
+    def sync(self):
+        self._chk(lib().avism_sync(self._h))
+
+    def counters(self):
+        """Work counters: ddens (particle, cell) pairs, PPART queries, VVEL queries, growth shells."""
+        out = np.zeros(8, np.int64)
+        self._chk(lib().avism_get_counters(self._h, _ptr(out), 8))
+        return dict(ddens_pairs=int(out[0]), ppart_queries=int(out[1]), vvel_queries=int(out[2]),
+                    growth_shells=int(out[3]))
 
     def particle_fields(self, np_total: int):
         pd = np.zeros(np_total, np.float32)
@@ -339,10 +371,15 @@ class Context:
         self._chk(lib().avism_get_catalogue(self._h, None, None, None, None, C.byref(st)))
         return st
 
-    def marca(self, inner_only=False):
+    def marca(self, inner_only=False, out=None):
+        """MARCA (ext grid, or the inner grid for the map file); out: a
+        caller-owned host array or tensor of that many int32 to fill."""
         gi = self.grid()
         n = gi.n ** 3 if inner_only else gi.next ** 3
-        out = np.zeros(n, np.int32)
+        if out is None:
+            out = np.zeros(n, np.int32)
+        elif (out.numel() if hasattr(out, "numel") else out.size) != n:
+            raise ValueError("MARCA output has the wrong size")
         self._chk(lib().avism_get_marca(self._h, _ptr(out), int(bool(inner_only)), AVISM_HOST))
         return out
 

@@ -65,6 +65,15 @@ struct Timing {
 struct avism_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // host<->device copies that overlap the kernels run on cstream (default
+  // priority, like stream): the particle velocities are uploaded there while
+  // binning and PPART_DENS run (ev_vel orders their first reader), and
+  // avism_get_fields(AVISM_HOST_ASYNC) downloads the maps there while the void
+  // stage runs (copy_pending: wait for cstream before the fields change)
+  hipStream_t cstream = nullptr;
+  hipEvent_t ev_vel = nullptr, ev_mark = nullptr;
+  bool copy_pending = false;
+  int64_t counters[8] = {0};                   // work counters (avism_get_counters)
   std::string err;
   avism::GridGeom g{};
   bool grid_set = false;
@@ -94,6 +103,7 @@ struct avism_ctx {
   bool has_marcap = false;
   avism::DBuf smask;                           // inner SMASK (int8), when has_mask
   bool has_mask = false;
+  int64_t mask_observed = 0;                   // cells with SMASK != 0
   avism::DBuf reqp;                            // REQP of the parent level, ids 1..nreqp
   int32_t nreqp = 0;
   bool has_reqp = false;
@@ -176,24 +186,30 @@ int cnn_propose(avism_ctx* c);
 
 // the small device scalars: counters and flags in the first 4 KB, RADX at
 // 4096, then 8-byte slots for kRedSlices partial sums
-inline size_t scalars_slices_offset(const avism_ctx* c) { return (4096 + c->g.rad.size() * 4 + 127) & ~(size_t)63; }
-inline size_t scalars_bytes(const avism_ctx* c) { return scalars_slices_offset(c) + 512 * 8; }
+inline size_t scalars_slices_offset(const GridGeom& g) { return (4096 + g.rad.size() * 4 + 127) & ~(size_t)63; }
+inline size_t scalars_bytes(const GridGeom& g) { return scalars_slices_offset(g) + 512 * 8; }
+inline size_t scalars_slices_offset(const avism_ctx* c) { return scalars_slices_offset(c->g); }
+inline size_t scalars_bytes(const avism_ctx* c) { return scalars_bytes(c->g); }
 
-// scoped stage timer on the context stream
+// wait (on the host) for async map downloads before their source changes
+int finish_copies(avism_ctx* c);
+
+// scoped stage timer on the context stream (or another stream of the context)
 struct StageTimer {
   avism_ctx* c;
   const char* name;
+  hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
-  StageTimer(avism_ctx* c_, const char* n_) : c(c_), name(n_) {
+  StageTimer(avism_ctx* c_, const char* n_, hipStream_t s_ = nullptr) : c(c_), name(n_), s(s_ ? s_ : c_->stream) {
     if (c->timing_on) {
       (void)hipEventCreate(&a);
       (void)hipEventCreate(&b);
-      (void)hipEventRecord(a, c->stream);
+      (void)hipEventRecord(a, s);
     }
   }
   ~StageTimer() {
     if (c->timing_on) {
-      (void)hipEventRecord(b, c->stream);
+      (void)hipEventRecord(b, s);
       add_timing(c, name, a, b);
     }
   }

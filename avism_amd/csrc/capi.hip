@@ -42,7 +42,13 @@ int avism_create(avism_ctx** out, int32_t device) {
   if (hipSetDevice(device) != hipSuccess) return AVISM_ERR_HIP;
   avism_ctx* c = new avism_ctx();
   c->device = device;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  // both streams at the default priority
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_vel, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_mark, hipEventDisableTiming) != hipSuccess) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->cstream) (void)hipStreamDestroy(c->cstream);
     delete c;
     return AVISM_ERR_HIP;
   }
@@ -64,7 +70,11 @@ int avism_destroy(avism_ctx* c) {
   if (!c) return AVISM_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->cstream);
   drop_timings(c);
+  (void)hipEventDestroy(c->ev_vel);
+  (void)hipEventDestroy(c->ev_mark);
+  (void)hipStreamDestroy(c->cstream);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return AVISM_OK;
@@ -81,24 +91,30 @@ int avism_set_timing(avism_ctx* c, int32_t on) {
 
 int avism_get_timings(avism_ctx* c, char* names, int32_t names_len, float* ms, int32_t* count) {
   CTX_GUARD(c);
-  int32_t cap = count ? *count : 0;
-  int32_t nwr = 0;
+  if (!count) AV_FAIL(c, AVISM_ERR_ARG, "count must not be NULL");
+  const int32_t cap = *count;
+  const int32_t nwr = (int32_t)c->timings.size();
   std::string all;
   for (auto& t : c->timings) {
-    float tm = 0.0f;
-    (void)hipEventSynchronize(t.b);
-    (void)hipEventElapsedTime(&tm, t.a, t.b);
-    if (nwr < cap && ms) ms[nwr] = tm;
     all += t.name;
     all += ';';
-    nwr++;
   }
-  if (names && names_len > 0) {
-    size_t nb = std::min((size_t)names_len - 1, all.size());
-    memcpy(names, all.data(), nb);
-    names[nb] = 0;
+  *count = nwr;
+  // too small a buffer: nothing is consumed, the caller retries with *count
+  // entries and a names buffer of the size the message gives
+  if (!ms || cap < nwr || !names || (size_t)names_len < all.size() + 1) {
+    c->err = "timings buffer too small: " + std::to_string(nwr) + " entries, " +
+             std::to_string(all.size() + 1) + " name bytes";
+    return AVISM_ERR_LIMIT;
   }
-  if (count) *count = nwr;
+  for (int32_t i = 0; i < nwr; ++i) {
+    float tm = 0.0f;
+    AV_CHECK_HIP(c, hipEventSynchronize(c->timings[i].b));
+    AV_CHECK_HIP(c, hipEventElapsedTime(&tm, c->timings[i].a, c->timings[i].b));
+    ms[i] = tm;
+  }
+  memcpy(names, all.data(), all.size());
+  names[all.size()] = 0;
   drop_timings(c);
   return AVISM_OK;
 }
@@ -108,16 +124,24 @@ int avism_set_grid(avism_ctx* c, int32_t n, float L0, int32_t periodic) {
   if (n < 4) AV_FAIL(c, AVISM_ERR_ARG, "grid size must be at least 4 cells per axis");
   if (!(L0 > 0.0f)) AV_FAIL(c, AVISM_ERR_ARG, "box size must be positive");
   if (periodic != 0 && periodic != 1) AV_FAIL(c, AVISM_ERR_ARG, "periodic must be 0 or 1");
-  int64_t next = 0;
-  c->g = make_grid(n, L0, periodic);
-  next = c->g.next;
+  if (int rc = finish_copies(c)) return rc;
+  // the new geometry is validated and its scalars allocated before any state
+  // changes: a rejected grid leaves the context unusable (AVISM_ERR_STATE on
+  // every later stage call) rather than holding extents its buffers lack
+  GridGeom g = make_grid(n, L0, periodic);
+  c->grid_set = false;
+  c->fields_ready = false;
+  c->post_ready = false;
+  const int64_t next = g.next;
   if (next * next * next >= (int64_t)0x7fffffff)
     AV_FAIL(c, AVISM_ERR_LIMIT, "extended grid above 2^31 cells is not supported by this build");
   // small device scalars: counters and sums in the first 4 KiB, the cell-centre
   // table RADX from byte 4096 on (read by the field and shape kernels)
-  if (!c->scalars.ensure(avism::scalars_bytes(c)))
+  if (!c->scalars.ensure(avism::scalars_bytes(g)))
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (scalars)");
+  c->g = std::move(g);
   c->grid_set = true;
+  c->nrand = 0;  // survey randoms belong to the previous grid's box
   c->fields_ready = false;
   c->post_ready = false;
   c->np = 0;
@@ -213,12 +237,23 @@ int avism_set_mask(avism_ctx* c, const int8_t* smask, int32_t where) {
   if (!c->smask.ensure(b)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
   AV_CHECK_HIP(c, hipMemcpy(c->smask.p, smask, b,
                             where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  // observed cells, for the query counter
+  int64_t nobs = 0;
+  if (where == AVISM_DEVICE) {
+    std::vector<int8_t> h(b);
+    AV_CHECK_HIP(c, hipMemcpy(h.data(), c->smask.p, b, hipMemcpyDeviceToHost));
+    for (int8_t v : h) nobs += v != 0;
+  } else {
+    for (size_t i = 0; i < b; ++i) nobs += smask[i] != 0;
+  }
+  c->mask_observed = nobs;
   c->has_mask = true;
   return AVISM_OK;
 }
 
 int avism_next_level(avism_ctx* c) {
   CTX_GUARD(c);
+  if (int rc = finish_copies(c)) return rc;
   return next_level(c);
 }
 
@@ -274,6 +309,9 @@ int avism_load_void_fields(avism_ctx* c, const float* u1co, const float* div, fl
   DEVICE_ARGS(c, where);
   if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "avism_set_grid must be called first");
   if (!u1co || !div) AV_FAIL(c, AVISM_ERR_ARG, "null field array");
+  if (int rc = finish_copies(c)) return rc;
+  c->nvoid = 0;
+  c->post_ready = false;
   size_t b = (size_t)c->g.n * c->g.n * c->g.n * 4;
   if (!c->u1co.ensure(b) || !c->div.ensure(b) || !c->fu.ensure(b) || !c->fv.ensure(b) || !c->fw.ensure(b))
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
@@ -293,14 +331,40 @@ int avism_get_fields(avism_ctx* c, float* u1co, float* div, float* u, float* v, 
                      int32_t where) {
   CTX_GUARD(c);
   DEVICE_ARGS(c, where);
+  if (where != AVISM_HOST && where != AVISM_DEVICE && where != AVISM_HOST_ASYNC)
+    AV_FAIL(c, AVISM_ERR_ARG, "where must be AVISM_HOST, AVISM_DEVICE or AVISM_HOST_ASYNC");
   if (!c->fields_ready) AV_FAIL(c, AVISM_ERR_STATE, "fields not computed");
   size_t b = (size_t)c->g.n * c->g.n * c->g.n * 4;
   hipMemcpyKind kind = where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   float* dst[5] = {u1co, div, u, v, w};
   void* src[5] = {c->u1co.p, c->div.p, c->fu.p, c->fv.p, c->fw.p};
+  if (where == AVISM_HOST_ASYNC) {  // on the copy stream, after the work queued so far
+    AV_CHECK_HIP(c, hipEventRecord(c->ev_mark, c->stream));
+    AV_CHECK_HIP(c, hipStreamWaitEvent(c->cstream, c->ev_mark, 0));
+    StageTimer t(c, "download_fields", c->cstream);
+    for (int i = 0; i < 5; i++)
+      if (dst[i]) AV_CHECK_HIP(c, hipMemcpyAsync(dst[i], src[i], b, kind, c->cstream));
+    c->copy_pending = true;
+    return AVISM_OK;
+  }
   for (int i = 0; i < 5; i++)
     if (dst[i]) AV_CHECK_HIP(c, hipMemcpyAsync(dst[i], src[i], b, kind, c->stream));
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  return AVISM_OK;
+}
+
+int avism_sync(avism_ctx* c) {
+  CTX_GUARD(c);
+  AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  return finish_copies(c);
+}
+
+int avism_get_counters(avism_ctx* c, int64_t* out, int32_t n) {
+  CTX_GUARD(c);
+  if (!out || n < 0) AV_FAIL(c, AVISM_ERR_ARG, "bad counter request");
+  const int32_t m = n < 8 ? n : 8;
+  for (int32_t i = 0; i < m; ++i) out[i] = c->counters[i];
+  for (int32_t i = m; i < n; ++i) out[i] = 0;
   return AVISM_OK;
 }
 

@@ -115,23 +115,20 @@ struct Kahan {
 };
 
 // ----------------------------------------------------------------- kernels
-// unit conversion (reader.f90:457-461), the box check (avism.f90:573-575) and
-// the largest |mass| (for the DDENS fixed point), and whether every mass
-// equals the first (m0raw: the first mass as given, copied aside before the
-// conversion in place); grid-stride with one atomic per block, since
-// same-address atomics serialise
-__global__ void __launch_bounds__(256) k_convert_check(int64_t np, float* vx, float* vy, float* vz, float* m,
-                                                      const float* x, const float* y, const float* z, float half,
-                                                      int* bad, unsigned* mmax, const float* m0raw, int* mixed) {
+// unit conversion of the masses (reader.f90:457-461), the box check
+// (avism.f90:573-575), the largest |mass| (for the DDENS fixed point), and
+// whether every mass equals the first (m0raw: the first mass as given, copied
+// aside before the conversion in place); grid-stride with one atomic per
+// block, since same-address atomics serialise
+__global__ void __launch_bounds__(256) k_convert_check(int64_t np, float* m, const float* x, const float* y,
+                                                      const float* z, float half, int* bad, unsigned* mmax,
+                                                      const float* m0raw, int* mixed) {
   unsigned mb = 0;
   bool out = false, diff = false;
   const float m0 = *m0raw / kUM;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
     const float mi = m[i] / kUM;
     m[i] = mi;
-    vx[i] = vx[i] / kUV;
-    vy[i] = vy[i] / kUV;
-    vz[i] = vz[i] / kUV;
     const float a = x[i], b = y[i], c = z[i];
     out = out || (a < -half || a > half || b < -half || b > half || c < -half || c > half);
     mb = max(mb, __float_as_uint(fabsf(mi)));  // non-negative floats order as their bits
@@ -152,6 +149,15 @@ __global__ void __launch_bounds__(256) k_convert_check(int64_t np, float* vx, fl
     if (bm) atomicMax(mmax, bm);
     if (wbad[0] | wbad[1] | wbad[2] | wbad[3]) atomicOr(bad, 1);
     if (wdiff[0] | wdiff[1] | wdiff[2] | wdiff[3]) atomicOr(mixed, 1);
+  }
+}
+
+// velocities to code units (reader.f90:457-459), on the copy stream
+__global__ void __launch_bounds__(256) k_convert_vel(int64_t np, float* vx, float* vy, float* vz) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+    vx[i] = vx[i] / kUV;
+    vy[i] = vy[i] / kUV;
+    vz[i] = vz[i] / kUV;
   }
 }
 
@@ -504,7 +510,8 @@ constexpr int DTX = 8, DTY = 8, DTZ = 4, DR = 4;
 constexpr int DBX = DTX + 2 * DR, DBY = DTY + 2 * DR, DBZ = DTZ + 2 * DR;
 
 __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, const float* __restrict__ hpart,
-                                                      double scale, unsigned long long* acc, float* norm) {
+                                                      double scale, unsigned long long* acc, float* norm,
+                                                      unsigned long long* npairs) {
   __shared__ unsigned long long box[DBX * DBY * DBZ];
   __shared__ int qrow[DTY * DTZ + 1], qbeg[DTY * DTZ];
   const int n = G.n;
@@ -535,6 +542,7 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
   // widened tile in 1-based cells
   const int bx0 = tx0 + 1 - DR, by0 = ty0 + 1 - DR, bz0 = tz0 + 1 - DR;
   const int nq = qrow[DTY * DTZ];
+  unsigned pairs = 0;  // (particle, cell) pairs within h: the work of the gather model
   for (int q = tid; q < nq; q += 256) {
     int r = 0;
     while (qrow[r + 1] <= q) ++r;
@@ -603,6 +611,7 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
             const float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);
             if (!ok) continue;
             if (pass == 0) {
+              ++pairs;
               nrm = nrm + wgt;  // particles.f90:520-547, in the reference's loop order
             } else {
               const float v = (wgt * p.w) / nrm;  // particles.f90:599-600
@@ -613,6 +622,8 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
       }
     }
   }
+  for (int o = 32; o > 0; o >>= 1) pairs += (unsigned)__shfl_xor((int)pairs, o);
+  if ((tid & 63) == 0 && pairs) atomicAdd(npairs, (unsigned long long)pairs);
   __syncthreads();
   for (int c = tid; c < DBX * DBY * DBZ; c += 256) {
     const unsigned long long v = box[c];
@@ -778,33 +789,64 @@ static Bins make_bins(avism_ctx* c) {
   return B;
 }
 
+int finish_copies(avism_ctx* c) {
+  if (!c->copy_pending) return AVISM_OK;
+  AV_CHECK_HIP(c, hipStreamSynchronize(c->cstream));
+  c->copy_pending = false;
+  return AVISM_OK;
+}
+
+// Positions and masses go up on the compute stream, which binning and
+// PPART_DENS need first; then velocities on the copy stream, so that with
+// pinned host arrays their transfer overlaps binning and PPART_DENS (ev_vel
+// orders their first reader).  Returns once positions and masses are checked.
 int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, const float* z,
                    const float* vx, const float* vy, const float* vz, const float* m, int where) {
   if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "avism_set_grid must be called first");
   if (np <= 0) AV_FAIL(c, AVISM_ERR_ARG, "np must be positive");
   if (np >= (int64_t)0x7fffffff) AV_FAIL(c, AVISM_ERR_LIMIT, "more than 2^31-1 particles per call");
+  // earlier copies on the copy stream end before any buffer is reused
+  AV_CHECK_HIP(c, hipStreamSynchronize(c->cstream));
+  c->copy_pending = false;
   size_t b = (size_t)np * 4;
   DBuf* bufs[7] = {&c->px, &c->py, &c->pz, &c->pvx, &c->pvy, &c->pvz, &c->pm};
   const float* src[7] = {x, y, z, vx, vy, vz, m};
   hipMemcpyKind kind = where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  for (int i = 0; i < 7; i++) {
+  for (int i = 0; i < 7; i++)
     if (!bufs[i]->ensure(b)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (particles)");
-    if (src[i])
-      AV_CHECK_HIP(c, hipMemcpyAsync(bufs[i]->p, src[i], b, kind, c->stream));
-    else  // no velocities (READ_BINARY_PART_2 data): zero, as the reader leaves them
-      AV_CHECK_HIP(c, hipMemsetAsync(bufs[i]->p, 0, b, c->stream));
-  }
   c->np = np;
   c->fields_ready = false;
   // [0] out-of-box flag, [1] max |mass| bits, [2] the first mass as given, [3] masses differ
   int* bad = (int*)c->scalars.p;
-  AV_CHECK_HIP(c, hipMemsetAsync(bad, 0, 16, c->stream));
-  AV_CHECK_HIP(c, hipMemcpyAsync(bad + 2, c->pm.p, 4, hipMemcpyDeviceToDevice, c->stream));
-  k_convert_check<<<(unsigned)std::min<int64_t>(nblk(np), 2048), 256, 0, c->stream>>>(np, c->pvx.as<float>(), c->pvy.as<float>(),
-                                                   c->pvz.as<float>(), c->pm.as<float>(),
-                                                   c->px.as<float>(), c->py.as<float>(),
-                                                   c->pz.as<float>(), c->g.L0 / 2.0f, bad,
-                                                   (unsigned*)bad + 1, (const float*)(bad + 2), bad + 3);
+  {
+    StageTimer t(c, "upload_pos_mass");
+    for (int i : {0, 1, 2, 6}) AV_CHECK_HIP(c, hipMemcpyAsync(bufs[i]->p, src[i], b, kind, c->stream));
+    // the velocities follow on the copy stream once positions and masses are
+    // in (sharing the host link would delay what binning needs first), and
+    // after all earlier work, which may still read the previous particles
+    AV_CHECK_HIP(c, hipEventRecord(c->ev_mark, c->stream));
+    AV_CHECK_HIP(c, hipStreamWaitEvent(c->cstream, c->ev_mark, 0));
+    AV_CHECK_HIP(c, hipMemsetAsync(bad, 0, 16, c->stream));
+    AV_CHECK_HIP(c, hipMemcpyAsync(bad + 2, c->pm.p, 4, hipMemcpyDeviceToDevice, c->stream));
+    k_convert_check<<<(unsigned)std::min<int64_t>(nblk(np), 2048), 256, 0, c->stream>>>(
+        np, c->pm.as<float>(), c->px.as<float>(), c->py.as<float>(), c->pz.as<float>(), c->g.L0 / 2.0f, bad,
+        (unsigned*)bad + 1, (const float*)(bad + 2), bad + 3);
+    AV_CHECK_HIP(c, hipGetLastError());
+  }
+  {
+    StageTimer t(c, "upload_vel", c->cstream);
+    for (int i = 3; i < 6; i++) {
+      if (src[i])
+        AV_CHECK_HIP(c, hipMemcpyAsync(bufs[i]->p, src[i], b, kind, c->cstream));
+      else  // no velocities (READ_BINARY_PART_2 data): zero, as the reader leaves them
+        AV_CHECK_HIP(c, hipMemsetAsync(bufs[i]->p, 0, b, c->cstream));
+    }
+    if (vx)
+      k_convert_vel<<<(unsigned)std::min<int64_t>(nblk(np), 4096), 256, 0, c->cstream>>>(
+          np, c->pvx.as<float>(), c->pvy.as<float>(), c->pvz.as<float>());
+    AV_CHECK_HIP(c, hipGetLastError());
+  }
+  AV_CHECK_HIP(c, hipEventRecord(c->ev_vel, c->cstream));
   int hbad[4] = {0, 0, 0, 0};
   AV_CHECK_HIP(c, hipMemcpyAsync(hbad, bad, 16, hipMemcpyDeviceToHost, c->stream));
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
@@ -904,6 +946,10 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   const int64_t np = c->np;
   const int n = c->g.n;
   const int64_t ncell = (int64_t)n * n * n;
+  if (int rc = finish_copies(c)) return rc;
+  for (int64_t& v : c->counters) v = 0;
+  c->nvoid = 0;  // cubes, MARCA and the catalogue of earlier fields are stale
+  c->post_ready = false;
   int rc = build_bins(c);
   if (rc) return rc;
   if (!c->pdens.ensure((size_t)np * 4) || !c->hk.ensure((size_t)np * 4) ||
@@ -934,6 +980,8 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
       });
       AV_CHECK_HIP(c, hipGetLastError());
     }
+    c->counters[1] = np;
+    AV_CHECK_HIP(c, hipStreamWaitEvent(c->stream, c->ev_vel, 0));  // velocities uploaded
     k_velrec<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), c->spos4.as<float4>(),
                                               c->pdens.as<float>(), c->pvx.as<float>(), c->pvy.as<float>(),
                                               c->pvz.as<float>(), c->vel4.as<float4>());
@@ -950,6 +998,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
         constexpr int K = decltype(KC)::value;
         k_vvel<K><<<vvel_tiles(n, K), kq_of(K), 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
       });
+      c->counters[2] = smask ? c->mask_observed : ncell;
       AV_CHECK_HIP(c, hipGetLastError());
     }
     {
@@ -966,6 +1015,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     if (!c->tsc_acc.ensure((size_t)ncell * 32)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (TSC)");
     AV_CHECK_HIP(c, hipMemsetAsync(c->pdens.p, 0, (size_t)np * 4, c->stream));
     StageTimer t(c, "vel_tsc");
+    AV_CHECK_HIP(c, hipStreamWaitEvent(c->stream, c->ev_vel, 0));  // velocities uploaded
     k_fill_u64<<<nblk(ncell * 4), 256, 0, c->stream>>>(ncell * 4, c->tsc_acc.as<unsigned long long>(), 0ull);
     k_tsc<<<nblk(np), 256, 0, c->stream>>>(T, np, c->px.as<float>(), c->py.as<float>(), c->pz.as<float>(),
                                            c->pm.as<float>(), c->pvx.as<float>(), c->pvy.as<float>(),
@@ -976,6 +1026,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   }
   if (dens_interp == 1 && vel_interp != 1) {  // H_DISTANCE (particles.f90:276-351), then FILL_H_ZEROS
     StageTimer t(c, "h_distance");
+    c->counters[2] = smask ? c->mask_observed : ncell;
     const VvelOut O{nullptr, c->hpart.as<uint32_t>(), nullptr, nullptr, nullptr, 0};
     with_k(k, [&](auto KC) {
       constexpr int K = decltype(KC)::value, BQ = kq_of(K);
@@ -999,12 +1050,15 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   int e2 = 0;
   std::frexp(bound, &e2);  // bound < 2^e2
   const double scale = std::ldexp(1.0, 62 - e2);
+  unsigned long long* dpairs = (unsigned long long*)((char*)c->scalars.p + 1024);
   if (dens_interp == 1) {
     StageTimer t(c, "ddens_scatter");
     k_fill_u64<<<nblk(ncell), 256, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 0ull);
     const unsigned ntiles = (unsigned)(((n + DTX - 1) / DTX) * ((n + DTY - 1) / DTY) * ((n + DTZ - 1) / DTZ));
+    AV_CHECK_HIP(c, hipMemsetAsync(dpairs, 0, 8, c->stream));
     k_ddens_scatter<<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
-                                                   c->mass_fix.as<unsigned long long>(), c->norm.as<float>());
+                                                   c->mass_fix.as<unsigned long long>(), c->norm.as<float>(),
+                                                   dpairs);
     AV_CHECK_HIP(c, hipGetLastError());
   } else {  // DENS_INTERP_TSC: FP64 cell masses in mass_fix (zero bits are +0.0)
     StageTimer t(c, "dens_tsc");
@@ -1044,10 +1098,14 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     AV_CHECK_HIP(c, hipGetLastError());
   }
   float hmean = 0;
+  unsigned long long hpairs = 0;
+  if (dens_interp == 1)
+    AV_CHECK_HIP(c, hipMemcpyAsync(&hpairs, dpairs, 8, hipMemcpyDeviceToHost, c->stream));
   AV_CHECK_HIP(c, hipMemcpyAsync(&hmean, dmean, 4, hipMemcpyDeviceToHost, c->stream));
   AV_CHECK_HIP(c, hipMemcpyAsync(&c->meansum, dsum, 8, hipMemcpyDeviceToHost, c->stream));
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   c->meandens_raw = hmean;
+  c->counters[0] = (int64_t)hpairs;
   float ul3 = (kUL * kUL) * kUL;
   c->meandens = hmean * (kUM / ul3);  // avism.f90:1409
   c->fields_ready = true;
@@ -1204,6 +1262,9 @@ int survey_density(avism_ctx* c, int32_t k, int32_t dens_interp) {
 int fields_from_grid(avism_ctx* c, const float* delta, const float* vx, const float* vy,
                      const float* vz, int where) {
   if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "avism_set_grid must be called first");
+  if (int rc = finish_copies(c)) return rc;
+  c->nvoid = 0;
+  c->post_ready = false;
   const int n = c->g.n;
   const int64_t ncell = (int64_t)n * n * n;
   if (!c->fu.ensure((size_t)ncell * 4) || !c->fv.ensure((size_t)ncell * 4) ||

@@ -1147,6 +1147,7 @@ __global__ void k_req(int nv, const int32_t* __restrict__ uvoid, const float* __
 
 // vf layout on the device: 12 blocks of nv floats (gxc gyc gzc xc yc zc volnew umean mtot eps ip req)
 int merge_and_post(avism_ctx* c, int32_t stop_after) {
+  if (!c->grid_set || !c->fields_ready) AV_FAIL(c, AVISM_ERR_STATE, "fields and voids of this grid not computed");
   const int nv = c->nvoid;
   const GridGeom& g = c->g;
   ExtG G{g.n, g.low1, g.low2, g.next, g.periodic};
@@ -1535,6 +1536,7 @@ int next_level(avism_ctx* c) {
     AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
     std::swap(c->smask.p, m2.p);
     std::swap(c->smask.bytes, m2.bytes);
+    c->mask_observed *= 8;  // each observed cell becomes 2 x 2 x 2
   }
   AV_CHECK_HIP(c, hipGetLastError());
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));

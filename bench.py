@@ -4,8 +4,15 @@
 Workload: 512^3 Zel'dovich particles (P(k) ~ k^-1.5 exp(-k^2/0.25), rms
 displacement 3 Mpc/h, f = 0.5) in a periodic 1000 Mpc/h box, SPH with k = 32
 onto a 512^3 grid (extended to 640^3 by the periodic shell), thresholds of
-config/voids.dat.  One step = particles resident in HBM -> fields -> marking,
-growth, acceptance -> merge -> post-processing -> catalogue and MARCA in HBM.
+config/voids.dat.
+
+One step (the headline, `value`) runs from the particle SoA resident in pinned
+host RAM to the catalogue table and the MARCA / U1CO / DIVERCO maps resident
+in pinned host RAM: upload (velocities overlapped with binning and
+PPART_DENS), fields, marking, growth, acceptance, merge, post-processing, and
+the downloads (U1CO and DIVERCO overlapped with the void stage).  The same
+step with the particles already in HBM and the results left there is
+reported beside it (`device_resident_*`); its stage timings give the roofline.
 
 Both growth modes run on the same particles: the plain method and the
 learned-proposal method (3D CNN half-width proposals, exactly re-verified);
@@ -23,6 +30,7 @@ Prints one JSON line (see the keys below).
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -33,6 +41,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# kernel sources whose HBM traffic profiles/hbm_traffic.json records: a figure
+# measured on other sources is not reported
+TRAFFIC_SOURCES = ["avism_amd/csrc/fields.hip", "avism_amd/csrc/knn_search.h", "avism_amd/csrc/voids.hip"]
 
 
 def parse():
@@ -46,10 +57,18 @@ def parse():
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed-mass", action="store_true", help="skip the mixed-mass field-stage timing")
-    ap.add_argument("--cpu-n", type=int, default=64, help="side of the CPU-baseline sample")
+    ap.add_argument("--no-host", action="store_true", help="time the HBM-resident step only")
+    ap.add_argument("--cpu-n", type=int, default=128, help="side of the CPU-baseline sample")
     ap.add_argument("--mode", choices=["both", "plain", "learned"], default="both")
     ap.add_argument("--levels", type=int, default=1, help="void-in-void levels per step (N, 2N, ...)")
     return ap.parse_args()
+
+
+def sources_hash() -> str:
+    h = hashlib.sha256()
+    for s in TRAFFIC_SOURCES:
+        h.update((ROOT / s).read_bytes())
+    return h.hexdigest()[:16]
 
 
 def stage_table(tim):
@@ -59,34 +78,63 @@ def stage_table(tim):
     return out
 
 
-def run_step(ctx, parts, k, growth_mode, proposer=None, levels=1, grid=None, per_level=None):
-    from avism_amd._lib import AVISM_DEVICE
+class HostOut:
+    """Pinned host buffers the host-RAM step downloads into, per grid size."""
+
+    def __init__(self):
+        self.bufs = {}
+
+    def get(self, n):
+        import torch
+        if n not in self.bufs:
+            n3 = n ** 3
+            self.bufs[n] = dict(u1co=torch.empty(n3, dtype=torch.float32).pin_memory(),
+                                div=torch.empty(n3, dtype=torch.float32).pin_memory(),
+                                marca=torch.empty(n3, dtype=torch.int32).pin_memory())
+        return self.bufs[n]
+
+
+def run_step(ctx, parts, k, growth_mode, levels=1, grid=None, per_level=None, host_out=None):
+    """One step.  host_out None: particles from HBM, results left in HBM.
+    Otherwise particles from pinned host RAM (parts holds host tensors) and
+    the catalogue and the maps downloaded into host_out's pinned buffers."""
+    from avism_amd._lib import AVISM_DEVICE, AVISM_HOST
     if levels > 1:
         ctx.set_grid(*grid)  # back to the base level
-    ctx.load_particles(parts.x, parts.y, parts.z, parts.vx, parts.vy, parts.vz, parts.m, where=AVISM_DEVICE)
+    ctx.load_particles(parts.x, parts.y, parts.z, parts.vx, parts.vy, parts.vz, parts.m,
+                       where=AVISM_HOST if host_out is not None else AVISM_DEVICE)
+    cat = None
     for lev in range(levels):
         t0 = time.perf_counter()
         if lev:
             ctx.next_level()
         ctx.particles_to_grid(k=k)
+        n = ctx.grid().n
+        if host_out is not None:
+            hb = host_out.get(n)
+            ctx.fields_async(hb["u1co"], hb["div"])  # overlaps the void stage
         nv = ctx.find_voids(growth_mode)
         ctx.merge_and_post()
+        if host_out is not None:
+            cat = ctx.catalogue(nv)
+            ctx.marca(inner_only=True, out=hb["marca"])
+            ctx.sync()
         if per_level is not None:  # the library calls return with their work complete
             st = ctx.stats()
-            rec = per_level.setdefault(lev, dict(n=ctx.grid().n, ms=0.0, stages={}))
+            rec = per_level.setdefault(lev, dict(n=n, ms=0.0, stages={}))
             rec["ms"] += (time.perf_counter() - t0) * 1e3
             rec.update(ncand=int(st.ncand), ncubes=int(nv), nvoids=int(st.nvoidt))
             for name, ms in ctx.timings():
                 rec["stages"][name] = rec["stages"].get(name, 0.0) + ms
-    return nv
+    return nv, cat
 
 
 def cpu_baseline(n_side: int, box_per_cell: float, k: int):
-    """The reference's own routines (oracle/_ref: the reference's Fortran
-    sources built with amdflang -O2 -fopenmp, driven by the oracle harness) on
-    a bounded sample of the same workload: n_side^3 particles and cells, same
-    cell size, all host cores.  Falls back to the oracle (C port, 1 thread)
-    when the reference build is absent."""
+    """The reference's own routines (oracle/_ref: its Fortran sources built
+    with amdflang -O2 -fopenmp, particles.f90 included, driven by the oracle
+    harness) on a bounded sample of the same workload: n_side^3 particles and
+    cells, same cell size, all host cores.  Falls back to the oracle (C port,
+    1 thread) when the reference build is absent."""
     import tempfile
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
@@ -97,14 +145,17 @@ def cpu_baseline(n_side: int, box_per_cell: float, k: int):
     if O.ref_available(1):
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         os.environ["OMP_NUM_THREADS"] = str(cores)
+        omp_fields = O.ref_fields_openmp()
         with tempfile.TemporaryDirectory() as d:
             t0 = time.perf_counter()
             r = O.run_reference(d, 1, n_side, L0, 1, O.VoidParams(nvoid_max=4_000_000), parts=arr, k=k)
             dt = time.perf_counter() - t0
+        how = ("every routine OpenMP" if omp_fields else
+               "field routines (particles.f90) on 1 thread, the rest OpenMP")
         return {"value": n_side ** 3 / dt, "unit": "cells/s", "cores": cores, "kind": "reference",
-                "sample": f"reference routines (amdflang -O2 -fopenmp, {cores} threads) full level on "
-                          f"{n_side}^3 Zel'dovich particles / {n_side}^3 cells, cell size {box_per_cell:.4g} "
-                          f"Mpc/h, k={k}: {dt:.2f} s wall incl. a few MB of file I/O; "
+                "sample": f"reference routines (amdflang -O2 -fopenmp, {how}, {cores} threads) full level on "
+                          f"{n_side}^3 Zel'dovich particles / {n_side}^3 cells, periodic {L0:g} Mpc/h (same cell "
+                          f"size), k={k}: {dt:.2f} s wall incl. file I/O of the harness; "
                           f"{r['nvoid']} cubes, {r['nvoidt']} voids"}
     t0 = time.perf_counter()
     f = O.fields_from_particles(*arr, n_side, L0, 1, k=k)
@@ -117,12 +168,31 @@ def cpu_baseline(n_side: int, box_per_cell: float, k: int):
                       f"{n_side}^3 cells, cell size {box_per_cell:.4g} Mpc/h, k={k}: {dt:.2f} s"}
 
 
+def timed(ctx, fn, warmup, steps):
+    import torch
+    for _ in range(warmup):
+        fn(None)
+    torch.cuda.synchronize()
+    ctx.timings()  # drop the warm-up's records
+    ctx.set_timing(True)
+    per_level = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn(per_level)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    tim = ctx.timings()
+    ctx.set_timing(False)
+    return dt, tim, per_level, out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this library drives a single "
               "device, and a sharded multi-GPU run is not implemented.", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     import torch
     from avism_amd import synth
     from avism_amd._lib import Context
@@ -134,6 +204,11 @@ def main():
     parts = synth.zeldovich(n, args.box, seed=args.seed, device=dev)
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t_gen
+    parts_h = None
+    if not args.no_host:  # the same particles in pinned host RAM
+        f = lambda t: t.cpu().pin_memory()
+        parts_h = synth.Particles(f(parts.x), f(parts.y), f(parts.z), f(parts.vx), f(parts.vy), f(parts.vz),
+                                  f(parts.m), parts.L0, parts.periodic)
     ctx = Context(0)
     ctx.set_grid(n, args.box, 1)
     ctx.set_cosmology(0.678, 0.31, 0.0)
@@ -143,38 +218,41 @@ def main():
     if proposer is not None:
         proposer.install(ctx)
 
-    results = {}
-    catalogues = {}
-    levels = {}
     GL = 2  # GROW_LEARNED
     modes = {"both": [0, GL], "plain": [0], "learned": [GL]}[args.mode]
+    grid = (n, args.box, 1)
+    host_out = HostOut()
+    dres, hres, catalogues, levels = {}, {}, {}, {}
     for mode in modes:
-        if mode == GL and proposer is None:
-            continue
-        grid = (n, args.box, 1)
-        for _ in range(args.warmup):
-            run_step(ctx, parts, args.k, mode, proposer, args.levels, grid)
-        torch.cuda.synchronize()
-        ctx.set_timing(True)
-        per_level = {} if args.levels > 1 else None
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            nv = run_step(ctx, parts, args.k, mode, proposer, args.levels, grid, per_level)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
-        tim = stage_table(ctx.timings())
-        if per_level is not None:
-            levels[mode] = [dict(n=v["n"], ms=round(v["ms"] / args.steps, 3), ncand=v["ncand"], ncubes=v["ncubes"],
-                                 nvoids=v["nvoids"],
+        lv = args.levels
+        # HBM-resident step: stage timings and roofline
+        dt, tim, per_level, _ = timed(
+            ctx, lambda pl: run_step(ctx, parts, args.k, mode, lv, grid, pl if lv > 1 else None),
+            args.warmup, args.steps)
+        st = ctx.stats()
+        counters = ctx.counters()
+        stages = {s: v / args.steps for s, v in stage_table(tim).items()}
+        if lv > 1:
+            levels[mode] = [dict(n=v["n"], ms=round(v["ms"] / args.steps, 3), ncand=v["ncand"],
+                                 ncubes=v["ncubes"], nvoids=v["nvoids"],
                                  stages_ms={k2: round(x / args.steps, 3) for k2, x in v["stages"].items()})
                             for _, v in sorted(per_level.items())]
-            tim = stage_table([(k2, x * args.steps) for k2, x in levels[mode][0]["stages_ms"].items()])
-        ctx.set_timing(False)
-        st = ctx.stats()
-        cat = ctx.catalogue(nv)
-        catalogues[mode] = (ctx.marca(), cat["uvoid"].copy(), cat["indice"].copy())
-        results[mode] = dict(ms=dt * 1e3, stages={k: v / args.steps for k, v in tim.items()},
-                             ncubes=int(st.ncubes), nvoids=int(st.nvoidt), ncand=int(st.ncand))
+            stages = dict(levels[mode][0]["stages_ms"])
+        dres[mode] = dict(ms=dt * 1e3, stages=stages, ncubes=int(st.ncubes), nvoids=int(st.nvoidt),
+                          ncand=int(st.ncand), counters=counters)
+        if parts_h is not None:
+            dt_h, tim_h, _, out = timed(
+                ctx, lambda pl: run_step(ctx, parts_h, args.k, mode, lv, grid, None, host_out),
+                args.warmup, args.steps)
+            hres[mode] = dict(ms=dt_h * 1e3, stages={s: v / args.steps for s, v in stage_table(tim_h).items()})
+            cat = out[1]
+            hb = host_out.get(ctx.grid().n)
+            catalogues[mode] = (hb["marca"].numpy().copy(), cat["uvoid"].copy(), cat["indice"].copy(),
+                                hb["u1co"].numpy().copy(), hb["div"].numpy().copy())
+        else:
+            st_nv = ctx.stats().ncubes
+            cat = ctx.catalogue(int(st_nv))
+            catalogues[mode] = (ctx.marca(), cat["uvoid"].copy(), cat["indice"].copy())
     # The workload's particles all have the same mass (as N-body snapshots do),
     # so PPART takes k*m as the exact mass sum.  For the record, the field stage
     # once more with masses spread over [0.5, 1.5] x m (the neighbour-sum path).
@@ -193,44 +271,68 @@ def main():
         del m2
     identical = None
     if 0 in catalogues and GL in catalogues:
-        import numpy as np
         identical = all(np.array_equal(a, b) for a, b in zip(catalogues[0], catalogues[GL]))
 
-    main_mode = GL if GL in results else 0
-    r = results[main_mode]
+    main_mode = GL if GL in dres else 0
+    r = dres[main_mode]
     cells = n ** 3
-    value = cells / (r["ms"] / 1e3)
+    use_host = main_mode in hres
+    ms_step = hres[main_mode]["ms"] if use_host else r["ms"]
+    value = cells / (ms_step / 1e3)
+
+    # Σ stages of the HBM-resident step against its wall time: the stage
+    # timers cover the whole step (gaps are host work between launches)
+    stages = r["stages"]
+    stage_sum = sum(stages.values())
+    coverage = stage_sum / r["ms"]
+    if args.levels == 1 and not (0.95 <= coverage <= 1.02):
+        print(f"warning: stage timings sum to {stage_sum:.1f} ms of a {r['ms']:.1f} ms step", file=sys.stderr)
 
     # roofline of the dominant kernel under the algorithmic byte model
     # (per-unit figures: neighbour records gathered per query, cell records
     # streamed per cell)
     k = args.k
+    cnt = r["counters"]
     byte_model = {
         # gather model: k neighbour records (xyz, m, rho, velocity, HPART) + 3 outputs
-        "vvel_sph_vw": cells * (k * 32 + 12 + k * 8),
+        "vvel_sph_vw": cnt["vvel_queries"] * (k * 32 + 12 + k * 8),
         # gather model: k neighbour records (xyz, m) + 1 output
-        "ppart_dens": n ** 3 * (k * 16 + 4),
-        # compulsory: particle records (x,y,z,m,h) and NORM out, cell sums filled and read
-        "ddens_scatter": n ** 3 * 24 + cells * 16,
-        "ddens_finalize": cells * 8,
-        "mark_flags": gi.next ** 3 * 9 + cells * 8,
+        "ppart_dens": cnt["ppart_queries"] * (k * 16 + 4),
+        # gather model: 24 bytes per (particle, cell) pair within h + the cell sums
+        "ddens_scatter": cnt["ddens_pairs"] * 24 + cells * 4,
+        # compulsory: read the int64 cell sums, write U1
+        "ddens_finalize": cells * 8 + cells * 4,
+        # compulsory: read 3 velocity fields + U1, write U1CO and DIVERCO
         "mean_div": cells * 24,
+        # the reference's bytes per extended cell: read U1CO, DIVERCO, MARCAP2
+        # (4 B each) and SMASK (1 B), write the 8 flag bytes
+        "mark_flags": gi.next ** 3 * 21,
     }
-    stages = r["stages"]
     dom = max((s for s in stages if s in byte_model), key=lambda s: stages[s])
     achieved = byte_model[dom] / (stages[dom] / 1e3) / 1e9
-    traffic = None  # HBM bytes per launch of that kernel, from the committed counter profile
+    traffic = None  # HBM bytes per launch of that kernel, from counters measured on these sources
     tfile = ROOT / "profiles" / "hbm_traffic.json"
+    src_hash = sources_hash()
     if tfile.exists():
-        t = json.loads(tfile.read_text()).get("kernels", {}).get(dom)
-        if t and t.get("n") == n and t.get("k") == k:
+        tj = json.loads(tfile.read_text())
+        t = tj.get("kernels", {}).get(dom)
+        if t and t.get("n") == n and t.get("k") == k and tj.get("sources") == src_hash:
             traffic = t["bytes"]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_model": byte_model[dom], "kernel_ms": round(stages[dom], 3)}
-    # every modelled stage's algorithmic bytes over the whole step's wall time
+                "bytes_model": byte_model[dom], "kernel_ms": round(stages[dom], 3), "sources": src_hash}
     e2e_bytes = sum(byte_model[s2] for s2 in stages if s2 in byte_model)
     roofline["end_to_end_achieved"] = round(e2e_bytes / (r["ms"] / 1e3) / 1e9, 1)
+    roofline["per_kernel_frac"] = {s2: round(byte_model[s2] / (stages[s2] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                                   for s2 in byte_model if s2 in stages}
+    knn = {}
+    if "ppart_dens" in stages:
+        knn["ppart_queries_per_s"] = cnt["ppart_queries"] / (stages["ppart_dens"] / 1e3)
+    if "vvel_sph_vw" in stages:
+        knn["vvel_queries_per_s"] = cnt["vvel_queries"] / (stages["vvel_sph_vw"] / 1e3)
+    if "ppart_dens" in stages and "vvel_sph_vw" in stages:
+        knn["all_queries_per_s"] = ((cnt["ppart_queries"] + cnt["vvel_queries"]) /
+                                    ((stages["ppart_dens"] + stages["vvel_sph_vw"]) / 1e3))
     out = {
         "metric": "cells/s end-to-end at 512^3: learned-proposal vs plain GPU, identical voids",
         "value": value,
@@ -238,28 +340,42 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r["ms"],
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded Zel'dovich particles generated on the GPU)",
+        "data": "synthetic (seeded Zel'dovich particles generated on the GPU, copied to pinned host RAM)",
         "config": {"workload": f"zeldovich {n}^3 particles, {n}^3 grid (ext {gi.next}^3), periodic "
                                f"{args.box:g} Mpc/h, SPH k={k}"
                                + (f", {args.levels} void-in-void levels ({n}^3 .. {n * 2 ** (args.levels - 1)}^3 cells)"
                                   if args.levels > 1 else ""),
-                   "growth": "learned" if main_mode else "plain"},
-        "plain_ms_per_step": results.get(0, {}).get("ms"),
-        "learned_ms_per_step": results.get(GL, {}).get("ms"),
-        "learned_over_plain": (results[0]["ms"] / results[GL]["ms"]) if (0 in results and GL in results) else None,
+                   "growth": "learned" if main_mode else "plain",
+                   "boundary": ("particle SoA in pinned host RAM -> catalogue + MARCA/U1CO/DIVERCO maps in pinned "
+                                "host RAM" if use_host else "particles in HBM -> catalogue and MARCA in HBM")},
+        "plain_ms_per_step": (hres if use_host else dres).get(0, {}).get("ms"),
+        "learned_ms_per_step": (hres if use_host else dres).get(GL, {}).get("ms"),
+        "learned_over_plain": None,
         "identical_voids": identical,
-        "stages_ms": {m: {k2: round(v, 3) for k2, v in results[m]["stages"].items()} for m in results},
+        "device_resident_ms_per_step": {("learned" if m else "plain"): round(v["ms"], 3) for m, v in dres.items()},
+        "device_resident_value": cells / (r["ms"] / 1e3),
+        "stage_sum_over_step": round(coverage, 4),
+        "stages_ms": {("learned" if m else "plain"): {k2: round(v, 3) for k2, v in dres[m]["stages"].items()}
+                      for m in dres},
+        "host_step_stages_ms": {("learned" if m else "plain"): {k2: round(v, 3) for k2, v in hres[m]["stages"].items()
+                                                              if k2.startswith(("upload", "download"))}
+                                for m in hres} or None,
         "ncand": r["ncand"], "ncubes": r["ncubes"], "nvoids": r["nvoids"],
+        "counters": cnt,
+        "knn": knn,
         "roofline": roofline,
         "field_stage_ms_mixed_masses": mixed,
         "levels": {("learned" if m else "plain"): v for m, v in levels.items()} or None,
         "input_gen_s": round(t_gen, 2),
     }
+    res = hres if use_host else dres
+    if 0 in res and GL in res:
+        out["learned_over_plain"] = res[0]["ms"] / res[GL]["ms"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.box / n, k)
     print(json.dumps(out))

@@ -1,11 +1,14 @@
-"""Summaries of rocprofv3 runs of bench.py (kernel times and HBM traffic).
+"""Summaries of rocprofv3 runs of bench.py (kernel times and HBM traffic);
+measurement tooling beside bench.py, not part of the library.
 
-    python -m avism_amd.profile_summary stats  <kernel_stats.csv> <out.md> <title> <command>
-    python -m avism_amd.profile_summary traffic <fetch_dir> <write_dir> <n> <k> <out.json> <command...>
-    python -m avism_amd.profile_summary counters <pmc_dir> <out.md> <title> <command...>
+    python bench_profile.py stats  <kernel_stats.csv> <out.md> <title> <command>
+    python bench_profile.py db     <results.db> <out.md> <title> <command>
+    python bench_profile.py traffic <fetch_dir> <write_dir> <n> <k> <out.json> <command...>
+    python bench_profile.py counters <pmc_dir> <out.md> <title> <command...>
 
 `stats` condenses a `rocprofv3 --kernel-trace --stats` kernel_stats CSV into a
-markdown table.  `counters` tabulates any `--pmc` run's counters per field kernel (mean per
+markdown table; `db` does the same from the rocpd SQLite database that
+rocprofv3 writes by default.  `counters` tabulates any `--pmc` run's counters per field kernel (mean per
 dispatch), with VALU instructions per wave.  `traffic` reads the counter_collection CSVs of two `--pmc`
 runs (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch) and scales each
 counter by what it reads for a copy of known size: the n^3 x 4-byte
@@ -19,8 +22,11 @@ from __future__ import annotations
 import csv
 import json
 import re
+import sqlite3
 import sys
 from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
 
 # library kernel -> the stage name bench.py reports it under
 STAGE_OF = {
@@ -61,6 +67,31 @@ def stats(src: str, dst: str, title: str, cmd: str) -> None:
     Path(dst).write_text("\n".join(out) + "\n")
 
 
+def db_stats(src: str, dst: str, title: str, cmd: str) -> None:
+    """Kernel time table from a rocpd database (the `kernels` view: one row per
+    dispatch, durations in ns), with VGPRs and LDS bytes per workgroup."""
+    con = sqlite3.connect(src)
+    rows = con.execute("select name, duration, vgpr_count, accum_vgpr_count, lds_size from kernels").fetchall()
+    agg: dict[str, list] = {}
+    for name, dur, vg, ag, lds in rows:
+        a = agg.setdefault(short(name), [0, 0.0, vg, ag, lds])
+        a[0] += 1
+        a[1] += float(dur)
+    tot = sum(v[1] for v in agg.values())
+    out = [f"# {title}", "", "Command (run from the repository root, TMPDIR=/tmp):", "", "```", cmd, "```", "",
+           "| kernel | calls | total ms | avg ms | % | VGPR | AGPR | LDS B |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for name, (c, t, vg, ag, lds) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:30]:
+        out.append(f"| {name} | {c} | {t / 1e6:.3f} | {t / 1e6 / c:.3f} | {100 * t / tot:.1f} | {vg} | {ag} | {lds} |")
+    out.append(f"| (all kernels) | | {tot / 1e6:.3f} | | 100 | | | |")
+    Path(dst).write_text("\n".join(out) + "\n")
+
+
+def sources_hash() -> str:
+    sys.path.insert(0, str(ROOT))
+    import bench
+    return bench.sources_hash()
+
+
 def _counters(d: str, counter: str) -> list[tuple[int, str, float]]:
     f = next(Path(d).glob("**/*counter_collection.csv"))
     return [(int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(open(f))
@@ -90,7 +121,7 @@ def traffic(fetch_dir: str, write_dir: str, n: int, k: int, dst: str, cmd: str) 
         fk, wk = sum(fv) / len(fv), sum(wv) / len(wv)
         kernels[stage] = {"kernel": base, "n": n, "k": k, "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
                           "bytes": int(round((fk * ffac + wk * wfac) * 1024))}
-    out = {"command": cmd,
+    out = {"command": cmd, "sources": sources_hash(),
            "calibration": {"copy_bytes": n ** 3 * 4, "fetch_kib": round(fc, 1), "write_kib": round(wc, 1),
                            "fetch_factor": round(ffac, 4), "write_factor": round(wfac, 4)},
            "kernels": kernels}
@@ -122,6 +153,8 @@ def counters(pmc_dir: str, dst: str, title: str, cmd: str) -> None:
 def main(argv: list[str]) -> None:
     if argv[0] == "stats":
         stats(argv[1], argv[2], argv[3], argv[4])
+    elif argv[0] == "db":
+        db_stats(argv[1], argv[2], argv[3], " ".join(argv[4:]))
     elif argv[0] == "counters":
         counters(argv[1], argv[2], argv[3], " ".join(argv[4:]))
     elif argv[0] == "traffic":

@@ -42,6 +42,11 @@ extern "C" {
 
 #define AVISM_HOST 0
 #define AVISM_DEVICE 1
+/* avism_get_fields only: host pointers into pinned memory (hipHostMalloc); the
+ * copies are queued behind the work already queued and the call returns at
+ * once, so they overlap the void stage.  The data is complete after
+ * avism_sync; a call that changes the fields first waits for the copies. */
+#define AVISM_HOST_ASYNC 2
 
 /* growth_mode of avism_find_voids */
 #define AVISM_GROW_PLAIN 0     /* exact shell-by-shell growth of every candidate */
@@ -83,7 +88,11 @@ int avism_create(avism_ctx** ctx, int32_t device);
 int avism_destroy(avism_ctx* ctx);
 const char* avism_last_error(avism_ctx* ctx);
 int avism_set_timing(avism_ctx* ctx, int32_t on);
-/* names and milliseconds of the stages timed since the last call */
+/* names (';'-separated) and milliseconds of the stages timed since the last
+ * call.  *count holds the capacity of ms on entry and the number of stages on
+ * return.  If ms or names is too small (or NULL) nothing is consumed and the
+ * call returns AVISM_ERR_LIMIT with *count set: call again with larger
+ * buffers (avism_last_error gives the names length needed). */
 int avism_get_timings(avism_ctx* ctx, char* names, int32_t names_len, float* ms, int32_t* count);
 
 /* ---- set-up ------------------------------------------------------------ */
@@ -160,9 +169,20 @@ int avism_load_grid_fields(avism_ctx* ctx, const float* delta, const float* vx, 
  * meandens in Msun/Mpc^3 is used for the void masses. */
 int avism_load_void_fields(avism_ctx* ctx, const float* u1co, const float* div, float meandens,
                            int32_t where);
-/* inner fields: U1CO (1+delta), DIVERCO, and the velocity components; NULL skips */
+/* inner fields: U1CO (1+delta), DIVERCO, and the velocity components; NULL
+ * skips.  where: AVISM_HOST, AVISM_DEVICE or AVISM_HOST_ASYNC.  Stands for the
+ * map output's source arrays (avism.f90:2028-2040). */
 int avism_get_fields(avism_ctx* ctx, float* u1co, float* div, float* u, float* v, float* w,
                      int32_t where);
+/* waits for every copy and kernel the context has queued (no counterpart in
+ * the reference, whose calls are all synchronous) */
+int avism_sync(avism_ctx* ctx);
+/* work counters of the last stages run, for throughput reports (no
+ * counterpart in the reference): out[0] (particle, cell) pairs within h in
+ * DDENS_INTERP_SPH (its cells with d <= h, particles.f90:520-547), out[1] kNN
+ * queries of PPART_DENS, out[2] kNN queries of VVEL_INTERP_SPH_VW or
+ * H_DISTANCE (unmasked cells), out[3] growth shells checked; the rest 0 */
+int avism_get_counters(avism_ctx* ctx, int64_t* out, int32_t n);
 /* per-particle PART_DENS and HPART in the input order; NULL skips */
 int avism_get_particle_fields(avism_ctx* ctx, float* part_dens, float* hpart);
 

@@ -385,6 +385,12 @@ def ref_available(periodic: int) -> bool:
     return (REF_DIR / f"ref_harness_p{int(periodic)}.x").exists()
 
 
+def ref_fields_openmp() -> bool:
+    """True when oracle/_ref was built with the field routines (particles.f90)
+    under OpenMP, as the reference's own Makefile builds them."""
+    return (REF_DIR / "omp_fields").exists()
+
+
 def run_reference_indexx(jobdir: Path, keys: np.ndarray) -> np.ndarray:
     """INDEXX (num.f90:2-84) run by the reference routine itself (1-based output)."""
     jobdir = Path(jobdir)
