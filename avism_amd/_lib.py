@@ -59,7 +59,7 @@ EXPORTS = [
     "avism_set_proposals", "avism_get_candidates", "avism_get_cubes", "avism_indexx",
     "avism_merge_and_post", "avism_get_catalogue", "avism_get_marca", "avism_run_level",
     "avism_cnn_set_params", "avism_cnn_patches", "avism_cnn_forward", "avism_cnn_backward", "avism_cnn_adam",
-    "avism_sync", "avism_get_counters",
+    "avism_sync", "avism_get_counters", "avism_indexx_noalloc",
 ]
 
 _lib = None
@@ -121,6 +121,7 @@ def lib():
         "avism_cnn_adam": ([vp, vp, vp, vp, vp, f32, i32], C.c_int),
         "avism_sync": ([vp], C.c_int),
         "avism_get_counters": ([vp, vp, i32], C.c_int),
+        "avism_indexx_noalloc": ([vp, vp, i64, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -175,6 +175,8 @@ int void_find(avism_ctx* c, int32_t growth_mode);
 int merge_and_post(avism_ctx* c, int32_t stop_after);
 int next_level(avism_ctx* c);
 int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx);
+int noalloc_init(int device);
+int indexx_noalloc(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx);
 void add_timing(avism_ctx* c, const char* name, hipEvent_t a, hipEvent_t b);
 int cnn_patches(avism_ctx* c, const int64_t* ranks, int64_t n, uint16_t* out);
 int cnn_forward(avism_ctx* c, const float* params, int64_t n, const uint16_t* patches, float* logits,

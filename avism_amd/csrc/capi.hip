@@ -52,6 +52,15 @@ int avism_create(avism_ctx** out, int32_t device) {
     delete c;
     return AVISM_ERR_HIP;
   }
+  // the static scratch of the allocation-free entry points, on this device
+  if (noalloc_init(device) != AVISM_OK) {
+    (void)hipEventDestroy(c->ev_vel);
+    (void)hipEventDestroy(c->ev_mark);
+    (void)hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->cstream);
+    delete c;
+    return AVISM_ERR_HIP;
+  }
   c->vp = avism_void_params{-0.6f, 10.0f, 0.25f, 0.0f, 1.0f, 1.0f, 30000000};
   avism_set_cosmology(c, 0.678f, 0.31f, 0.0f);
   *out = c;
@@ -418,6 +427,13 @@ int avism_indexx(avism_ctx* c, const float* keys, int64_t n, int32_t* indx, int3
     AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   }
   return AVISM_OK;
+}
+
+int avism_indexx_noalloc(avism_ctx* c, const float* keys, int64_t n, int32_t* indx) {
+  CTX_GUARD(c);
+  DEVICE_ARGS(c, AVISM_DEVICE);
+  if (n > 0 && (!keys || !indx)) AV_FAIL(c, AVISM_ERR_ARG, "null key or index array");
+  return indexx_noalloc(c, keys, n, indx);
 }
 
 int avism_merge_and_post(avism_ctx* c, int32_t* nvoids) {

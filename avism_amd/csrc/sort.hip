@@ -20,6 +20,7 @@
 // so each level is a handful of element-parallel kernels.  Segments of at most
 // M+1 elements get the reference's insertion sort (num.f90:15-30).
 #include <cstring>
+#include <mutex>
 
 #include <rocprim/rocprim.hpp>
 
@@ -226,18 +227,34 @@ __global__ void k_relabel(int64_t n, SegTab T, int32_t* segid) {
 }
 
 inline unsigned nb(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// the level counters and the first segment [1, n]
+__global__ void k_seg_init(int64_t n, int32_t* l, int32_t* r, int32_t* counter) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  l[0] = 1;
+  r[0] = (int32_t)n;
+  counter[0] = 1;
+  counter[1] = 0;
+}
+
+// scratch layout for n keys: segid, gf, sf, gx, sx, gpos, spos (n+1 each),
+// two segment tables, the two level counters, then rocPRIM's scan state
+size_t indexx_tables_bytes(int64_t n) {
+  const size_t N1 = (size_t)n + 1, tab_entries = (size_t)n / 2 + 2;
+  return ((7 * N1 * 4 + 2 * 10 * tab_entries * 4 + 64) + 255) & ~(size_t)255;
+}
+
 }  // namespace
 
-int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx) {
-  StageTimer tm(c, "indexx");
-  if (n <= 0) return AVISM_OK;
+// INDEXX of d_keys[0..n) into d_indx (1-based) on the context stream, with
+// all scratch in [scratch, scratch + bytes): the tables, then tmpb bytes of
+// scan state.  Allocates nothing.
+static int indexx_core(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx, char* scratch, size_t bytes,
+                       size_t tmpb) {
   const size_t N1 = (size_t)n + 1;
-  // scratch: segid, gf, sf, gx, sx, gpos, spos (n+1 each) and two segment tables
   const size_t tab_entries = (size_t)n / 2 + 2;
-  size_t need = 7 * N1 * 4 + 2 * 10 * tab_entries * 4 + 64;  // + the two level counters
-  char* base = (char*)c->sort_misc.ensure(need);
-  if (!base) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (indexx)");
-  int32_t* segid = (int32_t*)base;
+  if (indexx_tables_bytes(n) + tmpb > bytes) AV_FAIL(c, AVISM_ERR_LIMIT, "indexx scratch too small");
+  int32_t* segid = (int32_t*)scratch;
   int32_t* gf = segid + N1;
   int32_t* sf = gf + N1;
   int32_t* gx = sf + N1;
@@ -245,6 +262,7 @@ int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx)
   int32_t* gpos = sx + N1;
   int32_t* spos = gpos + N1;
   int32_t* tabs = spos + N1;
+  void* tmp = scratch + indexx_tables_bytes(n);
   auto mk = [&](int32_t* b) {
     SegTab t;
     size_t e = tab_entries;
@@ -254,15 +272,8 @@ int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx)
   };
   SegTab A = mk(tabs), B = mk(tabs + 10 * tab_entries);
   int32_t* counter = (int32_t*)(tabs + 20 * tab_entries);
-  size_t tmpb = 0;
-  AV_CHECK_HIP(c, rocprim::exclusive_scan(nullptr, tmpb, gf, gx, 0, N1, rocprim::plus<int32_t>(),
-                                          c->stream));
-  if (!c->sort_tmp.ensure(tmpb)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (scan)");
   k_init_indx<<<nb(n), 256, 0, c->stream>>>(n, d_indx, segid);
-  int32_t one[3] = {1, (int32_t)n, 1};
-  AV_CHECK_HIP(c, hipMemcpyAsync(A.l, &one[0], 4, hipMemcpyHostToDevice, c->stream));
-  AV_CHECK_HIP(c, hipMemcpyAsync(A.r, &one[1], 4, hipMemcpyHostToDevice, c->stream));
-  AV_CHECK_HIP(c, hipMemcpyAsync(counter, &one[2], 4, hipMemcpyHostToDevice, c->stream));
+  k_seg_init<<<1, 64, 0, c->stream>>>(n, A.l, A.r, counter);
   // levels run in batches with the segment count kept on the device; the
   // host checks it between batches (a level with no segment does nothing)
   constexpr int kBatch = 4;
@@ -274,10 +285,8 @@ int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx)
       int32_t* nxt = counter + ((levels + 1) & 1);
       k_seg_setup<<<gseg, 256, 0, c->stream>>>(cur, A, d_keys, d_indx);
       k_flags<<<nb((int64_t)N1), 256, 0, c->stream>>>(n, A, segid, d_keys, d_indx, gf, sf);
-      AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmpb, gf, gx, 0, N1,
-                                              rocprim::plus<int32_t>(), c->stream));
-      AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmpb, sf, sx, 0, N1,
-                                              rocprim::plus<int32_t>(), c->stream));
+      AV_CHECK_HIP(c, rocprim::exclusive_scan(tmp, tmpb, gf, gx, 0, N1, rocprim::plus<int32_t>(), c->stream));
+      AV_CHECK_HIP(c, rocprim::exclusive_scan(tmp, tmpb, sf, sx, 0, N1, rocprim::plus<int32_t>(), c->stream));
       k_scatter_gs<<<nb(n), 256, 0, c->stream>>>(n, A, segid, gf, sf, gx, sx, gpos, spos);
       k_seg_partition<<<gseg, 256, 0, c->stream>>>(cur, A, gx, sx, gpos, spos);
       k_exchange<<<nb(n), 256, 0, c->stream>>>(n, A, segid, gf, gx, spos, d_indx);
@@ -294,6 +303,65 @@ int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx)
     if (levels > 4096) AV_FAIL(c, AVISM_ERR_LIMIT, "indexx did not converge");
   }
   return AVISM_OK;
+}
+
+static size_t scan_state_bytes(int64_t n, hipStream_t s) {
+  size_t tmpb = 0;
+  if (rocprim::exclusive_scan(nullptr, tmpb, (int32_t*)nullptr, (int32_t*)nullptr, 0, (size_t)n + 1,
+                              rocprim::plus<int32_t>(), s) != hipSuccess)
+    return ~(size_t)0;
+  return (tmpb + 255) & ~(size_t)255;
+}
+
+int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx) {
+  StageTimer tm(c, "indexx");
+  if (n <= 0) return AVISM_OK;
+  const size_t tmpb = scan_state_bytes(n, c->stream);
+  if (tmpb == ~(size_t)0) AV_FAIL(c, AVISM_ERR_HIP, "rocprim scan size query failed");
+  const size_t need = indexx_tables_bytes(n) + tmpb;
+  char* base = (char*)c->sort_misc.ensure(need);
+  if (!base) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (indexx)");
+  return indexx_core(c, d_keys, n, d_indx, base, need, tmpb);
+}
+
+// ---- INDEXX without allocation (callers that cannot allocate device memory)
+// The scratch is a file-scope device array: every GPU holds its own copy for
+// the life of the process, found by its per-device address (the array's name
+// on the host is only a stand-in).  Calls that overlap on one GPU race on it.
+constexpr int64_t kNoAllocMaxKeys = (int64_t)1 << 19;
+constexpr size_t kNoAllocScratch = ((7 * ((size_t)kNoAllocMaxKeys + 1) * 4 +
+                                     2 * 10 * ((size_t)kNoAllocMaxKeys / 2 + 2) * 4 + 64 + 255) & ~(size_t)255) +
+                                   ((size_t)1 << 20);  // + scan state
+static __device__ __align__(256) char scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+static char* scratch_ptr[kMaxDevices];  // per device, set by noalloc_init
+static std::mutex scratch_mu;
+
+// once per device, with that device current (avism_create)
+int noalloc_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return AVISM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (scratch_ptr[device]) return AVISM_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return AVISM_ERR_HIP;
+  scratch_ptr[device] = (char*)p;
+  return AVISM_OK;
+}
+
+int indexx_noalloc(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx) {
+  if (n < 0) AV_FAIL(c, AVISM_ERR_ARG, "negative length");
+  if (n > kNoAllocMaxKeys)
+    AV_FAIL(c, AVISM_ERR_LIMIT, "avism_indexx_noalloc takes at most 524288 keys (its static scratch)");
+  if (n == 0) return AVISM_OK;
+  char* base = nullptr;
+  if (c->device >= 0 && c->device < kMaxDevices) {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    base = scratch_ptr[c->device];
+  }
+  if (!base) AV_FAIL(c, AVISM_ERR_STATE, "static scratch of this device not set up (avism_create)");
+  const size_t tmpb = scan_state_bytes(n, c->stream);
+  if (tmpb == ~(size_t)0) AV_FAIL(c, AVISM_ERR_HIP, "rocprim scan size query failed");
+  return indexx_core(c, d_keys, n, d_indx, base, kNoAllocScratch, tmpb);
 }
 
 }  // namespace avism

@@ -202,6 +202,13 @@ int avism_get_cubes(avism_ctx* ctx, int32_t* ci, float* cf, int32_t* father);
 /* INDEXX (num.f90:2-84) of a float array, reproducing the Numerical Recipes
  * permutation including its tie order; indx is 1-based. */
 int avism_indexx(avism_ctx* ctx, const float* keys, int64_t n, int32_t* indx, int32_t where);
+/* avism_indexx on device pointers that allocates no device memory: no
+ * counterpart in the reference; for callers that cannot allocate (code under
+ * graph capture, tight loops, hosts that forbid it).  Its scratch is a static
+ * array of about 20 MiB held on each GPU for the life of the process, so n is
+ * at most 524288 (AVISM_ERR_LIMIT beyond); calls that overlap on one GPU race
+ * on that array.  Same permutation as avism_indexx. */
+int avism_indexx_noalloc(avism_ctx* ctx, const float* keys, int64_t n, int32_t* indx);
 
 /* ---- merge and post-processing (avism.f90:1727-1973) ------------------- */
 int avism_merge_and_post(avism_ctx* ctx, int32_t* nvoids);
