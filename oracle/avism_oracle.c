@@ -351,24 +351,42 @@ static inline float m4(float q) {
     return 0.0f;
 }
 
+/* The per-query loops below run under OpenMP (one query per iteration, each
+ * with its own buffers).  Every result is a function of its own query, and
+ * HPART is an order-free maximum, so the output does not depend on the
+ * thread count. */
 void orc_ppart_dens(orc_knn* t, int64_t np, const float* x, const float* y,
                     const float* z, const float* m, int32_t k,
                     float* part_dens, float* hk_out) {
-    int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * k);
-    float* d2 = (float*)malloc(sizeof(float) * k);
-    for (int64_t p = 0; p < np; p++) {
-        int32_t c = orc_knn_query(t, x[p], y[p], z[p], k, idx, d2);
-        float hk = sqrtf(d2[c - 1]);               /* kdtree.f90:539 */
-        double bas8 = 0.0;
-        for (int32_t j = 0; j < c; j++) bas8 += (double)m[idx[j]];
-        double vol = ((4.0 / 3.0) * PI_D) * (double)((hk * hk) * hk);
-        part_dens[p] = (float)(bas8 / vol);          /* particles.f90:794 */
-        if (hk_out) hk_out[p] = hk;
+    #pragma omp parallel
+    {
+        int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * k);
+        float* d2 = (float*)malloc(sizeof(float) * k);
+        #pragma omp for schedule(dynamic, 256)
+        for (int64_t p = 0; p < np; p++) {
+            int32_t c = orc_knn_query(t, x[p], y[p], z[p], k, idx, d2);
+            float hk = sqrtf(d2[c - 1]);               /* kdtree.f90:539 */
+            double bas8 = 0.0;
+            for (int32_t j = 0; j < c; j++) bas8 += (double)m[idx[j]];
+            double vol = ((4.0 / 3.0) * PI_D) * (double)((hk * hk) * hk);
+            part_dens[p] = (float)(bas8 / vol);          /* particles.f90:794 */
+            if (hk_out) hk_out[p] = hk;
+        }
+        free(idx); free(d2);
     }
-    free(idx); free(d2);
 }
 
-static inline void atomic_max_f(float* a, float v) { if (v > *a) *a = v; }
+/* HPART = max(HPART, v) (particles.f90:882-885): non-negative floats order as
+ * their bits, so an integer compare-and-swap loop gives the maximum in any
+ * order */
+static inline void atomic_max_f(float* a, float v) {
+    uint32_t nv, cur;
+    memcpy(&nv, &v, 4);
+    cur = __atomic_load_n((uint32_t*)a, __ATOMIC_RELAXED);
+    while (nv > cur && !__atomic_compare_exchange_n((uint32_t*)a, &cur, nv, 0, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED)) {
+    }
+}
 
 /* common cell query of VVEL_INTERP_SPH_VW / H_DISTANCE (particles.f90:858-885):
  * k neighbours, or the ball of radius DX when d_k <= DX. Returns the count. */
@@ -391,76 +409,121 @@ static int64_t cell_neighbours(orc_knn* t, const orc_grid* g, int32_t k, float q
     return n;
 }
 
+/* one cell (0-based linear inner index c) of VVEL_INTERP_SPH_VW
+ * (particles.f90:856-925); hpart may be NULL */
+static void vvel_cell(orc_knn* t, const orc_grid* g, int32_t k, const float* m, const float* part_dens,
+                      const float* vx, const float* vy, const float* vz, float* hpart, size_t c,
+                      float* u, float* v, float* w, int64_t** idx, float** d, int64_t* cap) {
+    int32_t n = g->n;
+    int32_t ix = (int32_t)(c % n) + 1, jy = (int32_t)((c / n) % n) + 1, kz = (int32_t)(c / ((size_t)n * n)) + 1;
+    float qx = orc_grid_rad(g, ix), qy = orc_grid_rad(g, jy), qz = orc_grid_rad(g, kz);
+    int64_t cnt = cell_neighbours(t, g, k, qx, qy, qz, idx, d, cap);
+    if (hpart)
+        for (int64_t j = 0; j < cnt; j++) atomic_max_f(&hpart[(*idx)[j]], (*d)[j]);
+    float hk = (*d)[cnt - 1];
+    double b = 0, bx = 0, by = 0, bz = 0;
+    for (int64_t j = 0; j < cnt; j++) {
+        int64_t q8 = (*idx)[j];
+        float q = (*d)[j] / (hk / 2.0f);           /* particles.f90:941 */
+        float wgt = m4(q);
+        wgt = wgt * (m[q8] / part_dens[q8]);       /* 897-900 */
+        double w8 = (double)wgt;
+        b += w8;
+        bx += w8 * (double)vx[q8];
+        by += w8 * (double)vy[q8];
+        bz += w8 * (double)vz[q8];
+    }
+    *u = (float)(bx / b); *v = (float)(by / b); *w = (float)(bz / b);
+}
+
 void orc_vvel_vw(orc_knn* t, const orc_grid* g, int32_t k, const float* m,
                  const float* part_dens, const float* vx, const float* vy,
                  const float* vz, const int8_t* smask, float* hpart,
                  float* u, float* v, float* w) {
-    int32_t n = g->n;
-    int64_t cap = k > 64 ? k : 64;
-    int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * cap);
-    float* d = (float*)malloc(sizeof(float) * cap);
-    for (int32_t kz = 1; kz <= n; kz++)
-        for (int32_t jy = 1; jy <= n; jy++)
-            for (int32_t ix = 1; ix <= n; ix++) {
-                size_t c = (size_t)(ix - 1) + (size_t)n * ((jy - 1) + (size_t)n * (kz - 1));
-                if (smask && smask[c] == 0) continue;
-                float qx = orc_grid_rad(g, ix), qy = orc_grid_rad(g, jy), qz = orc_grid_rad(g, kz);
-                int64_t cnt = cell_neighbours(t, g, k, qx, qy, qz, &idx, &d, &cap);
-                for (int64_t j = 0; j < cnt; j++) atomic_max_f(&hpart[idx[j]], d[j]);
-                float hk = d[cnt - 1];
-                double b = 0, bx = 0, by = 0, bz = 0;
-                for (int64_t j = 0; j < cnt; j++) {
-                    float q = d[j] / (hk / 2.0f);           /* particles.f90:941 */
-                    float wgt = m4(q);
-                    wgt = wgt * (m[idx[j]] / part_dens[idx[j]]);  /* 897-900 */
-                    double w8 = (double)wgt;
-                    b += w8;
-                    bx += w8 * (double)vx[idx[j]];
-                    by += w8 * (double)vy[idx[j]];
-                    bz += w8 * (double)vz[idx[j]];
-                }
-                u[c] = (float)(bx / b); v[c] = (float)(by / b); w[c] = (float)(bz / b);
-            }
-    free(idx); free(d);
+    int64_t ncell = (int64_t)g->n * g->n * g->n;
+    #pragma omp parallel
+    {
+        int64_t cap = k > 64 ? k : 64;
+        int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * cap);
+        float* d = (float*)malloc(sizeof(float) * cap);
+        #pragma omp for schedule(dynamic, 256)
+        for (int64_t c = 0; c < ncell; c++) {
+            if (smask && smask[c] == 0) continue;
+            vvel_cell(t, g, k, m, part_dens, vx, vy, vz, hpart, (size_t)c, &u[c], &v[c], &w[c], &idx, &d, &cap);
+        }
+        free(idx); free(d);
+    }
+}
+
+/* VVEL_INTERP_SPH_VW's velocities at the listed cells only (0-based linear
+ * inner indices), for sampled comparisons on large grids: part_dens needs to
+ * hold the neighbours' values */
+void orc_vvel_cells(orc_knn* t, const orc_grid* g, int32_t k, const float* m,
+                    const float* part_dens, const float* vx, const float* vy,
+                    const float* vz, int64_t ncells, const int64_t* cells,
+                    float* u, float* v, float* w) {
+    #pragma omp parallel
+    {
+        int64_t cap = k > 64 ? k : 64;
+        int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * cap);
+        float* d = (float*)malloc(sizeof(float) * cap);
+        #pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < ncells; i++)
+            vvel_cell(t, g, k, m, part_dens, vx, vy, vz, NULL, (size_t)cells[i], &u[i], &v[i], &w[i], &idx, &d,
+                      &cap);
+        free(idx); free(d);
+    }
 }
 
 void orc_h_distance(orc_knn* t, const orc_grid* g, int32_t k, const int8_t* smask,
                     float* hpart) {
     int32_t n = g->n;
-    int64_t cap = k > 64 ? k : 64;
-    int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * cap);
-    float* d = (float*)malloc(sizeof(float) * cap);
-    for (int32_t kz = 1; kz <= n; kz++)
-        for (int32_t jy = 1; jy <= n; jy++)
-            for (int32_t ix = 1; ix <= n; ix++) {
-                size_t c = (size_t)(ix - 1) + (size_t)n * ((jy - 1) + (size_t)n * (kz - 1));
-                if (smask && smask[c] == 0) continue;
-                int64_t cnt = cell_neighbours(t, g, k, orc_grid_rad(g, ix), orc_grid_rad(g, jy),
-                                              orc_grid_rad(g, kz), &idx, &d, &cap);
-                for (int64_t j = 0; j < cnt; j++) atomic_max_f(&hpart[idx[j]], d[j]);
-            }
-    free(idx); free(d);
+    int64_t ncell = (int64_t)n * n * n;
+    #pragma omp parallel
+    {
+        int64_t cap = k > 64 ? k : 64;
+        int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * cap);
+        float* d = (float*)malloc(sizeof(float) * cap);
+        #pragma omp for schedule(dynamic, 256)
+        for (int64_t c = 0; c < ncell; c++) {
+            if (smask && smask[c] == 0) continue;
+            int32_t ix = (int32_t)(c % n) + 1, jy = (int32_t)((c / n) % n) + 1, kz = (int32_t)(c / ((int64_t)n * n)) + 1;
+            int64_t cnt = cell_neighbours(t, g, k, orc_grid_rad(g, ix), orc_grid_rad(g, jy),
+                                          orc_grid_rad(g, kz), &idx, &d, &cap);
+            for (int64_t j = 0; j < cnt; j++) atomic_max_f(&hpart[idx[j]], d[j]);
+        }
+        free(idx); free(d);
+    }
 }
 
 void orc_fill_h_zeros(orc_knn* t, int64_t np, const float* x, const float* y,
                       const float* z, int32_t k, float* hpart) {
-    int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * k);
-    float* d2 = (float*)malloc(sizeof(float) * k);
-    for (int64_t p = 0; p < np; p++) {
-        if (hpart[p] > 0.0f) continue;
-        int32_t c = orc_knn_query(t, x[p], y[p], z[p], k, idx, d2);
-        hpart[p] = sqrtf(d2[c - 1]);
+    #pragma omp parallel
+    {
+        int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * k);
+        float* d2 = (float*)malloc(sizeof(float) * k);
+        #pragma omp for schedule(dynamic, 256)
+        for (int64_t p = 0; p < np; p++) {
+            if (hpart[p] > 0.0f) continue;
+            int32_t c = orc_knn_query(t, x[p], y[p], z[p], k, idx, d2);
+            hpart[p] = sqrtf(d2[c - 1]);
+        }
+        free(idx); free(d2);
     }
-    free(idx); free(d2);
 }
 
-void orc_ddens_sph(const orc_grid* g, int64_t np, const float* x, const float* y,
-                   const float* z, const float* hpart, const float* m,
-                   float* mass_out, double* mass64_out) {
+/* par = 0: particles in order, as the reference with one thread; par = 1:
+ * particles spread over OpenMP threads with atomic FP64 deposits, as the
+ * reference's own OMP ATOMIC (particles.f90:599-602), so the last bits of the
+ * sums depend on the schedule */
+static void ddens_sph(const orc_grid* g, int64_t np, const float* x, const float* y,
+                      const float* z, const float* hpart, const float* m,
+                      float* mass_out, double* mass64_out, int par) {
     int32_t n = g->n;
     float dx = g->dx, rx1 = g->rx1;
     size_t ncell = (size_t)n * n * n;
     double* u8 = (double*)calloc(ncell, sizeof(double));
+    #pragma omp parallel for schedule(dynamic, 256) if (par)
     for (int64_t i = 0; i < np; i++) {
         int32_t ix = (int32_t)((x[i] - rx1) / dx) + 1;   /* particles.f90:489-491 */
         int32_t jy = (int32_t)((y[i] - rx1) / dx) + 1;
@@ -484,15 +547,31 @@ void orc_ddens_sph(const orc_grid* g, int64_t np, const float* x, const float* y
                         if (d2 > h2max) continue;
                         float q = sqrtf(d2) * inv_h;
                         float wgt = m4(q);
-                        if (pass == 0) norm = norm + wgt;
-                        else u8[(size_t)(i2 - 1) + (size_t)n * ((j2 - 1) + (size_t)n * (k2 - 1))] +=
-                                 (double)((wgt * m[i]) / norm);
+                        if (pass == 0) {
+                            norm = norm + wgt;
+                        } else {
+                            double* dst = &u8[(size_t)(i2 - 1) + (size_t)n * ((j2 - 1) + (size_t)n * (k2 - 1))];
+                            double val = (double)((wgt * m[i]) / norm);
+                            if (par) {
+                                #pragma omp atomic
+                                *dst += val;
+                            } else {
+                                *dst += val;
+                            }
+                        }
                     }
                 }
             }
             if (pass == 0 && norm == 0.0f) {   /* particles.f90:549-557 */
-                if (ix >= 1 && ix <= n && jy >= 1 && jy <= n && kz >= 1 && kz <= n)
-                    u8[(size_t)(ix - 1) + (size_t)n * ((jy - 1) + (size_t)n * (kz - 1))] += (double)m[i];
+                if (ix >= 1 && ix <= n && jy >= 1 && jy <= n && kz >= 1 && kz <= n) {
+                    double* dst = &u8[(size_t)(ix - 1) + (size_t)n * ((jy - 1) + (size_t)n * (kz - 1))];
+                    if (par) {
+                        #pragma omp atomic
+                        *dst += (double)m[i];
+                    } else {
+                        *dst += (double)m[i];
+                    }
+                }
                 break;
             }
         }
@@ -502,6 +581,18 @@ void orc_ddens_sph(const orc_grid* g, int64_t np, const float* x, const float* y
         if (mass64_out) mass64_out[c] = u8[c];
     }
     free(u8);
+}
+
+void orc_ddens_sph(const orc_grid* g, int64_t np, const float* x, const float* y,
+                   const float* z, const float* hpart, const float* m,
+                   float* mass_out, double* mass64_out) {
+    ddens_sph(g, np, x, y, z, hpart, m, mass_out, mass64_out, 0);
+}
+
+void orc_ddens_sph_par(const orc_grid* g, int64_t np, const float* x, const float* y,
+                       const float* z, const float* hpart, const float* m,
+                       float* mass_out, double* mass64_out) {
+    ddens_sph(g, np, x, y, z, hpart, m, mass_out, mass64_out, 1);
 }
 
 /* TSC weights (particles.f90:77-96) */

@@ -64,6 +64,10 @@ void orc_vvel_vw(orc_knn* t, const orc_grid* g, int32_t k, const float* m,
                  const float* vz, const int8_t* smask, float* hpart,
                  float* u, float* v, float* w);
 /* H_DISTANCE (particles.f90:276-351) */
+void orc_vvel_cells(orc_knn* t, const orc_grid* g, int32_t k, const float* m,
+                    const float* part_dens, const float* vx, const float* vy,
+                    const float* vz, int64_t ncells, const int64_t* cells,
+                    float* u, float* v, float* w);
 void orc_h_distance(orc_knn* t, const orc_grid* g, int32_t k, const int8_t* smask,
                     float* hpart);
 /* FILL_H_ZEROS (particles.f90:398-431) */
@@ -73,6 +77,9 @@ void orc_fill_h_zeros(orc_knn* t, int64_t np, const float* x, const float* y,
 void orc_ddens_sph(const orc_grid* g, int64_t np, const float* x, const float* y,
                    const float* z, const float* hpart, const float* m,
                    float* mass_out, double* mass64_out);
+void orc_ddens_sph_par(const orc_grid* g, int64_t np, const float* x, const float* y,
+                       const float* z, const float* hpart, const float* m,
+                       float* mass_out, double* mass64_out);
 /* DENS_INTERP_TSC / VEL_INTERP_TSC (particles.f90:16-128, 132-272) */
 void orc_dens_tsc(const orc_grid* g, int64_t np, const float* x, const float* y,
                   const float* z, const float* m, float* mass_out);

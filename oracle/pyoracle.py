@@ -67,9 +67,12 @@ def lib():
     L.orc_ppart_dens.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, C.c_int32, _f32p, _f32p]
     L.orc_vvel_vw.argtypes = [vp, vp, C.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_void_p,
                               _f32p, _f32p, _f32p, _f32p]
+    L.orc_vvel_cells.argtypes = [vp, vp, C.c_int32, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_int64, _i64p,
+                                 _f32p, _f32p, _f32p]
     L.orc_h_distance.argtypes = [vp, vp, C.c_int32, C.c_void_p, _f32p]
     L.orc_fill_h_zeros.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, C.c_int32, _f32p]
     L.orc_ddens_sph.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f64p]
+    L.orc_ddens_sph_par.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f64p]
     L.orc_dens_tsc.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p]
     L.orc_vel_tsc.argtypes = [vp, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, C.c_int32, _f32p, _f32p,
                               _f32p]
@@ -200,8 +203,47 @@ class LevelResult:
     extra: dict = field(default_factory=dict)
 
 
+def fields_sampled(x, y, z, vx, vy, vz, m, n, L0, periodic, parts, cells, k=32):
+    """PART_DENS at the particles `parts` and the SPH velocities (u, v, w) at
+    the inner cells `cells` (0-based linear indices), by the same routines as
+    fields_from_particles (particles.f90:742-930) but evaluated only there: for
+    grids too large for the whole field stage on the CPU.  Inputs in file
+    units."""
+    L = lib()
+    g = Grid(n, L0, periodic)
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    vx, vy, vz, m = (np.array(a, np.float32, copy=True) for a in (vx, vy, vz, m))
+    L.orc_convert_units(x.size, vx, vy, vz, m)
+    knn = KNN(x, y, z, L0, periodic)
+    parts = np.ascontiguousarray(parts, np.int64)
+    cells = np.ascontiguousarray(cells, np.int64)
+    pd_s = np.zeros(parts.size, np.float32)
+    hk_s = np.zeros(parts.size, np.float32)
+    L.orc_ppart_dens(knn._p, parts.size, x[parts].copy(), y[parts].copy(), z[parts].copy(), m, k, pd_s, hk_s)
+    # PART_DENS of every neighbour of the sampled cells, for the SPH weights
+    rad = np.array([g.rad(i) for i in range(1, n + 1)], np.float32)
+    ix, iy, iz = cells % n, (cells // n) % n, cells // (n * n)
+    need = set()
+    for a, b, c in zip(rad[ix], rad[iy], rad[iz]):
+        idx, d2 = knn.query((a, b, c), k)
+        need.update(idx.tolist())
+        if d2[-1] <= g.dx * g.dx:  # a dense cell takes the ball of radius DX (particles.f90:868-878)
+            ball, _ = knn.ball((a, b, c), g.dx, cap=1 << 14)
+            need.update(ball.tolist())
+    need = np.array(sorted(need), np.int64)
+    pd_n = np.zeros(need.size, np.float32)
+    L.orc_ppart_dens(knn._p, need.size, x[need].copy(), y[need].copy(), z[need].copy(), m, k, pd_n,
+                     np.zeros(need.size, np.float32))
+    pdens = np.ones(x.size, np.float32)
+    pdens[need] = pd_n
+    u = np.zeros(cells.size, np.float32); v = np.zeros(cells.size, np.float32); w = np.zeros(cells.size, np.float32)
+    L.orc_vvel_cells(knn._p, g._p, k, m, pdens, vx, vy, vz, cells.size, cells, u, v, w)
+    return dict(part_dens=pd_s, hk=hk_s, u=u, v=v, w=w)
+
+
 def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.678,
-                          omega0=0.31, zeta=0.0, vel_interp=1, dens_interp=1, tsc_exact=False, smask=None):
+                          omega0=0.31, zeta=0.0, vel_interp=1, dens_interp=1, tsc_exact=False, smask=None,
+                          parallel_ddens=False):
     """Field stage (avism.f90:1177-1257, 1371-1414). Inputs in file units
     (reader.f90:447-461 conversion applied here).  Returns a dict of arrays.
     tsc_exact: TSC velocity sums in FP64 instead of the reference's REAL*4
@@ -209,7 +251,9 @@ def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.
     H_DISTANCE skip masked cells (particles.f90:312, 856), DDENS ignores it.
     vel_interp -1: velocities not available (FLAG_VEL_AVAILABLE = 0,
     avism.f90:1259-1300): density only, DIVERCO from linear theory
-    (avism.f90:1646-1650); vx, vy, vz may be None."""
+    (avism.f90:1646-1650); vx, vy, vz may be None.  parallel_ddens: DDENS on
+    every thread with atomic FP64 deposits (the last bits of the cell sums then
+    depend on the schedule, as in the reference's OpenMP run)."""
     L = lib()
     g = Grid(n, L0, periodic)
     x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
@@ -237,7 +281,7 @@ def fields_from_particles(x, y, z, vx, vy, vz, m, n, L0, periodic, k=32, ache=0.
         if vel_interp != 1:
             L.orc_h_distance(knn._p, g._p, k, sm, hpart)
             L.orc_fill_h_zeros(knn._p, npart, x, y, z, k, hpart)
-        L.orc_ddens_sph(g._p, npart, x, y, z, hpart, m, mass, mass64)
+        (L.orc_ddens_sph_par if parallel_ddens else L.orc_ddens_sph)(g._p, npart, x, y, z, hpart, m, mass, mass64)
     else:
         L.orc_dens_tsc(g._p, npart, x, y, z, m, mass)
     u1co = np.zeros(ncell, np.float32)

@@ -1,16 +1,27 @@
-"""The benchmark's parity configurations at their full sizes, checked through
-properties that do not depend on size (the oracle's field stage needs minutes
-there; its void stage does not):
+"""The benchmark's parity configurations at their full sizes:
 
 * mock voids: 10^6 particles (40 planted voids, non-periodic 200 Mpc/h),
   128^3 grid, k = 32, thresholds delta1 = -0.9, delta2 = -0.8,
-  div_th = 0, |grad delta|_th = 0.5;
-* Zel'dovich: 256^3 particles, periodic 500 Mpc/h, 256^3 grid, k = 32.
+  div_th = 0, |grad delta|_th = 0.5 -- every field against the oracle's whole
+  field stage (its per-query loops and DDENS on all host cores);
+* Zel'dovich: 256^3 particles, periodic 500 Mpc/h, 256^3 grid, k = 32 --
+  PART_DENS and the SPH velocities against the oracle at 10^4 sampled
+  particles and cells (its whole field stage needs minutes there);
+* Zel'dovich: 512^3 particles, periodic 1000 Mpc/h, 512^3 grid, k = 32, the
+  benchmark workload -- sampled fields as above, and the trained network's
+  proposals give the plain catalogue.
 
-For both, the void stage run on the GPU's own fields equals the oracle's on the
+For all, the void stage run on the GPU's own fields equals the oracle's on the
 same fields bit for bit, the learned-proposal growth equals the plain growth,
-and, in the periodic box, the density field holds the particles' total mass
-(DDENS conserves mass, particles.f90:549-604)."""
+and, in the periodic boxes, the density field holds the particles' total mass
+(DDENS conserves mass, particles.f90:549-604) with mean U1CO 1.
+
+The field tolerances: HPART bit-exact; PART_DENS and U1CO within 1e-5
+relative per particle or cell; u, v, w within 1e-5 of their RMS per cell;
+DIVERCO within DIV_TOL of its RMS per cell (central differences of the
+velocities, divergence.f90:2-50, amplify their FP64-sum-order differences
+where the velocity field is smooth: DIVERCO's RMS is far below u's RMS / DX).
+The measured maxima are printed."""
 from __future__ import annotations
 
 import time
@@ -24,6 +35,39 @@ from test_gpu_voids import VF_NAMES, compare, gpu_void_outputs
 pytestmark = pytest.mark.gpu
 
 UM, UL = np.float32(9.1717e18), np.float32(10.98)
+REL = 1e-5
+DIV_TOL = 1e-5
+
+
+def _rmsrel(a, ref):
+    ref = ref.astype(np.float64)
+    return float(np.max(np.abs(a.astype(np.float64) - ref)) / np.sqrt(np.mean(ref ** 2)))
+
+
+def _rel(a, ref):
+    ref = ref.astype(np.float64)
+    return float(np.max(np.abs(a.astype(np.float64) - ref) / np.abs(ref)))
+
+
+def _sampled_fields(c, arr, n, L0, per, k=32, nsample=10_000, seed=0):
+    """PART_DENS and u, v, w at sampled particles and cells against the oracle."""
+    rng = np.random.default_rng(seed)
+    parts = np.sort(rng.choice(arr[0].size, nsample, replace=False))
+    cells = np.sort(rng.choice(n ** 3, nsample, replace=False))
+    t = time.time()
+    ref = O.fields_sampled(*arr, n, L0, per, parts, cells, k=k)
+    f = c.fields()
+    pd, _ = c.particle_fields(arr[0].size)
+    err = dict(part_dens=_rel(pd[parts], ref["part_dens"]))
+    for q in ("u", "v", "w"):
+        rms = np.sqrt(np.mean(f[q].astype(np.float64) ** 2))
+        err[q] = float(np.max(np.abs(f[q][cells].astype(np.float64) - ref[q])) / rms)
+    print(f"  sampled fields vs oracle ({nsample} particles and cells, {time.time() - t:.1f} s): "
+          + ", ".join(f"{q} {e:.2e}" for q, e in err.items()), flush=True)
+    assert err["part_dens"] <= REL
+    for q in ("u", "v", "w"):
+        assert err[q] <= REL, q
+    return f
 
 
 def _ctx(n, L0, per, P):
@@ -72,6 +116,20 @@ def test_mock_voids_1e6_on_128():
     c = _ctx(n, L0, 0, P)
     c.load_particles(*arr)
     c.particles_to_grid(k=32)
+    # every field against the oracle's field stage
+    t = time.time()
+    ref = O.fields_from_particles(*arr, n, L0, 0, k=32, parallel_ddens=True)
+    f = c.fields()
+    pd, hp = c.particle_fields(arr[0].size)
+    err = dict(part_dens=_rel(pd, ref["part_dens"]), u1co=_rel(f["u1co"], ref["u1co"]),
+               u=_rmsrel(f["u"], ref["u"]), v=_rmsrel(f["v"], ref["v"]), w=_rmsrel(f["w"], ref["w"]),
+               div=_rmsrel(f["div"], ref["div"]), hpart_differ=int(np.sum(hp != ref["hpart"])))
+    print(f"  fields vs oracle (oracle {time.time() - t:.1f} s): "
+          + ", ".join(f"{q} {e:.2e}" if isinstance(e, float) else f"{q} {e}" for q, e in err.items()), flush=True)
+    assert err["hpart_differ"] == 0
+    for q in ("part_dens", "u1co", "u", "v", "w"):
+        assert err[q] <= REL, q
+    assert err["div"] <= DIV_TOL
     plain, _, _ = _check_void_stage(c, n, L0, 0, P)
     assert plain["nvoidt"] > 0
     # the planted voids: most of the large ones away from the faces have a
@@ -94,12 +152,41 @@ def test_zeldovich_256():
     c = _ctx(n, L0, 1, P)
     c.load_particles(*arr)
     c.particles_to_grid(k=32)
+    _sampled_fields(c, arr, n, L0, 1)
     _, f, meandens = _check_void_stage(c, n, L0, 1, P)
-    # mass: MEANDENS (Msun/Mpc^3) x the box volume in the code's units is the
-    # particles' total mass (avism.f90:1371-1409; periodic, so nothing leaves)
+    _check_mass(arr, n, L0, f, meandens)
+
+
+def _check_mass(arr, n, L0, f, meandens):
+    """MEANDENS (Msun/Mpc^3) x the box volume in the code's units is the
+    particles' total mass (avism.f90:1371-1409; periodic, so nothing leaves),
+    and U1CO averages to 1"""
     rote, rete, _ = O.cosmo(0.678, 0.31, 0.0)
     dx = np.float64(np.float32(L0) / np.float32(n))
     total = np.sum((arr[6] / UM).astype(np.float32).astype(np.float64))
     expect = total / n ** 3 / (dx * rete) ** 3 * np.float64(UM) / np.float64(UL) ** 3
     assert abs(meandens - expect) <= 2e-6 * expect
     assert abs(np.mean(f["u1co"].astype(np.float64)) - 1.0) <= 1e-5
+
+
+def test_zeldovich_512_learned():
+    """The benchmark workload: the trained network's proposals (GROW_LEARNED)
+    give the plain growth's catalogue; sampled fields against the oracle; the
+    void stage on the GPU's fields against the oracle's."""
+    from avism_amd import learned, synth
+    from avism_amd._lib import GROW_LEARNED
+    n, L0 = 512, 1000.0
+    P = O.VoidParams(nvoid_max=4_000_000)
+    p = synth.zeldovich(n, L0, seed=4, device="cuda")
+    arr = p.numpy()
+    print(f"[zeldovich 512] np={arr[0].size} n={n}", flush=True)
+    c = _ctx(n, L0, 1, P)
+    c.load_particles(*arr)
+    c.particles_to_grid(k=32)
+    _sampled_fields(c, arr, n, L0, 1)
+    learned.Proposer.load_default().install(c)
+    lrn = gpu_void_outputs(c, GROW_LEARNED)
+    plain, f, meandens = _check_void_stage(c, n, L0, 1, P)
+    for k in ("marca", "uvoid", "indice", "cubes_i", "cubes_f", "ncellv", "vf"):
+        assert np.array_equal(lrn[k], plain[k]), k
+    _check_mass(arr, n, L0, f, meandens)
