@@ -60,6 +60,8 @@ EXPORTS = [
     "avism_merge_and_post", "avism_get_catalogue", "avism_get_marca", "avism_run_level",
     "avism_cnn_set_params", "avism_cnn_patches", "avism_cnn_forward", "avism_cnn_backward", "avism_cnn_adam",
     "avism_sync", "avism_get_counters", "avism_indexx_noalloc",
+    "avism_batch_set", "avism_batch_load", "avism_batch_fields", "avism_batch_voids", "avism_batch_patch",
+    "avism_batch_get_fields", "avism_batch_info",
 ]
 
 _lib = None
@@ -122,6 +124,13 @@ def lib():
         "avism_sync": ([vp], C.c_int),
         "avism_get_counters": ([vp, vp, i32], C.c_int),
         "avism_indexx_noalloc": ([vp, vp, i64, vp], C.c_int),
+        "avism_batch_set": ([vp, i32, vp, vp, vp, vp], C.c_int),
+        "avism_batch_load": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, i32], C.c_int),
+        "avism_batch_fields": ([vp, i32], C.c_int),
+        "avism_batch_voids": ([vp, i32, i32], C.c_int),
+        "avism_batch_patch": ([vp, i32, C.POINTER(vp)], C.c_int),
+        "avism_batch_get_fields": ([vp, i32, vp, vp, vp, vp, vp, vp, vp, i32], C.c_int),
+        "avism_batch_info": ([vp, C.c_char_p, i32, C.POINTER(i64), vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -156,9 +165,9 @@ class Context:
             raise AvismError(rc, lib().avism_last_error(self._h).decode())
 
     def close(self):
-        if self._h:
+        if self._h and not getattr(self, "_borrowed", False):
             lib().avism_destroy(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -383,6 +392,68 @@ class Context:
             raise ValueError("MARCA output has the wrong size")
         self._chk(lib().avism_get_marca(self._h, _ptr(out), int(bool(inner_only)), AVISM_HOST))
         return out
+
+    # ---- batches of independent patches (each patch: one grid, its particles and mask)
+    def batch_set(self, n, L0, nps, periodic=None):
+        """Layout: patch p has n[p]^3 cells over a box of L0[p] and nps[p] particles."""
+        n = np.ascontiguousarray(n, np.int32)
+        L0 = np.ascontiguousarray(L0, np.float32)
+        nps = np.ascontiguousarray(nps, np.int64)
+        per = None if periodic is None else np.ascontiguousarray(periodic, np.int32)
+        self._batch_layout = (n.copy(), L0.copy(), nps.copy())
+        self._chk(lib().avism_batch_set(self._h, int(n.size), _ptr(n), _ptr(L0), _ptr(per), _ptr(nps)))
+
+    def batch_load(self, x, y, z, vx, vy, vz, m, smask=None, where=None):
+        """Every patch's particles back to back (numpy or torch, host or device);
+        smask: the patches' inner masks back to back (int8) or None."""
+        arrs = [x, y, z, vx, vy, vz, m]
+        if where is None:
+            where = AVISM_DEVICE if getattr(x, "is_cuda", False) else AVISM_HOST
+        if where == AVISM_HOST:
+            arrs = [None if a is None else np.ascontiguousarray(a if isinstance(a, np.ndarray) else a.numpy(),
+                                                                np.float32) for a in arrs]
+            if smask is not None:
+                smask = np.ascontiguousarray(smask if isinstance(smask, np.ndarray) else smask.numpy(), np.int8)
+        self._keep_batch = (arrs, smask)
+        self._chk(lib().avism_batch_load(self._h, *[_ptr(a) for a in arrs], _ptr(smask), int(where)))
+
+    def batch_fields(self, k=32):
+        self._chk(lib().avism_batch_fields(self._h, int(k)))
+
+    def batch_voids(self, growth_mode=GROW_PLAIN, threads=8):
+        self._chk(lib().avism_batch_voids(self._h, int(growth_mode), int(threads)))
+
+    def batch_patch(self, p: int) -> "Context":
+        """Patch p's context (owned by this one) for the usual getters."""
+        h = C.c_void_p()
+        self._chk(lib().avism_batch_patch(self._h, int(p), C.byref(h)))
+        sub = Context.__new__(Context)
+        sub._h = h
+        sub._borrowed = True
+        sub._parent = self
+        return sub
+
+    def batch_fields_of(self, p: int):
+        """Patch p's inner fields and, in its input order, PART_DENS and HPART."""
+        n, _, nps = self._batch_layout
+        n3, npp = int(n[p]) ** 3, int(nps[p])
+        out = {k: np.zeros(n3, np.float32) for k in ("u1co", "div", "u", "v", "w")}
+        out["part_dens"] = np.zeros(npp, np.float32)
+        out["hpart"] = np.zeros(npp, np.float32)
+        self._chk(lib().avism_batch_get_fields(self._h, int(p), *[_ptr(out[k]) for k in
+                                                                  ("u1co", "div", "u", "v", "w", "part_dens",
+                                                                   "hpart")], AVISM_HOST))
+        return out
+
+    def batch_info(self):
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        launches = C.c_int64()
+        nb = len(self._batch_layout[0])
+        md = np.zeros(nb, np.float32)
+        self._chk(lib().avism_batch_info(self._h, buf, len(buf), C.byref(launches), _ptr(md)))
+        txt = buf.value.decode()
+        return dict(tuning=json.loads(txt) if txt else None, graph_launches=int(launches.value), meandens=md)
 
     def run_level(self, k=32, vel_interp=1, dens_interp=1, growth_mode=GROW_PLAIN) -> LevelStats:
         st = LevelStats()

@@ -40,6 +40,8 @@ struct GridGeom {
   float radx(int i) const { return rad[i - rlo]; }
 };
 
+struct BatchState;  // a batch of patches (batch.hip)
+
 GridGeom make_grid(int32_t n, float L0, int32_t periodic);
 
 // Device buffer that only grows.
@@ -145,6 +147,7 @@ struct avism_ctx {
   bool post_ready = false;                     // MARCA and the catalogue are final
   double volt_clean = 0;
   int64_t growth_stats[8] = {0};
+  avism::BatchState* batch = nullptr;          // avism_batch_* state
 };
 
 #define AV_CHECK_HIP(ctx, expr)                                                        \
@@ -176,6 +179,18 @@ int merge_and_post(avism_ctx* c, int32_t stop_after);
 int next_level(avism_ctx* c);
 int indexx_device(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx);
 int noalloc_init(int device);
+// batches of independent patches (batch.hip)
+void batch_free(avism_ctx* c);
+int batch_set(avism_ctx* c, int32_t npatch, const int32_t* n, const float* L0, const int32_t* periodic,
+              const int64_t* np);
+int batch_load(avism_ctx* c, const float* x, const float* y, const float* z, const float* vx, const float* vy,
+               const float* vz, const float* m, const int8_t* smask, int where);
+int batch_fields(avism_ctx* c, int32_t k);
+int batch_voids(avism_ctx* c, int32_t growth_mode, int32_t nthreads);
+int batch_patch(avism_ctx* c, int32_t p, avism_ctx** out);
+int batch_get_fields(avism_ctx* c, int32_t p, float* u1co, float* div, float* u, float* v, float* w,
+                     float* part_dens, float* hpart, int where);
+int batch_info(avism_ctx* c, char* tuning, int32_t len, int64_t* graph_launches, float* meandens);
 int indexx_noalloc(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx);
 void add_timing(avism_ctx* c, const char* name, hipEvent_t a, hipEvent_t b);
 int cnn_patches(avism_ctx* c, const int64_t* ranks, int64_t n, uint16_t* out);

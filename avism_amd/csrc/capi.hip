@@ -80,6 +80,7 @@ int avism_destroy(avism_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   (void)hipStreamSynchronize(c->cstream);
+  batch_free(c);
   drop_timings(c);
   (void)hipEventDestroy(c->ev_vel);
   (void)hipEventDestroy(c->ev_mark);
@@ -498,6 +499,49 @@ int avism_cnn_adam(avism_ctx* c, float* params, const float* grads, float* m, fl
   DEVICE_ARGS(c, AVISM_DEVICE);
   if (!params || !grads || !m || !v || step < 1) AV_FAIL(c, AVISM_ERR_ARG, "bad Adam request");
   return cnn_adam(c, params, grads, m, v, lr, step);
+}
+
+int avism_batch_set(avism_ctx* c, int32_t npatch, const int32_t* n, const float* L0, const int32_t* periodic,
+                    const int64_t* np) {
+  CTX_GUARD(c);
+  return batch_set(c, npatch, n, L0, periodic, np);
+}
+
+int avism_batch_load(avism_ctx* c, const float* x, const float* y, const float* z, const float* vx,
+                     const float* vy, const float* vz, const float* m, const int8_t* smask, int32_t where) {
+  CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
+  if (!x || !y || !z || !m) AV_FAIL(c, AVISM_ERR_ARG, "null particle array");
+  if ((vx == nullptr) != (vy == nullptr) || (vx == nullptr) != (vz == nullptr))
+    AV_FAIL(c, AVISM_ERR_ARG, "velocities: all three arrays or none");
+  return batch_load(c, x, y, z, vx, vy, vz, m, smask, where);
+}
+
+int avism_batch_fields(avism_ctx* c, int32_t k) {
+  CTX_GUARD(c);
+  return batch_fields(c, k);
+}
+
+int avism_batch_voids(avism_ctx* c, int32_t growth_mode, int32_t nthreads) {
+  CTX_GUARD(c);
+  return batch_voids(c, growth_mode, nthreads);
+}
+
+int avism_batch_patch(avism_ctx* c, int32_t p, avism_ctx** patch_ctx) {
+  CTX_GUARD(c);
+  return batch_patch(c, p, patch_ctx);
+}
+
+int avism_batch_get_fields(avism_ctx* c, int32_t p, float* u1co, float* div, float* u, float* v, float* w,
+                           float* part_dens, float* hpart, int32_t where) {
+  CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
+  return batch_get_fields(c, p, u1co, div, u, v, w, part_dens, hpart, where);
+}
+
+int avism_batch_info(avism_ctx* c, char* tuning, int32_t len, int64_t* graph_launches, float* meandens) {
+  CTX_GUARD(c);
+  return batch_info(c, tuning, len, graph_launches, meandens);
 }
 
 int avism_run_level(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp,

@@ -26,6 +26,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "avism_internal.h"
+#include "fields_batch.h"
 #include "knn_search.h"
 
 namespace avism {
@@ -505,47 +506,63 @@ __device__ __forceinline__ float ddens_w(const DdensGeom& G, float x, float y, f
 // integer atomics), then adds that copy to HBM once per cell: at one particle
 // per cell a cell receives ~32 deposits but only a handful of tile flushes.
 // A particle whose cell box reaches past the widened tile (large h, in voids)
-// adds directly to HBM.
-constexpr int DTX = 8, DTY = 8, DTZ = 4, DR = 4;
-constexpr int DBX = DTX + 2 * DR, DBY = DTY + 2 * DR, DBZ = DTZ + 2 * DR;
+// adds directly to HBM.  ddens_tile is one tile (bt, in the grid's tile
+// order) for a whole block of 256 threads; DTY and DTZ must be multiples of
+// the bin size (1, 2 or 4 cells).
+template <int DTX, int DTY, int DTZ, int DR, int FM = 1>
+struct DdensTile {
+  static constexpr int DBX = DTX + 2 * DR, DBY = DTY + 2 * DR, DBZ = DTZ + 2 * DR;
+  static constexpr int LDS = DBX * DBY * DBZ;
+  // rows of bins of a tile, with up to FM bins per cell along an axis
+  static constexpr int ROWS = (DTY * FM) * (DTZ * FM);
+  static int64_t tiles(int n) {
+    return (int64_t)((n + DTX - 1) / DTX) * ((n + DTY - 1) / DTY) * ((n + DTZ - 1) / DTZ);
+  }
+};
 
-__global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, const float* __restrict__ hpart,
-                                                      double scale, unsigned long long* acc, float* norm,
-                                                      unsigned long long* npairs) {
-  __shared__ unsigned long long box[DBX * DBY * DBZ];
-  __shared__ int qrow[DTY * DTZ + 1], qbeg[DTY * DTZ];
+template <int DTX, int DTY, int DTZ, int DR, int FM>
+__device__ __forceinline__ void ddens_tile(const DdensGeom& G, const Bins& B, const float* __restrict__ hpart,
+                                           double scale, unsigned long long* acc, float* norm,
+                                           unsigned long long* npairs, int64_t bt, unsigned long long* box,
+                                           int* qrow, int* qbeg) {
+  using T = DdensTile<DTX, DTY, DTZ, DR, FM>;
+  constexpr int DBX = T::DBX, DBY = T::DBY, DBZ = T::DBZ;
   const int n = G.n;
   const int ntx = (n + DTX - 1) / DTX, nty = (n + DTY - 1) / DTY;
-  const int64_t bt = xcd_block(blockIdx.x, gridDim.x);
   const int tx0 = (int)(bt % ntx) * DTX, ty0 = (int)((bt / ntx) % nty) * DTY, tz0 = (int)(bt / ((int64_t)ntx * nty)) * DTZ;
   const int tid = threadIdx.x;
   for (int c = tid; c < DBX * DBY * DBZ; c += 256) box[c] = 0ull;
-  if (tid == 0) {  // the tile's particles: one contiguous range per row of bins (a bin is s^3 cells)
-    const int nb = B.nb, s = n / nb;
-    const int bty = DTY / s, btz = DTZ / s, bx0 = tx0 / s, by0 = ty0 / s, bz0 = tz0 / s;
-    int accn = 0;
-    for (int r = 0; r < bty * btz; ++r) {
-      const int y = by0 + r % bty, z = bz0 + r / bty;
-      int cnt = 0, b0 = 0;
-      if (y < nb && z < nb) {
-        const int rowb = nb * (y + nb * z);
-        b0 = B.start[rowb + bx0];
-        cnt = B.start[rowb + min(bx0 + DTX / s, nb)] - b0;
-      }
-      qrow[r] = accn;
-      qbeg[r] = b0;
-      accn += cnt;
-    }
-    for (int r = bty * btz; r <= DTY * DTZ; ++r) qrow[r] = accn;
+  // the tile's particles: one contiguous range per row of bins.  Cell i
+  // (0-based) holds bins [i nb / n, (i + 1) nb / n): a bin is s^3 cells (nb =
+  // n / s, s dividing the tile) or a cell f^3 bins (nb = n f, f <= FM).
+  const int nb = B.nb;
+  const int ib0 = (int)((int64_t)tx0 * nb / n), ib1 = (int)min((int64_t)(tx0 + DTX) * nb / n, (int64_t)nb);
+  const int jb0 = (int)((int64_t)ty0 * nb / n), jb1 = (int)min((int64_t)(ty0 + DTY) * nb / n, (int64_t)nb);
+  const int kb0 = (int)((int64_t)tz0 * nb / n), kb1 = (int)min((int64_t)(tz0 + DTZ) * nb / n, (int64_t)nb);
+  const int nry = jb1 - jb0, nr = nry * (kb1 - kb0);  // <= T::ROWS
+  for (int r = tid; r < nr; r += 256) {
+    const int rowb = nb * ((jb0 + r % nry) + nb * (kb0 + r / nry));
+    const int b0 = B.start[rowb + ib0];
+    qbeg[r] = b0;
+    qrow[r + 1] = B.start[rowb + ib1] - b0;
+  }
+  __syncthreads();
+  if (tid == 0) {  // counts -> offsets
+    qrow[0] = 0;
+    for (int r = 0; r < nr; ++r) qrow[r + 1] += qrow[r];
   }
   __syncthreads();
   // widened tile in 1-based cells
   const int bx0 = tx0 + 1 - DR, by0 = ty0 + 1 - DR, bz0 = tz0 + 1 - DR;
-  const int nq = qrow[DTY * DTZ];
+  const int nq = qrow[nr];
   unsigned pairs = 0;  // (particle, cell) pairs within h: the work of the gather model
   for (int q = tid; q < nq; q += 256) {
-    int r = 0;
-    while (qrow[r + 1] <= q) ++r;
+    int r = 0, rh = nr;  // the row holding q: the last r with qrow[r] <= q
+    while (rh - r > 1) {
+      const int mid = (r + rh) >> 1;
+      if (qrow[mid] <= q) r = mid;
+      else rh = mid;
+    }
     const int64_t i = qbeg[r] + (q - qrow[r]);
     const float4 p = B.pos[i];
     const float hh = hpart[i];
@@ -631,6 +648,37 @@ __global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, cons
     const int ci = bx0 + c % DBX, cj = by0 + (c / DBX) % DBY, ck = bz0 + c / (DBX * DBY);
     atomicAdd(&acc[(size_t)(ci - 1) + (size_t)n * ((size_t)(cj - 1) + (size_t)n * (ck - 1))], v);
   }
+}
+
+constexpr int DTX = 8, DTY = 8, DTZ = 4, DR = 4;  // the single-grid tile
+
+// FM: the most bins per cell along an axis the launch allows (1, or 4 where
+// dense particle sets split cells)
+template <int FM>
+__global__ void __launch_bounds__(256) k_ddens_scatter(DdensGeom G, Bins B, const float* __restrict__ hpart,
+                                                      double scale, unsigned long long* acc, float* norm,
+                                                      unsigned long long* npairs) {
+  using T = DdensTile<DTX, DTY, DTZ, DR, FM>;
+  __shared__ unsigned long long box[T::LDS];
+  __shared__ int qrow[T::ROWS + 1], qbeg[T::ROWS];
+  ddens_tile<DTX, DTY, DTZ, DR, FM>(G, B, hpart, scale, acc, norm, npairs, xcd_block(blockIdx.x, gridDim.x), box,
+                                    qrow, qbeg);
+}
+
+// bins per axis for np particles on n^3 cells, about one particle per bin:
+// bins of s^3 cells (s = 1, 2, 4, dividing n; sparse sets, and the finer
+// grids of void-in-void levels) or cells of f^3 bins (f = 2, 4; dense sets),
+// at most 1024 bins per axis
+int32_t bins_per_axis(int32_t n, int64_t np) {
+  const double ppc = (double)np / ((double)n * n * n);
+  if (ppc >= 8.0) {
+    int f = ppc >= 64.0 ? 4 : 2;
+    while (f > 1 && (int64_t)n * f > 1024) f /= 2;
+    return n * f;
+  }
+  int s = 1;
+  while (s < 4 && n % (2 * s) == 0 && (double)(8 * s * s * s) * ppc <= 1.5) s *= 2;
+  return n / s;
 }
 
 // cell sums -> U1 = mass/(dx^3 rete^3)/rote*rote (avism.f90:1249-1250, 1386)
@@ -772,6 +820,281 @@ __global__ void k_scatter_orig(int64_t np, const uint32_t* orig, const float* a_
   a_out[orig[i]] = a_sorted[i];
 }
 
+// ------------------------------------------------------- batches of patches
+// Persistent kernels over the work items of all patches of a batch (see
+// fields_batch.h): a block takes items from its XCD's queue until the queue
+// is empty, each item being the single-grid kernel's unit of work on one
+// patch, so every patch's results are those of the single-grid kernels.
+
+// the next item of this block's XCD queue, or -1 (every thread of the block
+// calls it; items are handed out once each)
+__device__ __forceinline__ int64_t next_item(const ItemQueue& q, int64_t* slot) {
+  const int x = blockIdx.x & 7;
+  const int64_t lo = q.total * x / 8, hi = q.total * (x + 1) / 8;
+  if (threadIdx.x == 0) *slot = lo + (int64_t)atomicAdd(&q.cnt[x], 1u);
+  __syncthreads();
+  const int64_t it = *slot;
+  __syncthreads();  // every thread has read it before thread 0 writes the next
+  return it < hi ? it : -1;
+}
+
+// the patch of item it: the last p with off[p] <= it
+__device__ __forceinline__ int item_patch(const ItemQueue& q, int64_t it) {
+  int lo = 0, hi = q.npatch;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (q.off[mid] <= it) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+constexpr int kBatchQ = 128;  // lanes of the batched kNN kernels (k <= 64)
+
+template <int K>
+__global__ void __launch_bounds__(kBatchQ) k_batch_ppart(const PatchDev* __restrict__ P, ItemQueue q, int k) {
+  static_assert(kq_of(K) == kBatchQ, "batched kNN kernels take k <= 64");
+  __shared__ uint32_t keys_s[K * kBatchQ], gmx_s[K / 4 * kBatchQ];
+  __shared__ int64_t item_s;
+  for (;;) {
+    const int64_t it = next_item(q, &item_s);
+    if (it < 0) break;
+    const int p = item_patch(q, it);
+    const PatchDev& D = P[p];
+    const int64_t i = (it - q.off[p]) * kBatchQ + threadIdx.x;
+    if (i >= D.np) continue;
+    if (D.B.periodic)
+      ppart_one<1, K>(D.B, D.np, k, D.r0, D.uniform, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, D.pdens, D.hk);
+    else
+      ppart_one<0, K>(D.B, D.np, k, D.r0, D.uniform, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, D.pdens, D.hk);
+  }
+}
+
+template <int TX, int TY, int TZ>
+struct VvelShape {
+  static_assert(TX * TY * TZ == kBatchQ, "one cell per lane");
+  static int64_t tiles(int n) {
+    return (int64_t)((n + TX - 1) / TX) * ((n + TY - 1) / TY) * ((n + TZ - 1) / TZ);
+  }
+};
+
+template <int K, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(kBatchQ) k_batch_vvel(const PatchDev* __restrict__ P, ItemQueue q, int k) {
+  __shared__ uint32_t keys_s[K * kBatchQ], gmx_s[K / 4 * kBatchQ];
+  __shared__ int64_t item_s;
+  const int tid = threadIdx.x;
+  for (;;) {
+    const int64_t it = next_item(q, &item_s);
+    if (it < 0) break;
+    const int p = item_patch(q, it);
+    const PatchDev& D = P[p];
+    const int n = D.n;
+    const int64_t bt = it - q.off[p];
+    const int ntx = (n + TX - 1) / TX, nty = (n + TY - 1) / TY;
+    const int ix = (int)(bt % ntx) * TX + tid % TX, iy = (int)((bt / ntx) % nty) * TY + (tid / TX) % TY,
+              iz = (int)(bt / ((int64_t)ntx * nty)) * TZ + tid / (TX * TY);
+    if (ix >= n || iy >= n || iz >= n) continue;
+    const size_t c = (size_t)ix + (size_t)n * ((size_t)iy + (size_t)n * iz);
+    if (D.smask && D.smask[c] == 0) {  // masked cells keep zero velocity (particles.f90:856)
+      D.u[c] = 0.0f;
+      D.v[c] = 0.0f;
+      D.w[c] = 0.0f;
+      continue;
+    }
+    const float qx = D.rad[ix + 1 - D.rlo], qy = D.rad[iy + 1 - D.rlo], qz = D.rad[iz + 1 - D.rlo];
+    const VvelOut O{D.vel, D.hbits, D.u, D.v, D.w, 1};
+    if (D.B.periodic)
+      vvel_one<1, K>(D.B, D.np, k, D.r0, D.dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
+    else
+      vvel_one<0, K>(D.B, D.np, k, D.r0, D.dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
+  }
+}
+
+template <int TX, int TY, int TZ, int R>
+__global__ void __launch_bounds__(256) k_batch_ddens(const PatchDev* __restrict__ P, ItemQueue q,
+                                                    unsigned long long* npairs) {
+  using T = DdensTile<TX, TY, TZ, R, 4>;
+  __shared__ unsigned long long box[T::LDS];
+  __shared__ int qrow[T::ROWS + 1], qbeg[T::ROWS];
+  __shared__ int64_t item_s;
+  for (;;) {
+    const int64_t it = next_item(q, &item_s);
+    if (it < 0) break;
+    const int p = item_patch(q, it);
+    const PatchDev& D = P[p];
+    const DdensGeom G{D.rad, D.rlo, D.n, D.dx, D.rx1};
+    ddens_tile<TX, TY, TZ, R, 4>(G, D.B, reinterpret_cast<const float*>(D.hbits), D.scale, D.acc, D.norm, npairs,
+                                 it - q.off[p], box, qrow, qbeg);
+    __syncthreads();  // the tile's LDS is flushed before the next tile clears it
+  }
+}
+
+__global__ void __launch_bounds__(kFinB) k_batch_finalize(const PatchDev* __restrict__ P, int p, int64_t ncell,
+                                                         float den, float rote, float* umass, double* partial) {
+  // the single-grid kernel's body with the patch's scale read here
+  const double inv_scale = 1.0 / P[p].scale;
+  const unsigned long long* acc = P[p].acc;
+  const int64_t c0 = ((int64_t)blockIdx.x * kFinB + threadIdx.x) * kFinV;
+  double contrib = 0.0;
+  if (c0 + kFinV <= ncell) {
+    const ulonglong2 a01 = *reinterpret_cast<const ulonglong2*>(acc + c0);
+    const ulonglong2 a23 = *reinterpret_cast<const ulonglong2*>(acc + c0 + 2);
+    float4 u;
+    u.x = fin_cell(a01.x, inv_scale, 0, den, rote);
+    u.y = fin_cell(a01.y, inv_scale, 0, den, rote);
+    u.z = fin_cell(a23.x, inv_scale, 0, den, rote);
+    u.w = fin_cell(a23.y, inv_scale, 0, den, rote);
+    *reinterpret_cast<float4*>(umass + c0) = u;
+    contrib = (((double)u.x + (double)u.y) + (double)u.z) + (double)u.w;
+  } else {
+    for (int64_t c = c0; c < ncell; ++c) {
+      const float u = fin_cell(acc[c], inv_scale, 0, den, rote);
+      umass[c] = u;
+      contrib += (double)u;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off);
+  __shared__ double wsum[kFinB / 64];
+  if (threadIdx.x % 64 == 0) wsum[threadIdx.x / 64] = contrib;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kFinB / 64; ++w) t += wsum[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_fill_u32(int64_t n, uint32_t* a, uint32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_scatter_orig_off(int64_t np, const uint32_t* orig, int64_t orig0, const float* a_sorted,
+                                   float* a_out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  a_out[(int64_t)orig[i] - orig0] = a_sorted[i];
+}
+
+const char* vvel_shape_name(int s) {
+  static const char* names[kVvelShapes] = {"8x4x4", "4x8x4", "4x4x8"};
+  return (s >= 0 && s < kVvelShapes) ? names[s] : "?";
+}
+const char* ddens_shape_name(int s) {
+  static const char* names[kDdensShapes] = {"8x8x4+4", "8x8x8+3", "16x8x4+4"};
+  return (s >= 0 && s < kDdensShapes) ? names[s] : "?";
+}
+int64_t vvel_shape_tiles(int s, int n) {
+  switch (s) {
+    case 0: return VvelShape<8, 4, 4>::tiles(n);
+    case 1: return VvelShape<4, 8, 4>::tiles(n);
+    default: return VvelShape<4, 4, 8>::tiles(n);
+  }
+}
+int64_t ddens_shape_tiles(int s, int n) {
+  switch (s) {
+    case 0: return DdensTile<8, 8, 4, 4>::tiles(n);
+    case 1: return DdensTile<8, 8, 8, 3>::tiles(n);
+    default: return DdensTile<16, 8, 4, 4>::tiles(n);
+  }
+}
+int64_t ppart_chunks(int64_t np) { return (np + kBatchQ - 1) / kBatchQ; }
+
+hipError_t launch_batch_ppart(hipStream_t st, const PatchDev* P, ItemQueue q, int k, unsigned nblocks) {
+  if (k <= 32) k_batch_ppart<32><<<nblocks, kBatchQ, 0, st>>>(P, q, k);
+  else k_batch_ppart<64><<<nblocks, kBatchQ, 0, st>>>(P, q, k);
+  return hipGetLastError();
+}
+
+template <int K>
+static void vvel_shape_launch(hipStream_t st, const PatchDev* P, ItemQueue q, int k, int shape, unsigned nb) {
+  switch (shape) {
+    case 0: k_batch_vvel<K, 8, 4, 4><<<nb, kBatchQ, 0, st>>>(P, q, k); break;
+    case 1: k_batch_vvel<K, 4, 8, 4><<<nb, kBatchQ, 0, st>>>(P, q, k); break;
+    default: k_batch_vvel<K, 4, 4, 8><<<nb, kBatchQ, 0, st>>>(P, q, k); break;
+  }
+}
+
+hipError_t launch_batch_vvel(hipStream_t st, const PatchDev* P, ItemQueue q, int k, int shape, unsigned nblocks) {
+  if (k <= 32) vvel_shape_launch<32>(st, P, q, k, shape, nblocks);
+  else vvel_shape_launch<64>(st, P, q, k, shape, nblocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_ddens(hipStream_t st, const PatchDev* P, ItemQueue q, int shape, unsigned nblocks,
+                              unsigned long long* npairs) {
+  switch (shape) {
+    case 0: k_batch_ddens<8, 8, 4, 4><<<nblocks, 256, 0, st>>>(P, q, npairs); break;
+    case 1: k_batch_ddens<8, 8, 8, 3><<<nblocks, 256, 0, st>>>(P, q, npairs); break;
+    default: k_batch_ddens<16, 8, 4, 4><<<nblocks, 256, 0, st>>>(P, q, npairs); break;
+  }
+  return hipGetLastError();
+}
+
+int64_t finalize_blocks(int64_t ncell) { return (ncell + (int64_t)kFinB * kFinV - 1) / ((int64_t)kFinB * kFinV); }
+
+hipError_t launch_batch_finalize(hipStream_t st, const PatchDev* P, int p, int64_t ncell, float den, float rote,
+                                 float* umass, double* partial) {
+  k_batch_finalize<<<(unsigned)finalize_blocks(ncell), kFinB, 0, st>>>(P, p, ncell, den, rote, umass, partial);
+  return hipGetLastError();
+}
+
+static inline unsigned nblk256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t launch_fill_u32(hipStream_t st, int64_t n, uint32_t* a, uint32_t v) {
+  if (n > 0) k_fill_u32<<<nblk256(n), 256, 0, st>>>(n, a, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u64(hipStream_t st, int64_t n, unsigned long long* a, unsigned long long v) {
+  if (n > 0) k_fill_u64<<<nblk256(n), 256, 0, st>>>(n, a, v);
+  return hipGetLastError();
+}
+hipError_t launch_velrec(hipStream_t st, int64_t np, const uint32_t* orig, const float4* pos, const float* pdens,
+                         const float* vx, const float* vy, const float* vz, float4* vel) {
+  if (np > 0) k_velrec<<<nblk256(np), 256, 0, st>>>(np, orig, pos, pdens, vx, vy, vz, vel);
+  return hipGetLastError();
+}
+hipError_t launch_fill_h(hipStream_t st, int64_t np, float* hpart, const float* hk) {
+  if (np > 0) k_fill_h<<<nblk256(np), 256, 0, st>>>(np, hpart, hk);
+  return hipGetLastError();
+}
+hipError_t launch_reduce_mean(hipStream_t st, const double* partial, int64_t nparts, int64_t ncell, double* slices,
+                              double* sum_out, float* mean_out) {
+  k_reduce_mean<<<kRedSlices, 256, 0, st>>>(partial, nparts, ncell, slices, 0, sum_out, mean_out);
+  k_reduce_mean<<<1, 256, 0, st>>>(slices, kRedSlices, ncell, nullptr, 1, sum_out, mean_out);
+  return hipGetLastError();
+}
+hipError_t launch_div_finalize(hipStream_t st, int n, float dx, int periodic, const float* u, const float* v,
+                               const float* w, const float* umass, const float* mean, float* u1co, float* div) {
+  dim3 g2((n + 255) / 256, n, n);
+  k_div_finalize<<<g2, 256, 0, st>>>(n, dx, periodic, u, v, w, umass, mean, u1co, div);
+  return hipGetLastError();
+}
+hipError_t launch_convert_check(hipStream_t st, int64_t np, float* m, const float* x, const float* y, const float* z,
+                                float half, int* flags) {
+  // flags: [0] out of the box, [1] max |mass| bits, [2] the first mass as given, [3] masses differ
+  hipError_t e = hipMemsetAsync(flags, 0, 16, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(flags + 2, m, 4, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return e;
+  k_convert_check<<<(unsigned)std::min<int64_t>(nblk256(np), 2048), 256, 0, st>>>(
+      np, m, x, y, z, half, flags, (unsigned*)flags + 1, (const float*)(flags + 2), flags + 3);
+  return hipGetLastError();
+}
+hipError_t launch_convert_vel(hipStream_t st, int64_t np, float* vx, float* vy, float* vz) {
+  if (np > 0) k_convert_vel<<<(unsigned)std::min<int64_t>(nblk256(np), 4096), 256, 0, st>>>(np, vx, vy, vz);
+  return hipGetLastError();
+}
+hipError_t launch_gather_sorted(hipStream_t st, int64_t np, const uint32_t* perm, const float* x, const float* y,
+                                const float* z, const float* m, float4* pos, uint32_t* orig) {
+  if (np > 0) k_gather_sorted<<<nblk256(np), 256, 0, st>>>(np, perm, x, y, z, m, pos, orig);
+  return hipGetLastError();
+}
+hipError_t launch_scatter_orig(hipStream_t st, int64_t np, const uint32_t* orig, int64_t orig0, const float* a,
+                               float* out) {
+  if (np > 0) k_scatter_orig_off<<<nblk256(np), 256, 0, st>>>(np, orig, orig0, a, out);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- host side
 static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
@@ -860,13 +1183,7 @@ int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, con
 static int build_bins(avism_ctx* c) {
   StageTimer t(c, "bin");
   const int64_t np = c->np;
-  // bins of s^3 cells (s = 1, 2 or 4, dividing n and the DDENS tile): about one
-  // particle per bin, also on the finer grids of void-in-void levels
-  const double ppc = (double)np / ((double)c->g.n * c->g.n * c->g.n);
-  int s = 1;
-  while (s < 4 && c->g.n % (2 * s) == 0 && (double)(8 * s * s * s) * ppc <= 1.5) s *= 2;
-  int nb = c->g.n / s;
-  if (nb < 1) nb = 1;
+  int nb = bins_per_axis(c->g.n, np);
   c->nb = nb;
   c->bsz = c->g.L0 / (float)nb;
   int64_t nbins = (int64_t)nb * nb * nb;
@@ -1056,9 +1373,14 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     k_fill_u64<<<nblk(ncell), 256, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 0ull);
     const unsigned ntiles = (unsigned)(((n + DTX - 1) / DTX) * ((n + DTY - 1) / DTY) * ((n + DTZ - 1) / DTZ));
     AV_CHECK_HIP(c, hipMemsetAsync(dpairs, 0, 8, c->stream));
-    k_ddens_scatter<<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
-                                                   c->mass_fix.as<unsigned long long>(), c->norm.as<float>(),
-                                                   dpairs);
+    if (c->nb > n)  // cells split into bins
+      k_ddens_scatter<4><<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
+                                                        c->mass_fix.as<unsigned long long>(), c->norm.as<float>(),
+                                                        dpairs);
+    else
+      k_ddens_scatter<1><<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
+                                                        c->mass_fix.as<unsigned long long>(), c->norm.as<float>(),
+                                                        dpairs);
     AV_CHECK_HIP(c, hipGetLastError());
   } else {  // DENS_INTERP_TSC: FP64 cell masses in mass_fix (zero bits are +0.0)
     StageTimer t(c, "dens_tsc");

@@ -23,7 +23,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-__all__ = ["Particles", "zeldovich", "mock_voids", "grid_fields", "particle_mass", "survey_mask"]
+__all__ = ["Particles", "zeldovich", "mock_voids", "grid_fields", "particle_mass", "survey_mask", "patch_batch"]
 
 RHO_CRIT = 2.775e11  # h^2 Msun / Mpc^3
 
@@ -153,12 +153,40 @@ def survey_mask(n: int, empty_frac: float = 0.2, seed: int = 0, rmin: float = 1.
     cells are empty (the shape the reference reads as SMASK, avism.f90:403-418)."""
     rng = np.random.default_rng(seed)
     m = np.ones((n, n, n), np.int8)  # [z, y, x]
-    zz, yy, xx = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
-    while (m == 0).mean() < empty_frac:
+    empty, target = 0, empty_frac * n ** 3
+    while empty < target:
         c = rng.uniform(0, n, size=3)
         r = rng.uniform(rmin, rmax)
-        m[(xx - c[0]) ** 2 + (yy - c[1]) ** 2 + (zz - c[2]) ** 2 < r * r] = 0
+        # only the ball's bounding box can change
+        lo = [max(int(np.floor(c[a] - r)), 0) for a in range(3)]
+        hi = [min(int(np.ceil(c[a] + r)) + 1, n) for a in range(3)]
+        zz, yy, xx = np.meshgrid(np.arange(lo[2], hi[2]), np.arange(lo[1], hi[1]), np.arange(lo[0], hi[0]),
+                                 indexing="ij")
+        sub = m[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+        cut = ((xx - c[0]) ** 2 + (yy - c[1]) ** 2 + (zz - c[2]) ** 2 < r * r) & (sub == 1)
+        empty += int(cut.sum())
+        sub[cut] = 0
     return m.ravel()
+
+
+def patch_batch(npatch: int, seed0: int = 3000, np_range=(2e5, 8e6), grids=(32, 64, 96, 128, 160),
+                cell: float = 1.953125, empty_frac: float = 0.2, device="cpu"):
+    """Independent survey-like patches: patch i (seed seed0 + i) is a Zel'dovich
+    field in a non-periodic box of N^3 cells of side `cell` Mpc/h, N drawn from
+    `grids`, with about N_p particles, N_p log-uniform in np_range (the nearest
+    cube of a lattice side), and a survey mask with empty_frac of its cells
+    empty.  Returns a list of dicts (n, L0, particles, mask)."""
+    out = []
+    for i in range(npatch):
+        rng = np.random.default_rng(seed0 + i)
+        n = int(rng.choice(grids))
+        npt = float(np.exp(rng.uniform(np.log(np_range[0]), np.log(np_range[1]))))
+        side = max(int(round(npt ** (1.0 / 3.0))), 4)
+        L0 = n * cell
+        p = zeldovich(side, L0, seed=seed0 + i, device=device)
+        p = Particles(p.x, p.y, p.z, p.vx, p.vy, p.vz, p.m, L0, 0)
+        out.append(dict(n=n, L0=L0, particles=p, mask=survey_mask(n, empty_frac, seed=seed0 + i)))
+    return out
 
 
 def grid_fields(n: int, L0: float, seed: int, sigma: float = 0.8, kc: float = 0.5,

@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=128, help="side of the CPU-baseline sample")
     ap.add_argument("--mode", choices=["both", "plain", "learned"], default="both")
     ap.add_argument("--levels", type=int, default=1, help="void-in-void levels per step (N, 2N, ...)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-patch batch record")
     return ap.parse_args()
 
 
@@ -166,6 +167,90 @@ def cpu_baseline(n_side: int, box_per_cell: float, k: int):
     return {"value": n_side ** 3 / dt, "unit": "cells/s", "cores": 1, "kind": "port",
             "sample": f"oracle (C, -O2, 1 thread) full level on {n_side}^3 Zel'dovich particles / "
                       f"{n_side}^3 cells, cell size {box_per_cell:.4g} Mpc/h, k={k}: {dt:.2f} s"}
+
+
+def batch_bench(dev, k, steps, warmup):
+    """64 independent survey-like patches in one batch (Zel'dovich fields,
+    particle counts log-uniform in 2e5..8e6, grids of 32^3..160^3 cells,
+    non-periodic, 20% of each patch's cells masked): one step = pinned host
+    particles and masks -> every patch's fields (one graph launch) -> every
+    patch's catalogue and MARCA (void stages on 8 host threads).  Beside it, the
+    same patches one at a time through the single-grid path."""
+    import numpy as np
+    import torch
+    from avism_amd import synth
+    from avism_amd._lib import Context
+    t = time.perf_counter()
+    # on the host: 64 lattice sides would need as many GPU FFT plans
+    pats = synth.patch_batch(64, seed0=3000, device="cpu")
+    names = ("x", "y", "z", "vx", "vy", "vz", "m")
+    host = [torch.cat([getattr(p["particles"], f) for p in pats]).pin_memory() for f in names]
+    mask = torch.from_numpy(np.concatenate([p["mask"] for p in pats])).pin_memory()
+    gen_s = time.perf_counter() - t
+    ns = [p["n"] for p in pats]
+    nps = [p["particles"].n for p in pats]
+    ctx = Context(0)
+    ctx.set_cosmology(0.678, 0.31, 0.0)
+    ctx.set_void_params(-0.6, 10.0, 0.25, 0.0, 1.0, 1.0, 30_000_000)
+    ctx.batch_set(ns, [p["L0"] for p in pats], nps)
+
+    def step():
+        ctx.batch_load(*host, smask=mask)
+        ctx.batch_fields(k=k)
+        t1 = time.perf_counter()
+        ctx.batch_voids(threads=8)
+        return time.perf_counter() - t1
+
+    t = time.perf_counter()
+    step()  # tunes and captures
+    first_s = time.perf_counter() - t
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    vs = 0.0
+    t = time.perf_counter()
+    for _ in range(steps):
+        vs += step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) / steps * 1e3
+    st = stage_table(ctx.timings())
+    ctx.set_timing(False)
+    marca_b = [ctx.batch_patch(i).marca(inner_only=True) for i in range(len(pats))]
+    # the same patches one at a time through the single-grid path
+    single = Context(0)
+    single.set_cosmology(0.678, 0.31, 0.0)
+    single.set_void_params(-0.6, 10.0, 0.25, 0.0, 1.0, 1.0, 30_000_000)
+    off = np.concatenate([[0], np.cumsum(nps)])
+    moff = np.concatenate([[0], np.cumsum([n ** 3 for n in ns])])
+
+    def one(i):
+        single.set_grid(ns[i], pats[i]["L0"], 0)
+        single.set_mask(mask[moff[i]:moff[i + 1]].numpy())
+        a, b = int(off[i]), int(off[i + 1])
+        single.load_particles(*[h[a:b] for h in host])
+        single.particles_to_grid(k=k)
+        single.find_voids(0)
+        single.merge_and_post()
+        return single.marca(inner_only=True)
+
+    one(0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    same = all(np.array_equal(one(i), marca_b[i]) for i in range(len(pats)))
+    unbatched_ms = (time.perf_counter() - t) * 1e3
+    info = ctx.batch_info()
+    cells = int(sum(n ** 3 for n in ns))
+    return {"patches": len(pats), "particles": int(sum(nps)), "cells": cells,
+            "grids": {str(n): ns.count(n) for n in sorted(set(ns))},
+            "ms_per_batch": round(ms, 2), "patches_per_s": round(len(pats) / (ms / 1e3), 1),
+            "cells_per_s": cells / (ms / 1e3),
+            "stages_ms": {k2: round(v / steps, 3) for k2, v in st.items()},
+            "void_stages_wall_ms": round(vs / steps * 1e3, 2),
+            "unbatched_ms": round(unbatched_ms, 2), "batched_over_unbatched": round(unbatched_ms / ms, 3),
+            "identical_to_unbatched": same, "graph_launches": info["graph_launches"],
+            "first_batch_s_incl_tuning": round(first_s, 2), "generation_s": round(gen_s, 1),
+            "tuning": info["tuning"]}
 
 
 def timed(ctx, fn, warmup, steps):
@@ -376,6 +461,10 @@ def main():
     res = hres if use_host else dres
     if 0 in res and GL in res:
         out["learned_over_plain"] = res[0]["ms"] / res[GL]["ms"]
+    if not args.no_batch and args.levels == 1:
+        del parts, parts_h
+        torch.cuda.empty_cache()
+        out["batch64"] = batch_bench(dev, k, steps=max(1, min(args.steps, 3)), warmup=1)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.box / n, k)
     print(json.dumps(out))

@@ -240,6 +240,40 @@ int avism_cnn_backward(avism_ctx* ctx, const float* params, int64_t n, const uin
 int avism_cnn_adam(avism_ctx* ctx, float* params, const float* grads, float* m, float* v, float lr,
                    int32_t step);
 
+/* ---- batches of independent patches ----------------------------------------
+ * No single counterpart in the reference: each patch is one run of it
+ * (avism.f90:1177-1973) on its own grid, particles and survey mask.  The
+ * field stage of every patch (PPART_DENS, VVEL_INTERP_SPH_VW, FILL_H_ZEROS,
+ * DDENS_INTERP_SPH, mean density, DIVER_UNIFORM; SPH only) runs in persistent
+ * launches over all patches, with tile shapes picked by an autotuner on the
+ * layout's first batch and the launch sequence captured in a graph that later
+ * batches of the same layout replay.  The void stage (avism_find_voids +
+ * avism_merge_and_post) of each patch runs on its own context. */
+/* layout: npatch patches of n[p]^3 cells over a box of L0[p] (periodic[p], or
+ * all non-periodic when NULL) holding np[p] particles */
+int avism_batch_set(avism_ctx* ctx, int32_t npatch, const int32_t* n, const float* L0, const int32_t* periodic,
+                    const int64_t* np);
+/* particles of every patch back to back (patch p at offset np[0]+..+np[p-1]),
+ * as avism_load_particles takes them; smask: the patches' inner SMASK grids back
+ * to back (n[p]^3 bytes each) or NULL */
+int avism_batch_load(avism_ctx* ctx, const float* x, const float* y, const float* z, const float* vx,
+                     const float* vy, const float* vz, const float* m, const int8_t* smask, int32_t where);
+/* the field stage of every patch, k <= 64 (AVISM_ERR_UNSUPPORTED above) */
+int avism_batch_fields(avism_ctx* ctx, int32_t k);
+/* the void stage of every patch (AVISM_GROW_PLAIN or AVISM_GROW_LEARNED with the
+ * network of avism_cnn_set_params), the patches spread over nthreads host
+ * threads (0: 8), with the context's cosmology and void parameters */
+int avism_batch_voids(avism_ctx* ctx, int32_t growth_mode, int32_t nthreads);
+/* patch p's context after avism_batch_voids: its catalogue, MARCA, cubes and
+ * fields through the usual getters; owned by ctx (do not destroy) */
+int avism_batch_patch(avism_ctx* ctx, int32_t p, avism_ctx** patch_ctx);
+/* patch p's inner fields and, in its input order, PART_DENS and HPART; NULL skips */
+int avism_batch_get_fields(avism_ctx* ctx, int32_t p, float* u1co, float* div, float* u, float* v, float* w,
+                           float* part_dens, float* hpart, int32_t where);
+/* the autotuner's record (JSON: candidates' milliseconds and the choices),
+ * the number of graph launches so far, MEANDENS per patch; NULL skips */
+int avism_batch_info(avism_ctx* ctx, char* tuning, int32_t len, int64_t* graph_launches, float* meandens);
+
 /* ---- whole level ---------------------------------------------------------- */
 /* particles already loaded: fields, voids, merge and post in one call */
 int avism_run_level(avism_ctx* ctx, int32_t k, int32_t vel_interp, int32_t dens_interp,
