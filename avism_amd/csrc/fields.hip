@@ -650,7 +650,10 @@ __device__ __forceinline__ void ddens_tile(const DdensGeom& G, const Bins& B, co
   }
 }
 
-constexpr int DTX = 8, DTY = 8, DTZ = 4, DR = 4;  // the single-grid tile
+// the single-grid tile: 8 x 8 x 8 cells widened by 3 (timed against 8 x 8 x 4
+// widened by 4 and 16 x 8 x 4 by 4 by the batch autotuner: fastest at one
+// particle per cell)
+constexpr int DTX = 8, DTY = 8, DTZ = 8, DR = 3;
 
 // FM: the most bins per cell along an axis the launch allows (1, or 4 where
 // dense particle sets split cells)
