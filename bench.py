@@ -72,6 +72,12 @@ def sources_hash() -> str:
     return h.hexdigest()[:16]
 
 
+def progress(msg):
+    """A line on stderr when a phase of the benchmark has ended (stdout holds
+    only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def stage_table(tim):
     out = {}
     for name, ms in tim:
@@ -325,11 +331,13 @@ def main():
             stages = dict(levels[mode][0]["stages_ms"])
         dres[mode] = dict(ms=dt * 1e3, stages=stages, ncubes=int(st.ncubes), nvoids=int(st.nvoidt),
                           ncand=int(st.ncand), counters=counters)
+        progress(f"growth mode {mode}: HBM-resident step {dt * 1e3:.1f} ms")
         if parts_h is not None:
             dt_h, tim_h, _, out = timed(
                 ctx, lambda pl: run_step(ctx, parts_h, args.k, mode, lv, grid, None, host_out),
                 args.warmup, args.steps)
             hres[mode] = dict(ms=dt_h * 1e3, stages={s: v / args.steps for s, v in stage_table(tim_h).items()})
+            progress(f"growth mode {mode}: host-RAM step {dt_h * 1e3:.1f} ms")
             cat = out[1]
             hb = host_out.get(ctx.grid().n)
             catalogues[mode] = (hb["marca"].numpy().copy(), cat["uvoid"].copy(), cat["indice"].copy(),
@@ -465,7 +473,9 @@ def main():
         del parts, parts_h
         torch.cuda.empty_cache()
         out["batch64"] = batch_bench(dev, k, steps=max(1, min(args.steps, 3)), warmup=1)
+        progress(f"batch of 64 patches: {out['batch64']['ms_per_batch']} ms")
     if not args.no_cpu_baseline:
+        progress(f"CPU baseline on {args.cpu_n}^3")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.box / n, k)
     print(json.dumps(out))
 
