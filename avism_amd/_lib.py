@@ -56,7 +56,7 @@ EXPORTS = [
     "avism_load_randoms",
     "avism_survey_density",
     "avism_load_grid_fields", "avism_load_void_fields", "avism_get_fields", "avism_get_particle_fields", "avism_find_voids",
-    "avism_set_proposals", "avism_get_candidates", "avism_get_cubes", "avism_indexx",
+    "avism_set_proposals", "avism_cnn_propose", "avism_get_candidates", "avism_get_cubes", "avism_indexx",
     "avism_merge_and_post", "avism_get_catalogue", "avism_get_marca", "avism_run_level",
     "avism_cnn_set_params", "avism_cnn_patches", "avism_cnn_forward", "avism_cnn_backward", "avism_cnn_adam",
     "avism_sync", "avism_get_counters", "avism_indexx_noalloc",
@@ -109,6 +109,7 @@ def lib():
         "avism_get_particle_fields": ([vp, vp, vp], C.c_int),
         "avism_find_voids": ([vp, i32, C.POINTER(i32)], C.c_int),
         "avism_set_proposals": ([vp, vp, i64, i32], C.c_int),
+        "avism_cnn_propose": ([vp, vp, i64, i32], C.c_int),
         "avism_get_candidates": ([vp, vp, vp, i64, C.POINTER(i64)], C.c_int),
         "avism_get_cubes": ([vp, vp, vp, vp], C.c_int),
         "avism_indexx": ([vp, vp, i64, vp, i32], C.c_int),
@@ -316,11 +317,13 @@ class Context:
         self._chk(lib().avism_sync(self._h))
 
     def counters(self):
-        """Work counters: ddens (particle, cell) pairs, PPART queries, VVEL queries, growth shells."""
+        """Work counters: DDENS (particle, cell) pairs, PPART queries, VVEL
+        queries; of the last find_voids, the face cells the growth examined and
+        the candidates grown by one wave each."""
         out = np.zeros(8, np.int64)
         self._chk(lib().avism_get_counters(self._h, _ptr(out), 8))
         return dict(ddens_pairs=int(out[0]), ppart_queries=int(out[1]), vvel_queries=int(out[2]),
-                    growth_shells=int(out[3]))
+                    growth_cells=int(out[3]), growth_wave_candidates=int(out[4]))
 
     def particle_fields(self, np_total: int):
         pd = np.zeros(np_total, np.float32)
@@ -484,6 +487,15 @@ class Context:
             import torch
             p = self._dev(params, torch.float32, (CNN_NPARAM,), "params")
             self._chk(lib().avism_cnn_set_params(self._h, p, AVISM_DEVICE))
+
+    def cnn_propose(self) -> np.ndarray:
+        """The installed network's proposal classes for the current candidates
+        (processing order), as GROW_LEARNED computes them."""
+        nc = C.c_int64()
+        self._chk(lib().avism_get_candidates(self._h, None, None, 0, C.byref(nc)))
+        out = np.zeros(nc.value, np.int8)
+        self._chk(lib().avism_cnn_propose(self._h, _ptr(out), int(nc.value), AVISM_HOST))
+        return out
 
     def cnn_patches(self, ranks):
         """bf16 feature patches [n, 3, 9, 9, 9] (as int16) of candidates at processing ranks."""

@@ -132,10 +132,13 @@ struct avism_ctx {
   avism::DBuf sort_idx, sort_a, sort_b, seg_a, seg_b, seg_small, sort_misc, sort_gpos, sort_spos;
   int64_t ncand = 0;
   avism::DBuf cand_r, cand_state, cand_flag2;  // growth results per sorted candidate
+  avism::DBuf cand_work;                       // growth work per sorted candidate (counters)
+  bool growth_pending = false;                 // cand_work not yet summed into counters
   avism::DBuf cover_acc, cover_und;            // ext grids for acceptance
   avism::DBuf acc_rank, misc_i32;
   avism::DBuf proposals;                       // half-width proposals (int8 per candidate)
   avism::DBuf cnn_params;                      // proposal network parameters (fp32)
+  avism::DBuf cnn_work;                        // DIVERCO^2 partial sums and their total (FP64)
   bool cnn_ready = false;
   int32_t nvoid = 0;                           // accepted cubes
   avism::DBuf cube_i, cube_f, cube_father;     // [9][nvoid], [7][nvoid], [nvoid]

@@ -8,6 +8,7 @@ int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, con
                    const float* vx, const float* vy, const float* vz, const float* m, int where);
 int get_particle_fields(avism_ctx* c, float* part_dens, float* hpart);
 int set_proposals(avism_ctx* c, const int8_t* r, int64_t n, int where);
+int growth_counters(avism_ctx* c);
 int get_candidates(avism_ctx* c, int64_t* lin, int32_t* r, int64_t cap, int64_t* n);
 int get_cubes(avism_ctx* c, int32_t* ci, float* cf, int32_t* father);
 int get_catalogue(avism_ctx* c, int32_t* indice, int32_t* uvoid, int32_t* ncellv, float* vf,
@@ -372,6 +373,10 @@ int avism_sync(avism_ctx* c) {
 int avism_get_counters(avism_ctx* c, int64_t* out, int32_t n) {
   CTX_GUARD(c);
   if (!out || n < 0) AV_FAIL(c, AVISM_ERR_ARG, "bad counter request");
+  if (n > 3) {
+    int rc = growth_counters(c);
+    if (rc) return rc;
+  }
   const int32_t m = n < 8 ? n : 8;
   for (int32_t i = 0; i < m; ++i) out[i] = c->counters[i];
   for (int32_t i = m; i < n; ++i) out[i] = 0;
@@ -465,6 +470,21 @@ int avism_cnn_set_params(avism_ctx* c, const float* params, int32_t where) {
   AV_CHECK_HIP(c, hipMemcpy(c->cnn_params.p, params, b,
                             where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   c->cnn_ready = true;
+  return AVISM_OK;
+}
+
+int avism_cnn_propose(avism_ctx* c, int8_t* out, int64_t n, int32_t where) {
+  CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
+  if (n != c->ncand) AV_FAIL(c, AVISM_ERR_ARG, "proposal count differs from the candidate count");
+  if (n > 0 && !out) AV_FAIL(c, AVISM_ERR_ARG, "null proposal array");
+  if (c->ncand <= 0) return AVISM_OK;
+  int rc = cnn_propose(c);
+  if (rc) return rc;
+  AV_CHECK_HIP(c, hipMemcpyAsync(out, c->proposals.p, (size_t)n,
+                                 where == AVISM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                 c->stream));
+  AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   return AVISM_OK;
 }
 

@@ -1268,6 +1268,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   const int64_t ncell = (int64_t)n * n * n;
   if (int rc = finish_copies(c)) return rc;
   for (int64_t& v : c->counters) v = 0;
+  c->growth_pending = false;
   c->nvoid = 0;  // cubes, MARCA and the catalogue of earlier fields are stale
   c->post_ready = false;
   int rc = build_bins(c);

@@ -312,9 +312,12 @@ __device__ __forceinline__ bool blocked(const ExtGeom& G, const uint8_t* __restr
 }
 
 // is any face cell of the half-width-s cube around (ci,cj,ck) blocked?
+// nchk gains the shell's face cells (counting each one as it is examined
+// made the lane kernel a third slower)
 __device__ bool shell_blocked(const ExtGeom& G, const uint8_t* __restrict__ flags,
                               const int32_t* __restrict__ marcap, int indv, int ci, int cj, int ck,
-                              int s) {
+                              int s, int& nchk) {
+  nchk += 6 * (2 * s + 1) * (2 * s + 1);
   for (int u = -s; u <= s; ++u)
     for (int v = -s; v <= s; ++v) {
       if (blocked(G, flags, marcap, indv, 0, ci + s, cj + u, ck + v)) return true;
@@ -326,6 +329,11 @@ __device__ bool shell_blocked(const ExtGeom& G, const uint8_t* __restrict__ flag
     }
   return false;
 }
+
+// growth work per candidate (cw[t]): the face cells of the shells examined
+// (a lane counts a whole shell; a wave the cells its lanes examined), bit 31
+// set when a wave grew it; summed only when the counters are read
+constexpr uint32_t kGrownByWave = 0x80000000u;
 
 __device__ __forceinline__ void decode(const ExtGeom& G, int e, int& I, int& J, int& K) {
   I = e % G.ne + G.low1;
@@ -372,12 +380,12 @@ struct ParentRadii {
   float rmin_sub;
 };
 
-__global__ void k_grow_lane(int64_t ncand, ExtGeom G, const int32_t* __restrict__ order,
-                            const int32_t* __restrict__ lin, const uint8_t* __restrict__ flags,
-                            const int32_t* __restrict__ marcap, ParentRadii PR,
-                            const int8_t* __restrict__ prop, int smax, int8_t* status, int16_t* rr) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncand) return;
+// one candidate per lane; returns the face cells examined
+__device__ int grow_lane(int64_t t, const ExtGeom& G, const int32_t* __restrict__ order,
+                         const int32_t* __restrict__ lin, const uint8_t* __restrict__ flags,
+                         const int32_t* __restrict__ marcap, const ParentRadii& PR,
+                         const int8_t* __restrict__ prop, int smax, int wave_from, int8_t* status, int16_t* rr) {
+  int nchk = 0;
   int e = lin[order[t]];
   int I, J, K;
   decode(G, e, I, J, K);
@@ -386,7 +394,7 @@ __global__ void k_grow_lane(int64_t ncand, ExtGeom G, const int32_t* __restrict_
       J >= G.low2 - kNsec || K >= G.low2 - kNsec) {
     status[t] = ST_SKIP;
     rr[t] = -1;
-    return;
+    return 0;
   }
   int indv = marcap ? marcap[G.inner(I, J, K)] : -1;
   // FLAG1_GRID (voidfinding.f90:267-274, 491): a parent void smaller than
@@ -394,32 +402,34 @@ __global__ void k_grow_lane(int64_t ncand, ExtGeom G, const int32_t* __restrict_
   if (indv > 0 && PR.reqp && indv <= PR.n && PR.reqp[indv - 1] < PR.rmin_sub) {
     status[t] = ST_INVALID;
     rr[t] = 0;
-    return;
+    return 0;
   }
   int szone;
   bool zf2;
   zone_info(G, I, J, K, szone, zf2);
   int s0 = 0;
   if (prop) {
-    // learned proposal r_hat: check shells 0..r_hat in one sweep; the first
-    // blocked shell found is exact because every lower shell was checked
+    // learned proposal r_hat.  A large one goes to the wave kernel, whose
+    // lanes check the faces of shells 0, 1, ... together; a small one is
+    // checked here, shells 0..r_hat in one sweep, and the first blocked shell
+    // found is exact because every lower shell was checked
     int rh = prop[t];
     if (rh < 0) rh = 0;
-    if (rh >= smax) {  // predicted large: the wave kernel grows it from shell 0
+    if (rh >= wave_from) {
       status[t] = ST_DEFER;
       rr[t] = 0;
-      return;
+      return 0;
     }
     if (rh > szone - 1) rh = szone - 1;
     int first = -1;
     for (int s = 0; s <= rh && first < 0; ++s)
-      if (shell_blocked(G, flags, marcap, indv, I, J, K, s)) first = s;
+      if (shell_blocked(G, flags, marcap, indv, I, J, K, s, nchk)) first = s;
     if (first >= 0) {
       int8_t st; int16_t r;
       finish_growth(first, szone, zf2, st, r);
       status[t] = st;
       rr[t] = r;
-      return;
+      return nchk;
     }
     s0 = rh + 1;  // under-prediction: keep growing
   }
@@ -427,26 +437,43 @@ __global__ void k_grow_lane(int64_t ncand, ExtGeom G, const int32_t* __restrict_
     if (s >= smax) {  // continue in the wave kernel
       status[t] = ST_DEFER;
       rr[t] = (int16_t)s;
-      return;
+      return nchk;
     }
-    if (shell_blocked(G, flags, marcap, indv, I, J, K, s)) {
+    if (shell_blocked(G, flags, marcap, indv, I, J, K, s, nchk)) {
       int8_t st; int16_t r;
       finish_growth(s, szone, zf2, st, r);
       status[t] = st;
       rr[t] = r;
-      return;
+      return nchk;
     }
   }
   int8_t st; int16_t r;
   finish_growth(szone, szone, zf2, st, r);
   status[t] = st;
   rr[t] = r;
+  return nchk;
 }
 
-// large cubes: one wave per candidate, face cells spread over the lanes
+__global__ void k_grow_lane(int64_t ncand, ExtGeom G, const int32_t* __restrict__ order,
+                            const int32_t* __restrict__ lin, const uint8_t* __restrict__ flags,
+                            const int32_t* __restrict__ marcap, ParentRadii PR,
+                            const int8_t* __restrict__ prop, int smax, int wave_from, int8_t* status, int16_t* rr,
+                            uint32_t* cw) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ncand) cw[t] = (uint32_t)grow_lane(t, G, order, lin, flags, marcap, PR, prop, smax, wave_from, status, rr);
+}
+
+// large cubes: one wave per candidate.  The face cells of shells rr[t],
+// rr[t] + 1, ... are taken in that order, 64 at a time across the lanes (a
+// chunk may end one shell and start the next); the first chunk with a blocked
+// cell gives the first blocked shell, that of its lowest blocked lane, since
+// every cell of every lower shell has been examined.  cw[t] gains the face
+// cells examined here and the grown-by-a-wave bit.
+__device__ __forceinline__ int shell_cells(int s) { return 6 * (2 * s + 1) * (2 * s + 1); }
+
 __global__ void k_grow_wave(int64_t ncand, ExtGeom G, const int32_t* __restrict__ order,
                             const int32_t* __restrict__ lin, const uint8_t* __restrict__ flags,
-                            const int32_t* __restrict__ marcap, int8_t* status, int16_t* rr) {
+                            const int32_t* __restrict__ marcap, int8_t* status, int16_t* rr, uint32_t* cw) {
   int64_t t = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   int lane = threadIdx.x % 64;
   if (t >= ncand) return;
@@ -459,32 +486,46 @@ __global__ void k_grow_wave(int64_t ncand, ExtGeom G, const int32_t* __restrict_
   bool zf2;
   zone_info(G, I, J, K, szone, zf2);
   int sblock = szone;
-  for (int s = rr[t]; s < szone; ++s) {
-    int side = 2 * s + 1;
-    int nface = side * side;
-    bool any = false;
-    for (int base = 0; base < 6 * nface; base += 64) {
-      int idx = base + lane;
-      bool b = false;
-      if (idx < 6 * nface) {
-        int dir = idx / nface, f = idx % nface;
-        int u = f % side - s, v = f / side - s;
-        int a = I, bb = J, cc = K;
-        switch (dir) {
-          case 0: a = I + s; bb = J + u; cc = K + v; break;
-          case 1: a = I - s; bb = J + u; cc = K + v; break;
-          case 2: a = I + u; bb = J + s; cc = K + v; break;
-          case 3: a = I + u; bb = J - s; cc = K + v; break;
-          case 4: a = I + u; bb = J + v; cc = K + s; break;
-          default: a = I + u; bb = J + v; cc = K - s; break;
-        }
-        b = blocked(G, flags, marcap, indv, dir, a, bb, cc);
-      }
-      if (__any(b)) { any = true; break; }
+  int nchk = 0;
+  // chunk start: shell sb, cell ob within it
+  int sb = rr[t], ob = 0;
+  while (sb < szone) {
+    int s = sb, o = ob + lane;
+    while (s < szone && o >= shell_cells(s)) {
+      o -= shell_cells(s);
+      ++s;
     }
-    if (any) { sblock = s; break; }
+    bool b = false;
+    if (s < szone) {
+      ++nchk;
+      const int side = 2 * s + 1, nface = side * side;
+      const int dir = o / nface, f = o % nface;
+      const int u = f % side - s, v = f / side - s;
+      int a = I, bb = J, cc = K;
+      switch (dir) {
+        case 0: a = I + s; bb = J + u; cc = K + v; break;
+        case 1: a = I - s; bb = J + u; cc = K + v; break;
+        case 2: a = I + u; bb = J + s; cc = K + v; break;
+        case 3: a = I + u; bb = J - s; cc = K + v; break;
+        case 4: a = I + u; bb = J + v; cc = K + s; break;
+        default: a = I + u; bb = J + v; cc = K - s; break;
+      }
+      b = blocked(G, flags, marcap, indv, dir, a, bb, cc);
+    }
+    const unsigned long long m = __ballot(b);
+    if (m) {
+      sblock = __shfl(s, __ffsll((long long)m) - 1);
+      break;
+    }
+    ob += 64;
+    while (sb < szone && ob >= shell_cells(sb)) {
+      ob -= shell_cells(sb);
+      ++sb;
+    }
   }
+  for (int o = 32; o > 0; o >>= 1) nchk += __shfl_xor(nchk, o);
   if (lane == 0) {
+    cw[t] = (cw[t] + (uint32_t)nchk) | kGrownByWave;
     int8_t st; int16_t r;
     finish_growth(sblock, szone, zf2, st, r);
     status[t] = st;
@@ -688,6 +729,10 @@ int void_find(avism_ctx* c, int32_t growth_mode) {
   AV_CHECK_HIP(c, hipGetLastError());
   int8_t* status = c->cand_state.as<int8_t>();
   int16_t* rr = c->cand_r.as<int16_t>();
+  // growth work per candidate (summed by avism_get_counters)
+  if (!c->cand_work.ensure((size_t)(nc + 1) * 4)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (growth)");
+  uint32_t* cw = c->cand_work.as<uint32_t>();
+  c->growth_pending = true;
   const int8_t* prop = nullptr;
   if (growth_mode == AVISM_GROW_LEARNED) {
     rc = cnn_propose(c);
@@ -702,10 +747,14 @@ int void_find(avism_ctx* c, int32_t growth_mode) {
   {
     StageTimer t(c, prop ? "grow_proposal" : "grow");
     ParentRadii PR{c->has_reqp && marcap ? c->reqp.as<float>() : nullptr, c->nreqp, c->vp.rmin_sub};
+    // plain: shells 0..3 in a lane, larger cubes continue in the wave kernel;
+    // with proposals, a candidate proposed at 4 or more goes to the wave
+    // kernel from shell 0 (routing from 2 up sent 59% of the candidates of a
+    // 512^3 Zel'dovich box to waves and took 1.9 ms against 0.5)
     k_grow_lane<<<nbk(nc), 256, 0, c->stream>>>(nc, G, order, c->cand_lin.as<int32_t>(),
-                                                c->flags.as<uint8_t>(), marcap, PR, prop, 4, status, rr);
+                                                c->flags.as<uint8_t>(), marcap, PR, prop, 4, 4, status, rr, cw);
     k_grow_wave<<<nbk(nc, 4), 256, 0, c->stream>>>(nc, G, order, c->cand_lin.as<int32_t>(),
-                                                   c->flags.as<uint8_t>(), marcap, status, rr);
+                                                   c->flags.as<uint8_t>(), marcap, status, rr, cw);
     AV_CHECK_HIP(c, hipGetLastError());
   }
   // acceptance rounds
@@ -763,6 +812,44 @@ int void_find(avism_ctx* c, int32_t growth_mode) {
                                                 c->cube_f.as<float>(), c->cube_father.as<int32_t>());
   AV_CHECK_HIP(c, hipGetLastError());
   AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  return AVISM_OK;
+}
+
+namespace {
+__global__ void k_sum_growth(int64_t n, const uint32_t* __restrict__ cw, unsigned long long* out) {
+  unsigned long long cells = 0, waves = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = cw[i];
+    cells += v & ~kGrownByWave;
+    waves += v >> 31;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    cells += __shfl_xor(cells, o);
+    waves += __shfl_xor(waves, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(out, cells);
+    atomicAdd(out + 1, waves);
+  }
+}
+}  // namespace
+
+// counters[3], counters[4] from the per-candidate growth work of the last
+// avism_find_voids (once per growth)
+int growth_counters(avism_ctx* c) {
+  if (!c->growth_pending) return AVISM_OK;
+  c->growth_pending = false;
+  c->counters[3] = c->counters[4] = 0;
+  if (c->ncand <= 0) return AVISM_OK;
+  unsigned long long* d = (unsigned long long*)((char*)c->scalars.ensure(4096) + 3584);
+  AV_CHECK_HIP(c, hipMemsetAsync(d, 0, 16, c->stream));
+  k_sum_growth<<<64, 256, 0, c->stream>>>(c->ncand, c->cand_work.as<uint32_t>(), d);
+  AV_CHECK_HIP(c, hipGetLastError());
+  unsigned long long h[2] = {0, 0};
+  AV_CHECK_HIP(c, hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, c->stream));
+  AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  c->counters[3] = (int64_t)h[0];
+  c->counters[4] = (int64_t)h[1];
   return AVISM_OK;
 }
 

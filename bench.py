@@ -460,6 +460,13 @@ def main():
                                 for m in hres} or None,
         "ncand": r["ncand"], "ncubes": r["ncubes"], "nvoids": r["nvoids"],
         "counters": cnt,
+        # what the proposals change in the growth: its time, the face cells it
+        # examines and the candidates grown by one wave each (HBM-resident step)
+        "growth": {("learned" if m else "plain"): {
+            "grow_ms": round(v["stages"].get("grow_proposal" if m else "grow", 0.0), 3),
+            "propose_ms": round(v["stages"].get("cnn_propose", 0.0), 3),
+            "cells_examined": v["counters"]["growth_cells"],
+            "wave_candidates": v["counters"]["growth_wave_candidates"]} for m, v in dres.items()},
         "knn": knn,
         "roofline": roofline,
         "field_stage_ms_mixed_masses": mixed,

@@ -181,7 +181,9 @@ int avism_sync(avism_ctx* ctx);
  * counterpart in the reference): out[0] (particle, cell) pairs within h in
  * DDENS_INTERP_SPH (its cells with d <= h, particles.f90:520-547), out[1] kNN
  * queries of PPART_DENS, out[2] kNN queries of VVEL_INTERP_SPH_VW or
- * H_DISTANCE (unmasked cells), out[3] growth shells checked; the rest 0 */
+ * H_DISTANCE (unmasked cells); of the last avism_find_voids: out[3] face
+ * cells of the shells the cube growth examined, out[4] candidates grown by
+ * one wave each; the rest 0 */
 int avism_get_counters(avism_ctx* ctx, int64_t* out, int32_t n);
 /* per-particle PART_DENS and HPART in the input order; NULL skips */
 int avism_get_particle_fields(avism_ctx* ctx, float* part_dens, float* hpart);
@@ -227,6 +229,10 @@ int avism_get_marca(avism_ctx* ctx, int32_t* marca, int32_t inner_only, int32_t 
  * uint16), logits[n][8] fp32, cache[n][AVISM_CNN_CACHE] bf16, labels[n]. */
 /* network used by AVISM_GROW_LEARNED (copied into the context) */
 int avism_cnn_set_params(avism_ctx* ctx, const float* params, int32_t where);
+/* the network's proposals (int8 classes, processing order) for the current
+ * candidates, as AVISM_GROW_LEARNED computes them; after avism_find_voids and
+ * before avism_merge_and_post; n = the candidate count */
+int avism_cnn_propose(avism_ctx* ctx, int8_t* out, int64_t n, int32_t where);
 /* feature patches of the current candidates at processing ranks ranks[n] */
 int avism_cnn_patches(avism_ctx* ctx, const int64_t* ranks, int64_t n, uint16_t* patches);
 /* forward: logits, and the activations the backward needs (cache may be NULL) */

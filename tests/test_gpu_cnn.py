@@ -6,8 +6,8 @@ Tolerances.  Both sides round inputs, weights and activations to bf16; the GPU
 accumulates in fp32 (MFMA) where the reference uses FP64, so an activation can
 land on the other side of a bf16 rounding boundary.  Logits: max |diff| <=
 1e-2 * max |logit|.  Gradients (the backward feeds dZ to the MFMA as a bf16
-hi/lo pair): relative L2 error per parameter block <= 1e-2 and cosine >=
-0.9999.
+hi/lo pair, given the GPU's own activations): relative L2 error per
+parameter block <= 1e-5 (measured: up to 3.6e-6) and cosine >= 1 - 1e-9.
 """
 from __future__ import annotations
 
@@ -19,7 +19,7 @@ import cnn_ref as R
 pytestmark = pytest.mark.gpu
 
 LOGIT_TOL = 1e-2
-GRAD_REL_TOL = 1e-2
+GRAD_REL_TOL = 1e-5
 
 
 def _bf16_bits_to_f32(t):
@@ -109,6 +109,34 @@ def test_forward_matches_reference(batch, seed):
     assert np.array_equal(got.argmax(1)[clear], ref.argmax(1)[clear])
 
 
+@pytest.mark.parametrize("periodic", [1, 0])
+def test_proposal_kernel_matches_forward(periodic):
+    """The proposal pass (its own feature gather and forward, on the 16x16x32
+    MFMA) against the training forward on the patches of the same candidates:
+    the classes agree wherever the forward's top two logits are apart by more
+    than the two kernels' accumulation-order difference can move them."""
+    import torch
+    c = _fields_ctx(n=40, L0=80.0, seed=303, periodic=periodic)
+    c.find_voids(0)
+    _, r = c.candidates()
+    flat = R.flatten(_params(11))
+    c.cnn_set_params(flat)
+    prop = c.cnn_propose()
+    assert prop.size == r.size > 64
+    ranks = torch.arange(r.size, dtype=torch.int64, device="cuda")
+    patches = c.cnn_patches(ranks)
+    logits, _ = c.cnn_forward(torch.from_numpy(flat).cuda(), patches, want_cache=False)
+    lg = logits.cpu().numpy().astype(np.float64)
+    top2 = np.sort(lg, 1)[:, -2:]
+    gap = top2[:, 1] - top2[:, 0]
+    clear = gap > 1e-3 * max(1.0, float(np.max(np.abs(lg))))
+    same = prop == lg.argmax(1)
+    print(f"[propose periodic={periodic}] {r.size} candidates: {int(np.sum(~same))} differ from the forward's "
+          f"argmax, {int(np.sum(~same & clear))} of them with a clear gap", flush=True)
+    assert np.all(same[clear])
+    assert np.mean(clear) > 0.9
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_backward_matches_reference(batch, seed):
     import torch
@@ -134,7 +162,7 @@ def test_backward_matches_reference(batch, seed):
         rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
         cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
         print(f"[backward seed {seed}] {k}: rel {rel:.3e} cos {cos:.6f}", flush=True)
-        assert rel <= GRAD_REL_TOL and cos >= 0.9999, k
+        assert rel <= GRAD_REL_TOL and cos >= 1 - 1e-9, k
 
 
 def test_adam_step_matches_numpy():
@@ -173,8 +201,14 @@ def test_learned_growth_gives_identical_catalogue(which):
         if prop is None:
             pytest.skip("no trained weights in the package")
         params = prop.params
+    cnt_plain = c.counters()
     c.cnn_set_params(params)
     out = gpu_void_outputs(c, 2)
+    cnt = c.counters()
+    print(f"[growth {which}] face cells examined plain {cnt_plain['growth_cells']}, learned "
+          f"{cnt['growth_cells']}; candidates grown by a wave plain {cnt_plain['growth_wave_candidates']}, "
+          f"learned {cnt['growth_wave_candidates']}", flush=True)
+    assert cnt_plain["growth_cells"] > 0 and cnt["growth_cells"] > 0
     for k in ("nvoid", "nvoid2", "nvoidt", "volt_clean"):
         assert out[k] == plain[k], k
     for k in ("cubes_i", "cubes_f", "indice", "uvoid", "ncellv", "marca", "vf"):
