@@ -41,6 +41,13 @@ class VoidParams(C.Structure):
                 ("nvoid_max", C.c_int32)]
 
 
+class RefineInfo(C.Structure):
+    _fields_ = [("depth", C.c_int32), ("n_level", C.c_int32), ("natlas", C.c_int32), ("atlas", C.c_int32),
+                ("side", C.c_int32), ("nwindows", C.c_int32), ("window_cells", C.c_int64), ("dx", C.c_float),
+                ("rx1_level", C.c_float), ("dx_atlas", C.c_float), ("rx1_atlas", C.c_float),
+                ("meandens", C.c_float)]
+
+
 class LevelStats(C.Structure):
     _fields_ = [("ncand", C.c_int64), ("ncubes", C.c_int32), ("nvoid2", C.c_int32),
                 ("nvoidt", C.c_int32), ("nperiodic", C.c_int32), ("volt_clean", C.c_double),
@@ -62,6 +69,7 @@ EXPORTS = [
     "avism_sync", "avism_get_counters", "avism_indexx_noalloc",
     "avism_batch_set", "avism_batch_load", "avism_batch_fields", "avism_batch_voids", "avism_batch_patch",
     "avism_batch_get_fields", "avism_batch_info",
+    "avism_refine_plan", "avism_refine_load", "avism_refine_windows", "avism_refine_pop",
 ]
 
 _lib = None
@@ -132,6 +140,10 @@ def lib():
         "avism_batch_patch": ([vp, i32, C.POINTER(vp)], C.c_int),
         "avism_batch_get_fields": ([vp, i32, vp, vp, vp, vp, vp, vp, vp, i32], C.c_int),
         "avism_batch_info": ([vp, C.c_char_p, i32, C.POINTER(i64), vp], C.c_int),
+        "avism_refine_plan": ([vp, i32, i64, C.POINTER(i32)], C.c_int),
+        "avism_refine_load": ([vp, i32, i32], C.c_int),
+        "avism_refine_windows": ([vp, vp, i32, C.POINTER(i32), C.POINTER(RefineInfo)], C.c_int),
+        "avism_refine_pop": ([vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -234,6 +246,30 @@ class Context:
         """Void-in-void hand-off: MARCAP, REQP and the grid of 2N (avism.f90:2123-2161)."""
         self._chk(lib().avism_next_level(self._h))
         return self.grid()
+
+    # ---- refined levels in windows around the parent voids
+    def refine_plan(self, margin: int = 8, max_cells: int = 0) -> int:
+        """Plan the next level's windows from this level's voids; the atlas count."""
+        na = C.c_int32()
+        self._chk(lib().avism_refine_plan(self._h, int(margin), int(max_cells), C.byref(na)))
+        return na.value
+
+    def refine_load(self, atlas: int, k: int = 32):
+        """Atlas `atlas` of the top plan becomes this context's grid, with its fields."""
+        self._chk(lib().avism_refine_load(self._h, int(atlas), int(k)))
+
+    def refine_windows(self):
+        """(windows, info): windows[i] = parent, side, origin x, y, z in the level's
+        cells, origin x, y, z in the atlas; info: RefineInfo as a dict."""
+        n = C.c_int32()
+        info = RefineInfo()
+        self._chk(lib().avism_refine_windows(self._h, None, 0, C.byref(n), C.byref(info)))
+        tab = np.zeros((max(n.value, 1), 8), np.int32)
+        self._chk(lib().avism_refine_windows(self._h, _ptr(tab), int(n.value), C.byref(n), None))
+        return tab[: n.value], {f: getattr(info, f) for f, _ in RefineInfo._fields_}
+
+    def refine_pop(self):
+        self._chk(lib().avism_refine_pop(self._h))
 
     def set_timing(self, on: bool):
         self._chk(lib().avism_set_timing(self._h, int(bool(on))))

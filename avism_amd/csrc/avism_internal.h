@@ -40,7 +40,8 @@ struct GridGeom {
   float radx(int i) const { return rad[i - rlo]; }
 };
 
-struct BatchState;  // a batch of patches (batch.hip)
+struct BatchState;   // a batch of patches (batch.hip)
+struct RefineState;  // windows of refined void-in-void levels (refine.hip)
 
 GridGeom make_grid(int32_t n, float L0, int32_t periodic);
 
@@ -48,6 +49,23 @@ GridGeom make_grid(int32_t n, float L0, int32_t periodic);
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;  // one owner per allocation
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DBuf();
   void* ensure(size_t nbytes);
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
@@ -87,6 +105,11 @@ struct avism_ctx {
 
   // particles (original order) and the binned copies
   int64_t np = 0;
+  // the particles' box (the grid's box, except while an atlas of windows is
+  // loaded, whose grid is not the box): bins and kNN wrap over it
+  float box_L0 = 0;
+  int32_t box_periodic = 0;
+  bool ppart_valid = false;                    // pdens, hk and vel4 hold the loaded particles' PPART
   avism::DBuf px, py, pz, pvx, pvy, pvz, pm;   // converted units after upload
   avism::DBuf sbin_key, sbin_val, sbin_key2, sbin_val2, sort_tmp;
   avism::DBuf spos4, sorig;                    // sorted by bin
@@ -151,6 +174,7 @@ struct avism_ctx {
   double volt_clean = 0;
   int64_t growth_stats[8] = {0};
   avism::BatchState* batch = nullptr;          // avism_batch_* state
+  avism::RefineState* refine = nullptr;        // avism_refine_* state
 };
 
 #define AV_CHECK_HIP(ctx, expr)                                                        \
@@ -195,6 +219,13 @@ int batch_get_fields(avism_ctx* c, int32_t p, float* u1co, float* div, float* u,
                      float* part_dens, float* hpart, int where);
 int batch_info(avism_ctx* c, char* tuning, int32_t len, int64_t* graph_launches, float* meandens);
 int indexx_noalloc(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx);
+// refined levels restricted to windows around the parent voids (refine.hip)
+void refine_free(avism_ctx* c);
+bool refine_grid_is_atlas(const avism_ctx* c);  // the grid is an atlas of windows
+int refine_plan(avism_ctx* c, int32_t margin, int64_t max_cells, int32_t* natlas);
+int refine_load(avism_ctx* c, int32_t atlas, int32_t k);
+int refine_windows(avism_ctx* c, int32_t* tab, int32_t cap, int32_t* n, avism_refine_info* info);
+int refine_pop(avism_ctx* c);
 void add_timing(avism_ctx* c, const char* name, hipEvent_t a, hipEvent_t b);
 int cnn_patches(avism_ctx* c, const int64_t* ranks, int64_t n, uint16_t* out);
 int cnn_forward(avism_ctx* c, const float* params, int64_t n, const uint16_t* patches, float* logits,

@@ -82,6 +82,7 @@ int avism_destroy(avism_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   (void)hipStreamSynchronize(c->cstream);
   batch_free(c);
+  refine_free(c);
   drop_timings(c);
   (void)hipEventDestroy(c->ev_vel);
   (void)hipEventDestroy(c->ev_mark);
@@ -140,6 +141,7 @@ int avism_set_grid(avism_ctx* c, int32_t n, float L0, int32_t periodic) {
   // changes: a rejected grid leaves the context unusable (AVISM_ERR_STATE on
   // every later stage call) rather than holding extents its buffers lack
   GridGeom g = make_grid(n, L0, periodic);
+  refine_free(c);
   c->grid_set = false;
   c->fields_ready = false;
   c->post_ready = false;
@@ -264,7 +266,10 @@ int avism_set_mask(avism_ctx* c, const int8_t* smask, int32_t where) {
 
 int avism_next_level(avism_ctx* c) {
   CTX_GUARD(c);
+  if (refine_grid_is_atlas(c))
+    AV_FAIL(c, AVISM_ERR_STATE, "the grid is an atlas of windows: use avism_refine_plan for its next level");
   if (int rc = finish_copies(c)) return rc;
+  refine_free(c);
   return next_level(c);
 }
 
@@ -562,6 +567,27 @@ int avism_batch_get_fields(avism_ctx* c, int32_t p, float* u1co, float* div, flo
 int avism_batch_info(avism_ctx* c, char* tuning, int32_t len, int64_t* graph_launches, float* meandens) {
   CTX_GUARD(c);
   return batch_info(c, tuning, len, graph_launches, meandens);
+}
+
+int avism_refine_plan(avism_ctx* c, int32_t margin, int64_t max_cells, int32_t* natlas) {
+  CTX_GUARD(c);
+  if (int rc = finish_copies(c)) return rc;
+  return refine_plan(c, margin, max_cells, natlas);
+}
+
+int avism_refine_load(avism_ctx* c, int32_t atlas, int32_t k) {
+  CTX_GUARD(c);
+  return refine_load(c, atlas, k);
+}
+
+int avism_refine_windows(avism_ctx* c, int32_t* tab, int32_t cap, int32_t* n, avism_refine_info* info) {
+  CTX_GUARD(c);
+  return refine_windows(c, tab, cap, n, info);
+}
+
+int avism_refine_pop(avism_ctx* c) {
+  CTX_GUARD(c);
+  return refine_pop(c);
 }
 
 int avism_run_level(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp,

@@ -28,6 +28,7 @@
 #include "avism_internal.h"
 #include "fields_batch.h"
 #include "knn_search.h"
+#include "refine.h"
 
 namespace avism {
 
@@ -1107,11 +1108,11 @@ static Bins make_bins(avism_ctx* c) {
   B.orig = c->sorig.as<uint32_t>();
   B.start = c->bin_start.as<int32_t>();
   B.nb = c->nb;
-  B.lo = -c->g.L0 / 2.0f;
+  B.lo = -c->box_L0 / 2.0f;
   B.bsz = c->bsz;
-  B.L = c->g.L0;
-  B.eps = c->g.L0 * 2e-6f + c->bsz * 1e-5f;
-  B.periodic = c->g.periodic;
+  B.L = c->box_L0;
+  B.eps = c->box_L0 * 2e-6f + c->bsz * 1e-5f;
+  B.periodic = c->box_periodic;
   return B;
 }
 
@@ -1129,6 +1130,8 @@ int finish_copies(avism_ctx* c) {
 int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, const float* z,
                    const float* vx, const float* vy, const float* vz, const float* m, int where) {
   if (!c->grid_set) AV_FAIL(c, AVISM_ERR_STATE, "avism_set_grid must be called first");
+  if (refine_grid_is_atlas(c))
+    AV_FAIL(c, AVISM_ERR_STATE, "the grid is an atlas of windows, not the particles' box: avism_set_grid first");
   if (np <= 0) AV_FAIL(c, AVISM_ERR_ARG, "np must be positive");
   if (np >= (int64_t)0x7fffffff) AV_FAIL(c, AVISM_ERR_LIMIT, "more than 2^31-1 particles per call");
   // earlier copies on the copy stream end before any buffer is reused
@@ -1142,6 +1145,8 @@ int load_particles(avism_ctx* c, int64_t np, const float* x, const float* y, con
     if (!bufs[i]->ensure(b)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (particles)");
   c->np = np;
   c->fields_ready = false;
+  c->ppart_valid = false;
+  refine_free(c);  // plans hold the previous particles' parent maps
   // [0] out-of-box flag, [1] max |mass| bits, [2] the first mass as given, [3] masses differ
   int* bad = (int*)c->scalars.p;
   {
@@ -1188,7 +1193,9 @@ static int build_bins(avism_ctx* c) {
   const int64_t np = c->np;
   int nb = bins_per_axis(c->g.n, np);
   c->nb = nb;
-  c->bsz = c->g.L0 / (float)nb;
+  c->box_L0 = c->g.L0;
+  c->box_periodic = c->g.periodic;
+  c->bsz = c->box_L0 / (float)nb;
   int64_t nbins = (int64_t)nb * nb * nb;
   if (!c->bin_start.ensure((size_t)(nbins + 1) * 4) || !c->sbin_key.ensure((size_t)np * 4) ||
       !c->sbin_val.ensure((size_t)np * 4) || !c->sbin_key2.ensure((size_t)np * 4) ||
@@ -1256,6 +1263,8 @@ static unsigned vvel_tiles(int n, int K) {
 
 int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp) {
   if (c->np <= 0) AV_FAIL(c, AVISM_ERR_STATE, "no particles loaded");
+  if (refine_grid_is_atlas(c))
+    AV_FAIL(c, AVISM_ERR_STATE, "the grid is an atlas of windows: its fields come from avism_refine_load");
   if (k <= 0) AV_FAIL(c, AVISM_ERR_INPUT, "KNEIGHBOURS must be positive");
   if (k > 256) AV_FAIL(c, AVISM_ERR_UNSUPPORTED, "KNEIGHBOURS above 256 is not supported by this build");
   if (k > c->np) AV_FAIL(c, AVISM_ERR_INPUT, "KNEIGHBOURS exceeds the number of particles");
@@ -1271,6 +1280,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   c->growth_pending = false;
   c->nvoid = 0;  // cubes, MARCA and the catalogue of earlier fields are stale
   c->post_ready = false;
+  c->ppart_valid = false;
   int rc = build_bins(c);
   if (rc) return rc;
   if (!c->pdens.ensure((size_t)np * 4) || !c->hk.ensure((size_t)np * 4) ||
@@ -1302,6 +1312,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
       AV_CHECK_HIP(c, hipGetLastError());
     }
     c->counters[1] = np;
+    c->ppart_valid = true;
     AV_CHECK_HIP(c, hipStreamWaitEvent(c->stream, c->ev_vel, 0));  // velocities uploaded
     k_velrec<<<nblk(np), 256, 0, c->stream>>>(np, c->sorig.as<uint32_t>(), c->spos4.as<float4>(),
                                               c->pdens.as<float>(), c->pvx.as<float>(), c->pvy.as<float>(),
@@ -1446,6 +1457,7 @@ static void swap_buf(DBuf& a, DBuf& b) {
 }
 
 void swap_particle_sets(avism_ctx* c) {
+  c->ppart_valid = false;
   swap_buf(c->px, c->rpx); swap_buf(c->py, c->rpy); swap_buf(c->pz, c->rpz);
   swap_buf(c->pvx, c->rpvx); swap_buf(c->pvy, c->rpvy); swap_buf(c->pvz, c->rpvz); swap_buf(c->pm, c->rpm);
   swap_buf(c->spos4, c->rspos4); swap_buf(c->sorig, c->rsorig); swap_buf(c->vel4, c->rvel4);
@@ -1637,6 +1649,317 @@ int get_particle_fields(avism_ctx* c, float* part_dens, float* hpart) {
     AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   }
   return AVISM_OK;
+}
+
+// ------------------------------------------------ windows of a refined level
+// The single-grid kernel bodies on the cells of cubic windows of a level's
+// grid, written into an atlas (refine.h).  A window cell's query point,
+// neighbours, weights, NORM and deposits are those the whole-grid kernels
+// compute for the same cell of the level's grid.
+
+__device__ __forceinline__ int wrapn(int v, int n) {
+  v %= n;
+  return v < 0 ? v + n : v;
+}
+
+// the window of item it: the last W with off[W] <= it
+__device__ __forceinline__ int item_window(const int64_t* __restrict__ off, int nwin, int64_t it) {
+  int lo = 0, hi = nwin;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= it) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+int64_t win_vvel_items(int32_t w, int32_t k) {
+  const int tz = (k <= 64 ? 128 : 64) / 32;  // kq_of of the slot class of k, over 32
+  return (int64_t)((w + 7) / 8) * ((w + 3) / 4) * ((w + tz - 1) / tz);
+}
+int64_t win_ddens_items(const WinDev& wd) {
+  return (int64_t)((wd.bnx + kWinBins - 1) / kWinBins) * ((wd.bny + kWinBins - 1) / kWinBins) *
+         ((wd.bnz + kWinBins - 1) / kWinBins);
+}
+int64_t win_cell_items(int32_t w) { return ((int64_t)w * w * w + 255) / 256; }
+
+// VVEL_INTERP_SPH_VW on the windows: tiles of 8 x 4 x (lanes / 32) window cells
+template <int K>
+__global__ void __launch_bounds__(kq_of(K)) k_win_vvel(Bins B, int64_t np, const WinDev* __restrict__ W,
+                                                      const int64_t* __restrict__ off, int nwin, VirtG V, int k,
+                                                      float r0, VvelOut O) {
+  constexpr int BQ = kq_of(K), TZ = BQ / 32;
+  __shared__ uint32_t keys_s[K * BQ], gmx_s[K / 4 * BQ];
+  const int64_t it = xcd_block(blockIdx.x, gridDim.x);
+  const int wi = item_window(off, nwin, it);
+  const WinDev wd = W[wi];
+  const int64_t t = it - off[wi];
+  const int w = wd.w, ntx = (w + 7) / 8, nty = (w + 3) / 4;
+  const int tid = threadIdx.x;
+  const int lx = (int)(t % ntx) * 8 + tid % 8, ly = (int)((t / ntx) % nty) * 4 + (tid / 8) % 4,
+            lz = (int)(t / ((int64_t)ntx * nty)) * TZ + tid / 32;
+  if (lx >= w || ly >= w || lz >= w) return;
+  int gx = wd.ox + lx, gy = wd.oy + ly, gz = wd.oz + lz;
+  if (V.periodic) {
+    gx = wrapn(gx, V.n);
+    gy = wrapn(gy, V.n);
+    gz = wrapn(gz, V.n);
+  }
+  const float qx = V.rad[gx + 1 - V.rlo], qy = V.rad[gy + 1 - V.rlo], qz = V.rad[gz + 1 - V.rlo];
+  const size_t c =
+      (size_t)(wd.ax + lx) + (size_t)V.A * ((size_t)(wd.ay + ly) + (size_t)V.A * (size_t)(wd.az + lz));
+  if (B.periodic)
+    vvel_one<1, K>(B, np, k, r0, V.dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
+  else
+    vvel_one<0, K>(B, np, k, r0, V.dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
+}
+
+// cell c (1-based) of the level's grid -> its index in the window, or -1
+__device__ __forceinline__ int win_local(int c, int o, int w, int n, int periodic) {
+  int l = c - 1 - o;
+  if (periodic) l = wrapn(l, n);
+  return (l >= 0 && l < w) ? l : -1;
+}
+// some cell of lo..hi (1-based, within 1..n) lies in the window
+__device__ __forceinline__ bool win_hits(int lo, int hi, int o, int w, int n, int periodic) {
+  if (lo > hi) return false;
+  if (!periodic) return max(lo - 1, o) < min(hi, o + w);
+  const int l0 = wrapn(lo - 1 - o, n);
+  return l0 < w || l0 + (hi - lo + 1) > n;
+}
+
+// DDENS_INTERP_SPH into the windows.  A block takes the particles of 4^3
+// bins of a window's particle range (the bins within the largest DDENS reach
+// of it); a particle whose cell box (or, with NORM = 0, home cell) meets the
+// window computes its NORM over the whole box on the level's grid, exactly as
+// ddens_tile does, then adds the deposits that fall in the window into the
+// atlas's int64 fixed-point cell sums.
+__global__ void __launch_bounds__(256) k_win_ddens(VirtG V, Bins B, const float* __restrict__ hpart, double scale,
+                                                  unsigned long long* acc, const WinDev* __restrict__ W,
+                                                  const int64_t* __restrict__ off, int nwin,
+                                                  unsigned long long* npairs) {
+  constexpr int TB = kWinBins, NR = 2 * TB * TB;  // rows of bins, each in at most two runs
+  __shared__ int qrow[NR + 1], qbeg[NR];
+  const int64_t it = xcd_block(blockIdx.x, gridDim.x);
+  const int wi = item_window(off, nwin, it);
+  const WinDev wd = W[wi];
+  const int64_t t = it - off[wi];
+  const int ntx = (wd.bnx + TB - 1) / TB, nty = (wd.bny + TB - 1) / TB;
+  const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), tz = (int)(t / ((int64_t)ntx * nty));
+  const int bxa = wd.bx0 + tx * TB, bxb = wd.bx0 + min(tx * TB + TB, wd.bnx);
+  const int bya = wd.by0 + ty * TB, byb = wd.by0 + min(ty * TB + TB, wd.bny);
+  const int bza = wd.bz0 + tz * TB, bzb = wd.bz0 + min(tz * TB + TB, wd.bnz);
+  const int nb = B.nb, ry = byb - bya, nrows = ry * (bzb - bza);
+  const int tid = threadIdx.x;
+  if (tid < NR) {
+    const int r = tid >> 1, part = tid & 1;
+    int beg = 0, cnt = 0;
+    if (r < nrows) {
+      const int by = wrapn(bya + r % ry, nb), bz = wrapn(bza + r / ry, nb);
+      const int s = wrapn(bxa, nb), len = bxb - bxa;
+      const int a = part == 0 ? s : 0;
+      const int b = part == 0 ? min(s + len, nb) : max(s + len - nb, 0);
+      if (b > a) {
+        const int64_t rowb = (int64_t)nb * ((int64_t)by + (int64_t)nb * bz);
+        beg = B.start[rowb + a];
+        cnt = B.start[rowb + b] - beg;
+      }
+    }
+    qbeg[tid] = beg;
+    qrow[tid + 1] = cnt;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    qrow[0] = 0;
+    for (int r = 0; r < NR; ++r) qrow[r + 1] += qrow[r];
+  }
+  __syncthreads();
+  const int n = V.n, A = V.A, per = V.periodic;
+  const DdensGeom G{V.rad, V.rlo, n, V.dx, V.rx1};
+  const int nq = qrow[NR];
+  unsigned pairs = 0;
+  for (int q = tid; q < nq; q += 256) {
+    int r = 0, rh = NR;
+    while (rh - r > 1) {
+      const int mid = (r + rh) >> 1;
+      if (qrow[mid] <= q) r = mid;
+      else rh = mid;
+    }
+    const int64_t i = qbeg[r] + (q - qrow[r]);
+    const float4 p = B.pos[i];
+    const float hh = hpart[i];
+    int ix, iy, iz, inc, reach;
+    float inv_h, h2max;
+    ddens_setup(G, p.x, p.y, p.z, hh, ix, iy, iz, inv_h, h2max, inc, reach);
+    int k0 = max(iz - reach, 1), k1 = min(iz + reach, n);
+    int j0 = max(iy - reach, 1), j1 = min(iy + reach, n);
+    const int i0 = max(ix - reach, 1), i1 = min(ix + reach, n);
+    const float hm2 = hh * hh * 1.00001f;
+    const float rdx = 1.0f / G.dx;
+    const float ux = (p.x - G.rx1) * rdx + 1.0f, uy = (p.y - G.rx1) * rdx + 1.0f, uz = (p.z - G.rx1) * rdx + 1.0f;
+    const float hc = hh * 1.00001f * rdx + 1e-3f;
+    k0 = max(k0, (int)ceilf(uz - hc));
+    k1 = min(k1, (int)floorf(uz + hc));
+    j0 = max(j0, (int)ceilf(uy - hc));
+    j1 = min(j1, (int)floorf(uy + hc));
+    const int ia = max(i0, (int)ceilf(ux - hc));
+    const int ib = min(i1, (int)floorf(ux + hc));
+    const bool box_in = win_hits(ia, ib, wd.ox, wd.w, n, per) && win_hits(j0, j1, wd.oy, wd.w, n, per) &&
+                        win_hits(k0, k1, wd.oz, wd.w, n, per);
+    const int hx = (ix >= 1 && ix <= n) ? win_local(ix, wd.ox, wd.w, n, per) : -1;
+    const int hy = (iy >= 1 && iy <= n) ? win_local(iy, wd.oy, wd.w, n, per) : -1;
+    const int hz = (iz >= 1 && iz <= n) ? win_local(iz, wd.oz, wd.w, n, per) : -1;
+    const bool home_in = hx >= 0 && hy >= 0 && hz >= 0;
+    if (!box_in && !home_in) continue;
+    float nrm = 0.0f;  // particles.f90:520-547, in the reference's loop order
+    for (int k2 = k0; k2 <= k1; ++k2) {
+      const float tz2 = p.z - G.rad[k2 - G.rlo];
+      for (int j2 = j0; j2 <= j1; ++j2) {
+        const float ty2 = p.y - G.rad[j2 - G.rlo];
+        const float rem = hm2 - (ty2 * ty2 + tz2 * tz2);
+        if (rem < 0.0f) continue;
+        const float rc = bound_sqrt(rem) * rdx + 1e-3f;
+        const int a = max(i0, (int)ceilf(ux - rc));
+        const int b = min(i1, (int)floorf(ux + rc));
+        for (int i2 = a; i2 <= b; ++i2) {
+          bool ok;
+          const float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);
+          if (!ok) continue;
+          ++pairs;
+          nrm = nrm + wgt;
+        }
+      }
+    }
+    if (nrm == 0.0f) {  // particles.f90:549-557: all the mass to the home cell
+      if (home_in)
+        atomicAdd(&acc[(size_t)(wd.ax + hx) + (size_t)A * ((size_t)(wd.ay + hy) + (size_t)A * (size_t)(wd.az + hz))],
+                  (unsigned long long)llrint((double)p.w * scale));
+      continue;
+    }
+    if (!box_in) continue;
+    for (int k2 = k0; k2 <= k1; ++k2) {
+      const int lz = win_local(k2, wd.oz, wd.w, n, per);
+      if (lz < 0) continue;
+      const float tz2 = p.z - G.rad[k2 - G.rlo];
+      for (int j2 = j0; j2 <= j1; ++j2) {
+        const int ly = win_local(j2, wd.oy, wd.w, n, per);
+        if (ly < 0) continue;
+        const float ty2 = p.y - G.rad[j2 - G.rlo];
+        const float rem = hm2 - (ty2 * ty2 + tz2 * tz2);
+        if (rem < 0.0f) continue;
+        const float rc = bound_sqrt(rem) * rdx + 1e-3f;
+        const int a = max(i0, (int)ceilf(ux - rc));
+        const int b = min(i1, (int)floorf(ux + rc));
+        const size_t row = (size_t)A * ((size_t)(wd.ay + ly) + (size_t)A * (size_t)(wd.az + lz));
+        for (int i2 = a; i2 <= b; ++i2) {
+          const int lx = win_local(i2, wd.ox, wd.w, n, per);
+          if (lx < 0) continue;
+          bool ok;
+          const float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);
+          if (!ok) continue;
+          const float v = (wgt * p.w) / nrm;  // particles.f90:599-600
+          if (v != 0.0f) atomicAdd(&acc[(size_t)(wd.ax + lx) + row], (unsigned long long)llrint((double)v * scale));
+        }
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) pairs += (unsigned)__shfl_xor((int)pairs, o);
+  if ((tid & 63) == 0 && pairs) atomicAdd(npairs, (unsigned long long)pairs);
+}
+
+// MARCAP of the windows: the parent level's MARCA at cell floor((o + l) / 2),
+// kept where it is the window's parent
+__global__ void __launch_bounds__(256) k_win_marcap(const WinDev* __restrict__ W, const int64_t* __restrict__ off,
+                                                   int nwin, int A, const int32_t* __restrict__ pmarca, int pside,
+                                                   int32_t* marcap) {
+  const int64_t it = blockIdx.x;
+  const int wi = item_window(off, nwin, it);
+  const WinDev wd = W[wi];
+  const int w = wd.w;
+  const int64_t l = (it - off[wi]) * 256 + threadIdx.x;
+  if (l >= (int64_t)w * w * w) return;
+  const int lx = (int)(l % w), ly = (int)((l / w) % w), lz = (int)(l / ((int64_t)w * w));
+  int sx = ((wd.ox + lx) >> 1) - wd.sox, sy = ((wd.oy + ly) >> 1) - wd.soy, sz = ((wd.oz + lz) >> 1) - wd.soz;
+  if (wd.swrap) {
+    sx = wrapn(sx, wd.sw);
+    sy = wrapn(sy, wd.sw);
+    sz = wrapn(sz, wd.sw);
+  }
+  int32_t v = 0;
+  if (sx >= 0 && sx < wd.sw && sy >= 0 && sy < wd.sw && sz >= 0 && sz < wd.sw) {
+    const int32_t m = pmarca[(size_t)(wd.sax + sx) +
+                             (size_t)pside * ((size_t)(wd.say + sy) + (size_t)pside * (size_t)(wd.saz + sz))];
+    v = m == wd.parent ? m : 0;
+  }
+  marcap[(size_t)(wd.ax + lx) + (size_t)A * ((size_t)(wd.ay + ly) + (size_t)A * (size_t)(wd.az + lz))] = v;
+}
+
+// the largest of n float bits (non-negative floats: the largest value)
+__global__ void __launch_bounds__(256) k_max_bits(int64_t n, const uint32_t* __restrict__ a, uint32_t* out) {
+  uint32_t m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) m = max(m, a[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// FP64 sums of the particle masses over nparts contiguous slices (fixed order)
+__global__ void __launch_bounds__(256) k_mass_sum(int64_t np, const float* __restrict__ m, double* partial) {
+  __shared__ double red[256];
+  const int64_t per = (np + gridDim.x - 1) / gridDim.x;
+  const int64_t b0 = (int64_t)blockIdx.x * per, b1 = min(b0 + per, np);
+  double s = 0.0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) s += (double)m[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+hipError_t launch_win_vvel(hipStream_t st, const Bins& B, int64_t np, const WinDev* W, const int64_t* off,
+                           int32_t nwin, int64_t items, const VirtG& V, int32_t k, float r0, const float4* vel,
+                           uint32_t* hbits, float* u, float* v, float* w) {
+  if (items <= 0) return hipSuccess;
+  const VvelOut O{vel, hbits, u, v, w, vel ? 1 : 0};
+  with_k(k, [&](auto KC) {
+    constexpr int K = decltype(KC)::value;
+    k_win_vvel<K><<<(unsigned)items, kq_of(K), 0, st>>>(B, np, W, off, nwin, V, k, r0, O);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_win_ddens(hipStream_t st, const VirtG& V, const Bins& B, const float* hpart, double scale,
+                            unsigned long long* acc, const WinDev* W, const int64_t* off, int32_t nwin,
+                            int64_t items, unsigned long long* npairs) {
+  if (items <= 0) return hipSuccess;
+  k_win_ddens<<<(unsigned)items, 256, 0, st>>>(V, B, hpart, scale, acc, W, off, nwin, npairs);
+  return hipGetLastError();
+}
+
+hipError_t launch_win_marcap(hipStream_t st, const WinDev* W, const int64_t* off, int32_t nwin, int64_t items,
+                             int32_t A, const int32_t* pmarca, int32_t pside, int32_t* marcap) {
+  if (items <= 0) return hipSuccess;
+  k_win_marcap<<<(unsigned)items, 256, 0, st>>>(W, off, nwin, A, pmarca, pside, marcap);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_bits(hipStream_t st, int64_t n, const uint32_t* a, uint32_t* out) {
+  k_max_bits<<<(unsigned)std::max<int64_t>(1, std::min<int64_t>(nblk(n), 4096)), 256, 0, st>>>(n, a, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mass_sum(hipStream_t st, int64_t np, const float* m, double* partial, int32_t nparts) {
+  k_mass_sum<<<(unsigned)nparts, 256, 0, st>>>(np, m, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_ddens_finalize(hipStream_t st, int64_t ncell, const unsigned long long* acc, double inv_scale,
+                                 float den, float rote, float* umass, double* partial) {
+  const unsigned grid = (unsigned)((ncell + (int64_t)kFinB * kFinV - 1) / ((int64_t)kFinB * kFinV));
+  k_ddens_finalize<<<grid, kFinB, 0, st>>>(ncell, acc, inv_scale, 0, den, rote, umass, partial);
+  return hipGetLastError();
 }
 
 }  // namespace avism

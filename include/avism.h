@@ -280,6 +280,66 @@ int avism_batch_get_fields(avism_ctx* ctx, int32_t p, float* u1co, float* div, f
  * the number of graph launches so far, MEANDENS per patch; NULL skips */
 int avism_batch_info(avism_ctx* ctx, char* tuning, int32_t len, int64_t* graph_launches, float* meandens);
 
+/* ---- refined levels inside the parent voids --------------------------------
+ * Void-in-void levels (avism.f90:742-759, 957-968, 2123-2161) computed only
+ * around the parent voids, for boxes whose finer grids do not fit whole.
+ * The reference runs every level on the whole box and cannot run more than
+ * one level with particle input at all (its level loop frees the particles,
+ * avism.f90:1613-1629); these semantics are this library's:
+ *  - a level has 2x the parent level's cells per axis.  Each main void V of
+ *    the parent level with REQ(V) >= RMIN_SUB (smaller ones can hold no cube,
+ *    voidfinding.f90:267-274) gets one cubic window: the bounding box of V's
+ *    cells at the finer resolution, grown by `margin` cells per side;
+ *  - windows are packed side by side into atlases (non-periodic grids of at
+ *    most max_cells cells) that are processed one at a time, so memory is
+ *    bounded by the largest atlas, not by the level's whole grid;
+ *  - in a window, MARCAP is V on V's cells and 0 elsewhere, REQP(V) =
+ *    REQ(V): the void stage finds V's subvoids independently of other parents;
+ *  - fields: VVEL_INTERP_SPH_VW queries the window's cells over all loaded
+ *    particles (the box's periodic wrap); HPART is the largest distance at
+ *    which a particle served any window cell of the level; DDENS_INTERP_SPH
+ *    normalises each particle over its whole cell box on the level's grid
+ *    (particles.f90:520-547) and keeps the deposits that fall in a window;
+ *    MEANDENS is the total particle mass over the level's cells, which the
+ *    whole-grid mean equals because DDENS conserves mass (particles.f90:549-604).
+ * Call after avism_merge_and_post of the parent level (the whole box, or a
+ * loaded atlas of the previous plan).  avism_particles_to_grid must have run
+ * with SPH velocities on the loaded particles (its PPART_DENS is reused). */
+typedef struct {
+    int32_t depth;        /* plans on the stack (1: the first refined level) */
+    int32_t n_level;      /* cells per axis of the level's whole grid */
+    int32_t natlas;       /* atlases of the top plan */
+    int32_t atlas;        /* atlas loaded on the context, -1 if none */
+    int32_t side;         /* cells per axis of that atlas */
+    int32_t nwindows;     /* its windows */
+    int64_t window_cells; /* cells of its windows */
+    float dx;             /* cell size of the level's grid */
+    float rx1_level;      /* RADX(1) of the level's grid */
+    float dx_atlas;       /* cell size of the atlas grid (MALLA over side cells of dx; within an ulp of dx) */
+    float rx1_atlas;      /* RADX(1) of the atlas grid */
+    float meandens;       /* MEANDENS of the level, Msun/Mpc^3 */
+} avism_refine_info;
+/* plan the next level's windows from the current level's catalogue and MARCA
+ * (pushed on a stack of plans); margin >= 4 cells; max_cells <= 2^31-1 bounds
+ * each atlas (0: 2^30); *natlas = the atlases to process (0: no parent
+ * qualifies) */
+int avism_refine_plan(avism_ctx* ctx, int32_t margin, int64_t max_cells, int32_t* natlas);
+/* atlas a of the top plan becomes the context's grid (non-periodic, side^3
+ * cells) with its fields, MARCAP and REQP; then avism_find_voids,
+ * avism_merge_and_post and the getters work on it as on any grid (cube and
+ * catalogue coordinates in the atlas frame: avism_refine_windows maps them
+ * to the level's grid).  k: KNEIGHBOURS (SPH only). */
+int avism_refine_load(avism_ctx* ctx, int32_t atlas, int32_t k);
+/* the loaded atlas: tab[8*i .. 8*i+7] = parent id, side, origin x, y, z in
+ * the level's cells (0-based; in a periodic box it may lie outside 0..n_level-1
+ * and wraps), origin x, y, z in the atlas (0-based), for i < *n (cap: entries
+ * tab holds; tab may be NULL); info may be NULL.  A position X in the atlas
+ * frame is X - rx1_atlas - ax dx_atlas + rx1_level + ox dx in the level's. */
+int avism_refine_windows(avism_ctx* ctx, int32_t* tab, int32_t cap, int32_t* n, avism_refine_info* info);
+/* drop the top plan (its parent MARCA copy); the context's grid keeps the
+ * last loaded atlas until another one is loaded */
+int avism_refine_pop(avism_ctx* ctx);
+
 /* ---- whole level ---------------------------------------------------------- */
 /* particles already loaded: fields, voids, merge and post in one call */
 int avism_run_level(avism_ctx* ctx, int32_t k, int32_t vel_interp, int32_t dens_interp,
