@@ -62,7 +62,18 @@ def parse():
     ap.add_argument("--mode", choices=["both", "plain", "learned"], default="both")
     ap.add_argument("--levels", type=int, default=1, help="void-in-void levels per step (N, 2N, ...)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-patch batch record")
-    return ap.parse_args()
+    ap.add_argument("--refine", type=int, default=0,
+                    help="refined void-in-void levels in windows around the parent voids (avism_refine_*)")
+    ap.add_argument("--margin", type=int, default=8, help="window margin of the refined levels, cells")
+    ap.add_argument("--max-cells", type=int, default=1 << 30, help="cells per atlas of a refined level")
+    ap.add_argument("--config", choices=["c4", "c5"], default="c4",
+                    help="c5: 1024^3 particles in a periodic 2000 Mpc/h box, x2 and x4 refinement in the "
+                         "parent voids (the refined-level run)")
+    a = ap.parse_args()
+    if a.config == "c5":
+        a.n, a.box, a.seed = 1024, 2000.0, 5
+        a.refine = a.refine or 2
+    return a
 
 
 def sources_hash() -> str:
@@ -259,6 +270,135 @@ def batch_bench(dev, k, steps, warmup):
             "tuning": info["tuning"]}
 
 
+def refine_walk(ctx, k, mode, depth, margin, max_cells, recs, lev=1):
+    """Refined levels lev..depth in windows around the parent voids, depth
+    first: plan the level from the current catalogue, then each atlas's fields,
+    void stage and catalogue (downloaded), and its own sublevels."""
+    import torch
+    t = time.perf_counter()
+    na = ctx.refine_plan(margin=margin, max_cells=max_cells)
+    rec = recs.setdefault(lev, dict(atlases=0, windows=0, window_cells=0, atlas_cells=0, max_side=0, ncand=0,
+                                    ncubes=0, nvoids=0, ms=0.0, min_free_gb=1e9, stages={}))
+    rec["ms"] += (time.perf_counter() - t) * 1e3
+    for a in range(na):
+        t = time.perf_counter()
+        ctx.refine_load(a, k)
+        _, info = ctx.refine_windows()
+        nv = ctx.find_voids(mode)
+        ctx.merge_and_post()
+        cat = ctx.catalogue(nv)
+        st = cat["stats"]
+        rec["ms"] += (time.perf_counter() - t) * 1e3
+        rec["atlases"] += 1
+        rec["windows"] += int(info["nwindows"])
+        rec["window_cells"] += int(info["window_cells"])
+        rec["atlas_cells"] += int(info["side"]) ** 3
+        rec["max_side"] = max(rec["max_side"], int(info["side"]))
+        rec["ncand"] += int(st.ncand)
+        rec["ncubes"] += int(nv)
+        rec["nvoids"] += int(st.nvoidt)
+        rec["min_free_gb"] = min(rec["min_free_gb"], torch.cuda.mem_get_info()[0] / 2 ** 30)
+        for name, ms in ctx.timings():
+            rec["stages"][name] = rec["stages"].get(name, 0.0) + ms
+        progress(f"level {lev + 1} atlas {a + 1}/{na}: {int(info['nwindows'])} windows, side {int(info['side'])}, "
+                 f"{int(nv)} cubes, {int(st.nvoidt)} voids, {(time.perf_counter() - t) * 1e3:.0f} ms")
+        if lev < depth:
+            refine_walk(ctx, k, mode, depth, margin, max_cells, recs, lev + 1)
+    ctx.refine_pop()
+    return recs
+
+
+def refine_bench(args):
+    """The refined-level run: the base level on the whole box, then
+    args.refine levels (x2, x4, ...) computed only in windows around the
+    parent voids, atlas by atlas (memory bounded by the largest atlas).  One
+    step = particles in HBM -> base fields, voids and catalogue -> every
+    refined level's atlases and catalogues (catalogues downloaded)."""
+    import torch
+    from avism_amd import learned, synth
+    from avism_amd._lib import AVISM_DEVICE, Context
+    dev = torch.device("cuda:0")
+    n, k = args.n, args.k
+    t = time.perf_counter()
+    parts = synth.zeldovich(n, args.box, seed=args.seed, device=dev)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t
+    torch.cuda.empty_cache()
+    progress(f"generated {n}^3 particles in {gen_s:.1f} s")
+    ctx = Context(0)
+    ctx.set_grid(n, args.box, 1)
+    ctx.set_cosmology(0.678, 0.31, 0.0)
+    ctx.set_void_params(-0.6, 10.0, 0.25, 0.0, 1.0, 1.0, 30_000_000)
+    gi = ctx.grid()
+    GL = 2
+    mode = 0 if args.mode == "plain" else GL
+    if mode == GL:
+        learned.Proposer.load_default(dev).install(ctx)
+    total_gb = torch.cuda.mem_get_info()[1] / 2 ** 30
+
+    def step(recs):
+        ctx.set_grid(n, args.box, 1)
+        ctx.load_particles(parts.x, parts.y, parts.z, parts.vx, parts.vy, parts.vz, parts.m, where=AVISM_DEVICE)
+        t0 = time.perf_counter()
+        ctx.particles_to_grid(k=k)
+        nv = ctx.find_voids(mode)
+        ctx.merge_and_post()
+        cat = ctx.catalogue(nv)
+        st = cat["stats"]
+        base = dict(n=n, ms=(time.perf_counter() - t0) * 1e3, ncand=int(st.ncand), ncubes=int(nv),
+                    nvoids=int(st.nvoidt), free_gb=torch.cuda.mem_get_info()[0] / 2 ** 30,
+                    stages={})
+        for name, ms in ctx.timings():
+            base["stages"][name] = base["stages"].get(name, 0.0) + ms
+        progress(f"base level {n}^3: {base['ms']:.0f} ms, {int(nv)} cubes, {int(st.nvoidt)} voids")
+        refine_walk(ctx, k, mode, args.refine, args.margin, args.max_cells, recs)
+        return base
+
+    for _ in range(args.warmup):
+        step({})
+        progress("warm-up step done")
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    ctx.timings()
+    recs, bases = {}, []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        bases.append(step(recs))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    ctx.set_timing(False)
+    progress(f"refined-level step {dt:.2f} s")
+    s_ = args.steps
+    levels = [dict(n=n, ms=round(sum(b["ms"] for b in bases) / s_, 1), ncand=bases[-1]["ncand"],
+                   ncubes=bases[-1]["ncubes"], nvoids=bases[-1]["nvoids"],
+                   min_free_gb=round(min(b["free_gb"] for b in bases), 1),
+                   stages_ms={kk: round(sum(b["stages"].get(kk, 0.0) for b in bases) / s_, 2)
+                              for kk in bases[-1]["stages"]})]
+    for lev in sorted(recs):
+        r = recs[lev]
+        levels.append(dict(n=n * 2 ** lev, ms=round(r["ms"] / s_, 1), atlases=r["atlases"] // s_,
+                           windows=r["windows"] // s_, window_cells=r["window_cells"] // s_,
+                           atlas_cells=r["atlas_cells"] // s_, max_atlas_side=r["max_side"],
+                           whole_grid_cells=(n * 2 ** lev) ** 3,
+                           ncand=r["ncand"] // s_, ncubes=r["ncubes"] // s_, nvoids=r["nvoids"] // s_,
+                           min_free_gb=round(r["min_free_gb"], 1),
+                           stages_ms={kk: round(v / s_, 2) for kk, v in r["stages"].items()}))
+    peak_gb = total_gb - min(lv["min_free_gb"] for lv in levels)
+    return {
+        "metric": f"cells/s of a {n}^3-particle run with {args.refine} refined levels in the parent voids",
+        "value": n ** 3 / dt, "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded Zel'dovich particles generated on the GPU)",
+        "config": {"workload": f"zeldovich {n}^3 particles, {n}^3 base grid (ext {gi.next}^3), periodic "
+                               f"{args.box:g} Mpc/h, SPH k={k}; refined levels x2..x{2 ** args.refine} in windows "
+                               f"(margin {args.margin} cells, atlases of at most {args.max_cells} cells)",
+                   "growth": "learned" if mode else "plain",
+                   "boundary": "particles in HBM -> every level's catalogue in host RAM"},
+        "levels": levels, "device_memory_gb": round(total_gb, 1), "peak_device_use_gb": round(peak_gb, 1),
+        "input_gen_s": round(gen_s, 2),
+    }
+
+
 def timed(ctx, fn, warmup, steps):
     import torch
     for _ in range(warmup):
@@ -283,6 +423,9 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this library drives a single "
               "device, and a sharded multi-GPU run is not implemented.", file=sys.stderr)
         sys.exit(2)
+    if args.refine > 0:
+        print(json.dumps(refine_bench(args)))
+        return
     import numpy as np
     import torch
     from avism_amd import synth
