@@ -148,6 +148,29 @@ module avism_c
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx
     end function
+    ! refined levels in windows around the parent voids (include/avism.h)
+    integer(c_int) function avism_refine_plan(ctx, margin, max_cells, natlas) bind(C, name="avism_refine_plan")
+      import :: c_ptr, c_int, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: margin
+      integer(c_int64_t), value :: max_cells
+      integer(c_int32_t) :: natlas
+    end function
+    integer(c_int) function avism_refine_load(ctx, atlas, k) bind(C, name="avism_refine_load")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: atlas, k
+    end function
+    integer(c_int) function avism_refine_windows(ctx, tab, cap, n, info) bind(C, name="avism_refine_windows")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx, tab, info
+      integer(c_int32_t), value :: cap
+      integer(c_int32_t) :: n
+    end function
+    integer(c_int) function avism_refine_pop(ctx) bind(C, name="avism_refine_pop")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
     integer(c_int) function avism_cnn_set_params(ctx, params, where) bind(C, name="avism_cnn_set_params")
       import :: c_ptr, c_int, c_int32_t, c_float
       type(c_ptr), value :: ctx
