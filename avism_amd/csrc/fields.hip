@@ -763,37 +763,60 @@ __global__ void __launch_bounds__(256) k_reduce_mean(const double* partial, int6
   }
 }
 
-// DIVER_UNIFORM (divergence.f90:2-50) fused with U1CO = U1 / MEANDENS
-__global__ void k_div_finalize(int n, float dx, int periodic, const float* __restrict__ u,
-                               const float* __restrict__ v, const float* __restrict__ w,
-                               const float* __restrict__ umass, const float* __restrict__ mean,
-                               float* u1co, float* div) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0-based
-  int j = blockIdx.y, k = blockIdx.z;
-  if (i >= n) return;
+// DIVER_UNIFORM (divergence.f90:2-50) fused with U1CO = U1 / MEANDENS.  A
+// block takes a 64 x 8 (x, y) column of cells and marches kDivZ planes in z:
+// each cell's w at z-1, z, z+1 slides through registers (every w value is
+// read once per column), the x and y neighbours come from the same and the
+// block's adjacent rows (cache hits); boundary cells use the reference's
+// periodic or extrapolated differences with direct loads.
+constexpr int kDivX = 64, kDivY = 8, kDivZ = 16;
+
+__global__ void __launch_bounds__(kDivX * kDivY) k_div_finalize(int n, float dx, int periodic,
+                                                                const float* __restrict__ u,
+                                                                const float* __restrict__ v,
+                                                                const float* __restrict__ w,
+                                                                const float* __restrict__ umass,
+                                                                const float* __restrict__ mean, float* u1co,
+                                                                float* div) {
+  const int i = blockIdx.x * kDivX + (int)threadIdx.x % kDivX;  // 0-based
+  const int j = blockIdx.y * kDivY + (int)threadIdx.x / kDivX;
+  if (i >= n || j >= n) return;
+  const int z0 = blockIdx.z * kDivZ, z1 = min(z0 + kDivZ, n);
   auto A = [&](const float* f, int a, int b, int cc) -> float {
     return f[(size_t)a + (size_t)n * ((size_t)b + (size_t)n * cc)];
   };
-  size_t c = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k);
-  float fx, fy, fz;
   const int N1 = n - 1;
-  if (i >= 1 && i <= n - 2) fx = (A(u, i + 1, j, k) - A(u, i - 1, j, k)) / 2.0f;
-  else if (periodic) fx = (i == 0) ? (A(u, 1, j, k) - A(u, N1, j, k)) / 2.0f
-                                   : (A(u, 0, j, k) - A(u, n - 2, j, k)) / 2.0f;
-  else if (i == 0) fx = 2.0f * (A(u, 1, j, k) - A(u, 0, j, k)) - (A(u, 2, j, k) - A(u, 0, j, k)) / 2.0f;
-  else fx = 2.0f * (A(u, N1, j, k) - A(u, n - 2, j, k)) - (A(u, N1, j, k) - A(u, n - 3, j, k)) / 2.0f;
-  if (j >= 1 && j <= n - 2) fy = (A(v, i, j + 1, k) - A(v, i, j - 1, k)) / 2.0f;
-  else if (periodic) fy = (j == 0) ? (A(v, i, 1, k) - A(v, i, N1, k)) / 2.0f
-                                   : (A(v, i, 0, k) - A(v, i, n - 2, k)) / 2.0f;
-  else if (j == 0) fy = 2.0f * (A(v, i, 1, k) - A(v, i, 0, k)) - (A(v, i, 2, k) - A(v, i, 0, k)) / 2.0f;
-  else fy = 2.0f * (A(v, i, N1, k) - A(v, i, n - 2, k)) - (A(v, i, N1, k) - A(v, i, n - 3, k)) / 2.0f;
-  if (k >= 1 && k <= n - 2) fz = (A(w, i, j, k + 1) - A(w, i, j, k - 1)) / 2.0f;
-  else if (periodic) fz = (k == 0) ? (A(w, i, j, 1) - A(w, i, j, N1)) / 2.0f
-                                   : (A(w, i, j, 0) - A(w, i, j, n - 2)) / 2.0f;
-  else if (k == 0) fz = 2.0f * (A(w, i, j, 1) - A(w, i, j, 0)) - (A(w, i, j, 2) - A(w, i, j, 0)) / 2.0f;
-  else fz = 2.0f * (A(w, i, j, N1) - A(w, i, j, n - 2)) - (A(w, i, j, N1) - A(w, i, j, n - 3)) / 2.0f;
-  div[c] = (fx / dx + fy / dx) + fz / dx;
-  if (umass) u1co[c] = umass[c] / (*mean);
+  const float mn = umass ? *mean : 1.0f;
+  float wm = z0 >= 1 ? A(w, i, j, z0 - 1) : 0.0f, wc = A(w, i, j, z0);
+  for (int k = z0; k < z1; ++k) {
+    const float wp = k + 1 < n ? A(w, i, j, k + 1) : 0.0f;
+    float fx, fy, fz;
+    if (i >= 1 && i <= n - 2) fx = (A(u, i + 1, j, k) - A(u, i - 1, j, k)) / 2.0f;
+    else if (periodic) fx = (i == 0) ? (A(u, 1, j, k) - A(u, N1, j, k)) / 2.0f
+                                     : (A(u, 0, j, k) - A(u, n - 2, j, k)) / 2.0f;
+    else if (i == 0) fx = 2.0f * (A(u, 1, j, k) - A(u, 0, j, k)) - (A(u, 2, j, k) - A(u, 0, j, k)) / 2.0f;
+    else fx = 2.0f * (A(u, N1, j, k) - A(u, n - 2, j, k)) - (A(u, N1, j, k) - A(u, n - 3, j, k)) / 2.0f;
+    if (j >= 1 && j <= n - 2) fy = (A(v, i, j + 1, k) - A(v, i, j - 1, k)) / 2.0f;
+    else if (periodic) fy = (j == 0) ? (A(v, i, 1, k) - A(v, i, N1, k)) / 2.0f
+                                     : (A(v, i, 0, k) - A(v, i, n - 2, k)) / 2.0f;
+    else if (j == 0) fy = 2.0f * (A(v, i, 1, k) - A(v, i, 0, k)) - (A(v, i, 2, k) - A(v, i, 0, k)) / 2.0f;
+    else fy = 2.0f * (A(v, i, N1, k) - A(v, i, n - 2, k)) - (A(v, i, N1, k) - A(v, i, n - 3, k)) / 2.0f;
+    if (k >= 1 && k <= n - 2) fz = (wp - wm) / 2.0f;
+    else if (periodic) fz = (k == 0) ? (A(w, i, j, 1) - A(w, i, j, N1)) / 2.0f
+                                     : (A(w, i, j, 0) - A(w, i, j, n - 2)) / 2.0f;
+    else if (k == 0) fz = 2.0f * (A(w, i, j, 1) - A(w, i, j, 0)) - (A(w, i, j, 2) - A(w, i, j, 0)) / 2.0f;
+    else fz = 2.0f * (A(w, i, j, N1) - A(w, i, j, n - 2)) - (A(w, i, j, N1) - A(w, i, j, n - 3)) / 2.0f;
+    const size_t c = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k);
+    div[c] = (fx / dx + fy / dx) + fz / dx;
+    if (umass) u1co[c] = umass[c] / mn;
+    wm = wc;
+    wc = wp;
+  }
+}
+
+static inline dim3 div_grid(int n) {
+  return dim3((unsigned)((n + kDivX - 1) / kDivX), (unsigned)((n + kDivY - 1) / kDivY),
+              (unsigned)((n + kDivZ - 1) / kDivZ));
 }
 
 // velocities not available (FLAG_DIV = -1, avism.f90:1259-1263, 1646-1650):
@@ -1070,8 +1093,7 @@ hipError_t launch_reduce_mean(hipStream_t st, const double* partial, int64_t npa
 }
 hipError_t launch_div_finalize(hipStream_t st, int n, float dx, int periodic, const float* u, const float* v,
                                const float* w, const float* umass, const float* mean, float* u1co, float* div) {
-  dim3 g2((n + 255) / 256, n, n);
-  k_div_finalize<<<g2, 256, 0, st>>>(n, dx, periodic, u, v, w, umass, mean, u1co, div);
+  k_div_finalize<<<div_grid(n), kDivX * kDivY, 0, st>>>(n, dx, periodic, u, v, w, umass, mean, u1co, div);
   return hipGetLastError();
 }
 hipError_t launch_convert_check(hipStream_t st, int64_t np, float* m, const float* x, const float* y, const float* z,
@@ -1426,8 +1448,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
       k_div_linear<<<nblk(ncell), 256, 0, c->stream>>>(ncell, lin, c->u_mass.as<float>(), dmean,
                                                       c->u1co.as<float>(), c->div.as<float>());
     } else {
-      dim3 g2((n + 255) / 256, n, n);
-      k_div_finalize<<<g2, 256, 0, c->stream>>>(n, c->g.dx, c->g.periodic, c->fu.as<float>(),
+      k_div_finalize<<<div_grid(n), kDivX * kDivY, 0, c->stream>>>(n, c->g.dx, c->g.periodic, c->fu.as<float>(),
                                                 c->fv.as<float>(), c->fw.as<float>(),
                                                 c->u_mass.as<float>(), dmean, c->u1co.as<float>(),
                                                 c->div.as<float>());
@@ -1621,8 +1642,7 @@ int fields_from_grid(avism_ctx* c, const float* delta, const float* vx, const fl
                                                    c->pz.as<float>(), c->pm.as<float>(),
                                                    c->u1co.as<float>(), c->fu.as<float>(),
                                                    c->fv.as<float>(), c->fw.as<float>());
-  dim3 g2((n + 255) / 256, n, n);
-  k_div_finalize<<<g2, 256, 0, c->stream>>>(n, c->g.dx, c->g.periodic, c->fu.as<float>(),
+  k_div_finalize<<<div_grid(n), kDivX * kDivY, 0, c->stream>>>(n, c->g.dx, c->g.periodic, c->fu.as<float>(),
                                             c->fv.as<float>(), c->fw.as<float>(), nullptr, nullptr,
                                             nullptr, c->div.as<float>());
   AV_CHECK_HIP(c, hipGetLastError());
