@@ -70,6 +70,7 @@ EXPORTS = [
     "avism_batch_set", "avism_batch_load", "avism_batch_fields", "avism_batch_voids", "avism_batch_patch",
     "avism_batch_get_fields", "avism_batch_info",
     "avism_refine_plan", "avism_refine_load", "avism_refine_windows", "avism_refine_pop",
+    "avism_amr_set", "avism_amr_fields",
 ]
 
 _lib = None
@@ -144,6 +145,8 @@ def lib():
         "avism_refine_load": ([vp, i32, i32], C.c_int),
         "avism_refine_windows": ([vp, vp, i32, C.POINTER(i32), C.POINTER(RefineInfo)], C.c_int),
         "avism_refine_pop": ([vp], C.c_int),
+        "avism_amr_set": ([vp, i32, vp, i32, vp, vp, vp, i32], C.c_int),
+        "avism_amr_fields": ([vp, i32, i32, i32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -270,6 +273,22 @@ class Context:
 
     def refine_pop(self):
         self._chk(lib().avism_refine_pop(self._h))
+
+    # ---- MASCLET AMR input
+    def amr_set(self, nhyx: int, base, pint=None, preal=None, pdata=None):
+        """The gas of a MASCLET snapshot: base = [delta, vx, vy, vz] on nhyx^3
+        cells; level-1 patches as (pint, preal, pdata) (synth.amr_pack)."""
+        b = np.ascontiguousarray(np.concatenate([np.ravel(a) for a in base]), np.float32)
+        npa = 0 if pint is None else len(pint) // 6
+        arrs = [None if a is None else np.ascontiguousarray(a, t)
+                for a, t in ((pint, np.int32), (preal, np.float32), (pdata, np.float32))]
+        self._keep_amr = (b, *arrs)
+        self._chk(lib().avism_amr_set(self._h, int(nhyx), _ptr(b), int(npa), *[_ptr(a) for a in arrs], AVISM_HOST))
+
+    def amr_fields(self, ir: int, flag_dens: int = 2, flag_div: int = 2):
+        """U1CO and DIVERCO of level ir from the gas (and the dark matter of the
+        last particles_to_grid on this grid, per FLAG_DENS / FLAG_DIV)."""
+        self._chk(lib().avism_amr_fields(self._h, int(ir), int(flag_dens), int(flag_div)))
 
     def set_timing(self, on: bool):
         self._chk(lib().avism_set_timing(self._h, int(bool(on))))

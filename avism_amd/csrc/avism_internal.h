@@ -42,6 +42,7 @@ struct GridGeom {
 
 struct BatchState;   // a batch of patches (batch.hip)
 struct RefineState;  // windows of refined void-in-void levels (refine.hip)
+struct AmrState;     // MASCLET AMR gas (amr.hip)
 
 GridGeom make_grid(int32_t n, float L0, int32_t periodic);
 
@@ -175,6 +176,11 @@ struct avism_ctx {
   int64_t growth_stats[8] = {0};
   avism::BatchState* batch = nullptr;          // avism_batch_* state
   avism::RefineState* refine = nullptr;        // avism_refine_* state
+  avism::AmrState* amr = nullptr;              // avism_amr_* state
+  // the DDENS / TSC cell masses in mass_fix: int64 fixed point with this
+  // inverse scale, or FP64 (mass_fp64); mass_n: the grid they were made on
+  double mass_inv_scale = 0;
+  int32_t mass_fp64 = 0, mass_n = 0;
 };
 
 #define AV_CHECK_HIP(ctx, expr)                                                        \
@@ -221,6 +227,10 @@ int batch_info(avism_ctx* c, char* tuning, int32_t len, int64_t* graph_launches,
 int indexx_noalloc(avism_ctx* c, const float* d_keys, int64_t n, int32_t* d_indx);
 // refined levels restricted to windows around the parent voids (refine.hip)
 void refine_free(avism_ctx* c);
+void amr_free(avism_ctx* c);
+int amr_set(avism_ctx* c, int32_t nb, const float* base, int32_t npatch, const int32_t* pint, const float* preal,
+            const float* pdata, int where);
+int amr_fields(avism_ctx* c, int32_t ir, int32_t flag_dens, int32_t flag_div);
 bool refine_grid_is_atlas(const avism_ctx* c);  // the grid is an atlas of windows
 int refine_plan(avism_ctx* c, int32_t margin, int64_t max_cells, int32_t* natlas);
 int refine_load(avism_ctx* c, int32_t atlas, int32_t k);

@@ -83,6 +83,7 @@ int avism_destroy(avism_ctx* c) {
   (void)hipStreamSynchronize(c->cstream);
   batch_free(c);
   refine_free(c);
+  amr_free(c);
   drop_timings(c);
   (void)hipEventDestroy(c->ev_vel);
   (void)hipEventDestroy(c->ev_mark);
@@ -588,6 +589,20 @@ int avism_refine_windows(avism_ctx* c, int32_t* tab, int32_t cap, int32_t* n, av
 int avism_refine_pop(avism_ctx* c) {
   CTX_GUARD(c);
   return refine_pop(c);
+}
+
+int avism_amr_set(avism_ctx* c, int32_t nhyx, const float* base, int32_t npatch, const int32_t* pint,
+                  const float* preal, const float* pdata, int32_t where) {
+  CTX_GUARD(c);
+  DEVICE_ARGS(c, where);
+  if (!base) AV_FAIL(c, AVISM_ERR_ARG, "null base grid");
+  return amr_set(c, nhyx, base, npatch, pint, preal, pdata, where);
+}
+
+int avism_amr_fields(avism_ctx* c, int32_t ir, int32_t flag_dens, int32_t flag_div) {
+  CTX_GUARD(c);
+  if (refine_grid_is_atlas(c)) AV_FAIL(c, AVISM_ERR_STATE, "the grid is an atlas of windows");
+  return amr_fields(c, ir, flag_dens, flag_div);
 }
 
 int avism_run_level(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t dens_interp,

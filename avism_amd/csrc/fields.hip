@@ -1427,6 +1427,9 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
                                            c->pvz.as<float>(), c->mass_fix.as<double>(), nullptr);
     AV_CHECK_HIP(c, hipGetLastError());
   }
+  c->mass_inv_scale = 1.0 / scale;
+  c->mass_fp64 = dens_interp == 1 ? 0 : 1;
+  c->mass_n = n;
   {
     StageTimer t(c, "ddens_finalize");
     k_ddens_finalize<<<grid, kFinB, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 1.0 / scale,

@@ -543,6 +543,7 @@ int refine_load(avism_ctx* c, int32_t atlas, int32_t k) {
   if (!c->scalars.ensure(scalars_bytes(c))) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (scalars)");
   AV_CHECK_HIP(c, hipMemcpy((char*)c->scalars.p + 4096, c->g.rad.data(), c->g.rad.size() * 4, hipMemcpyHostToDevice));
   c->counters[0] = (int64_t)hpairs;
+  c->mass_n = 0;  // the cell masses are the atlas's, not a whole grid's
   c->has_marcap = true;
   c->has_reqp = true;
   c->nreqp = (int32_t)P.reqp.size();

@@ -23,7 +23,8 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-__all__ = ["Particles", "zeldovich", "mock_voids", "grid_fields", "particle_mass", "survey_mask", "patch_batch"]
+__all__ = ["Particles", "zeldovich", "mock_voids", "grid_fields", "particle_mass", "survey_mask", "patch_batch",
+           "amr_gas"]
 
 RHO_CRIT = 2.775e11  # h^2 Msun / Mpc^3
 
@@ -201,3 +202,56 @@ def grid_fields(n: int, L0: float, seed: int, sigma: float = 0.8, kc: float = 0.
     for kj in (kx, ky, kz):
         vel.append(_fortran_flat(torch.fft.irfftn(dl * (1j * f * H0 * kj / k2), s=(n, n, n)).float()))
     return _fortran_flat(delta.float()), vel[0], vel[1], vel[2]
+
+
+def amr_gas(nb: int, L0: float, seed: int, npatch: int = 12, nmin: int = 4, nmax: int = 12):
+    """A MASCLET-like gas snapshot: a log-normal density contrast and velocities
+    (units of c) on the base grid of nb^3 cells, and npatch level-1 patches
+    (twice the resolution) of even sides nmin..nmax fine cells, at least two
+    base cells from the box faces, some overlapping, with their own fields
+    (the base field refined plus small-scale noise).
+
+    Returns (base, patches): base = [delta, vx, vy, vz] float32 arrays of nb^3
+    (x fastest); patches = list of dicts with n (nx, ny, nz), x (PATCHX, PATCHY,
+    PATCHZ: 1-based base cell of the patch's first cells), r (PATCHRX, RY, RZ:
+    the first fine cell's centre plus a quarter cell) and fields [delta, vx,
+    vy, vz] float32 arrays of nx*ny*nz (x fastest), as READ_MASCLET reads them
+    (reader.f90:86-405)."""
+    rng = np.random.default_rng(seed)
+    d, vx, vy, vz = (a.numpy() for a in grid_fields(nb, L0, seed))
+    vx, vy, vz = (v / 299792.458 for v in (vx, vy, vz))
+    base = [np.ascontiguousarray(a, np.float32) for a in (d, vx, vy, vz)]
+    # MALLA of the level-1 grid (2 nb cells): the first fine cell's centre
+    n1 = 2 * nb
+    lado = np.float32(L0) - np.float32(L0) / np.float32(n1)
+    a = -lado / np.float32(2.0)
+    cstep = (lado / np.float32(2.0) - a) / np.float32(n1 - 1)
+    dxl = (a + cstep) - a
+    cube = [b.reshape(nb, nb, nb) for b in base]  # [z, y, x]
+    patches = []
+    for _ in range(npatch):
+        nn = [2 * int(rng.integers(nmin // 2, nmax // 2 + 1)) for _ in range(3)]
+        x0 = [int(rng.integers(2, nb - nn[i] // 2 - 1)) for i in range(3)]  # 1-based base cell
+        fields = []
+        for q in range(4):
+            sl = cube[q][x0[2] - 1:x0[2] - 1 + nn[2] // 2, x0[1] - 1:x0[1] - 1 + nn[1] // 2,
+                         x0[0] - 1:x0[0] - 1 + nn[0] // 2]
+            up = sl.repeat(2, 0).repeat(2, 1).repeat(2, 2)
+            noise = rng.normal(0.0, 0.05 if q == 0 else 2e-5, up.shape)
+            f = (up + noise).astype(np.float32)
+            if q == 0:
+                f = np.maximum(f, np.float32(-0.99))
+            fields.append(np.ascontiguousarray(f).ravel())
+        r = [float(a + np.float32(2 * x0[i] - 2) * dxl + np.float32(0.25) * dxl) for i in range(3)]
+        patches.append(dict(n=tuple(nn), x=tuple(x0), r=tuple(r), fields=fields))
+    return base, patches
+
+
+def amr_pack(patches):
+    """(pint int32[6 np], preal float32[3 np], pdata float32) of a patch list, as
+    avism_amr_set and the oracle take them."""
+    pint = np.array([list(p["n"]) + list(p["x"]) for p in patches], np.int32).ravel()
+    preal = np.array([list(p["r"]) for p in patches], np.float32).ravel()
+    pdata = np.concatenate([f for p in patches for f in p["fields"]]).astype(np.float32) if patches \
+        else np.zeros(1, np.float32)
+    return pint, preal, pdata

@@ -340,6 +340,35 @@ int avism_refine_windows(avism_ctx* ctx, int32_t* tab, int32_t cap, int32_t* n, 
  * last loaded atlas until another one is loaded */
 int avism_refine_pop(avism_ctx* ctx);
 
+/* ---- MASCLET AMR input (FLAG_DATA = 0) ------------------------------------
+ * READ_MASCLET (reader.f90:86-405) reads a base grid of NHYX^3 cells and AMR
+ * patches; the gas enters the void finder through a uniform grid per level
+ * and the dark-matter particles through the SPH path.  avism_amr_set takes
+ * the gas as the files hold it: base[4][nhyx^3] = density contrast, vx, vy,
+ * vz (x fastest); npatch level-1 patches with pint[6*npatch] = PATCHNX,
+ * PATCHNY, PATCHNZ, PATCHX, PATCHY, PATCHZ (1-based base cell of the first
+ * cells), preal[3*npatch] = PATCHRX, PATCHRY, PATCHRZ, and pdata = per patch
+ * density contrast, vx, vy, vz (nx*ny*nz each, x fastest), patches back to
+ * back.  Fails with AVISM_ERR_INPUT for a patch with no cells. */
+int avism_amr_set(avism_ctx* ctx, int32_t nhyx, const float* base, int32_t npatch, const int32_t* pint,
+                  const float* preal, const float* pdata, int32_t where);
+/* U1CO and DIVERCO of level ir on the context's grid (NHYX * 2^ir cells per
+ * axis, avism.f90:750): the gas by SMOOTH_2 (ir < 0, amr2uniform.f90:62-121)
+ * or the base grid (ir = 0) with DIVER_UNIFORM, or DIVER_FINA_GAS and VMESH
+ * (ir = 1, divergence.f90:55-337, amr2uniform.f90:130-248; VMESH averages a
+ * refined cell over 1 + its covering patches, as the reference's FLAGAMR
+ * counts from 1); then U1CO by FLAG_DENS (0: dark matter + gas, 1: dark
+ * matter, 2: gas) times ROTE over its FP64 mean, and DIVERCO by FLAG_DIV (0:
+ * the smaller of gas and dark matter, 1: dark matter, 2: gas, -1: linear
+ * theory), avism.f90:1369-1409, 1633-1649.  The dark matter is that of the
+ * last avism_particles_to_grid on this grid (required unless FLAG_DENS = 2
+ * and FLAG_DIV is 2 or -1); its divergence is 0 at ir = 1, as the reference
+ * computes none there.  ir > 1: AVISM_ERR_UNSUPPORTED (the reference's VMESH
+ * indexes its base-grid flags past level 1); AVISM_ERR_INPUT for a patch
+ * outside the level's grid or whose parent cells touch the base grid's faces
+ * (the reference would read outside its arrays). */
+int avism_amr_fields(avism_ctx* ctx, int32_t ir, int32_t flag_dens, int32_t flag_div);
+
 /* ---- whole level ---------------------------------------------------------- */
 /* particles already loaded: fields, voids, merge and post in one call */
 int avism_run_level(avism_ctx* ctx, int32_t k, int32_t vel_interp, int32_t dens_interp,

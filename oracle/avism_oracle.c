@@ -1615,3 +1615,156 @@ int32_t orc_merge_post(const orc_grid* g, const orc_void_params* p, int32_t nvoi
 #undef UV
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * MASCLET AMR gas onto a level's uniform grid (FLAG_DATA = 0):
+ *   READ_MASCLET adds 1 to the density contrasts (reader.f90:392-398);
+ *   level IR < 0: SMOOTH_2 (amr2uniform.f90:62-121), IR = 0: the base grid
+ *   (avism.f90:1077-1084), then DIVER_UNIFORM (avism.f90:1129-1146);
+ *   IR = 1: DIVER_FINA_GAS (divergence.f90:55-337) and VMESH
+ *   (amr2uniform.f90:130-248), U1GCO = UGR, DIVERGCO = DIVR (avism.f90:1100-1119).
+ * ------------------------------------------------------------------------ */
+int32_t orc_amr_gas(int32_t nb, float L0, int32_t periodic, int32_t ir, const float* base, int32_t npatch,
+                    const int32_t* pint, const float* preal, const float* pdata, float* u1gco, float* divgco) {
+    if (ir < -3 || ir > 1) return -1;
+    const size_t nb3 = (size_t)nb * nb * nb;
+    const int32_t n = ir >= 0 ? nb << ir : nb >> (-ir);
+    const size_t n3 = (size_t)n * n * n;
+    orc_grid* g0 = orc_grid_new(nb, L0, 0);
+    orc_grid* gl = orc_grid_new(n, L0, periodic);
+    const float dx0 = g0->dx, dxl = gl->dx, rx1 = orc_grid_rad(gl, 1);
+    float* U1G = malloc(nb3 * sizeof(float));
+    for (size_t c = 0; c < nb3; c++) U1G[c] = base[c] + 1.0f;
+    const float *U2G = base + nb3, *U3G = base + 2 * nb3, *U4G = base + 3 * nb3;
+#define B0(a, i, j, k) (a)[(size_t)((i) - 1) + (size_t)nb * (((j) - 1) + (size_t)nb * ((k) - 1))]
+    int32_t rc = 0;
+    if (ir <= 0) {
+        float* u = calloc(4 * n3, sizeof(float));
+        if (ir == 0) {
+            memcpy(u, U1G, n3 * sizeof(float));
+            memcpy(u + n3, U2G, 3 * n3 * sizeof(float));
+        } else {  /* SMOOTH_2: base cells summed into the coarse cell holding them, in loop order */
+            const float wei = ((dx0 * dx0) * dx0) / ((dxl * dxl) * dxl);
+            const float* src[4] = {U1G, U2G, U3G, U4G};
+            for (int32_t kk = 1; kk <= nb; kk++)
+                for (int32_t jj = 1; jj <= nb; jj++)
+                    for (int32_t ii = 1; ii <= nb; ii++) {
+                        const int32_t I = (int32_t)(((orc_grid_rad(g0, ii) - rx1) / dxl) + 0.5f) + 1;
+                        const int32_t J = (int32_t)(((orc_grid_rad(g0, jj) - rx1) / dxl) + 0.5f) + 1;
+                        const int32_t K = (int32_t)(((orc_grid_rad(g0, kk) - rx1) / dxl) + 0.5f) + 1;
+                        const size_t c = (size_t)(I - 1) + (size_t)n * ((J - 1) + (size_t)n * (K - 1));
+                        for (int a = 0; a < 4; a++) u[a * n3 + c] = u[a * n3 + c] + B0(src[a], ii, jj, kk) * wei;
+                    }
+        }
+        memcpy(u1gco, u, n3 * sizeof(float));
+        orc_diver_uniform(n, dxl, u + n3, u + 2 * n3, u + 3 * n3, periodic, divgco);
+        free(u);
+    } else {
+        /* DIVER0 on the base grid: interior differences over 2*DX0, edges copied
+         * from the neighbouring plane in x, then y, then z (= the interior cell
+         * at the clamped index) */
+        float* div0 = malloc(nb3 * sizeof(float));
+        for (int32_t k = 1; k <= nb; k++)
+            for (int32_t j = 1; j <= nb; j++)
+                for (int32_t i = 1; i <= nb; i++) {
+                    const int32_t a = i < 2 ? 2 : (i > nb - 1 ? nb - 1 : i);
+                    const int32_t b = j < 2 ? 2 : (j > nb - 1 ? nb - 1 : j);
+                    const int32_t c = k < 2 ? 2 : (k > nb - 1 ? nb - 1 : k);
+                    float b21 = B0(U2G, a + 1, b, c) - B0(U2G, a - 1, b, c);
+                    b21 = b21 / (2.0f * dx0);
+                    float b32 = B0(U3G, a, b + 1, c) - B0(U3G, a, b - 1, c);
+                    b32 = b32 / (2.0f * dx0);
+                    float b43 = B0(U4G, a, b, c + 1) - B0(U4G, a, b, c - 1);
+                    b43 = b43 / (2.0f * dx0);
+                    B0(div0, i, j, k) = (b21 + b32) + b43;
+                }
+        float* ugr = calloc(n3, sizeof(float));
+        float* divr = calloc(n3, sizeof(float));
+        int32_t* flagamr = malloc(n3 * sizeof(int32_t));
+        uint8_t* flagr = calloc(nb3, 1);
+        for (size_t c = 0; c < n3; c++) flagamr[c] = 1;
+        const float dxr = dx0 / 2.0f;
+        size_t off = 0;
+        for (int32_t p = 0; p < npatch && rc == 0; p++) {
+            const int32_t n1 = pint[6 * p], n2 = pint[6 * p + 1], n3p = pint[6 * p + 2];
+            const int32_t L1 = pint[6 * p + 3], L2 = pint[6 * p + 4], L3 = pint[6 * p + 5];
+            const size_t pc = (size_t)n1 * n2 * n3p;
+            const float *P1 = pdata + off, *P2 = P1 + pc, *P3 = P2 + pc, *P4 = P3 + pc;
+            off += 4 * pc;
+#define PA(a, i, j, k) (a)[(size_t)((i) - 1) + (size_t)n1 * (((j) - 1) + (size_t)n2 * ((k) - 1))]
+            const int32_t ixr1 = (int32_t)((((preal[3 * p] - 0.5f * dxl) - rx1) / dxl) + 0.5f);
+            const int32_t jyr1 = (int32_t)((((preal[3 * p + 1] - 0.5f * dxl) - rx1) / dxl) + 0.5f);
+            const int32_t kzr1 = (int32_t)((((preal[3 * p + 2] - 0.5f * dxl) - rx1) / dxl) + 0.5f);
+            for (int32_t kz = 1; kz <= n3p && rc == 0; kz++)
+                for (int32_t jy = 1; jy <= n2 && rc == 0; jy++)
+                    for (int32_t ix = 1; ix <= n1; ix++) {
+                        /* DIVER of the patch cell (divergence.f90:182-223, 228-334) */
+                        const int border = ix == 1 || ix == n1 || jy == 1 || jy == n2 || kz == 1 || kz == n3p;
+                        const int near = ix == 2 || ix == n1 - 1 || jy == 2 || jy == n2 - 1 || kz == 2 || kz == n3p - 1;
+                        float dv = 0.0f, aux = 0.0f;
+                        if (!border) {
+                            float b21 = PA(P2, ix + 1, jy, kz) - PA(P2, ix - 1, jy, kz);
+                            b21 = b21 / (2.0f * dxr);
+                            float b32 = PA(P3, ix, jy + 1, kz) - PA(P3, ix, jy - 1, kz);
+                            b32 = b32 / (2.0f * dxr);
+                            float b43 = PA(P4, ix, jy, kz + 1) - PA(P4, ix, jy, kz - 1);
+                            b43 = b43 / (2.0f * dxr);
+                            dv = (b21 + b32) + b43;
+                        }
+                        if (border || near) {
+                            const int32_t c1 = (ix + 1) / 2 + L1 - 1, c2 = (jy + 1) / 2 + L2 - 1, c3 = (kz + 1) / 2 + L3 - 1;
+                            if (c1 < 2 || c1 > nb - 1 || c2 < 2 || c2 > nb - 1 || c3 < 2 || c3 > nb - 1) {
+                                rc = -2;
+                                break;
+                            }
+                            float b21 = B0(U2G, c1 + 1, c2, c3) - B0(U2G, c1 - 1, c2, c3);
+                            b21 = b21 / (2.0f * dx0);
+                            float b32 = B0(U3G, c1, c2 + 1, c3) - B0(U3G, c1, c2 - 1, c3);
+                            b32 = b32 / (2.0f * dx0);
+                            float b43 = B0(U4G, c1, c2, c3 + 1) - B0(U4G, c1, c2, c3 - 1);
+                            b43 = b43 / (2.0f * dx0);
+                            aux = (b21 + b32) + b43;
+                        }
+                        if (border) dv = aux;
+                        else if (near) dv = 0.5f * (aux + dv);
+                        /* VMESH: sums over the patches, the cell counted from 1 */
+                        const int32_t X = ixr1 + ix, Y = jyr1 + jy, Z = kzr1 + kz;
+                        if (X < 1 || X > n || Y < 1 || Y > n || Z < 1 || Z > n) {
+                            rc = -3;
+                            break;
+                        }
+                        const size_t c = (size_t)(X - 1) + (size_t)n * ((Y - 1) + (size_t)n * (Z - 1));
+                        flagamr[c] += 1;
+                        ugr[c] = ugr[c] + (PA(P1, ix, jy, kz) + 1.0f);
+                        divr[c] = divr[c] + dv;
+                        const int32_t x0 = (int32_t)((float)(X - 1) / 2.0f) + 1, y0 = (int32_t)((float)(Y - 1) / 2.0f) + 1,
+                                      z0 = (int32_t)((float)(Z - 1) / 2.0f) + 1;
+                        B0(flagr, x0, y0, z0) = 1;
+                    }
+#undef PA
+        }
+        if (rc == 0) {
+            for (size_t c = 0; c < n3; c++) {
+                ugr[c] = ugr[c] / (float)flagamr[c];
+                divr[c] = divr[c] / (float)flagamr[c];
+            }
+            for (int32_t k = 1; k <= n; k++)
+                for (int32_t j = 1; j <= n; j++)
+                    for (int32_t i = 1; i <= n; i++) {
+                        const int32_t a = (i - 1) / 2 + 1, b = (j - 1) / 2 + 1, c0 = (k - 1) / 2 + 1;
+                        if (B0(flagr, a, b, c0)) continue;
+                        const size_t c = (size_t)(i - 1) + (size_t)n * ((j - 1) + (size_t)n * (k - 1));
+                        ugr[c] = B0(U1G, a, b, c0);
+                        divr[c] = B0(div0, a, b, c0);
+                    }
+            memcpy(u1gco, ugr, n3 * sizeof(float));
+            memcpy(divgco, divr, n3 * sizeof(float));
+        }
+        free(div0); free(ugr); free(divr); free(flagamr); free(flagr);
+    }
+#undef B0
+    free(U1G);
+    orc_grid_free(g0);
+    orc_grid_free(gl);
+    return rc;
+}

@@ -147,6 +147,16 @@ int32_t orc_merge_post(const orc_grid* g, const orc_void_params* p, int32_t nvoi
 void orc_cosmo(float ache, float omega0, float zeta, float* rote, float* rete,
                float* hte);
 
+/* ---- MASCLET AMR gas (FLAG_DATA = 0) onto the uniform grid of level ir
+ * (-3..1): base[4][nb^3] the base grid's gas density contrast and velocities as
+ * the files hold them; npatch level-1 patches, pint[6*npatch] = PATCHNX, NY, NZ,
+ * PATCHX, Y, Z (1-based base cell of the first cells), preal[3*npatch] =
+ * PATCHRX, RY, RZ, pdata = per patch density contrast, vx, vy, vz (x fastest).
+ * u1gco, divgco: n^3, n = nb * 2^ir.  0, or negative for an unsupported level
+ * or a patch whose stencils leave the grids. */
+int32_t orc_amr_gas(int32_t nb, float L0, int32_t periodic, int32_t ir, const float* base, int32_t npatch,
+                    const int32_t* pint, const float* preal, const float* pdata, float* u1gco, float* divgco);
+
 #ifdef __cplusplus
 }
 #endif

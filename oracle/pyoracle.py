@@ -79,6 +79,9 @@ def lib():
     L.orc_normalise.restype = C.c_float
     L.orc_normalise.argtypes = [vp, C.c_float, C.c_float, C.c_float, _f32p, _f32p, C.POINTER(C.c_double)]
     L.orc_diver_uniform.argtypes = [C.c_int32, C.c_float, _f32p, _f32p, _f32p, C.c_int32, _f32p]
+    L.orc_amr_gas.argtypes = [C.c_int32, C.c_float, C.c_int32, C.c_int32, _f32p, C.c_int32, _i32p, _f32p, _f32p,
+                              _f32p, _f32p]
+    L.orc_amr_gas.restype = C.c_int32
     L.orc_extend_f32.argtypes = [vp, _f32p, _f32p]
     L.orc_extend_i32.argtypes = [vp, _i32p, _i32p]
     L.orc_indexx.restype = C.c_int32
@@ -421,8 +424,78 @@ def next_level_inputs(g: Grid, marca_ext, uvoid, req):
     return np.ascontiguousarray(up, np.int32).ravel(), reqp
 
 
+def amr_gas(nb: int, L0: float, periodic: int, ir: int, base, pint, preal, pdata):
+    """MASCLET AMR gas onto the uniform grid of level ir (SMOOTH_2, DIVER_UNIFORM,
+    DIVER_FINA_GAS, VMESH; orc_amr_gas): (U1GCO, DIVERGCO) of n^3, n = nb 2^ir."""
+    n = nb << ir if ir >= 0 else nb >> (-ir)
+    b = np.ascontiguousarray(np.concatenate([np.ravel(a) for a in base]), np.float32)
+    u = np.zeros(n ** 3, np.float32)
+    d = np.zeros(n ** 3, np.float32)
+    rc = lib().orc_amr_gas(nb, float(L0), int(periodic), int(ir), b, len(pint) // 6,
+                           np.ascontiguousarray(pint, np.int32), np.ascontiguousarray(preal, np.float32),
+                           np.ascontiguousarray(pdata, np.float32), u, d)
+    if rc != 0:
+        raise RuntimeError(f"orc_amr_gas failed: {rc}")
+    return u, d
+
+
 # ----------------------------------------------------------------- reference
 REF_DIR = HERE / "_ref"
+
+
+def run_reference_amr(jobdir: Path, nb: int, L0: float, periodic: int, ir: int, base, pint, preal, pdata,
+                      timeout=300):
+    """The reference's SMOOTH_2 / DIVER_UNIFORM or DIVER_FINA_GAS + VMESH run by
+    the harness (mode 4) on MASCLET-like gas: (U1GCO, DIVERGCO)."""
+    jobdir = Path(jobdir)
+    jobdir.mkdir(parents=True, exist_ok=True)
+    npa = len(pint) // 6
+    with open(jobdir / "amr_base.bin", "wb") as fh:
+        for a in base:
+            fh.write(np.ascontiguousarray(a, np.float32).tobytes())
+    (jobdir / "amr_pint.bin").write_bytes(np.ascontiguousarray(pint, np.int32).tobytes())
+    (jobdir / "amr_preal.bin").write_bytes(np.ascontiguousarray(preal, np.float32).tobytes())
+    (jobdir / "amr_patch.bin").write_bytes(np.ascontiguousarray(pdata, np.float32)[:max(len(pdata), 0)].tobytes())
+    (jobdir / "amr.in").write_text(f"{ir} {npa}\n")
+    (jobdir / "job.in").write_text(f"4\n{nb} {L0!r} {periodic}\n-0.6 10.0 0.25 0.0 1.0 1.0 100\n"
+                                   f"0.678 0.31 0.0\n32 1 1 0\n")
+    exe = REF_DIR / f"ref_harness_p{int(periodic)}.x"
+    r = subprocess.run([str(exe), str(jobdir)], capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference AMR harness failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+    n = nb << ir if ir >= 0 else nb >> (-ir)
+    out = np.fromfile(jobdir / "out_amr.bin", np.float32)
+    return out[: n ** 3].copy(), out[n ** 3: 2 * n ** 3].copy()
+
+
+def run_reference_read_masclet(jobdir: Path, nb: int, it: int, npalev: int, nlevels: int, npmax: int, timeout=300):
+    """The reference's READ_MASCLET (reader.f90:86-405) on <jobdir>/simu_masclet/
+    files of iteration it: dict(ndxyz0, nl, npartt, gas = [U1G, U2G, U3G, U4G],
+    patches = [(ints6, reals3, [U11G, U12G, U13G, U14G])], particles = 7 arrays)."""
+    jobdir = Path(jobdir)
+    (jobdir / "amr.in").write_text(f"{it} {npalev} {nlevels}\n")
+    (jobdir / "job.in").write_text(f"5\n{nb} 1.0 0\n-0.6 10.0 0.25 0.0 1.0 1.0 100\n0.678 0.31 0.0\n32 1 1 {npmax}\n")
+    exe = REF_DIR / "ref_harness_p0.x"
+    r = subprocess.run([str(exe), str(jobdir.resolve())], capture_output=True, text=True, timeout=timeout,
+                       cwd=str(jobdir))
+    if r.returncode != 0:
+        raise RuntimeError(f"reference READ_MASCLET failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+    head = np.fromfile(jobdir / "out_masclet_head.bin", np.int32)
+    ndxyz0, nl, npartt, npatch = (int(v) for v in head[:4])
+    n3 = nb ** 3
+    gas = np.fromfile(jobdir / "out_masclet_gas.bin", np.float32)
+    raw = (jobdir / "out_masclet_patch.bin").read_bytes()
+    patches, off = [], 0
+    for _ in range(npatch):
+        ints = np.frombuffer(raw, np.int32, 6, off)
+        reals = np.frombuffer(raw, np.float32, 3, off + 24)
+        m = int(ints[0]) * int(ints[1]) * int(ints[2])
+        f = np.frombuffer(raw, np.float32, 4 * m, off + 36)
+        patches.append((ints.copy(), reals.copy(), [f[q * m:(q + 1) * m].copy() for q in range(4)]))
+        off += 36 + 16 * m
+    part = np.fromfile(jobdir / "out_masclet_part.bin", np.float32)
+    return dict(ndxyz0=ndxyz0, nl=nl, npartt=npartt, gas=[gas[q * n3:(q + 1) * n3] for q in range(4)],
+                patches=patches, particles=[part[q * npartt:(q + 1) * npartt] for q in range(7)])
 
 
 def ref_available(periodic: int) -> bool:

@@ -68,6 +68,16 @@ PROGRAM REF_HARNESS
       CALL RUN_INDEXX(NP4)
       STOP
    ENDIF
+   IF (MODE .EQ. 5) THEN
+      ! READ_MASCLET (reader.f90:86-405) on simu_masclet/ of the working directory
+      CALL RUN_READ_MASCLET(N, NP4)
+      STOP
+   ENDIF
+   IF (MODE .EQ. 4) THEN
+      ! MASCLET AMR gas onto a level's uniform grid (SMOOTH_2, DIVER_FINA_GAS, VMESH)
+      CALL RUN_AMR(N)
+      STOP
+   ENDIF
 
    PI = DACOS(-1.D0)
    ACHE0 = ACHE*3.66D-3
@@ -409,6 +419,144 @@ CONTAINS
       WRITE(3) IDX
       CLOSE(3)
    END SUBROUTINE RUN_INDEXX
+
+   ! MASCLET gas of a base grid of NB^3 cells and its level-1 patches onto the
+   ! uniform grid of level IR (amr.in: IR, NPATCH), as READ_MASCLET
+   ! (reader.f90:86-405) leaves the arrays and avism.f90:366-382, 750, 1064-1146
+   ! use them: IR < 0 SMOOTH_2, IR = 0 the base grid, then DIVER_UNIFORM;
+   ! IR >= 1 DIVER_FINA_GAS and VMESH.
+   !   amr_base.bin    delta, vx, vy, vz  float32 [NB^3] each
+   !   amr_pint.bin    int32 [6, NPATCH]: PATCHNX, PATCHNY, PATCHNZ, PATCHX, PATCHY, PATCHZ
+   !   amr_preal.bin   float32 [3, NPATCH]: PATCHRX, PATCHRY, PATCHRZ
+   !   amr_patch.bin   per patch delta, vx, vy, vz  float32 [nx*ny*nz] each
+   ! Output out_amr.bin: U1GCO, DIVERGCO  float32 [NXX^3] each.
+   SUBROUTINE RUN_AMR(NB)
+      INTEGER :: NB
+      INTEGER :: IRL, NPA, NXL, IP, N1, N2, N3, D1, D2, D3, D4
+      INTEGER, ALLOCATABLE :: PINT(:,:)
+      REAL*4, ALLOCATABLE :: PREAL(:,:), DIVB(:,:,:)
+      REAL*4 :: LADOB, DXL, RXL1
+      OPEN(1, FILE=TRIM(JOB)//'/amr.in', STATUS='OLD', ACTION='READ')
+      READ(1,*) IRL, NPA
+      CLOSE(1)
+      NHYX = NB; NHYY = NB; NHYZ = NB
+      NL2 = 1; NLEVELS = 1; NPALEV = MAX(NPA, 1)
+      ALLOCATE(NPATCH(0:NLEVELS), PARE(NPALEV))
+      ALLOCATE(PATCHNX(NPALEV), PATCHNY(NPALEV), PATCHNZ(NPALEV))
+      ALLOCATE(PATCHX(NPALEV), PATCHY(NPALEV), PATCHZ(NPALEV))
+      ALLOCATE(PATCHRX(NPALEV), PATCHRY(NPALEV), PATCHRZ(NPALEV))
+      NPATCH = 0; NPATCH(1) = NPA; PARE = 0
+      PATCHNX = 0; PATCHNY = 0; PATCHNZ = 0; PATCHX = 0; PATCHY = 0; PATCHZ = 0
+      PATCHRX = 0.; PATCHRY = 0.; PATCHRZ = 0.
+      IF (NPA .GT. 0) THEN
+         ALLOCATE(PINT(6, NPA), PREAL(3, NPA))
+         OPEN(2, FILE=TRIM(JOB)//'/amr_pint.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+         READ(2) PINT
+         CLOSE(2)
+         OPEN(2, FILE=TRIM(JOB)//'/amr_preal.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+         READ(2) PREAL
+         CLOSE(2)
+         PATCHNX(1:NPA) = PINT(1,:); PATCHNY(1:NPA) = PINT(2,:); PATCHNZ(1:NPA) = PINT(3,:)
+         PATCHX(1:NPA) = PINT(4,:); PATCHY(1:NPA) = PINT(5,:); PATCHZ(1:NPA) = PINT(6,:)
+         PATCHRX(1:NPA) = PREAL(1,:); PATCHRY(1:NPA) = PREAL(2,:); PATCHRZ(1:NPA) = PREAL(3,:)
+      ENDIF
+      ! the base grid (avism.f90:366-382)
+      LADOB = LADO0-(LADO0/NB)
+      ALLOCATE(RADX(0:NB+1), RADY(0:NB+1), RADZ(0:NB+1))
+      CALL MALLA(NB,NB,NB,LADOB)
+      ALLOCATE(RADX0(0:NB+1), RADY0(0:NB+1), RADZ0(0:NB+1))
+      RADX0 = RADX; RADY0 = RADY; RADZ0 = RADZ
+      DX0 = DX; DY0 = DY; DZ0 = DZ
+      DEALLOCATE(RADX, RADY, RADZ)
+      ! the gas as READ_MASCLET leaves it: density contrast + 1 (reader.f90:392-398)
+      ALLOCATE(U1G(NB,NB,NB), U2G(NB,NB,NB), U3G(NB,NB,NB), U4G(NB,NB,NB), U1S(NB,NB,NB), U1DM(NB,NB,NB))
+      OPEN(2, FILE=TRIM(JOB)//'/amr_base.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+      READ(2) U1G, U2G, U3G, U4G
+      CLOSE(2)
+      U1S = -1.; U1DM = -1.
+      U1G = U1G + 1.; U1S = U1S + 1.; U1DM = U1DM + 1.
+      D1 = MAX(MAXVAL(PATCHNX), 1); D2 = MAX(MAXVAL(PATCHNY), 1); D3 = MAX(MAXVAL(PATCHNZ), 1)
+      D4 = MAX(SUM(NPATCH(0:NL2)), 1)
+      ALLOCATE(U11G(D1,D2,D3,D4), U12G(D1,D2,D3,D4), U13G(D1,D2,D3,D4), U14G(D1,D2,D3,D4))
+      ALLOCATE(U11DM(D1,D2,D3,D4), U11S(D1,D2,D3,D4))
+      U11G = -1.; U11DM = -1.; U11S = -1.
+      U12G = 0.; U13G = 0.; U14G = 0.
+      IF (NPA .GT. 0) THEN
+         OPEN(2, FILE=TRIM(JOB)//'/amr_patch.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+         DO IP = 1, NPA
+            N1 = PATCHNX(IP); N2 = PATCHNY(IP); N3 = PATCHNZ(IP)
+            READ(2) U11G(1:N1,1:N2,1:N3,IP)
+            READ(2) U12G(1:N1,1:N2,1:N3,IP)
+            READ(2) U13G(1:N1,1:N2,1:N3,IP)
+            READ(2) U14G(1:N1,1:N2,1:N3,IP)
+         ENDDO
+         CLOSE(2)
+      ENDIF
+      U11G = U11G + 1.; U11DM = U11DM + 1.; U11S = U11S + 1.
+      ! the level's grid (avism.f90:750, 776-830)
+      NXL = INT(REAL(NB)*(2.**(IRL)))
+      LADOB = LADO0-(LADO0/NXL)
+      ALLOCATE(RADX(0:NXL+1), RADY(0:NXL+1), RADZ(0:NXL+1))
+      CALL MALLA(NXL,NXL,NXL,LADOB)
+      DXL = DX
+      RXL1 = RADX(1)
+      ALLOCATE(U1GCO(NXL,NXL,NXL), U2GCO(NXL,NXL,NXL), U3GCO(NXL,NXL,NXL), U4GCO(NXL,NXL,NXL))
+      ALLOCATE(U1SCO(NXL,NXL,NXL), U1DMCO(NXL,NXL,NXL), DIVERGCO(NXL,NXL,NXL), DIVERDMCO(NXL,NXL,NXL))
+      ALLOCATE(DIVB(NXL,NXL,NXL))
+      U1GCO = 0.; U2GCO = 0.; U3GCO = 0.; U4GCO = 0.; U1SCO = 0.; U1DMCO = 0.; DIVERGCO = 0.; DIVERDMCO = 0.
+      IF (IRL .LT. 0) CALL SMOOTH_2(NXL,NXL,NXL)
+      IF (IRL .EQ. 0) THEN
+         U1GCO = U1G; U2GCO = U2G; U3GCO = U3G; U4GCO = U4G
+      ENDIF
+      IF (IRL .LE. 0) THEN
+         CALL DIVER_UNIFORM(NXL,NXL,NXL,DXL,DXL,DXL,U2GCO,U3GCO,U4GCO,DIVB,FLAG_PERIOD)
+         DIVERGCO = DIVB
+      ELSE
+         CALL DIVER_FINA_GAS(IRL)
+         CALL VMESH(IRL, NXL, NXL, NXL, DXL, DXL, DXL, RXL1, RXL1, RXL1, NB, NB, NB)
+         U1GCO = UGR
+         DIVERGCO = DIVR
+      ENDIF
+      OPEN(3, FILE=TRIM(JOB)//'/out_amr.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+      WRITE(3) U1GCO, DIVERGCO
+      CLOSE(3)
+   END SUBROUTINE RUN_AMR
+
+   ! READ_MASCLET of iteration ITER (amr.in: ITER, NPALEV, NLEVELS) with a base
+   ! grid of NB^3 cells and room for NPMAX particles; NL2 = 1.  Writes
+   ! out_masclet_head.bin (NDXYZ0, NL, NPARTT, NPATCH(1)), out_masclet_gas.bin
+   ! (U1G, U2G, U3G, U4G as the reader leaves them), out_masclet_patch.bin (per
+   ! level-1 patch PATCHNX..PATCHZ, PATCHRX..PATCHRZ, U11G..U14G) and
+   ! out_masclet_part.bin (RXPA, RYPA, RZPA, U2PA, U3PA, U4PA, MASAP).
+   SUBROUTINE RUN_READ_MASCLET(NB, NPMAX)
+      INTEGER :: NB, NPMAX
+      INTEGER :: ITER, NDXYZ0, NLR, IP, N1, N2, N3
+      REAL*4 :: T, ZETAR
+      OPEN(1, FILE=TRIM(JOB)//'/amr.in', STATUS='OLD', ACTION='READ')
+      READ(1,*) ITER, NPALEV, NLEVELS
+      CLOSE(1)
+      NHYX = NB; NHYY = NB; NHYZ = NB
+      PARTIRED = NPMAX
+      NL2 = 1
+      CALL READ_MASCLET(ITER, NDXYZ0, NLR, T, ZETAR)
+      OPEN(3, FILE=TRIM(JOB)//'/out_masclet_head.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+      WRITE(3) NDXYZ0, NLR, INT(NPARTT, KIND=4), NPATCH(1)
+      CLOSE(3)
+      OPEN(3, FILE=TRIM(JOB)//'/out_masclet_gas.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+      WRITE(3) U1G, U2G, U3G, U4G
+      CLOSE(3)
+      OPEN(3, FILE=TRIM(JOB)//'/out_masclet_patch.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+      DO IP = 1, NPATCH(1)
+         N1 = PATCHNX(IP); N2 = PATCHNY(IP); N3 = PATCHNZ(IP)
+         WRITE(3) N1, N2, N3, PATCHX(IP), PATCHY(IP), PATCHZ(IP), PATCHRX(IP), PATCHRY(IP), PATCHRZ(IP)
+         WRITE(3) U11G(1:N1,1:N2,1:N3,IP), U12G(1:N1,1:N2,1:N3,IP), U13G(1:N1,1:N2,1:N3,IP), U14G(1:N1,1:N2,1:N3,IP)
+      ENDDO
+      CLOSE(3)
+      OPEN(3, FILE=TRIM(JOB)//'/out_masclet_part.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+      WRITE(3) RXPA(1:NPARTT), RYPA(1:NPARTT), RZPA(1:NPARTT), U2PA(1:NPARTT), U3PA(1:NPARTT), U4PA(1:NPARTT), &
+               MASAP(1:NPARTT)
+      CLOSE(3)
+   END SUBROUTINE RUN_READ_MASCLET
 
    SUBROUTINE DROP_VOID(ID)
       INTEGER :: ID

@@ -30,7 +30,13 @@ Each fixture holds the inputs and the reference's outputs for one level:
                     H_DISTANCE skip masked cells, MARK_ALL and the flag stencil
                     read it (voidfinding.f90:33, 283-300), DDENS ignores it
 
-`python tests/golden/make_golden.py tsc` (or `lev`, `mask`) writes only those fixtures.
+  amr_{p0,p1}_ir{m1,0,1}
+                  : MASCLET-like AMR gas (base grid and level-1 patches, some
+                    overlapping) onto the uniform grid of level IR: SMOOTH_2 or
+                    the base grid, then DIVER_UNIFORM (IR <= 0); DIVER_FINA_GAS
+                    and VMESH (IR = 1) -> U1GCO and DIVERGCO
+
+`python tests/golden/make_golden.py tsc` (or `lev`, `mask`, `amr`) writes only those fixtures.
 """
 from __future__ import annotations
 
@@ -183,6 +189,19 @@ def survey_fields_fixtures(td: Path):
                                       dens_interp=di))
 
 
+AMR_CASES = [("amr_p0_ir1", 24, 48.0, 0, 1, 31), ("amr_p1_ir1", 24, 48.0, 1, 1, 32),
+             ("amr_p1_ir0", 24, 48.0, 1, 0, 33), ("amr_p0_irm1", 32, 64.0, 0, -1, 34)]
+
+
+def amr_fixtures(td: Path):
+    for name, nb, L0, per, ir, seed in AMR_CASES:
+        base, patches = synth.amr_gas(nb, L0, seed)
+        pint, preal, pdata = synth.amr_pack(patches)
+        u, d = O.run_reference_amr(td / name, nb, L0, per, ir, base, pint, preal, pdata)
+        _save(name, dict(delta=base[0], vx=base[1], vy=base[2], vz=base[3], pint=pint, preal=preal, pdata=pdata),
+              dict(u1gco=u, divgco=d), dict(nb=nb, L0=L0, periodic=per, ir=ir, seed=seed))
+
+
 def main():
     for per in (0, 1):
         if not O.ref_available(per):
@@ -207,12 +226,16 @@ def main():
         if sys.argv[1:] == ["survey"]:
             survey_fields_fixtures(td)
             return
+        if sys.argv[1:] == ["amr"]:
+            amr_fixtures(td)
+            return
         tsc_fixtures(td)
         lev_fixtures(td)
         mask_fixtures(td)
         kbig_fixtures(td)
         novel_fixtures(td)
         survey_fields_fixtures(td)
+        amr_fixtures(td)
         for per, seed in ((0, 11), (1, 12)):
             n, L0 = 24, 48.0
             d, vx, vy, vz = (a.numpy() for a in synth.grid_fields(n, L0, seed))
