@@ -55,10 +55,11 @@ struct MarkParams {
 // walks kMarkZ planes of it in z, kMarkP planes per step: the planes' U1CO
 // (halo 3) and DIVERCO (halo 1) are staged in LDS for the x and y stencils,
 // and each cell's z line (U1CO at z-3..z+3, DIVERCO at z-1..z+1) slides
-// through registers.  Ext cells read the inner fields through the periodic
+// through registers.  The next step's planes are loaded into registers while
+// the current step is computed from LDS.  Ext cells read the inner fields through the periodic
 // wrap; halo cells outside LOW1..LOW2 are clamped into it (the stencil never
 // uses them).
-constexpr int kMarkX = 64, kMarkY = 8, kMarkZ = 32, kMarkP = 4;
+constexpr int kMarkX = 64, kMarkY = 8, kMarkZ = 32, kMarkP = 2;
 constexpr int kMarkT = kMarkX * kMarkY;                          // threads per block
 constexpr int kMarkU = (kMarkY + 6) * (kMarkX + 6);              // U1CO tile entries per plane
 constexpr int kMarkD = (kMarkY + 2) * (kMarkX + 2);              // DIVERCO tile entries per plane
@@ -157,20 +158,53 @@ __global__ void __launch_bounds__(kMarkT) k_mark(ExtGeom G, MarkParams P, const 
   for (int o = 0; o < kMarkP + 6; ++o) zu[o] = u1co[cxy + plane(z0 + L1 + o - 3)];
 #pragma unroll
   for (int o = 0; o < kMarkP + 2; ++o) zd[o] = div[cxy + plane(z0 + L1 + o - 1)];
-  for (int z = z0; z < z1; z += kMarkP) {
+  // the step's planes go through registers: the next step's global loads are
+  // issued before the current step is computed, so their latency is hidden
+  float pu[kMarkP][kMarkUP], pd[kMarkP][kMarkDP];
+  int8_t pm[MASK ? kMarkP : 1][MASK ? kMarkDP : 1];
+  float nzu[kMarkP], nzd[kMarkP];
+  int8_t nzm[MASK ? kMarkP : 1];
+  auto fetch = [&](int z) {
 #pragma unroll
     for (int pl = 0; pl < kMarkP; ++pl) {
       const size_t pk = plane(z + pl + L1);
 #pragma unroll
       for (int m = 0; m < kMarkUP; ++m) {
         const int q = threadIdx.x + m * kMarkT;
-        if (q < kMarkU) (&su[pl][0][0])[q] = u1co[uo[m] + pk];
+        pu[pl][m] = q < kMarkU ? u1co[uo[m] + pk] : 0.0f;
       }
 #pragma unroll
       for (int m = 0; m < kMarkDP; ++m) {
         const int q = threadIdx.x + m * kMarkT;
-        if (q < kMarkD) (&sd[pl][0][0])[q] = div[dof[m] + pk];
-        if (MASK && q < kMarkD) (&sm[pl][0][0])[q] = mask_at(mo[m], z + pl + L1);
+        pd[pl][m] = q < kMarkD ? div[dof[m] + pk] : 0.0f;
+        if (MASK) pm[pl][m] = q < kMarkD ? mask_at(mo[m], z + pl + L1) : (int8_t)0;
+      }
+    }
+  };
+  fetch(z0);
+  for (int z = z0; z < z1; z += kMarkP) {
+#pragma unroll
+    for (int pl = 0; pl < kMarkP; ++pl) {
+#pragma unroll
+      for (int m = 0; m < kMarkUP; ++m) {
+        const int q = threadIdx.x + m * kMarkT;
+        if (q < kMarkU) (&su[pl][0][0])[q] = pu[pl][m];
+      }
+#pragma unroll
+      for (int m = 0; m < kMarkDP; ++m) {
+        const int q = threadIdx.x + m * kMarkT;
+        if (q < kMarkD) (&sd[pl][0][0])[q] = pd[pl][m];
+        if (MASK && q < kMarkD) (&sm[pl][0][0])[q] = pm[pl][m];
+      }
+    }
+    const bool more = z + kMarkP < z1;
+    if (more) {  // the next step's planes and z line, in flight while this step is computed
+      fetch(z + kMarkP);
+#pragma unroll
+      for (int o = 0; o < kMarkP; ++o) {
+        nzu[o] = u1co[cxy + plane(z + kMarkP + L1 + o + 3)];
+        nzd[o] = div[cxy + plane(z + kMarkP + L1 + o + 1)];
+        if (MASK) nzm[o] = mask_at(mxy, z + kMarkP + L1 + o + 1);
       }
     }
     __syncthreads();
@@ -215,20 +249,20 @@ __global__ void __launch_bounds__(kMarkT) k_mark(ExtGeom G, MarkParams P, const 
       }
     }
     // slide the z line kMarkP planes on
-    if (z + kMarkP < z1) {
+    if (more) {
 #pragma unroll
       for (int o = 0; o < 6; ++o) zu[o] = zu[o + kMarkP];
 #pragma unroll
-      for (int o = 6; o < kMarkP + 6; ++o) zu[o] = u1co[cxy + plane(z + kMarkP + L1 + o - 3)];
+      for (int o = 6; o < kMarkP + 6; ++o) zu[o] = nzu[o - 6];
 #pragma unroll
       for (int o = 0; o < 2; ++o) zd[o] = zd[o + kMarkP];
 #pragma unroll
-      for (int o = 2; o < kMarkP + 2; ++o) zd[o] = div[cxy + plane(z + kMarkP + L1 + o - 1)];
+      for (int o = 2; o < kMarkP + 2; ++o) zd[o] = nzd[o - 2];
       if (MASK) {
 #pragma unroll
         for (int o = 0; o < 2; ++o) zm[o] = zm[o + kMarkP];
 #pragma unroll
-        for (int o = 2; o < kMarkP + 2; ++o) zm[o] = mask_at(mxy, z + kMarkP + L1 + o - 1);
+        for (int o = 2; o < kMarkP + 2; ++o) zm[o] = nzm[o - 2];
       }
     }
     __syncthreads();
