@@ -252,7 +252,12 @@ struct VvelOut {
   int want_vel;
 };
 
-template <int PER, int K>
+// CHK: read HPART before the atomic and skip it when the value is not larger.
+// For grids finer than the particles, where each particle serves many cells;
+// a stale read is never above the true maximum, so no update is lost.
+inline bool vvel_chk(int64_t ncell, int64_t np) { return ncell >= 4 * np; }
+
+template <int PER, int K, bool CHK = false>
 __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float r0, float dx, float qx, float qy,
                                          float qz, size_t c, uint32_t* keys, uint32_t* gmx, const VvelOut& O) {
   Sel<kq_of(K), K / 4> h;
@@ -266,7 +271,8 @@ __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float
     // not wait for it (reading HPART first to skip it costs a load latency
     // per neighbour, which measured slower).
     const float4 vv = O.want_vel ? O.vel[j] : make_float4(0.f, 0.f, 0.f, 0.f);  // vx, vy, vz, m/rho
-    atomicMax(&O.hbits[j], __float_as_uint(d));
+    const uint32_t hb = __float_as_uint(d);
+    if (!CHK || hb > O.hbits[j]) atomicMax(&O.hbits[j], hb);
     if (O.want_vel) {
       const float qq = d / (hh / 2.0f);                     // particles.f90:941
       const float wgt = m4(qq) * vv.w;                      // 897-900
@@ -296,7 +302,7 @@ __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float
   }
 }
 
-template <int K>
+template <int K, bool CHK>
 __global__ void __launch_bounds__(kq_of(K)) k_vvel(Bins B, int64_t np, int n, const float* __restrict__ rad,
                                                   int rlo, float dx, int k, float r0,
                                                   const int8_t* __restrict__ smask, VvelOut O) {
@@ -312,9 +318,9 @@ __global__ void __launch_bounds__(kq_of(K)) k_vvel(Bins B, int64_t np, int n, co
   if (smask && smask[c] == 0) return;
   const float qx = rad[ix + 1 - rlo], qy = rad[iy + 1 - rlo], qz = rad[iz + 1 - rlo];
   if (B.periodic)
-    vvel_one<1, K>(B, np, k, r0, dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
+    vvel_one<1, K, CHK>(B, np, k, r0, dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
   else
-    vvel_one<0, K>(B, np, k, r0, dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
+    vvel_one<0, K, CHK>(B, np, k, r0, dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
 }
 
 // the velocity record of every particle, bin-sorted: (vx, vy, vz, m/rho), m/rho
@@ -1350,7 +1356,12 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
       }
       with_k(k, [&](auto KC) {
         constexpr int K = decltype(KC)::value;
-        k_vvel<K><<<vvel_tiles(n, K), kq_of(K), 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
+        if (vvel_chk(ncell, np))
+          k_vvel<K, true><<<vvel_tiles(n, K), kq_of(K), 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0,
+                                                                        smask, O);
+        else
+          k_vvel<K, false><<<vvel_tiles(n, K), kq_of(K), 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0,
+                                                                         smask, O);
       });
       c->counters[2] = smask ? c->mask_observed : ncell;
       AV_CHECK_HIP(c, hipGetLastError());
@@ -1384,7 +1395,10 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     const VvelOut O{nullptr, c->hpart.as<uint32_t>(), nullptr, nullptr, nullptr, 0};
     with_k(k, [&](auto KC) {
       constexpr int K = decltype(KC)::value, BQ = kq_of(K);
-      k_vvel<K><<<vvel_tiles(n, K), BQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
+      if (vvel_chk(ncell, np))
+        k_vvel<K, true><<<vvel_tiles(n, K), BQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
+      else
+        k_vvel<K, false><<<vvel_tiles(n, K), BQ, 0, c->stream>>>(B, np, n, rad, c->g.rlo, c->g.dx, k, r0, smask, O);
       if (c->g.periodic)
         k_fill_h_knn<1, K><<<nblk(np, BQ), BQ, 0, c->stream>>>(B, np, k, r0, c->hpart.as<float>());
       else
@@ -1707,7 +1721,7 @@ int64_t win_ddens_items(const WinDev& wd) {
 int64_t win_cell_items(int32_t w) { return ((int64_t)w * w * w + 255) / 256; }
 
 // VVEL_INTERP_SPH_VW on the windows: tiles of 8 x 4 x (lanes / 32) window cells
-template <int K>
+template <int K, bool CHK>
 __global__ void __launch_bounds__(kq_of(K)) k_win_vvel(Bins B, int64_t np, const WinDev* __restrict__ W,
                                                       const int64_t* __restrict__ off, int nwin, VirtG V, int k,
                                                       float r0, VvelOut O) {
@@ -1732,9 +1746,9 @@ __global__ void __launch_bounds__(kq_of(K)) k_win_vvel(Bins B, int64_t np, const
   const size_t c =
       (size_t)(wd.ax + lx) + (size_t)V.A * ((size_t)(wd.ay + ly) + (size_t)V.A * (size_t)(wd.az + lz));
   if (B.periodic)
-    vvel_one<1, K>(B, np, k, r0, V.dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
+    vvel_one<1, K, CHK>(B, np, k, r0, V.dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
   else
-    vvel_one<0, K>(B, np, k, r0, V.dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
+    vvel_one<0, K, CHK>(B, np, k, r0, V.dx, qx, qy, qz, c, keys_s + tid, gmx_s + tid, O);
 }
 
 // cell c (1-based) of the level's grid -> its index in the window, or -1
@@ -1946,9 +1960,13 @@ hipError_t launch_win_vvel(hipStream_t st, const Bins& B, int64_t np, const WinD
                            uint32_t* hbits, float* u, float* v, float* w) {
   if (items <= 0) return hipSuccess;
   const VvelOut O{vel, hbits, u, v, w, vel ? 1 : 0};
+  const bool chk = vvel_chk((int64_t)V.n * V.n * V.n, np);
   with_k(k, [&](auto KC) {
     constexpr int K = decltype(KC)::value;
-    k_win_vvel<K><<<(unsigned)items, kq_of(K), 0, st>>>(B, np, W, off, nwin, V, k, r0, O);
+    if (chk)
+      k_win_vvel<K, true><<<(unsigned)items, kq_of(K), 0, st>>>(B, np, W, off, nwin, V, k, r0, O);
+    else
+      k_win_vvel<K, false><<<(unsigned)items, kq_of(K), 0, st>>>(B, np, W, off, nwin, V, k, r0, O);
   });
   return hipGetLastError();
 }
