@@ -18,6 +18,7 @@
 //  * DDENS deposits go into int64 fixed-point cell sums through an LDS copy of
 //    each block's tile (integer atomics: the sums do not depend on the order).
 //    HPART uses integer atomicMax on the float bits (order-free).
+#include <climits>
 #include <cstring>
 #include <algorithm>
 #include <cmath>
@@ -1424,7 +1425,34 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     k_fill_u64<<<nblk(ncell), 256, 0, c->stream>>>(ncell, c->mass_fix.as<unsigned long long>(), 0ull);
     const unsigned ntiles = (unsigned)(((n + DTX - 1) / DTX) * ((n + DTY - 1) / DTY) * ((n + DTZ - 1) / DTZ));
     AV_CHECK_HIP(c, hipMemsetAsync(dpairs, 0, 8, c->stream));
-    if (c->nb > n)  // cells split into bins
+    if (vvel_chk(ncell, np) && c->nb <= n) {
+      // a grid finer than the particles: the window path with one window, the
+      // whole grid (NORM per particle, then the deposits tile by tile in LDS)
+      uint32_t* dmax = (uint32_t*)((char*)c->scalars.p + 1792);
+      AV_CHECK_HIP(c, launch_fill_u32(c->stream, 1, dmax, 0u));
+      AV_CHECK_HIP(c, launch_max_bits(c->stream, np, c->hpart.as<uint32_t>(), dmax));
+      uint32_t hbits = 0;
+      AV_CHECK_HIP(c, hipMemcpyAsync(&hbits, dmax, 4, hipMemcpyDeviceToHost, c->stream));
+      AV_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+      float hmax = 0.0f;
+      memcpy(&hmax, &hbits, 4);
+      struct {
+        WinDev w;
+        int64_t off[4];
+      } h{};
+      h.w.w = n;
+      h.w.bnx = h.w.bny = h.w.bnz = c->nb;
+      WinDev* dw = (WinDev*)((char*)c->scalars.p + 1536);
+      const int64_t items = win_ddens_items(h.w), dep = win_dep_items(n);
+      h.off[1] = items;
+      h.off[3] = dep;
+      AV_CHECK_HIP(c, hipMemcpy(dw, &h, sizeof(h), hipMemcpyHostToDevice));
+      const int64_t* doff = (const int64_t*)(dw + 1);
+      const VirtG V{rad, c->g.rlo, n, c->g.periodic, c->g.dx, c->g.rx1, n};
+      AV_CHECK_HIP(c, launch_win_ddens(c->stream, V, B, np, c->hpart.as<float>(), scale,
+                                       c->mass_fix.as<unsigned long long>(), dw, doff, 1, items, dpairs,
+                                       c->norm.as<float>(), hmax, doff + 2, dep));
+    } else if (c->nb > n)  // cells split into bins
       k_ddens_scatter<4><<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
                                                         c->mass_fix.as<unsigned long long>(), c->norm.as<float>(),
                                                         dpairs);
@@ -1765,16 +1793,17 @@ __device__ __forceinline__ bool win_hits(int lo, int hi, int o, int w, int n, in
   return l0 < w || l0 + (hi - lo + 1) > n;
 }
 
-// DDENS_INTERP_SPH into the windows.  A block takes the particles of 4^3
-// bins of a window's particle range (the bins within the largest DDENS reach
-// of it); a particle whose cell box (or, with NORM = 0, home cell) meets the
-// window computes its NORM over the whole box on the level's grid, exactly as
-// ddens_tile does, then adds the deposits that fall in the window into the
-// atlas's int64 fixed-point cell sums.
-__global__ void __launch_bounds__(256) k_win_ddens(VirtG V, Bins B, const float* __restrict__ hpart, double scale,
-                                                  unsigned long long* acc, const WinDev* __restrict__ W,
-                                                  const int64_t* __restrict__ off, int nwin,
-                                                  unsigned long long* npairs) {
+// DDENS_INTERP_SPH into the windows, first part: NORM.  A block takes the
+// particles of 4^3 bins of a window's particle range (the bins within the
+// largest DDENS reach of it); a particle whose cell box (or, with NORM = 0,
+// home cell) meets the window computes its NORM over the whole box on the
+// level's grid, exactly as ddens_tile does, into norm[i] (the same value from
+// every window that holds it), and a particle with NORM = 0 adds its mass to
+// its home cell if the window holds it.  k_win_dep makes the other deposits.
+__global__ void __launch_bounds__(256) k_win_norm(VirtG V, Bins B, const float* __restrict__ hpart, double scale,
+                                                 unsigned long long* acc, const WinDev* __restrict__ W,
+                                                 const int64_t* __restrict__ off, int nwin,
+                                                 unsigned long long* npairs, float* norm) {
   constexpr int TB = kWinBins, NR = 2 * TB * TB;  // rows of bins, each in at most two runs
   __shared__ int qrow[NR + 1], qbeg[NR];
   const int64_t it = xcd_block(blockIdx.x, gridDim.x);
@@ -1867,41 +1896,243 @@ __global__ void __launch_bounds__(256) k_win_ddens(VirtG V, Bins B, const float*
         }
       }
     }
-    if (nrm == 0.0f) {  // particles.f90:549-557: all the mass to the home cell
-      if (home_in)
-        atomicAdd(&acc[(size_t)(wd.ax + hx) + (size_t)A * ((size_t)(wd.ay + hy) + (size_t)A * (size_t)(wd.az + hz))],
-                  (unsigned long long)llrint((double)p.w * scale));
-      continue;
-    }
-    if (!box_in) continue;
-    for (int k2 = k0; k2 <= k1; ++k2) {
-      const int lz = win_local(k2, wd.oz, wd.w, n, per);
-      if (lz < 0) continue;
-      const float tz2 = p.z - G.rad[k2 - G.rlo];
-      for (int j2 = j0; j2 <= j1; ++j2) {
-        const int ly = win_local(j2, wd.oy, wd.w, n, per);
-        if (ly < 0) continue;
-        const float ty2 = p.y - G.rad[j2 - G.rlo];
-        const float rem = hm2 - (ty2 * ty2 + tz2 * tz2);
-        if (rem < 0.0f) continue;
-        const float rc = bound_sqrt(rem) * rdx + 1e-3f;
-        const int a = max(i0, (int)ceilf(ux - rc));
-        const int b = min(i1, (int)floorf(ux + rc));
-        const size_t row = (size_t)A * ((size_t)(wd.ay + ly) + (size_t)A * (size_t)(wd.az + lz));
-        for (int i2 = a; i2 <= b; ++i2) {
-          const int lx = win_local(i2, wd.ox, wd.w, n, per);
-          if (lx < 0) continue;
-          bool ok;
-          const float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);
-          if (!ok) continue;
-          const float v = (wgt * p.w) / nrm;  // particles.f90:599-600
-          if (v != 0.0f) atomicAdd(&acc[(size_t)(wd.ax + lx) + row], (unsigned long long)llrint((double)v * scale));
-        }
-      }
-    }
+    norm[i] = nrm;
+    if (nrm == 0.0f && home_in)  // particles.f90:549-557: all the mass to the home cell
+      atomicAdd(&acc[(size_t)(wd.ax + hx) + (size_t)A * ((size_t)(wd.ay + hy) + (size_t)A * (size_t)(wd.az + hz))],
+                (unsigned long long)llrint((double)p.w * scale));
   }
   for (int o = 32; o > 0; o >>= 1) pairs += (unsigned)__shfl_xor((int)pairs, o);
   if ((tid & 63) == 0 && pairs) atomicAdd(npairs, (unsigned long long)pairs);
+}
+
+// The deposits of DDENS_INTERP_SPH into the windows (particles.f90:560-618),
+// tile by tile: a block takes kDepX x kDepY x kDepZ cells of a window, keeps
+// their int64 fixed-point sums in LDS, and gathers the particles whose cell
+// box (clipped to the cells whose centre lies within h) meets the tile: those
+// binned within the level's largest h of the tile's cell centres.  Each wave
+// takes one such particle at a time and its lanes go through the particle's
+// cells in the tile, adding m W / NORM (NORM from k_win_norm) with LDS
+// atomics; cells past h get M4 = 0 and add nothing.  The tile's sums then go
+// to the atlas once per cell.  On grids finer than the particles each
+// particle deposits into hundreds of cells, and the sums stay in LDS instead
+// of costing one HBM atomic per (particle, cell) pair.  Integer sums: the
+// result does not depend on the order.
+constexpr int kDepX = 16, kDepY = 16, kDepZ = 8, kDepT = 256, kDepL = 768;
+
+int64_t win_dep_items(int32_t w) {
+  return (int64_t)((w + kDepX - 1) / kDepX) * ((w + kDepY - 1) / kDepY) * ((w + kDepZ - 1) / kDepZ);
+}
+
+// one axis of a tile: window-local cells [t0, t1) as up to two runs of level
+// cells (0-based) g .. g + len - 1, the first at window-local cell l
+struct CellRun {
+  int g, len, l;
+};
+__device__ __forceinline__ int tile_runs(int o, int t0, int t1, int n, int per, CellRun* r) {
+  if (t1 <= t0) return 0;
+  if (!per) {  // the windows of a box that is not periodic lie inside the grid
+    r[0] = CellRun{o + t0, t1 - t0, t0};
+    return 1;
+  }
+  const int s = wrapn(o + t0, n), len = t1 - t0;
+  if (s + len <= n) {
+    r[0] = CellRun{s, len, t0};
+    return 1;
+  }
+  r[0] = CellRun{s, n - s, t0};
+  r[1] = CellRun{0, s + len - n, t0 + (n - s)};
+  return 2;
+}
+
+// particle i's cells (1-based level cells, x a..b, y c..d, z e..f) as the
+// deposit loop of ddens_tile bounds them: the reference's box clipped to the
+// grid and to the centres within h
+struct DepBox {
+  float4 p;
+  float h, inv_h, h2max, nrm;
+  int a, b, c, d, e, f;
+};
+__device__ __forceinline__ void dep_box(const DdensGeom& G, const Bins& B, const float* __restrict__ hpart,
+                                        int64_t i, DepBox& q) {
+  q.p = B.pos[i];
+  const float hh = hpart[i];
+  q.h = hh;
+  int ix, iy, iz, inc, reach;
+  ddens_setup(G, q.p.x, q.p.y, q.p.z, hh, ix, iy, iz, q.inv_h, q.h2max, inc, reach);
+  const int n = G.n;
+  const float rdx = 1.0f / G.dx;
+  const float ux = (q.p.x - G.rx1) * rdx + 1.0f, uy = (q.p.y - G.rx1) * rdx + 1.0f, uz = (q.p.z - G.rx1) * rdx + 1.0f;
+  const float hc = hh * 1.00001f * rdx + 1e-3f;
+  q.a = max(max(ix - reach, 1), (int)ceilf(ux - hc));
+  q.b = min(min(ix + reach, n), (int)floorf(ux + hc));
+  q.c = max(max(iy - reach, 1), (int)ceilf(uy - hc));
+  q.d = min(min(iy + reach, n), (int)floorf(uy + hc));
+  q.e = max(max(iz - reach, 1), (int)ceilf(uz - hc));
+  q.f = min(min(iz + reach, n), (int)floorf(uz + hc));
+}
+
+__global__ void __launch_bounds__(kDepT) k_win_dep(VirtG V, Bins B, const float* __restrict__ hpart,
+                                                 const float* __restrict__ norm, double scale, float hm,
+                                                 int big, unsigned long long* acc, const WinDev* __restrict__ W,
+                                                 const int64_t* __restrict__ off, int nwin) {
+  __shared__ unsigned long long box[kDepZ][kDepY][kDepX];
+  // list: particles with fewer than big cells in the tile from the front (s_ns
+  // of them, a thread each), the others from the back (s_nb, a wave each)
+  __shared__ int qcnt[kDepT + 1], qbeg[kDepT], list[kDepL], s_ns, s_nb;
+  const int64_t it = xcd_block(blockIdx.x, gridDim.x);
+  const int wi = item_window(off, nwin, it);
+  const WinDev wd = W[wi];
+  const int64_t t = it - off[wi];
+  const int w = wd.w, ntx = (w + kDepX - 1) / kDepX, nty = (w + kDepY - 1) / kDepY;
+  const int tx0 = (int)(t % ntx) * kDepX, ty0 = (int)((t / ntx) % nty) * kDepY,
+            tz0 = (int)(t / ((int64_t)ntx * nty)) * kDepZ;
+  const int tx1 = min(tx0 + kDepX, w), ty1 = min(ty0 + kDepY, w), tz1 = min(tz0 + kDepZ, w);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int c = tid; c < kDepX * kDepY * kDepZ; c += kDepT) (&box[0][0][0])[c] = 0ull;
+  if (tid == 0) s_ns = s_nb = 0;
+  const int n = V.n, per = V.periodic, nb = B.nb;
+  const DdensGeom G{V.rad, V.rlo, n, V.dx, V.rx1};
+  CellRun rx[2], ry[2], rz[2];
+  const int nrx = tile_runs(wd.ox, tx0, tx1, n, per, rx), nry = tile_runs(wd.oy, ty0, ty1, n, per, ry),
+            nrz = tile_runs(wd.oz, tz0, tz1, n, per, rz);
+  // the bins of particles within hm of a run's cell centres (one bin of margin)
+  auto bins_of = [&](const CellRun& R, int& b0, int& b1) {
+    const float lo = G.rad[R.g + 1 - G.rlo] - hm, hi = G.rad[R.g + R.len - G.rlo] + hm;
+    b0 = max((int)floorf((lo - B.lo) / B.bsz) - 1, 0);
+    b1 = min((int)floorf((hi - B.lo) / B.bsz) + 1, nb - 1);
+  };
+  for (int sz = 0; sz < nrz; ++sz)
+    for (int sy = 0; sy < nry; ++sy)
+      for (int sx = 0; sx < nrx; ++sx) {
+        const CellRun X = rx[sx], Y = ry[sy], Z = rz[sz];
+        // the sub-tile in 1-based level cells, and its offset into box[][][]
+        const int xa = X.g + 1, xb = X.g + X.len, ya = Y.g + 1, yb = Y.g + Y.len, za = Z.g + 1, zb = Z.g + Z.len;
+        const int ox = X.l - tx0 - xa, oy = Y.l - ty0 - ya, oz = Z.l - tz0 - za;
+        int bx0, bx1, by0, by1, bz0, bz1;
+        bins_of(X, bx0, bx1);
+        bins_of(Y, by0, by1);
+        bins_of(Z, bz0, bz1);
+        const int nyr = by1 - by0 + 1, nrows = nyr * (bz1 - bz0 + 1);
+        // the listed particles' deposits
+        auto deposit_thread = [&](int nl) {  // a particle per thread, rows clipped to the centres within h
+          for (int e = tid; e < nl; e += kDepT) {
+            DepBox q;
+            dep_box(G, B, hpart, list[e], q);
+            const float nrm = norm[list[e]];
+            const int a0 = max(q.a, xa), b0 = min(q.b, xb), c0 = max(q.c, ya), d0 = min(q.d, yb);
+            const int e0 = max(q.e, za), f0 = min(q.f, zb);
+            const float hm2 = q.h * q.h * 1.00001f, rdx = 1.0f / G.dx;
+            const float ux = (q.p.x - G.rx1) * rdx + 1.0f;
+            for (int k2 = e0; k2 <= f0; ++k2) {
+              const float tz2 = q.p.z - G.rad[k2 - G.rlo];
+              for (int j2 = c0; j2 <= d0; ++j2) {
+                const float ty2 = q.p.y - G.rad[j2 - G.rlo];
+                const float rem = hm2 - (ty2 * ty2 + tz2 * tz2);
+                if (rem < 0.0f) continue;
+                const float rc = bound_sqrt(rem) * rdx + 1e-3f;
+                const int a = max(a0, (int)ceilf(ux - rc)), b = min(b0, (int)floorf(ux + rc));
+                for (int i2 = a; i2 <= b; ++i2) {
+                  bool ok;
+                  const float wgt = ddens_w(G, q.p.x, q.p.y, q.p.z, q.inv_h, q.h2max, i2, j2, k2, ok);
+                  if (!ok) continue;
+                  const float v = (wgt * q.p.w) / nrm;  // particles.f90:599-600
+                  if (v != 0.0f)
+                    atomicAdd(&box[k2 + oz][j2 + oy][i2 + ox], (unsigned long long)llrint((double)v * scale));
+                }
+              }
+            }
+          }
+        };
+        auto deposit_wave = [&](int nl) {  // a particle per wave, its cells across the lanes
+          for (int e = wave; e < nl; e += kDepT / 64) {
+            const int64_t i = list[kDepL - 1 - e];
+            DepBox q;
+            dep_box(G, B, hpart, i, q);
+            q.nrm = norm[i];
+            const int a = max(q.a, xa), b = min(q.b, xb), c = max(q.c, ya), d = min(q.d, yb);
+            const int e0 = max(q.e, za), f = min(q.f, zb);
+            const int mx = b - a + 1, my = d - c + 1, cells = mx * my * (f - e0 + 1);
+            for (int u = lane; u < cells; u += 64) {
+              const int i2 = a + u % mx, j2 = c + (u / mx) % my, k2 = e0 + u / (mx * my);
+              bool ok;
+              const float wgt = ddens_w(G, q.p.x, q.p.y, q.p.z, q.inv_h, q.h2max, i2, j2, k2, ok);
+              if (!ok) continue;
+              const float v = (wgt * q.p.w) / q.nrm;  // particles.f90:599-600
+              if (v != 0.0f)
+                atomicAdd(&box[k2 + oz][j2 + oy][i2 + ox], (unsigned long long)llrint((double)v * scale));
+            }
+          }
+        };
+        auto deposit = [&](int ns, int nbig) {
+          deposit_thread(ns);
+          deposit_wave(nbig);
+        };
+        for (int r0 = 0; r0 < nrows; r0 += kDepT) {
+          __syncthreads();  // qcnt and qbeg of the previous rows are no longer read
+          {
+            const int r = r0 + tid;
+            int beg = 0, cnt = 0;
+            if (r < nrows) {
+              const int64_t rowb = (int64_t)nb * ((int64_t)(by0 + r % nyr) + (int64_t)nb * (bz0 + r / nyr));
+              beg = B.start[rowb + bx0];
+              cnt = B.start[rowb + bx1 + 1] - beg;
+            }
+            qbeg[tid] = beg;
+            qcnt[tid + 1] = cnt;
+          }
+          __syncthreads();
+          if (tid == 0) {
+            qcnt[0] = 0;
+            for (int r = 0; r < kDepT; ++r) qcnt[r + 1] += qcnt[r];
+          }
+          __syncthreads();
+          const int nq = qcnt[kDepT];
+          for (int q0 = 0; q0 < nq; q0 += kDepT) {
+            const int qi = q0 + tid;
+            if (qi < nq) {
+              int r = 0, rh = kDepT;
+              while (rh - r > 1) {
+                const int mid = (r + rh) >> 1;
+                if (qcnt[mid] <= qi) r = mid;
+                else rh = mid;
+              }
+              const int64_t i = qbeg[r] + (qi - qcnt[r]);
+              DepBox q;
+              dep_box(G, B, hpart, i, q);
+              const int mx = min(q.b, xb) - max(q.a, xa) + 1, my = min(q.d, yb) - max(q.c, ya) + 1,
+                        mz = min(q.f, zb) - max(q.e, za) + 1;
+              if (mx > 0 && my > 0 && mz > 0 && norm[i] != 0.0f) {
+                if (mx * my * mz < big)
+                  list[atomicAdd(&s_ns, 1)] = (int)i;
+                else
+                  list[kDepL - 1 - atomicAdd(&s_nb, 1)] = (int)i;
+              }
+            }
+            __syncthreads();
+            const int ns = s_ns, nbig = s_nb;
+            if (ns + nbig > kDepL - kDepT) {  // flush before the list can overflow
+              deposit(ns, nbig);
+              __syncthreads();
+              if (tid == 0) s_ns = s_nb = 0;
+            }
+            __syncthreads();
+          }
+        }
+        __syncthreads();
+        const int ns = s_ns, nbig = s_nb;
+        if (ns + nbig > 0) deposit(ns, nbig);
+        __syncthreads();
+        if (tid == 0) s_ns = s_nb = 0;
+      }
+  __syncthreads();
+  for (int c = tid; c < kDepX * kDepY * kDepZ; c += kDepT) {
+    const unsigned long long v = (&box[0][0][0])[c];
+    if (v == 0ull) continue;
+    const int lx = c % kDepX, ly = (c / kDepX) % kDepY, lz = c / (kDepX * kDepY);
+    const size_t ai = (size_t)(wd.ax + tx0 + lx) +
+                      (size_t)V.A * ((size_t)(wd.ay + ty0 + ly) + (size_t)V.A * (size_t)(wd.az + tz0 + lz));
+    atomicAdd(&acc[ai], v);
+  }
 }
 
 // MARCAP of the windows: the parent level's MARCA at cell floor((o + l) / 2),
@@ -1971,11 +2202,18 @@ hipError_t launch_win_vvel(hipStream_t st, const Bins& B, int64_t np, const WinD
   return hipGetLastError();
 }
 
-hipError_t launch_win_ddens(hipStream_t st, const VirtG& V, const Bins& B, const float* hpart, double scale,
-                            unsigned long long* acc, const WinDev* W, const int64_t* off, int32_t nwin,
-                            int64_t items, unsigned long long* npairs) {
+hipError_t launch_win_ddens(hipStream_t st, const VirtG& V, const Bins& B, int64_t np, const float* hpart,
+                            double scale, unsigned long long* acc, const WinDev* W, const int64_t* off, int32_t nwin,
+                            int64_t items, unsigned long long* npairs, float* norm, float hmax,
+                            const int64_t* dep_off, int64_t dep_items) {
   if (items <= 0) return hipSuccess;
-  k_win_ddens<<<(unsigned)items, 256, 0, st>>>(V, B, hpart, scale, acc, W, off, nwin, npairs);
+  k_win_norm<<<(unsigned)items, 256, 0, st>>>(V, B, hpart, scale, acc, W, off, nwin, npairs, norm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || dep_items <= 0) return e;
+  const float hm = hmax * 1.00001f + V.dx;
+  // a wave per particle pays off where particles reach thousands of cells
+  const int big = (double)V.n * V.n * V.n >= 32.0 * (double)np ? 256 : INT_MAX;
+  k_win_dep<<<(unsigned)dep_items, kDepT, 0, st>>>(V, B, hpart, norm, scale, hm, big, acc, W, dep_off, nwin);
   return hipGetLastError();
 }
 

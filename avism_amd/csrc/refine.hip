@@ -382,7 +382,8 @@ int refine_load(avism_ctx* c, int32_t atlas, int32_t k) {
       !c->u_mass.ensure((size_t)ncell * 4) || !c->u1co.ensure((size_t)ncell * 4) ||
       !c->div.ensure((size_t)ncell * 4) || !c->mass_fix.ensure((size_t)ncell * 8) ||
       !c->marcap.ensure((size_t)ncell * 4) || !c->partial64.ensure((size_t)finalize_blocks(ncell) * 8) ||
-      !c->hpart.ensure((size_t)np * 4) || !R.dwin.ensure(sizeof(WinDev) * (size_t)std::max(nw, 1)) ||
+      !c->hpart.ensure((size_t)np * 4) || !c->norm.ensure((size_t)np * 4) ||
+      !R.dwin.ensure(sizeof(WinDev) * (size_t)std::max(nw, 1)) ||
       !R.drad.ensure(gl.rad.size() * 4) || !R.dsum.ensure(1024 * 8) ||
       !c->reqp.ensure((P.reqp.size() + 1) * 4))
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (refine)");
@@ -395,11 +396,12 @@ int refine_load(avism_ctx* c, int32_t atlas, int32_t k) {
   AV_CHECK_HIP(c, hipMemcpyAsync(R.drad.p, gl.rad.data(), gl.rad.size() * 4, hipMemcpyHostToDevice, c->stream));
   AV_CHECK_HIP(c, hipMemcpyAsync(R.dwin.p, wd.data(), sizeof(WinDev) * (size_t)nw, hipMemcpyHostToDevice, c->stream));
   // item offsets: VVEL tiles, then (after the reach is known) DDENS bin tiles,
-  // then MARCAP cell blocks, each nw + 1 entries
-  std::vector<int64_t> off(3 * ((size_t)nw + 1), 0);
+  // then MARCAP cell blocks, then DDENS deposit tiles, each nw + 1 entries
+  std::vector<int64_t> off(4 * ((size_t)nw + 1), 0);
   for (int32_t i = 0; i < nw; ++i) {
     off[i + 1] = off[i] + win_vvel_items(lw[i].w, k);
     off[2 * (nw + 1) + i + 1] = off[2 * (nw + 1) + i] + win_cell_items(lw[i].w);
+    off[3 * (nw + 1) + i + 1] = off[3 * (nw + 1) + i] + win_dep_items(lw[i].w);
   }
   if (!R.doff.ensure(off.size() * 8)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (refine)");
   AV_CHECK_HIP(c, hipMemcpyAsync(R.doff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice, c->stream));
@@ -506,9 +508,10 @@ int refine_load(avism_ctx* c, int32_t atlas, int32_t k) {
     StageTimer t(c, "refine_ddens");
     AV_CHECK_HIP(c, launch_fill_u64(c->stream, ncell, c->mass_fix.as<unsigned long long>(), 0ull));
     AV_CHECK_HIP(c, launch_fill_u64(c->stream, 1, dpairs, 0ull));
-    AV_CHECK_HIP(c, launch_win_ddens(c->stream, V, B, c->hpart.as<float>(), scale,
+    AV_CHECK_HIP(c, launch_win_ddens(c->stream, V, B, np, c->hpart.as<float>(), scale,
                                      c->mass_fix.as<unsigned long long>(), R.dwin.as<WinDev>(), d_off + (nw + 1),
-                                     nw, off[2 * (nw + 1) - 1], dpairs));
+                                     nw, off[2 * (nw + 1) - 1], dpairs, c->norm.as<float>(), hmax,
+                                     d_off + 3 * (nw + 1), off[4 * (nw + 1) - 1]));
   }
   // U1, MEANDENS = total mass / the level's cells, U1CO and DIVERCO on the atlas
   const float den = ((gl.dx * gl.dx) * gl.dx) * ((c->rete * c->rete) * c->rete);
