@@ -1584,12 +1584,16 @@ int get_marca(avism_ctx* c, int32_t* marca, int32_t inner_only, int32_t where) {
     AV_CHECK_HIP(c, hipMemcpy(marca, c->marca.p, (size_t)ne3 * 4, kind));
     return AVISM_OK;
   }
-  // inner box = ext rows starting at (1,1,1)
+  // inner box = the n^3 ext cells from (1,1,1): one strided 3-D copy
   const int n = g.n, off = 1 - g.low1;
-  for (int k = 0; k < n; ++k)
-    AV_CHECK_HIP(c, hipMemcpy2D(marca + (size_t)k * n * n, (size_t)n * 4,
-                                c->marca.as<int32_t>() + ((size_t)off + (size_t)g.next * ((size_t)off + (size_t)g.next * (k + off))),
-                                (size_t)g.next * 4, (size_t)n * 4, (size_t)n, kind));
+  hipMemcpy3DParms p{};
+  p.srcPtr = make_hipPitchedPtr(c->marca.p, (size_t)g.next * 4, (size_t)g.next, (size_t)g.next);
+  p.srcPos = make_hipPos((size_t)off * 4, (size_t)off, (size_t)off);
+  p.dstPtr = make_hipPitchedPtr(marca, (size_t)n * 4, (size_t)n, (size_t)n);
+  p.dstPos = make_hipPos(0, 0, 0);
+  p.extent = make_hipExtent((size_t)n * 4, (size_t)n, (size_t)n);
+  p.kind = kind;
+  AV_CHECK_HIP(c, hipMemcpy3D(&p));
   return AVISM_OK;
 }
 

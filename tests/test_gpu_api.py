@@ -116,6 +116,34 @@ def test_host_upload_and_async_maps_match(periodic):
         assert np.array_equal(u1.numpy(), f1["u1co"])
 
 
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_inner_marca_is_the_ext_box(periodic):
+    """avism_get_marca(inner_only) is the n^3 block of the ext map that starts
+    at cell (1,1,1), read into host memory and into device memory."""
+    import ctypes as C
+    import torch
+    from avism_amd._lib import AVISM_DEVICE, lib
+    n, L0 = 24, 48.0
+    p = _particles(n, L0, 21 + periodic)
+    c = _ctx()
+    c.set_grid(n, L0, periodic)
+    c.load_particles(*p.numpy())
+    c.particles_to_grid(k=32)
+    nv = c.find_voids(0)
+    c.merge_and_post()
+    gi = c.grid()
+    print(f"[inner marca] periodic={periodic} n={n} ext={gi.next} cubes={nv}", flush=True)
+    ext = c.marca().reshape(gi.next, gi.next, gi.next)
+    o = 1 - gi.low1
+    want = ext[o:o + n, o:o + n, o:o + n].reshape(-1)
+    assert (want > 0).any()
+    assert np.array_equal(c.marca(inner_only=True), want)
+    d = torch.full((n ** 3,), -7, dtype=torch.int32, device="cuda")
+    assert lib().avism_get_marca(c._h, C.c_void_p(d.data_ptr()), 1, AVISM_DEVICE) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(d.cpu().numpy(), want)
+
+
 def test_counters():
     n, L0 = 20, 40.0
     p = _particles(n, L0, 13)
