@@ -710,14 +710,18 @@ __global__ void __launch_bounds__(256) k_cc_local(ExtG G, const int32_t* __restr
 }
 
 // joins across the low faces of the blocks: one wave per 8 x 8 block face
-// (grid: blocks, 3 directions), lane (u, v) on the face.  A pair of local
-// labels equal to its left or lower neighbour's on the face is already joined
-// by that neighbour (the first lane of each distinct pair has neither), so a
-// face whose two sides are one component each makes one union, not 64.
-__global__ void __launch_bounds__(64) k_cc_faces(ExtG G, int nbc, int32_t* lab) {
-  const int dir = blockIdx.y, b = blockIdx.x;
+// (item = block * 3 + direction, four items per workgroup), lane (u, v) on the
+// face.  A pair of local labels equal to its left or lower neighbour's on the
+// face is already joined by that neighbour (the first lane of each distinct
+// pair has neither), so a face whose two sides are one component each makes
+// one union, not 64.
+constexpr int kFaceW = 4;  // waves (faces) per workgroup
+__global__ void __launch_bounds__(64 * kFaceW) k_cc_faces(ExtG G, int nbc, int32_t* lab) {
+  const int64_t item = (int64_t)blockIdx.x * kFaceW + (threadIdx.x >> 6);
+  if (item >= (int64_t)nbc * nbc * nbc * 3) return;
+  const int dir = (int)(item % 3), b = (int)(item / 3);
   const int bx = b % nbc, by = (b / nbc) % nbc, bz = b / (nbc * nbc);
-  const int l = threadIdx.x, u = l & (CCB - 1), v = l / CCB;
+  const int l = threadIdx.x & 63, u = l & (CCB - 1), v = l / CCB;
   int i, j, k;
   int64_t step;
   if (dir == 0) {
@@ -1351,7 +1355,8 @@ int merge_and_post(avism_ctx* c, int32_t stop_after) {
     uint8_t* isrep = c->paintK.as<uint8_t>();  // free after the merge
     k_cc_local<<<(unsigned)((int64_t)nbc * nbc * nbc), 256, 0, c->stream>>>(G, marca, lab, isrep);
     const dim3 g3((unsigned)((G.ne + 255) / 256), (unsigned)G.ne, (unsigned)G.ne);
-    k_cc_faces<<<dim3((unsigned)((int64_t)nbc * nbc * nbc), 3), 64, 0, c->stream>>>(G, nbc, lab);
+    k_cc_faces<<<(unsigned)(((int64_t)nbc * nbc * nbc * 3 + kFaceW - 1) / kFaceW), 64 * kFaceW, 0, c->stream>>>(
+        G, nbc, lab);
     k_cc_flatten<<<nbk(ne3), 256, 0, c->stream>>>(ne3, isrep, lab);
     AV_CHECK_HIP(c, hipMemsetAsync(small + 2, 0, 4, c->stream));
     k_fill_holes<<<g3, 256, 0, c->stream>>>(G, lab, marca, small + 2);
