@@ -976,6 +976,8 @@ __device__ __forceinline__ void vp_decide(int ind, int ind2, int cnt, int cnt2, 
   grown[(ind2 - 1) >> 5] |= 1u << ((ind2 - 1) & 31);
 }
 
+constexpr int kVpCh = 32;
+
 // one workgroup walks the flagged main voids in INDICE order (each id occurs
 // once).  Thread 0 decides every void whose counts follow from its histogram;
 // a void that gained members, or whose histogram overflowed, is counted again
@@ -989,6 +991,10 @@ __global__ void __launch_bounds__(1024) k_void_periodic(
     int32_t* lmap, int32_t* mnext, int32_t* mtail, int32_t* selfm, int32_t* stats) {
   extern __shared__ uint32_t dyn[];
   __shared__ int s_cnt, s_cnt2, s_go, s_ind, s_ind2, s_sx, s_sy, s_sz, s_next;
+  // the next kVpCh entries of the list and their histograms, staged by the
+  // whole workgroup so that thread 0's ordered pass reads LDS, not HBM
+  __shared__ int4 s_list[kVpCh];
+  __shared__ int32_t s_pre[kVpCh * VPW];
   const int tid = threadIdx.x;
   const int nl = *nlist;
   const int nw = (nv + 32) / 32;
@@ -1002,13 +1008,19 @@ __global__ void __launch_bounds__(1024) k_void_periodic(
   __syncthreads();
   const VpMap M{lm16, lmap};
   for (;;) {
+    const int base = s_next;  // set before the last barrier
+    if (base >= nl) break;
+    const int nch = min(kVpCh, nl - base);
+    for (int e = tid; e < nch; e += blockDim.x) s_list[e] = list[base + e];
+    for (int e = tid; e < nch * VPW; e += blockDim.x) s_pre[e] = pre[(size_t)base * VPW + e];
+    __syncthreads();
     if (tid == 0) {
-      int L = s_next;
+      int L = base;
       s_go = 0;
-      for (; L < nl; ++L) {
-        const int4 in = list[L];
+      for (; L < base + nch; ++L) {
+        const int4 in = s_list[L - base];
         const int id = in.x;
-        const int32_t* pr = pre + (size_t)L * VPW;
+        const int32_t* pr = s_pre + (L - base) * VPW;
         const int ind2 = M.resolve(in.y);
         const bool gr = (grown[(id - 1) >> 5] >> ((id - 1) & 31)) & 1u;
         const int nlab = pr[1];
@@ -1031,7 +1043,7 @@ __global__ void __launch_bounds__(1024) k_void_periodic(
       s_cnt2 = 0;
     }
     __syncthreads();
-    if (!s_go) break;
+    if (!s_go) continue;  // the chunk is done: stage the next one
     {
       const int ind = s_ind, ind2 = s_ind2;
       int c1 = 0, c2 = 0;
