@@ -116,7 +116,6 @@ struct avism_ctx {
   avism::DBuf spos4, sorig;                    // sorted by bin
   avism::DBuf vel4;                            // (vx, vy, vz, m/rho), sorted by bin
   avism::DBuf bin_start;                       // nb^3 + 1
-  avism::DBuf bin_reach;                       // nb^3, DDENS reach per bin
   int32_t nb = 0;
   float bsz = 0;
   avism::DBuf pdens, hk, hpart, norm;          // sorted order
@@ -144,7 +143,7 @@ struct avism_ctx {
   // survey randoms (FLAG_SURVEY = 1): a second particle set, swapped with the
   // first by swap_particle_sets; raw_mass: DDENS leaves cell masses unscaled
   int64_t nrand = 0;
-  avism::DBuf rpx, rpy, rpz, rpvx, rpvy, rpvz, rpm, rspos4, rsorig, rvel4, rbin_start, rbin_reach;
+  avism::DBuf rpx, rpy, rpz, rpvx, rpvy, rpvz, rpm, rspos4, rsorig, rvel4, rbin_start;
   avism::DBuf rpdens, rhk, rhpart, rnorm, u_rand;
   int32_t rnb = 0;
   float rbsz = 0, rmass_max = 0;

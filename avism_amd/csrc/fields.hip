@@ -170,13 +170,34 @@ __global__ void k_fill_u64(int64_t n, unsigned long long* a, unsigned long long 
 }
 
 __global__ void k_bin_keys(int64_t np, const float* x, const float* y, const float* z, Bins B,
-                           uint32_t* key, uint32_t* val, int32_t* count) {
+                           uint32_t* key, uint32_t* val) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   int b = B.bin_of(x[i]) + B.nb * (B.bin_of(y[i]) + B.nb * B.bin_of(z[i]));
   key[i] = (uint32_t)b;
   val[i] = (uint32_t)i;
-  atomicAdd(&count[b], 1);
+}
+
+// bin starts from the sorted keys: position i starts every bin after key[i-1]
+// up to key[i] (bin nbins at np), so each of the nbins + 1 entries is written
+// once.  A thread writes at most kRun of its run; the rest of a longer run
+// (empty regions) goes to a list that k_bin_runs fills with whole blocks.
+constexpr int kRun = 256;
+__global__ void k_bin_starts(int64_t np, int64_t nbins, const uint32_t* __restrict__ key, int32_t* start,
+                             int4* longrun, int32_t* nlong) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > np) return;
+  const int64_t lo = i == 0 ? 0 : (int64_t)key[i - 1] + 1, hi = i == np ? nbins : (int64_t)key[i];
+  const int64_t end = min(hi, lo + kRun - 1);
+  for (int64_t b = lo; b <= end; ++b) start[b] = (int32_t)i;
+  if (end < hi) longrun[atomicAdd(nlong, 1)] = make_int4((int)(end + 1), (int)hi, (int)i, 0);
+}
+__global__ void k_bin_runs(const int4* __restrict__ longrun, const int32_t* __restrict__ nlong, int32_t* start) {
+  const int n = *nlong;
+  for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    const int4 r = longrun[e];
+    for (int b = r.x + (int)threadIdx.x; b <= r.y; b += blockDim.x) start[b] = r.z;
+  }
 }
 
 __global__ void k_gather_sorted(int64_t np, const uint32_t* perm, const float* x, const float* y,
@@ -1229,15 +1250,12 @@ static int build_bins(avism_ctx* c) {
   if (!c->bin_start.ensure((size_t)(nbins + 1) * 4) || !c->sbin_key.ensure((size_t)np * 4) ||
       !c->sbin_val.ensure((size_t)np * 4) || !c->sbin_key2.ensure((size_t)np * 4) ||
       !c->sbin_val2.ensure((size_t)np * 4) || !c->spos4.ensure((size_t)np * 16) ||
-      !c->vel4.ensure((size_t)np * 16) || !c->sorig.ensure((size_t)np * 4) ||
-      !c->bin_reach.ensure((size_t)nbins * 4))
+      !c->vel4.ensure((size_t)np * 16) || !c->sorig.ensure((size_t)np * 4))
     AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (bins)");
-  int32_t* count = c->bin_reach.as<int32_t>();  // reused as the histogram
-  AV_CHECK_HIP(c, hipMemsetAsync(count, 0, (size_t)nbins * 4, c->stream));
   Bins B = make_bins(c);
   k_bin_keys<<<nblk(np), 256, 0, c->stream>>>(np, c->px.as<float>(), c->py.as<float>(),
                                               c->pz.as<float>(), B, c->sbin_key.as<uint32_t>(),
-                                              c->sbin_val.as<uint32_t>(), count);
+                                              c->sbin_val.as<uint32_t>());
   unsigned bits = 1;
   while ((1ll << bits) < nbins) bits++;
   size_t tmp = 0;
@@ -1245,23 +1263,20 @@ static int build_bins(avism_ctx* c) {
                                             c->sbin_key2.as<uint32_t>(), c->sbin_val.as<uint32_t>(),
                                             c->sbin_val2.as<uint32_t>(), (size_t)np, 0, bits,
                                             c->stream));
-  if (!c->sort_tmp.ensure(tmp)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (sort)");
+  // the sort's scratch also holds, afterwards, the list of long runs of empty
+  // bins (at most nbins / kRun + 2 entries) and its count
+  const size_t lr_bytes = ((size_t)(nbins / kRun) + 2) * sizeof(int4);
+  if (!c->sort_tmp.ensure(std::max(tmp, lr_bytes + 16))) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (sort)");
   AV_CHECK_HIP(c, rocprim::radix_sort_pairs(c->sort_tmp.p, tmp, c->sbin_key.as<uint32_t>(),
                                             c->sbin_key2.as<uint32_t>(), c->sbin_val.as<uint32_t>(),
                                             c->sbin_val2.as<uint32_t>(), (size_t)np, 0, bits,
                                             c->stream));
-  size_t tmp2 = 0;
-  AV_CHECK_HIP(c, rocprim::exclusive_scan(nullptr, tmp2, count, c->bin_start.as<int32_t>(), 0,
-                                          (size_t)nbins + 1, rocprim::plus<int32_t>(), c->stream));
-  if (!c->sort_tmp.ensure(std::max(tmp, tmp2))) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory");
-  // the scan reads nbins+1 inputs: make the extra one a zero
-  AV_CHECK_HIP(c, hipMemsetAsync(c->sbin_key.p, 0, 4, c->stream));
-  // exclusive scan of counts over nbins entries, final total appended
-  AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmp2, count, c->bin_start.as<int32_t>(), 0,
-                                          (size_t)nbins, rocprim::plus<int32_t>(), c->stream));
-  int32_t total = (int32_t)np;
-  AV_CHECK_HIP(c, hipMemcpyAsync(c->bin_start.as<int32_t>() + nbins, &total, 4,
-                                 hipMemcpyHostToDevice, c->stream));
+  int4* longrun = c->sort_tmp.as<int4>();
+  int32_t* nlong = (int32_t*)((char*)c->sort_tmp.p + lr_bytes);
+  AV_CHECK_HIP(c, hipMemsetAsync(nlong, 0, 4, c->stream));
+  k_bin_starts<<<nblk(np + 1), 256, 0, c->stream>>>(np, nbins, c->sbin_key2.as<uint32_t>(),
+                                                    c->bin_start.as<int32_t>(), longrun, nlong);
+  k_bin_runs<<<1024, 256, 0, c->stream>>>(longrun, nlong, c->bin_start.as<int32_t>());
   k_gather_sorted<<<nblk(np), 256, 0, c->stream>>>(
       np, c->sbin_val2.as<uint32_t>(), c->px.as<float>(), c->py.as<float>(), c->pz.as<float>(),
       c->pm.as<float>(), c->spos4.as<float4>(), c->sorig.as<uint32_t>());
@@ -1527,7 +1542,7 @@ void swap_particle_sets(avism_ctx* c) {
   swap_buf(c->px, c->rpx); swap_buf(c->py, c->rpy); swap_buf(c->pz, c->rpz);
   swap_buf(c->pvx, c->rpvx); swap_buf(c->pvy, c->rpvy); swap_buf(c->pvz, c->rpvz); swap_buf(c->pm, c->rpm);
   swap_buf(c->spos4, c->rspos4); swap_buf(c->sorig, c->rsorig); swap_buf(c->vel4, c->rvel4);
-  swap_buf(c->bin_start, c->rbin_start); swap_buf(c->bin_reach, c->rbin_reach);
+  swap_buf(c->bin_start, c->rbin_start);
   swap_buf(c->pdens, c->rpdens); swap_buf(c->hk, c->rhk); swap_buf(c->hpart, c->rhpart); swap_buf(c->norm, c->rnorm);
   std::swap(c->np, c->nrand); std::swap(c->nb, c->rnb); std::swap(c->bsz, c->rbsz);
   std::swap(c->mass_max, c->rmass_max); std::swap(c->mass_uniform, c->rmass_uniform);
