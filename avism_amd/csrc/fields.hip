@@ -509,6 +509,18 @@ __device__ __forceinline__ void ddens_setup(const DdensGeom& G, float x, float y
   reach = rr < inc ? rr : inc;
 }
 
+// ddens_w with the row's squared y and z separations given (dy2 = ty * ty,
+// dz2 = tz * tz, as ddens_w forms them): the same FP32 operations
+__device__ __forceinline__ float ddens_w_row(const DdensGeom& G, float x, float dy2, float dz2, float inv_h,
+                                            float h2max, int i2, bool& ok) {
+  float tx = x - G.rad[i2 - G.rlo];
+  float dx2 = tx * tx;
+  float d2 = (dx2 + dy2) + dz2;
+  ok = !(d2 > h2max);
+  float q = sqrtf(d2) * inv_h;
+  return m4(q);
+}
+
 __device__ __forceinline__ float ddens_w(const DdensGeom& G, float x, float y, float z, float inv_h,
                                         float h2max, int i2, int j2, int k2, bool& ok) {
   float tz = z - G.rad[k2 - G.rlo];
@@ -652,9 +664,10 @@ __device__ __forceinline__ void ddens_tile(const DdensGeom& G, const Bins& B, co
           const float rc = bound_sqrt(rem) * rdx + 1e-3f;
           const int a = max(i0, (int)ceilf(ux - rc));
           const int b = min(i1, (int)floorf(ux + rc));
+          const float dy2 = ty * ty, dz2 = tz * tz;
           for (int i2 = a; i2 <= b; ++i2) {
             bool ok;
-            const float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);
+            const float wgt = ddens_w_row(G, p.x, dy2, dz2, inv_h, h2max, i2, ok);
             if (!ok) continue;
             if (pass == 0) {
               ++pairs;
@@ -1902,9 +1915,10 @@ __global__ void __launch_bounds__(256) k_win_norm(VirtG V, Bins B, const float* 
         const float rc = bound_sqrt(rem) * rdx + 1e-3f;
         const int a = max(i0, (int)ceilf(ux - rc));
         const int b = min(i1, (int)floorf(ux + rc));
+        const float dy2 = ty2 * ty2, dz2 = tz2 * tz2;
         for (int i2 = a; i2 <= b; ++i2) {
           bool ok;
-          const float wgt = ddens_w(G, p.x, p.y, p.z, inv_h, h2max, i2, j2, k2, ok);
+          const float wgt = ddens_w_row(G, p.x, dy2, dz2, inv_h, h2max, i2, ok);
           if (!ok) continue;
           ++pairs;
           nrm = nrm + wgt;
@@ -2046,9 +2060,10 @@ __global__ void __launch_bounds__(kDepT) k_win_dep(VirtG V, Bins B, const float*
                 if (rem < 0.0f) continue;
                 const float rc = bound_sqrt(rem) * rdx + 1e-3f;
                 const int a = max(a0, (int)ceilf(ux - rc)), b = min(b0, (int)floorf(ux + rc));
+                const float dy2 = ty2 * ty2, dz2 = tz2 * tz2;
                 for (int i2 = a; i2 <= b; ++i2) {
                   bool ok;
-                  const float wgt = ddens_w(G, q.p.x, q.p.y, q.p.z, q.inv_h, q.h2max, i2, j2, k2, ok);
+                  const float wgt = ddens_w_row(G, q.p.x, dy2, dz2, q.inv_h, q.h2max, i2, ok);
                   if (!ok) continue;
                   const float v = (wgt * q.p.w) / nrm;  // particles.f90:599-600
                   if (v != 0.0f)
