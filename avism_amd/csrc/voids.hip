@@ -56,9 +56,9 @@ struct MarkParams {
 // (halo 3) and DIVERCO (halo 1) are staged in LDS for the x and y stencils,
 // and each cell's z line (U1CO at z-3..z+3, DIVERCO at z-1..z+1) slides
 // through registers.  The next step's planes are loaded into registers while
-// the current step is computed from LDS.  Ext cells read the inner fields through the periodic
-// wrap; halo cells outside LOW1..LOW2 are clamped into it (the stencil never
-// uses them).
+// the current step is computed from LDS.  Ext cells read the inner fields
+// through the periodic wrap; halo cells outside LOW1..LOW2 are clamped into
+// it (the stencil never uses them).
 constexpr int kMarkX = 64, kMarkY = 8, kMarkZ = 32, kMarkP = 2;
 constexpr int kMarkT = kMarkX * kMarkY;                          // threads per block
 constexpr int kMarkU = (kMarkY + 6) * (kMarkX + 6);              // U1CO tile entries per plane
