@@ -180,3 +180,37 @@ def test_batch_rejects():
     with pytest.raises(AvismError) as e:
         c.batch_load(*arr)
     assert e.value.code == -3 and "patch 0" in str(e.value)
+
+
+def test_batch_64_patches_at_full_size():
+    """Configuration 3 at its full size: 64 patches, 2e5..8e6 particles each,
+    grids of 32^3..160^3 cells, 20% of each mask empty, one batch.  Every
+    patch's fields equal the single-grid path's bit for bit and its catalogue
+    is the single-grid context's; the three smallest patches' void stages
+    equal the oracle's on the batch's fields."""
+    from avism_amd import synth
+    pats = synth.patch_batch(64, seed0=3000, device="cpu")
+    ns = [p["n"] for p in pats]
+    print(f"[batch 64] particles {sum(p['particles'].n for p in pats)}, cells {sum(n ** 3 for n in ns)}",
+          flush=True)
+    c = _ctx()
+    _layout(c, pats)
+    _load(c, pats)
+    c.batch_fields(k=32)
+    c.batch_voids(threads=8)
+    small = set(np.argsort([n ** 3 + p["particles"].n for n, p in zip(ns, pats)], kind="stable")[:3].tolist())
+    for i, pat in enumerate(pats):
+        fb = c.batch_fields_of(i)
+        fs, cs = _single(pat)
+        for q in FIELDS:
+            assert np.array_equal(fb[q], fs[q]), (i, q)
+        out = gpu_void_outputs_from(c.batch_patch(i))
+        single = gpu_void_outputs(cs)
+        for k in ("marca", "uvoid", "indice", "cubes_i", "ncellv", "vf"):
+            assert np.array_equal(out[k], single[k]), (i, k)
+        if i in small:
+            ref = O.void_stage(O.Grid(pat["n"], pat["L0"], 0), fb["u1co"], fb["div"], PARAMS, smask=pat["mask"],
+                               meandens=c.batch_patch(i).stats().meandens, rete=O.cosmo(0.678, 0.31, 0.0)[1])
+            compare(out, lambda k, r=ref: _as_ref(r)[k])
+        del fb, fs, cs
+    assert c.batch_info()["graph_launches"] == 1
