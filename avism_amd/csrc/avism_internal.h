@@ -119,6 +119,7 @@ struct avism_ctx {
   int32_t nb = 0;
   float bsz = 0;
   avism::DBuf pdens, hk, hpart, norm;          // sorted order
+  avism::DBuf pmark;                           // np bytes: particles marked for a NORM (fine-grid DDENS)
   avism::DBuf scalars;                         // small device scalars
   // inner fields
   avism::DBuf u_mass;                          // mass -> density (before /MEANDENS)

@@ -1456,6 +1456,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
     if (vvel_chk(ncell, np) && c->nb <= n) {
       // a grid finer than the particles: the window path with one window, the
       // whole grid (NORM per particle, then the deposits tile by tile in LDS)
+      if (!c->pmark.ensure((size_t)np)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (DDENS marks)");
       uint32_t* dmax = (uint32_t*)((char*)c->scalars.p + 1792);
       AV_CHECK_HIP(c, launch_fill_u32(c->stream, 1, dmax, 0u));
       AV_CHECK_HIP(c, launch_max_bits(c->stream, np, c->hpart.as<uint32_t>(), dmax));
@@ -1479,7 +1480,8 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
       const VirtG V{rad, c->g.rlo, n, c->g.periodic, c->g.dx, c->g.rx1, n};
       AV_CHECK_HIP(c, launch_win_ddens(c->stream, V, B, np, c->hpart.as<float>(), scale,
                                        c->mass_fix.as<unsigned long long>(), dw, doff, 1, items, dpairs,
-                                       c->norm.as<float>(), hmax, doff + 2, dep));
+                                       c->norm.as<float>(), c->pmark.as<uint8_t>(), c->sbin_val.as<int32_t>(),
+                                       (int32_t*)((char*)c->scalars.p + 1800), hmax, doff + 2, dep));
     } else if (c->nb > n)  // cells split into bins
       k_ddens_scatter<4><<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
                                                         c->mass_fix.as<unsigned long long>(), c->norm.as<float>(),
@@ -1821,17 +1823,45 @@ __device__ __forceinline__ bool win_hits(int lo, int hi, int o, int w, int n, in
   return l0 < w || l0 + (hi - lo + 1) > n;
 }
 
-// DDENS_INTERP_SPH into the windows, first part: NORM.  A block takes the
-// particles of 4^3 bins of a window's particle range (the bins within the
-// largest DDENS reach of it); a particle whose cell box (or, with NORM = 0,
-// home cell) meets the window computes its NORM over the whole box on the
-// level's grid, exactly as ddens_tile does, into norm[i] (the same value from
-// every window that holds it), and a particle with NORM = 0 adds its mass to
-// its home cell if the window holds it.  k_win_dep makes the other deposits.
-__global__ void __launch_bounds__(256) k_win_norm(VirtG V, Bins B, const float* __restrict__ hpart, double scale,
-                                                 unsigned long long* acc, const WinDev* __restrict__ W,
-                                                 const int64_t* __restrict__ off, int nwin,
-                                                 unsigned long long* npairs, float* norm) {
+// A particle's cell box on the level's grid (1-based cells, the reference's
+// box clipped to the grid and to the centres within h, as in ddens_tile) and
+// its home cell.
+struct NormBox {
+  float4 p;
+  float h, inv_h, h2max;
+  int ix, iy, iz;          // home cell (may lie outside 1..n)
+  int i0, i1, ia, ib, j0, j1, k0, k1;
+};
+__device__ __forceinline__ void norm_box(const DdensGeom& G, const Bins& B, const float* __restrict__ hpart,
+                                         int64_t i, NormBox& q) {
+  q.p = B.pos[i];
+  q.h = hpart[i];
+  int inc, reach;
+  ddens_setup(G, q.p.x, q.p.y, q.p.z, q.h, q.ix, q.iy, q.iz, q.inv_h, q.h2max, inc, reach);
+  const int n = G.n;
+  q.k0 = max(q.iz - reach, 1), q.k1 = min(q.iz + reach, n);
+  q.j0 = max(q.iy - reach, 1), q.j1 = min(q.iy + reach, n);
+  q.i0 = max(q.ix - reach, 1), q.i1 = min(q.ix + reach, n);
+  const float rdx = 1.0f / G.dx;
+  const float ux = (q.p.x - G.rx1) * rdx + 1.0f, uy = (q.p.y - G.rx1) * rdx + 1.0f, uz = (q.p.z - G.rx1) * rdx + 1.0f;
+  const float hc = q.h * 1.00001f * rdx + 1e-3f;
+  q.k0 = max(q.k0, (int)ceilf(uz - hc));
+  q.k1 = min(q.k1, (int)floorf(uz + hc));
+  q.j0 = max(q.j0, (int)ceilf(uy - hc));
+  q.j1 = min(q.j1, (int)floorf(uy + hc));
+  q.ia = max(q.i0, (int)ceilf(ux - hc));
+  q.ib = min(q.i1, (int)floorf(ux + hc));
+}
+
+// DDENS_INTERP_SPH into the windows, first part: which particles need a
+// NORM.  A block takes the particles of 4^3 bins of a window's particle range
+// (the bins within the largest DDENS reach of it) and marks need[i] for each
+// whose cell box or home cell meets the window.  Windows overlap in their
+// ranges, so this pass is cheap where the NORMs are not: k_norm_marked then
+// computes each marked particle's NORM once.
+__global__ void __launch_bounds__(256) k_win_mark(VirtG V, Bins B, const float* __restrict__ hpart,
+                                                 const WinDev* __restrict__ W, const int64_t* __restrict__ off,
+                                                 int nwin, uint8_t* need) {
   constexpr int TB = kWinBins, NR = 2 * TB * TB;  // rows of bins, each in at most two runs
   __shared__ int qrow[NR + 1], qbeg[NR];
   const int64_t it = xcd_block(blockIdx.x, gridDim.x);
@@ -1868,10 +1898,9 @@ __global__ void __launch_bounds__(256) k_win_norm(VirtG V, Bins B, const float* 
     for (int r = 0; r < NR; ++r) qrow[r + 1] += qrow[r];
   }
   __syncthreads();
-  const int n = V.n, A = V.A, per = V.periodic;
+  const int n = V.n, per = V.periodic;
   const DdensGeom G{V.rad, V.rlo, n, V.dx, V.rx1};
   const int nq = qrow[NR];
-  unsigned pairs = 0;
   for (int q = tid; q < nq; q += 256) {
     int r = 0, rh = NR;
     while (rh - r > 1) {
@@ -1880,45 +1909,61 @@ __global__ void __launch_bounds__(256) k_win_norm(VirtG V, Bins B, const float* 
       else rh = mid;
     }
     const int64_t i = qbeg[r] + (q - qrow[r]);
-    const float4 p = B.pos[i];
-    const float hh = hpart[i];
-    int ix, iy, iz, inc, reach;
-    float inv_h, h2max;
-    ddens_setup(G, p.x, p.y, p.z, hh, ix, iy, iz, inv_h, h2max, inc, reach);
-    int k0 = max(iz - reach, 1), k1 = min(iz + reach, n);
-    int j0 = max(iy - reach, 1), j1 = min(iy + reach, n);
-    const int i0 = max(ix - reach, 1), i1 = min(ix + reach, n);
-    const float hm2 = hh * hh * 1.00001f;
+    NormBox b;
+    norm_box(G, B, hpart, i, b);
+    const bool box_in = win_hits(b.ia, b.ib, wd.ox, wd.w, n, per) && win_hits(b.j0, b.j1, wd.oy, wd.w, n, per) &&
+                        win_hits(b.k0, b.k1, wd.oz, wd.w, n, per);
+    const bool home_in = b.ix >= 1 && b.ix <= n && b.iy >= 1 && b.iy <= n && b.iz >= 1 && b.iz <= n &&
+                         win_local(b.ix, wd.ox, wd.w, n, per) >= 0 && win_local(b.iy, wd.oy, wd.w, n, per) >= 0 &&
+                         win_local(b.iz, wd.oz, wd.w, n, per) >= 0;
+    if (box_in || home_in) need[i] = 1;
+  }
+}
+
+// the marked particles as a list (a wave's marked lanes take consecutive
+// slots, so neighbours in the bin order stay together)
+__global__ void __launch_bounds__(256) k_list_marked(int64_t np, const uint8_t* __restrict__ need, int32_t* list,
+                                                    int32_t* count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool take = i < np && need[i] != 0;
+  const unsigned long long m = __ballot(take);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63, first = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == first) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, first);
+  if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)i;
+}
+
+// NORM of every listed particle (particles.f90:520-547, in the reference's
+// loop order, as ddens_tile computes it), one thread each
+__global__ void __launch_bounds__(256) k_norm_marked(VirtG V, Bins B, const float* __restrict__ hpart,
+                                                    const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+                                                    float* norm, unsigned long long* npairs) {
+  const int nl = *count;
+  unsigned pairs = 0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nl; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = list[t];
+    const DdensGeom G{V.rad, V.rlo, V.n, V.dx, V.rx1};
+    NormBox b;
+    norm_box(G, B, hpart, i, b);
+    const float hm2 = b.h * b.h * 1.00001f;
     const float rdx = 1.0f / G.dx;
-    const float ux = (p.x - G.rx1) * rdx + 1.0f, uy = (p.y - G.rx1) * rdx + 1.0f, uz = (p.z - G.rx1) * rdx + 1.0f;
-    const float hc = hh * 1.00001f * rdx + 1e-3f;
-    k0 = max(k0, (int)ceilf(uz - hc));
-    k1 = min(k1, (int)floorf(uz + hc));
-    j0 = max(j0, (int)ceilf(uy - hc));
-    j1 = min(j1, (int)floorf(uy + hc));
-    const int ia = max(i0, (int)ceilf(ux - hc));
-    const int ib = min(i1, (int)floorf(ux + hc));
-    const bool box_in = win_hits(ia, ib, wd.ox, wd.w, n, per) && win_hits(j0, j1, wd.oy, wd.w, n, per) &&
-                        win_hits(k0, k1, wd.oz, wd.w, n, per);
-    const int hx = (ix >= 1 && ix <= n) ? win_local(ix, wd.ox, wd.w, n, per) : -1;
-    const int hy = (iy >= 1 && iy <= n) ? win_local(iy, wd.oy, wd.w, n, per) : -1;
-    const int hz = (iz >= 1 && iz <= n) ? win_local(iz, wd.oz, wd.w, n, per) : -1;
-    const bool home_in = hx >= 0 && hy >= 0 && hz >= 0;
-    if (!box_in && !home_in) continue;
-    float nrm = 0.0f;  // particles.f90:520-547, in the reference's loop order
-    for (int k2 = k0; k2 <= k1; ++k2) {
-      const float tz2 = p.z - G.rad[k2 - G.rlo];
-      for (int j2 = j0; j2 <= j1; ++j2) {
-        const float ty2 = p.y - G.rad[j2 - G.rlo];
+    const float ux = (b.p.x - G.rx1) * rdx + 1.0f;
+    float nrm = 0.0f;
+    for (int k2 = b.k0; k2 <= b.k1; ++k2) {
+      const float tz2 = b.p.z - G.rad[k2 - G.rlo];
+      for (int j2 = b.j0; j2 <= b.j1; ++j2) {
+        const float ty2 = b.p.y - G.rad[j2 - G.rlo];
         const float rem = hm2 - (ty2 * ty2 + tz2 * tz2);
         if (rem < 0.0f) continue;
         const float rc = bound_sqrt(rem) * rdx + 1e-3f;
-        const int a = max(i0, (int)ceilf(ux - rc));
-        const int b = min(i1, (int)floorf(ux + rc));
+        const int a = max(b.i0, (int)ceilf(ux - rc));
+        const int e = min(b.i1, (int)floorf(ux + rc));
         const float dy2 = ty2 * ty2, dz2 = tz2 * tz2;
-        for (int i2 = a; i2 <= b; ++i2) {
+        for (int i2 = a; i2 <= e; ++i2) {
           bool ok;
-          const float wgt = ddens_w_row(G, p.x, dy2, dz2, inv_h, h2max, i2, ok);
+          const float wgt = ddens_w_row(G, b.p.x, dy2, dz2, b.inv_h, b.h2max, i2, ok);
           if (!ok) continue;
           ++pairs;
           nrm = nrm + wgt;
@@ -1926,12 +1971,9 @@ __global__ void __launch_bounds__(256) k_win_norm(VirtG V, Bins B, const float* 
       }
     }
     norm[i] = nrm;
-    if (nrm == 0.0f && home_in)  // particles.f90:549-557: all the mass to the home cell
-      atomicAdd(&acc[(size_t)(wd.ax + hx) + (size_t)A * ((size_t)(wd.ay + hy) + (size_t)A * (size_t)(wd.az + hz))],
-                (unsigned long long)llrint((double)p.w * scale));
   }
   for (int o = 32; o > 0; o >>= 1) pairs += (unsigned)__shfl_xor((int)pairs, o);
-  if ((tid & 63) == 0 && pairs) atomicAdd(npairs, (unsigned long long)pairs);
+  if ((threadIdx.x & 63) == 0 && pairs) atomicAdd(npairs, (unsigned long long)pairs);
 }
 
 // The deposits of DDENS_INTERP_SPH into the windows (particles.f90:560-618),
@@ -1940,7 +1982,7 @@ __global__ void __launch_bounds__(256) k_win_norm(VirtG V, Bins B, const float* 
 // box (clipped to the cells whose centre lies within h) meets the tile: those
 // binned within the level's largest h of the tile's cell centres.  Each wave
 // takes one such particle at a time and its lanes go through the particle's
-// cells in the tile, adding m W / NORM (NORM from k_win_norm) with LDS
+// cells in the tile, adding m W / NORM (NORM from k_norm_marked) with LDS
 // atomics; cells past h get M4 = 0 and add nothing.  The tile's sums then go
 // to the atlas once per cell.  On grids finer than the particles each
 // particle deposits into hundreds of cells, and the sums stay in LDS instead
@@ -1980,14 +2022,16 @@ struct DepBox {
   float4 p;
   float h, inv_h, h2max, nrm;
   int a, b, c, d, e, f;
+  int ix, iy, iz;  // home cell
 };
 __device__ __forceinline__ void dep_box(const DdensGeom& G, const Bins& B, const float* __restrict__ hpart,
                                         int64_t i, DepBox& q) {
   q.p = B.pos[i];
   const float hh = hpart[i];
   q.h = hh;
-  int ix, iy, iz, inc, reach;
-  ddens_setup(G, q.p.x, q.p.y, q.p.z, hh, ix, iy, iz, q.inv_h, q.h2max, inc, reach);
+  int inc, reach;
+  ddens_setup(G, q.p.x, q.p.y, q.p.z, hh, q.ix, q.iy, q.iz, q.inv_h, q.h2max, inc, reach);
+  const int ix = q.ix, iy = q.iy, iz = q.iz;
   const int n = G.n;
   const float rdx = 1.0f / G.dx;
   const float ux = (q.p.x - G.rx1) * rdx + 1.0f, uy = (q.p.y - G.rx1) * rdx + 1.0f, uz = (q.p.z - G.rx1) * rdx + 1.0f;
@@ -2131,11 +2175,19 @@ __global__ void __launch_bounds__(kDepT) k_win_dep(VirtG V, Bins B, const float*
               dep_box(G, B, hpart, i, q);
               const int mx = min(q.b, xb) - max(q.a, xa) + 1, my = min(q.d, yb) - max(q.c, ya) + 1,
                         mz = min(q.f, zb) - max(q.e, za) + 1;
-              if (mx > 0 && my > 0 && mz > 0 && norm[i] != 0.0f) {
-                if (mx * my * mz < big)
-                  list[atomicAdd(&s_ns, 1)] = (int)i;
-                else
-                  list[kDepL - 1 - atomicAdd(&s_nb, 1)] = (int)i;
+              const bool meets = mx > 0 && my > 0 && mz > 0;
+              const bool home = q.ix >= xa && q.ix <= xb && q.iy >= ya && q.iy <= yb && q.iz >= za && q.iz <= zb;
+              if (meets || home) {
+                if (norm[i] != 0.0f) {
+                  if (meets) {
+                    if (mx * my * mz < big)
+                      list[atomicAdd(&s_ns, 1)] = (int)i;
+                    else
+                      list[kDepL - 1 - atomicAdd(&s_nb, 1)] = (int)i;
+                  }
+                } else if (home) {  // particles.f90:549-557: NORM = 0, all the mass to the home cell
+                  atomicAdd(&box[q.iz + oz][q.iy + oy][q.ix + ox], (unsigned long long)llrint((double)q.p.w * scale));
+                }
               }
             }
             __syncthreads();
@@ -2234,11 +2286,16 @@ hipError_t launch_win_vvel(hipStream_t st, const Bins& B, int64_t np, const WinD
 
 hipError_t launch_win_ddens(hipStream_t st, const VirtG& V, const Bins& B, int64_t np, const float* hpart,
                             double scale, unsigned long long* acc, const WinDev* W, const int64_t* off, int32_t nwin,
-                            int64_t items, unsigned long long* npairs, float* norm, float hmax,
-                            const int64_t* dep_off, int64_t dep_items) {
+                            int64_t items, unsigned long long* npairs, float* norm, uint8_t* need,
+                            int32_t* list, int32_t* count, float hmax, const int64_t* dep_off, int64_t dep_items) {
   if (items <= 0) return hipSuccess;
-  k_win_norm<<<(unsigned)items, 256, 0, st>>>(V, B, hpart, scale, acc, W, off, nwin, npairs, norm);
-  hipError_t e = hipGetLastError();
+  hipError_t e = hipMemsetAsync(need, 0, (size_t)np, st);
+  if (e == hipSuccess) e = hipMemsetAsync(count, 0, 4, st);
+  if (e != hipSuccess) return e;
+  k_win_mark<<<(unsigned)items, 256, 0, st>>>(V, B, hpart, W, off, nwin, need);
+  k_list_marked<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(np, need, list, count);
+  k_norm_marked<<<8192, 256, 0, st>>>(V, B, hpart, list, count, norm, npairs);
+  e = hipGetLastError();
   if (e != hipSuccess || dep_items <= 0) return e;
   const float hm = hmax * 1.00001f + V.dx;
   // a wave per particle pays off where particles reach thousands of cells

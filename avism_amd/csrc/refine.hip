@@ -382,7 +382,7 @@ int refine_load(avism_ctx* c, int32_t atlas, int32_t k) {
       !c->u_mass.ensure((size_t)ncell * 4) || !c->u1co.ensure((size_t)ncell * 4) ||
       !c->div.ensure((size_t)ncell * 4) || !c->mass_fix.ensure((size_t)ncell * 8) ||
       !c->marcap.ensure((size_t)ncell * 4) || !c->partial64.ensure((size_t)finalize_blocks(ncell) * 8) ||
-      !c->hpart.ensure((size_t)np * 4) || !c->norm.ensure((size_t)np * 4) ||
+      !c->hpart.ensure((size_t)np * 4) || !c->norm.ensure((size_t)np * 4) || !c->pmark.ensure((size_t)np) ||
       !R.dwin.ensure(sizeof(WinDev) * (size_t)std::max(nw, 1)) ||
       !R.drad.ensure(gl.rad.size() * 4) || !R.dsum.ensure(1024 * 8) ||
       !c->reqp.ensure((P.reqp.size() + 1) * 4))
@@ -510,7 +510,9 @@ int refine_load(avism_ctx* c, int32_t atlas, int32_t k) {
     AV_CHECK_HIP(c, launch_fill_u64(c->stream, 1, dpairs, 0ull));
     AV_CHECK_HIP(c, launch_win_ddens(c->stream, V, B, np, c->hpart.as<float>(), scale,
                                      c->mass_fix.as<unsigned long long>(), R.dwin.as<WinDev>(), d_off + (nw + 1),
-                                     nw, off[2 * (nw + 1) - 1], dpairs, c->norm.as<float>(), hmax,
+                                     nw, off[2 * (nw + 1) - 1], dpairs, c->norm.as<float>(),
+                                     c->pmark.as<uint8_t>(), c->sbin_val.as<int32_t>(),
+                                     (int32_t*)((char*)c->scalars.p + 1800), hmax,
                                      d_off + 3 * (nw + 1), off[4 * (nw + 1) - 1]));
   }
   // U1, MEANDENS = total mass / the level's cells, U1CO and DIVERCO on the atlas
