@@ -1481,7 +1481,7 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
       AV_CHECK_HIP(c, launch_win_ddens(c->stream, V, B, np, c->hpart.as<float>(), scale,
                                        c->mass_fix.as<unsigned long long>(), dw, doff, 1, items, dpairs,
                                        c->norm.as<float>(), c->pmark.as<uint8_t>(), c->sbin_val.as<int32_t>(),
-                                       (int32_t*)((char*)c->scalars.p + 1800), hmax, doff + 2, dep));
+                                       (int32_t*)((char*)c->scalars.p + 1800), c->sort_tmp, hmax, doff + 2, dep));
     } else if (c->nb > n)  // cells split into bins
       k_ddens_scatter<4><<<ntiles, 256, 0, c->stream>>>(G, B, c->hpart.as<float>(), scale,
                                                         c->mass_fix.as<unsigned long long>(), c->norm.as<float>(),
@@ -1920,23 +1920,9 @@ __global__ void __launch_bounds__(256) k_win_mark(VirtG V, Bins B, const float* 
   }
 }
 
-// the marked particles as a list (a wave's marked lanes take consecutive
-// slots, so neighbours in the bin order stay together)
-__global__ void __launch_bounds__(256) k_list_marked(int64_t np, const uint8_t* __restrict__ need, int32_t* list,
-                                                    int32_t* count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool take = i < np && need[i] != 0;
-  const unsigned long long m = __ballot(take);
-  if (m == 0ull) return;
-  const int lane = threadIdx.x & 63, first = __ffsll((long long)m) - 1;
-  int base = 0;
-  if (lane == first) base = atomicAdd(count, __popcll(m));
-  base = __shfl(base, first);
-  if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)i;
-}
-
 // NORM of every listed particle (particles.f90:520-547, in the reference's
-// loop order, as ddens_tile computes it), one thread each
+// loop order, as ddens_tile computes it), one thread each; the list keeps the
+// bin order, so a wave's particles are neighbours
 __global__ void __launch_bounds__(256) k_norm_marked(VirtG V, Bins B, const float* __restrict__ hpart,
                                                     const int32_t* __restrict__ list, const int32_t* __restrict__ count,
                                                     float* norm, unsigned long long* npairs) {
@@ -2287,13 +2273,19 @@ hipError_t launch_win_vvel(hipStream_t st, const Bins& B, int64_t np, const WinD
 hipError_t launch_win_ddens(hipStream_t st, const VirtG& V, const Bins& B, int64_t np, const float* hpart,
                             double scale, unsigned long long* acc, const WinDev* W, const int64_t* off, int32_t nwin,
                             int64_t items, unsigned long long* npairs, float* norm, uint8_t* need,
-                            int32_t* list, int32_t* count, float hmax, const int64_t* dep_off, int64_t dep_items) {
+                            int32_t* list, int32_t* count, DBuf& sel_tmp, float hmax, const int64_t* dep_off,
+                            int64_t dep_items) {
   if (items <= 0) return hipSuccess;
   hipError_t e = hipMemsetAsync(need, 0, (size_t)np, st);
-  if (e == hipSuccess) e = hipMemsetAsync(count, 0, 4, st);
   if (e != hipSuccess) return e;
   k_win_mark<<<(unsigned)items, 256, 0, st>>>(V, B, hpart, W, off, nwin, need);
-  k_list_marked<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(np, need, list, count);
+  // the marked particles' positions, in order (rocPRIM's flagged select)
+  size_t tb = 0;
+  e = rocprim::select(nullptr, tb, rocprim::counting_iterator<int32_t>(0), need, list, count, (size_t)np, st);
+  if (e != hipSuccess) return e;
+  if (!sel_tmp.ensure(tb)) return hipErrorOutOfMemory;
+  e = rocprim::select(sel_tmp.p, tb, rocprim::counting_iterator<int32_t>(0), need, list, count, (size_t)np, st);
+  if (e != hipSuccess) return e;
   k_norm_marked<<<8192, 256, 0, st>>>(V, B, hpart, list, count, norm, npairs);
   e = hipGetLastError();
   if (e != hipSuccess || dep_items <= 0) return e;

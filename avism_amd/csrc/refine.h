@@ -42,14 +42,15 @@ hipError_t launch_win_vvel(hipStream_t st, const Bins& B, int64_t np, const WinD
                            int32_t nwin, int64_t items, const VirtG& V, int32_t k, float r0, const float4* vel,
                            uint32_t* hbits, float* u, float* v, float* w);
 // the particles that meet a window marked in need (np bytes; items: the
-// windows' bin tiles, off) and listed in list (np ints, the length in count),
-// the NORM of each into norm, then the deposits tile by tile in LDS
-// (dep_items: the windows' cell tiles, dep_off), each tile gathering the
-// particles within hmax of its cells
+// windows' bin tiles, off) and listed in list (np ints, the length in count,
+// sel_tmp the selection's scratch), the NORM of each into norm, then the
+// deposits tile by tile in LDS (dep_items: the windows' cell tiles, dep_off),
+// each tile gathering the particles within hmax of its cells
 hipError_t launch_win_ddens(hipStream_t st, const VirtG& V, const Bins& B, int64_t np, const float* hpart,
                             double scale, unsigned long long* acc, const WinDev* W, const int64_t* off, int32_t nwin,
                             int64_t items, unsigned long long* npairs, float* norm, uint8_t* need,
-                            int32_t* list, int32_t* count, float hmax, const int64_t* dep_off, int64_t dep_items);
+                            int32_t* list, int32_t* count, DBuf& sel_tmp, float hmax, const int64_t* dep_off,
+                            int64_t dep_items);
 hipError_t launch_win_marcap(hipStream_t st, const WinDev* W, const int64_t* off, int32_t nwin, int64_t items,
                              int32_t A, const int32_t* pmarca, int32_t pside, int32_t* marcap);
 hipError_t launch_max_bits(hipStream_t st, int64_t n, const uint32_t* a, uint32_t* out);

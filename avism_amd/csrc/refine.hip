@@ -512,7 +512,7 @@ int refine_load(avism_ctx* c, int32_t atlas, int32_t k) {
                                      c->mass_fix.as<unsigned long long>(), R.dwin.as<WinDev>(), d_off + (nw + 1),
                                      nw, off[2 * (nw + 1) - 1], dpairs, c->norm.as<float>(),
                                      c->pmark.as<uint8_t>(), c->sbin_val.as<int32_t>(),
-                                     (int32_t*)((char*)c->scalars.p + 1800), hmax,
+                                     (int32_t*)((char*)c->scalars.p + 1800), c->sort_tmp, hmax,
                                      d_off + 3 * (nw + 1), off[4 * (nw + 1) - 1]));
   }
   // U1, MEANDENS = total mass / the level's cells, U1CO and DIVERCO on the atlas
