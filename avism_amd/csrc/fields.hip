@@ -169,6 +169,53 @@ __global__ void k_fill_u64(int64_t n, unsigned long long* a, unsigned long long 
   if (i < n) a[i] = v;
 }
 
+// Query order of PPART: the particles grouped by blocks of 4^3 bins, in bin
+// order inside a block, so that a wave's 64 queries (about one block at one
+// particle per bin) are neighbours in 3-D rather than a run of 64 bins along
+// x; their searches then take similar paths (10% fewer PPART instructions).
+// A block's particles are its 16 rows of 4 bins, each a contiguous range.
+constexpr int kPB = 4;  // bins per block side
+__device__ __forceinline__ void pblock_rows(const Bins& B, int64_t b, int nbb, int& x0, int& x1, int& y0, int& z0,
+                                            int& ny, int& nz) {
+  const int X = (int)(b % nbb), Y = (int)((b / nbb) % nbb), Z = (int)(b / ((int64_t)nbb * nbb));
+  x0 = X * kPB;
+  x1 = min(x0 + kPB, B.nb);
+  y0 = Y * kPB;
+  z0 = Z * kPB;
+  ny = min(kPB, B.nb - y0);
+  nz = min(kPB, B.nb - z0);
+}
+__global__ void k_pblock_count(Bins B, int nbb, int32_t* cnt) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= (int64_t)nbb * nbb * nbb) return;
+  int x0, x1, y0, z0, ny, nz;
+  pblock_rows(B, b, nbb, x0, x1, y0, z0, ny, nz);
+  int c = 0;
+  for (int dz = 0; dz < nz; ++dz)
+    for (int dy = 0; dy < ny; ++dy) {
+      const int64_t row = (int64_t)B.nb * ((int64_t)(y0 + dy) + (int64_t)B.nb * (z0 + dz));
+      c += B.start[row + x1] - B.start[row + x0];
+    }
+  cnt[b] = c;
+}
+// one wave per block: its rows' particles, in order, from off[b]
+__global__ void __launch_bounds__(256) k_pblock_write(Bins B, int nbb, const int32_t* __restrict__ off,
+                                                     uint32_t* perm) {
+  const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= (int64_t)nbb * nbb * nbb) return;
+  const int lane = threadIdx.x & 63;
+  int x0, x1, y0, z0, ny, nz;
+  pblock_rows(B, b, nbb, x0, x1, y0, z0, ny, nz);
+  int o = off[b];
+  for (int dz = 0; dz < nz; ++dz)
+    for (int dy = 0; dy < ny; ++dy) {
+      const int64_t row = (int64_t)B.nb * ((int64_t)(y0 + dy) + (int64_t)B.nb * (z0 + dz));
+      const int s0 = B.start[row + x0], s1 = B.start[row + x1];
+      for (int q = lane; q < s1 - s0; q += 64) perm[o + q] = (uint32_t)(s0 + q);
+      o += s1 - s0;
+    }
+}
+
 __global__ void k_bin_keys(int64_t np, const float* x, const float* y, const float* z, Bins B,
                            uint32_t* key, uint32_t* val) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -252,11 +299,12 @@ __device__ __forceinline__ void ppart_one(const Bins& B, int64_t np, int k, floa
 
 template <int K>
 __global__ void __launch_bounds__(kq_of(K)) k_ppart(Bins B, int64_t np, int k, float r0, int uniform,
-                                                   float* pdens, float* hk) {
+                                                   float* pdens, float* hk, const uint32_t* __restrict__ perm) {
   constexpr int BQ = kq_of(K);
   __shared__ uint32_t keys_s[K * BQ], gmx_s[K / 4 * BQ];
-  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * BQ + threadIdx.x;
-  if (i >= np) return;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * BQ + threadIdx.x;
+  if (t >= np) return;
+  const int64_t i = perm ? (int64_t)perm[t] : t;
   if (B.periodic)
     ppart_one<1, K>(B, np, k, r0, uniform, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
   else
@@ -1361,10 +1409,29 @@ int fields_from_particles(avism_ctx* c, int32_t k, int32_t vel_interp, int32_t d
   if (vel_interp == 1) {
     {
       StageTimer t(c, "ppart_dens");
+      // the query order: blocks of 4^3 bins (counts, offsets, then positions)
+      const int nbb = (c->nb + kPB - 1) / kPB;
+      const int64_t nblocks = (int64_t)nbb * nbb * nbb;
+      // (in the binning scratch, free here; with far more bins than particles
+      // the counts do not fit there and the queries keep the bin order)
+      const uint32_t* perm = nullptr;
+      if (nblocks < np) {
+        int32_t* bcnt = c->sbin_key.as<int32_t>();
+        int32_t* boff = c->sbin_val.as<int32_t>();
+        k_pblock_count<<<nblk(nblocks), 256, 0, c->stream>>>(B, nbb, bcnt);
+        size_t tmp = 0;
+        AV_CHECK_HIP(c, rocprim::exclusive_scan(nullptr, tmp, bcnt, boff, 0, (size_t)nblocks,
+                                                rocprim::plus<int32_t>(), c->stream));
+        if (!c->sort_tmp.ensure(tmp)) AV_FAIL(c, AVISM_ERR_HIP, "out of device memory (scan)");
+        AV_CHECK_HIP(c, rocprim::exclusive_scan(c->sort_tmp.p, tmp, bcnt, boff, 0, (size_t)nblocks,
+                                                rocprim::plus<int32_t>(), c->stream));
+        k_pblock_write<<<nblk(nblocks, 4), 256, 0, c->stream>>>(B, nbb, boff, c->sbin_val2.as<uint32_t>());
+        perm = c->sbin_val2.as<uint32_t>();
+      }
       with_k(k, [&](auto KC) {
         constexpr int K = decltype(KC)::value;
         k_ppart<K><<<nblk(np, kq_of(K)), kq_of(K), 0, c->stream>>>(B, np, k, r0, c->mass_uniform ? 1 : 0,
-                                                                   c->pdens.as<float>(), c->hk.as<float>());
+                                                                   c->pdens.as<float>(), c->hk.as<float>(), perm);
       });
       AV_CHECK_HIP(c, hipGetLastError());
     }
