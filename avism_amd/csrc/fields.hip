@@ -191,11 +191,14 @@ __global__ void k_pblock_count(Bins B, int nbb, int32_t* cnt) {
   int x0, x1, y0, z0, ny, nz;
   pblock_rows(B, b, nbb, x0, x1, y0, z0, ny, nz);
   int c = 0;
-  for (int dz = 0; dz < nz; ++dz)
-    for (int dy = 0; dy < ny; ++dy) {
+#pragma unroll
+  for (int r = 0; r < kPB * kPB; ++r) {  // the 16 rows' loads all in flight
+    const int dz = r / kPB, dy = r % kPB;
+    if (dy < ny && dz < nz) {
       const int64_t row = (int64_t)B.nb * ((int64_t)(y0 + dy) + (int64_t)B.nb * (z0 + dz));
       c += B.start[row + x1] - B.start[row + x0];
     }
+  }
   cnt[b] = c;
 }
 // one wave per block: its rows' particles, in order, from off[b]
@@ -206,14 +209,22 @@ __global__ void __launch_bounds__(256) k_pblock_write(Bins B, int nbb, const int
   const int lane = threadIdx.x & 63;
   int x0, x1, y0, z0, ny, nz;
   pblock_rows(B, b, nbb, x0, x1, y0, z0, ny, nz);
-  int o = off[b];
-  for (int dz = 0; dz < nz; ++dz)
-    for (int dy = 0; dy < ny; ++dy) {
+  // lanes 0..15 load row dz * 4 + dy's range, then every lane writes
+  int s0 = 0, s1 = 0;
+  if (lane < kPB * kPB) {
+    const int dz = lane / kPB, dy = lane % kPB;
+    if (dy < ny && dz < nz) {
       const int64_t row = (int64_t)B.nb * ((int64_t)(y0 + dy) + (int64_t)B.nb * (z0 + dz));
-      const int s0 = B.start[row + x0], s1 = B.start[row + x1];
-      for (int q = lane; q < s1 - s0; q += 64) perm[o + q] = (uint32_t)(s0 + q);
-      o += s1 - s0;
+      s0 = B.start[row + x0];
+      s1 = B.start[row + x1];
     }
+  }
+  int o = off[b];
+  for (int r = 0; r < kPB * kPB; ++r) {
+    const int a = __shfl(s0, r), e = __shfl(s1, r);
+    for (int q = lane; q < e - a; q += 64) perm[o + q] = (uint32_t)(a + q);
+    o += e - a;
+  }
 }
 
 __global__ void k_bin_keys(int64_t np, const float* x, const float* y, const float* z, Bins B,
