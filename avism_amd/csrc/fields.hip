@@ -338,10 +338,17 @@ struct VvelOut {
 // a stale read is never above the true maximum, so no update is lost.
 inline bool vvel_chk(int64_t ncell, int64_t np) { return ncell >= 4 * np; }
 
+// the selection keeps its slots' candidates in registers (k <= 64), so that
+// the k nearest are known without a second sweep when no tie at the k-th
+// distance was turned away
+constexpr bool vvel_idx(int K) { return K <= 64; }
+
 template <int PER, int K, bool CHK = false>
 __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float r0, float dx, float qx, float qy,
                                          float qz, size_t c, uint32_t* keys, uint32_t* gmx, const VvelOut& O) {
-  Sel<kq_of(K), K / 4> h;
+  constexpr int BS = kq_of(K);
+  constexpr bool IDX = vvel_idx(K);
+  Sel<BS, K / 4, IDX> h;
   h.init(keys, gmx, k);
   bool scanned;
   const float dk2 = knn_kth<PER>(B, np, qx, qy, qz, r0, k, h, scanned);
@@ -365,8 +372,14 @@ __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float
     }
   };
   if (dk > dx) {  // k neighbours (particles.f90:866-867)
-    knn_each<PER>(B, np, qx, qy, qz, k, dk2, scanned, h, PosLoad{B.pos},
-                  [&](int j, float d2, const float4&) { serve(j, sqrtf(d2), dk); });
+    if (IDX && !scanned && h.rej == 0) {
+#pragma unroll
+      for (int t = 0; t < K; ++t)
+        if (t < k) serve((int)h.idr[t], sqrtf(u2f(keys[t * BS])), dk);
+    } else {
+      knn_each<PER>(B, np, qx, qy, qz, k, dk2, scanned, h, PosLoad{B.pos},
+                    [&](int j, float d2, const float4&) { serve(j, sqrtf(d2), dk); });
+    }
   } else {  // ball of radius DX (particles.f90:868-878): every particle with d <= DX
     const float r2 = dx * dx;
     float maxd2 = 0.0f;

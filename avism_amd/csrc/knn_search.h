@@ -116,12 +116,21 @@ AVK_DEV float u2f(uint32_t u) {
 // writes two entries, with no data-dependent branches.  Slots at or beyond k
 // hold 0 and never become a maximum.  The same structure selects the m
 // smallest input indices among tied candidates.
-template <int BS, int NG>
+//
+// IDX: each slot's candidate (its sorted position) is also kept, in
+// registers (idr, written with a select per slot so that it never leaves
+// them), and rej counts the candidates equal to the worst value that were
+// turned away or evicted since the worst took that value.  A search that ends
+// with rej == 0 holds every candidate at the k-th distance, so its slots are
+// exactly the k neighbours.
+template <int BS, int NG, bool IDX = false>
 struct Sel {
   uint32_t* keys;  // this lane's 4*NG slots in LDS
   uint32_t* gmx;   // this lane's NG group maxima in LDS
   uint32_t worst;  // largest value held once full (~0 while filling)
   int wg, k, cnt;
+  uint32_t rej;
+  uint32_t idr[IDX ? 4 * NG : 1];
 
   AVK_DEV void init(uint32_t* ks, uint32_t* gm, int k_) {
     keys = ks;
@@ -133,8 +142,22 @@ struct Sel {
     cnt = 0;
     worst = ~0u;
     wg = 0;
+    rej = 0;
 #pragma unroll
     for (int t = 0; t < 4 * NG; ++t) keys[t * BS] = 0u;
+  }
+  // slot = 4 * g + s: one compare per group and per position, their products
+  // formed on the scalar unit
+  AVK_DEV void set_idx(int g, int s, uint32_t j) {
+    if (IDX) {
+      bool mg[NG], ms[4];
+#pragma unroll
+      for (int t = 0; t < NG; ++t) mg[t] = t == g;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) ms[t] = t == s;
+#pragma unroll
+      for (int t = 0; t < 4 * NG; ++t) idr[t] = mg[t / 4] && ms[t % 4] ? j : idr[t];
+    }
   }
   AVK_DEV bool full() const { return cnt >= k; }
   // the k-th smallest value as a float (INFINITY while filling)
@@ -152,10 +175,14 @@ struct Sel {
     worst = w;
     wg = g0;
   }
-  AVK_DEV void consider(uint32_t key) {
-    if (key >= worst) return;  // never true while filling
+  AVK_DEV void consider(uint32_t key, uint32_t j = 0) {
+    if (key >= worst) {  // never true while filling
+      if (IDX && key == worst) ++rej;
+      return;
+    }
     if (cnt < k) {
       keys[cnt * BS] = key;
+      set_idx(cnt >> 2, cnt & 3, j);
       if (++cnt == k) {  // full: the group maxima, then the worst
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
@@ -182,11 +209,15 @@ struct Sel {
 #pragma unroll
     for (int s = 0; s < 4; ++s) x[s] = s == p ? key : x[s];
     grp[p * BS] = key;
+    set_idx(wg, p, j);
     uint32_t m = x[0];
 #pragma unroll
     for (int s = 1; s < 4; ++s) m = x[s] > m ? x[s] : m;
     gmx[wg * BS] = m;
+    const uint32_t w0 = worst;
     find_worst();
+    // the evicted entry held w0: still the worst, it is a tie turned away
+    if (IDX) rej = worst == w0 ? rej + 1 : 0u;
   }
 };
 
@@ -388,8 +419,8 @@ struct KthVisitor {
   S& sel;
   float R2, lo2;
   AVK_DEV float bound() const { return fminf(R2, sel.wd()); }
-  AVK_DEV void cand(float d2, int, const Rec&) {
-    if (d2 <= R2 && d2 > lo2) sel.consider(f2u(d2));
+  AVK_DEV void cand(float d2, int j, const Rec&) {
+    if (d2 <= R2 && d2 > lo2) sel.consider(f2u(d2), (uint32_t)j);
   }
 };
 
@@ -413,7 +444,7 @@ AVK_DEV float knn_kth(const Bins& B, int64_t np, float qx, float qy, float qz, f
   sel.reset(k);
   for (int64_t j = 0; j < np; ++j) {
     const float4 p = B.pos[j];
-    sel.consider(f2u(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, PER)));
+    sel.consider(f2u(dist2(p.x, p.y, p.z, qx, qy, qz, B.L, PER)), (uint32_t)j);
   }
   return sel.wd();
 }
