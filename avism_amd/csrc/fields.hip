@@ -354,13 +354,12 @@ __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float
   const float dk2 = knn_kth<PER>(B, np, qx, qy, qz, r0, k, h, scanned);
   const float dk = sqrtf(dk2);
   double sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-  auto serve = [&](int j, float d, float hh) {
-    // particles.f90:882-885.  The atomic returns nothing, so the lane does
-    // not wait for it (reading HPART first to skip it costs a load latency
-    // per neighbour, which measured slower).
+  // particles.f90:882-885; old: HPART as read before (0: not read).  The
+  // atomic returns nothing, so the lane does not wait for it.
+  auto serve_v = [&](int j, float d, float hh, uint32_t old) {
     const float4 vv = O.want_vel ? O.vel[j] : make_float4(0.f, 0.f, 0.f, 0.f);  // vx, vy, vz, m/rho
     const uint32_t hb = __float_as_uint(d);
-    if (!CHK || hb > O.hbits[j]) atomicMax(&O.hbits[j], hb);
+    if (hb > old) atomicMax(&O.hbits[j], hb);
     if (O.want_vel) {
       const float qq = d / (hh / 2.0f);                     // particles.f90:941
       const float wgt = m4(qq) * vv.w;                      // 897-900
@@ -371,11 +370,21 @@ __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float
       sz += w8 * (double)vv.z;
     }
   };
+  // one neighbour found by a sweep (reading HPART first there costs a load
+  // latency per neighbour, so only CHK does)
+  auto serve = [&](int j, float d, float hh) { serve_v(j, d, hh, CHK ? O.hbits[j] : 0u); };
   if (dk > dx) {  // k neighbours (particles.f90:866-867)
     if (IDX && !scanned && h.rej == 0) {
+      // the k neighbours are known: their HPART values are read first, all
+      // loads in flight together, and an atomic is skipped where HPART
+      // already holds at least this distance (a stale value is never above
+      // the true maximum)
+      uint32_t old[K];
+#pragma unroll
+      for (int t = 0; t < K; ++t) old[t] = t < k ? O.hbits[h.idr[t]] : 0u;
 #pragma unroll
       for (int t = 0; t < K; ++t)
-        if (t < k) serve((int)h.idr[t], sqrtf(u2f(keys[t * BS])), dk);
+        if (t < k) serve_v((int)h.idr[t], sqrtf(u2f(keys[t * BS])), dk, old[t]);
     } else {
       knn_each<PER>(B, np, qx, qy, qz, k, dk2, scanned, h, PosLoad{B.pos},
                     [&](int j, float d2, const float4&) { serve(j, sqrtf(d2), dk); });
