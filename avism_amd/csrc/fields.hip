@@ -274,6 +274,10 @@ __global__ void k_gather_sorted(int64_t np, const uint32_t* perm, const float* x
 // threads per block of the kNN kernels: a lane keeps its k selection slots
 // in LDS (5K/4 words), so K = 128 and 256 run 64 lanes a block (40 and 80 KB)
 constexpr int kq_of(int K) { return K <= 64 ? 128 : 64; }
+// the selection keeps its slots' candidates in registers (k <= 64), so that
+// the k nearest are known without a second sweep when no tie at the k-th
+// distance was turned away
+constexpr bool knn_idx(int K) { return K <= 64; }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
 // and each XCD has its own L2.  Renumbering the blocks so that every XCD gets
@@ -287,21 +291,31 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
 
 // uniform: every particle has the same mass, so the FP64 sum of the k
 // nearest masses is k * m exactly (each partial sum j * m, j <= 64, needs at
-// most 30 significant bits) and the neighbours themselves are not needed
-template <int PER, int K>
+// most 30 significant bits) and the neighbours themselves are not needed.
+// Otherwise (IDX) the masses of the selection's slots are summed when they
+// are the k nearest, else those of a second sweep.
+template <int PER, int K, bool IDX>
 __device__ __forceinline__ void ppart_one(const Bins& B, int64_t np, int k, float r0, int uniform, int64_t i,
                                           uint32_t* keys, uint32_t* gmx, float* pdens, float* hk) {
   const float4 pq = B.pos[i];
-  Sel<kq_of(K), K / 4> h;
+  Sel<kq_of(K), K / 4, IDX> h;
   h.init(keys, gmx, k);
   bool scanned;
   const float dk2 = knn_kth<PER>(B, np, pq.x, pq.y, pq.z, r0, k, h, scanned);
   double bas8 = 0.0;  // particles.f90:779-788
-  if (uniform)
+  if (uniform) {
     bas8 = (double)k * (double)pq.w;
-  else
+  } else if (IDX && !scanned && h.rej == 0) {
+    float m[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) m[t] = t < k ? B.pos[h.idr[t]].w : 0.0f;
+#pragma unroll
+    for (int t = 0; t < K; ++t)
+      if (t < k) bas8 += (double)m[t];
+  } else {
     knn_each<PER>(B, np, pq.x, pq.y, pq.z, k, dk2, scanned, h, PosLoad{B.pos},
                   [&](int, float, const float4& p) { bas8 += (double)p.w; });
+  }
   const float dk = sqrtf(dk2);  // kdtree.f90:539
   hk[i] = dk;
   const double vol = (4.0 / 3.0 * kPi) * (double)((dk * dk) * dk);
@@ -316,10 +330,17 @@ __global__ void __launch_bounds__(kq_of(K)) k_ppart(Bins B, int64_t np, int k, f
   const int64_t t = xcd_block(blockIdx.x, gridDim.x) * BQ + threadIdx.x;
   if (t >= np) return;
   const int64_t i = perm ? (int64_t)perm[t] : t;
-  if (B.periodic)
-    ppart_one<1, K>(B, np, k, r0, uniform, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
-  else
-    ppart_one<0, K>(B, np, k, r0, uniform, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
+  if (B.periodic) {
+    if (uniform)
+      ppart_one<1, K, false>(B, np, k, r0, 1, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
+    else
+      ppart_one<1, K, knn_idx(K)>(B, np, k, r0, 0, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
+  } else {
+    if (uniform)
+      ppart_one<0, K, false>(B, np, k, r0, 1, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
+    else
+      ppart_one<0, K, knn_idx(K)>(B, np, k, r0, 0, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, pdens, hk);
+  }
 }
 
 // VVEL_INTERP_SPH_VW (particles.f90:806-930): per cell the k nearest (or,
@@ -338,16 +359,11 @@ struct VvelOut {
 // a stale read is never above the true maximum, so no update is lost.
 inline bool vvel_chk(int64_t ncell, int64_t np) { return ncell >= 4 * np; }
 
-// the selection keeps its slots' candidates in registers (k <= 64), so that
-// the k nearest are known without a second sweep when no tie at the k-th
-// distance was turned away
-constexpr bool vvel_idx(int K) { return K <= 64; }
-
 template <int PER, int K, bool CHK = false>
 __device__ __forceinline__ void vvel_one(const Bins& B, int64_t np, int k, float r0, float dx, float qx, float qy,
                                          float qz, size_t c, uint32_t* keys, uint32_t* gmx, const VvelOut& O) {
   constexpr int BS = kq_of(K);
-  constexpr bool IDX = vvel_idx(K);
+  constexpr bool IDX = knn_idx(K);
   Sel<BS, K / 4, IDX> h;
   h.init(keys, gmx, k);
   bool scanned;
@@ -1012,10 +1028,18 @@ __global__ void __launch_bounds__(kBatchQ) k_batch_ppart(const PatchDev* __restr
     const PatchDev& D = P[p];
     const int64_t i = (it - q.off[p]) * kBatchQ + threadIdx.x;
     if (i >= D.np) continue;
-    if (D.B.periodic)
-      ppart_one<1, K>(D.B, D.np, k, D.r0, D.uniform, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, D.pdens, D.hk);
-    else
-      ppart_one<0, K>(D.B, D.np, k, D.r0, D.uniform, i, keys_s + threadIdx.x, gmx_s + threadIdx.x, D.pdens, D.hk);
+    uint32_t *ks = keys_s + threadIdx.x, *gm = gmx_s + threadIdx.x;
+    if (D.B.periodic) {
+      if (D.uniform)
+        ppart_one<1, K, false>(D.B, D.np, k, D.r0, 1, i, ks, gm, D.pdens, D.hk);
+      else
+        ppart_one<1, K, knn_idx(K)>(D.B, D.np, k, D.r0, 0, i, ks, gm, D.pdens, D.hk);
+    } else {
+      if (D.uniform)
+        ppart_one<0, K, false>(D.B, D.np, k, D.r0, 1, i, ks, gm, D.pdens, D.hk);
+      else
+        ppart_one<0, K, knn_idx(K)>(D.B, D.np, k, D.r0, 0, i, ks, gm, D.pdens, D.hk);
+    }
   }
 }
 
