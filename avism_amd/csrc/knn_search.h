@@ -380,6 +380,9 @@ AVK_DEV bool ring_sweep(const Bins& B, float qx, float qy, float qz, V& v) {
       adv();
       typename V::Rec p{};
       if (j < je) p = v.ld.load(j);
+      // two candidates per trip: the loaded record alternates between two
+      // register sets instead of being copied each candidate
+#pragma unroll 2
       while (j < je) {
         const typename V::Rec pc = p;
         const int jc = j;
