@@ -269,8 +269,9 @@ __global__ void k_gather_sorted(int64_t np, const uint32_t* perm, const float* x
 
 // PPART_DENS (particles.f90:742-802): rho_p = (FP64 sum of the k nearest
 // masses, self included) / (4/3 pi d_k^3); d_k is kept for FILL_H_ZEROS.
-// One particle per lane, in bin-sorted order (a wave's queries are neighbours).
-// The search returns input indices, so the masses are read in input order.
+// One particle per lane, the queries taken in blocks of 4^3 bins (a wave's
+// queries are neighbours); the masses come with the bin-sorted records.
+
 // threads per block of the kNN kernels: a lane keeps its k selection slots
 // in LDS (5K/4 words), so K = 128 and 256 run 64 lanes a block (40 and 80 KB)
 constexpr int kq_of(int K) { return K <= 64 ? 128 : 64; }
@@ -354,9 +355,12 @@ struct VvelOut {
   int want_vel;
 };
 
-// CHK: read HPART before the atomic and skip it when the value is not larger.
-// For grids finer than the particles, where each particle serves many cells;
-// a stale read is never above the true maximum, so no update is lost.
+// CHK: also where the neighbours come from a sweep (ties at d_k, ball mode),
+// read HPART before the atomic and skip it when the value is not larger.  For
+// grids finer than the particles, where each particle serves many cells; a
+// stale read is never above the true maximum, so no update is lost.  (When
+// the selection's slots are the neighbours, their reads are always made, all
+// together, before the atomics.)
 inline bool vvel_chk(int64_t ncell, int64_t np) { return ncell >= 4 * np; }
 
 template <int PER, int K, bool CHK = false>
