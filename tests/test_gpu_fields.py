@@ -284,11 +284,14 @@ def test_particles_outside_box_rejected():
     assert e.value.code == -3
 
 
+@pytest.mark.parametrize("mixed", [0, 1])
 @pytest.mark.parametrize("per", [0, 1])
-def test_fields_lattice_ties(per):
+def test_fields_lattice_ties(per, mixed):
     """Particles on the cell centres of a perfect lattice: every query has many
     neighbours at exactly equal distances, so the tie rule (lower input index
-    wins) decides the neighbour sets; HPART must still match bit for bit."""
+    wins) decides the neighbour sets; HPART must still match bit for bit.
+    With unequal masses PART_DENS sums the masses of exactly those sets, so a
+    wrong member among the ties shows in it."""
     import pyoracle as O
     n, L0 = 16, 32.0
     d = L0 / n
@@ -299,7 +302,9 @@ def test_fields_lattice_ties(per):
     arr = [a.reshape(-1)[perm].astype(np.float32) for a in (x, y, z)]
     arr += [rng.standard_normal(n ** 3).astype(np.float32) * 100 for _ in range(3)]
     arr += [np.full(n ** 3, 1e10, np.float32)]
-    print(f"[fields lattice periodic={per}]", flush=True)
+    if mixed:
+        arr[6] = (arr[6] * rng.uniform(0.5, 1.5, n ** 3)).astype(np.float32)
+    print(f"[fields lattice periodic={per} mixed={mixed}]", flush=True)
     ref = O.fields_from_particles(*arr, n, L0, per, k=32)
     _, f, pd, hp = _gpu_fields(arr, n, L0, per, 32)
     assert np.array_equal(hp, ref["hpart"])
